@@ -1,0 +1,907 @@
+// fs_host.cpp -- C ABI of libflexshell: contexts, models, CSR pattern and
+// scatter-plan construction, and the launch sequences of the hot path.
+//
+// Nothing here computes element or matrix values on the host: the host only
+// builds integer structure (node adjacency, CSR pattern, pair tables) and
+// small per-material factors; all floating-point work of stiffness, mass,
+// normals and the time loop runs in the kernels of fs_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/flexshell.h"
+#include "fs_device.h"
+
+namespace fs {
+hipError_t launch_element_blocks(const ElemParams &p, const double *x, const double *y, const double *z,
+                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
+                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st);
+hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, int nblk, double *vals, hipStream_t st);
+hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const double *y, const double *z,
+                          const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
+                          double *nx, double *ny, double *nz, uint8_t *valid, hipStream_t st);
+hipError_t launch_mass(int nen, int64_t N, int E, int composite, double rho, double t, const double *g_inertia,
+                       const int32_t *elem_group, const double *x, const double *y, const double *z,
+                       const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, const int64_t *node_dof,
+                       int64_t nrows, double *diag, hipStream_t st);
+hipError_t launch_cd_init(int64_t n, double dt, double g0, const double *invMC, const double *Fmag, const double *U0,
+                          const double *V, double *A, double *U1, hipStream_t st);
+hipError_t launch_cd_step(int tpr, int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval,
+                          const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
+                          const double *Cd, const double *Fmag, double gstep, double dt, int damped,
+                          const int32_t *probe_slot, double *probe_dst, hipStream_t st);
+hipError_t launch_spmv(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval, const double *x,
+                       double *y, hipStream_t st);
+hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, const double *d, double *part,
+                      double *out, hipStream_t st);
+hipError_t launch_scale(int64_t n, const double *ss, const double *d, const double *x, double *y, hipStream_t st);
+}  // namespace fs
+
+struct fs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+struct Block {
+    bool built = false;
+    int64_t nrows = 0;
+    int64_t nnz = 0;
+    std::vector<int64_t> rowptr;   // nrows + 1
+    std::vector<int32_t> colval;   // nnz
+    // scatter plan (device)
+    int64_t npairs = 0;
+    DevBuf pair_I, pair_J, pair_off, contrib_ptr, contrib, node_rowstart, node_code;
+    // device CSR (int32 for the time loop) and values
+    DevBuf rowptr32, colidx32, kval, mdiag;
+    bool k_ready = false, m_ready = false;
+    void reset() {
+        built = k_ready = m_ready = false;
+        nrows = nnz = npairs = 0;
+        rowptr.clear();
+        colval.clear();
+        for (DevBuf *b : {&pair_I, &pair_J, &pair_off, &contrib_ptr, &contrib, &node_rowstart, &node_code, &rowptr32,
+                          &colidx32, &kval, &mdiag})
+            b->release();
+    }
+};
+
+}  // namespace
+
+struct fs_model {
+    fs_ctx *ctx = nullptr;
+    int nen = 0;
+    int64_t N = 0, E = 0;
+    std::vector<int32_t> conn;       // E x nen row-major, 0-based (host)
+    std::vector<int32_t> ne_ptr, ne_list;  // node -> elements (ascending)
+    std::vector<int32_t> nbr_ptr, nbr_list;  // node -> neighbour nodes (sorted)
+    DevBuf x, y, z, dconn, dne_ptr, dne_list, nx, ny, nz, valid, node_dof, Kb, status;
+    bool normals_ready = false;
+    bool dofs_ready = false;
+    std::vector<int64_t> dof;  // N x 6 row-major, 0-based
+    int64_t nall = 0, nfree = 0;
+    Block blk[2];
+    // group data for the last stiffness/mass call
+    DevBuf g_Labd, g_Lh, g_csys, g_t, g_group, g_inertia, m_group;
+    double timing[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(fs_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define FS_HIP(ctx, call)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return fail((ctx), FS_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+int dalloc(fs_ctx *c, DevBuf &b, size_t bytes) {
+    b.release();
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, FS_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return FS_OK;
+}
+
+template <class T>
+int upload(fs_ctx *c, DevBuf &b, const T *src, size_t n) {
+    int s = dalloc(c, b, n * sizeof(T));
+    if (s) return s;
+    if (n) FS_HIP(c, hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return FS_OK;
+}
+
+// Upper Cholesky factor of a symmetric n x n matrix (row-major): M = L' L.
+bool chol_upper(int n, const double *M, double *L) {
+    for (int i = 0; i < n * n; ++i) L[i] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double d = M[i * n + i];
+        for (int k = 0; k < i; ++k) d -= L[k * n + i] * L[k * n + i];
+        if (!(d > 0.0)) return false;
+        const double lii = std::sqrt(d);
+        L[i * n + i] = lii;
+        for (int j = i + 1; j < n; ++j) {
+            double s = M[i * n + j];
+            for (int k = 0; k < i; ++k) s -= L[k * n + i] * L[k * n + j];
+            L[i * n + j] = s / lii;
+        }
+    }
+    return true;
+}
+
+// column-major n x n -> row-major
+void cm2rm(int n, const double *cm, double *rm) {
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) rm[i * n + j] = cm[j * n + i];
+}
+
+int build_block(fs_model *m, int which) {
+    fs_ctx *c = m->ctx;
+    Block &B = m->blk[which];
+    if (B.built) return FS_OK;
+    if (!m->dofs_ready) return fail(c, FS_ERR_STATE, "pattern: equation numbers not set (fs_model_set_dofs)");
+    const int64_t N = m->N;
+    const int64_t nrows = which == FS_BLOCK_FF ? m->nfree : m->nall;
+    const int64_t nfree = m->nfree;
+    // per-node codes: offset of each comp within the node's free / fixed run
+    std::vector<int8_t> code(N * 6, -1);
+    std::vector<int8_t> nfr(N, 0), nfx(N, 0);
+    std::vector<int64_t> minfr(N, INT64_MAX), minfx(N, INT64_MAX);
+    for (int64_t I = 0; I < N; ++I) {
+        int64_t fr[6], fx[6];
+        int a = 0, b = 0;
+        for (int cc = 0; cc < 6; ++cc) {
+            const int64_t d = m->dof[I * 6 + cc];
+            if (d >= nrows) continue;
+            if (d < nfree)
+                fr[a++] = d;
+            else
+                fx[b++] = d;
+        }
+        std::sort(fr, fr + a);
+        std::sort(fx, fx + b);
+        if (a && fr[a - 1] - fr[0] + 1 != a)
+            return fail(c, FS_ERR_UNSUPPORTED, "node %lld: free dofs are not consecutive numbers", (long long)I);
+        if (b && fx[b - 1] - fx[0] + 1 != b)
+            return fail(c, FS_ERR_UNSUPPORTED, "node %lld: fixed dofs are not consecutive numbers", (long long)I);
+        for (int cc = 0; cc < 6; ++cc) {
+            const int64_t d = m->dof[I * 6 + cc];
+            if (d >= nrows) continue;
+            if (d < nfree)
+                code[I * 6 + cc] = (int8_t)(d - fr[0]);
+            else
+                code[I * 6 + cc] = (int8_t)(8 + (d - fx[0]));
+        }
+        nfr[I] = (int8_t)a;
+        nfx[I] = (int8_t)b;
+        if (a) minfr[I] = fr[0];
+        if (b) minfx[I] = fx[0];
+    }
+    // per node: sorted column list = runs of the neighbours ordered by first dof
+    // (runs are contiguous integer ranges, so sorting runs sorts the columns)
+    std::vector<int32_t> rowlen(N, 0);
+    std::vector<int32_t> nb_offfree(m->nbr_list.size(), -1), nb_offfix(m->nbr_list.size(), -1);
+    {
+        std::vector<std::pair<int64_t, int64_t>> runs;  // (first dof, nbr slot*2 + kind)
+        for (int64_t I = 0; I < N; ++I) {
+            runs.clear();
+            for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
+                const int J = m->nbr_list[k];
+                if (nfr[J]) runs.push_back({minfr[J], (int64_t)k * 2 + 0});
+                if (nfx[J]) runs.push_back({minfx[J], (int64_t)k * 2 + 1});
+            }
+            std::sort(runs.begin(), runs.end());
+            int32_t pos = 0;
+            for (auto &r : runs) {
+                const int64_t k = r.second / 2;
+                const int J = m->nbr_list[k];
+                if (r.second % 2 == 0) {
+                    nb_offfree[k] = pos;
+                    pos += nfr[J];
+                } else {
+                    nb_offfix[k] = pos;
+                    pos += nfx[J];
+                }
+            }
+            rowlen[I] = pos;
+        }
+    }
+    // row owners
+    std::vector<int64_t> owner(nrows, -1);
+    for (int64_t I = 0; I < N; ++I)
+        for (int cc = 0; cc < 6; ++cc) {
+            const int64_t d = m->dof[I * 6 + cc];
+            if (d < nrows) owner[d] = I;
+        }
+    B.rowptr.assign(nrows + 1, 0);
+    for (int64_t d = 0; d < nrows; ++d) {
+        if (owner[d] < 0) return fail(c, FS_ERR_ARG, "dof %lld is not used by any node", (long long)d);
+        B.rowptr[d + 1] = B.rowptr[d] + rowlen[owner[d]];
+    }
+    const int64_t nnz = B.rowptr[nrows];
+    try {
+        B.colval.assign(nnz, 0);
+    } catch (const std::bad_alloc &) {
+        return fail(c, FS_ERR_NOMEM, "host pattern of %lld entries", (long long)nnz);
+    }
+    {
+        std::vector<int32_t> cols;
+        for (int64_t I = 0; I < N; ++I) {
+            cols.assign(rowlen[I], 0);
+            for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
+                const int J = m->nbr_list[k];
+                for (int cc = 0; cc < 6; ++cc) {
+                    const int cd = code[(int64_t)J * 6 + cc];
+                    if (cd < 0) continue;
+                    const int pos = cd < 8 ? nb_offfree[k] + cd : nb_offfix[k] + cd - 8;
+                    cols[pos] = (int32_t)m->dof[(int64_t)J * 6 + cc];
+                }
+            }
+            for (int cc = 0; cc < 6; ++cc) {
+                const int64_t d = m->dof[I * 6 + cc];
+                if (d >= nrows) continue;
+                std::copy(cols.begin(), cols.end(), B.colval.begin() + B.rowptr[d]);
+            }
+        }
+    }
+    // node row starts
+    std::vector<int64_t> rowstart(N * 6, -1);
+    for (int64_t I = 0; I < N; ++I)
+        for (int cc = 0; cc < 6; ++cc) {
+            const int64_t d = m->dof[I * 6 + cc];
+            if (d < nrows) rowstart[I * 6 + cc] = B.rowptr[d];
+        }
+    // upper pairs and their offsets
+    std::vector<int64_t> pair_base(N + 1, 0);
+    for (int64_t I = 0; I < N; ++I) {
+        int cnt = 0;
+        for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k)
+            if (m->nbr_list[k] >= I) ++cnt;
+        pair_base[I + 1] = pair_base[I] + cnt;
+    }
+    const int64_t npairs = pair_base[N];
+    std::vector<int32_t> pI(npairs), pJ(npairs), poff(npairs * 4);
+    auto slot_of = [&](int64_t I, int64_t J) -> int {
+        const int32_t *b = m->nbr_list.data() + m->nbr_ptr[I];
+        const int32_t *e = m->nbr_list.data() + m->nbr_ptr[I + 1];
+        const int32_t *it = std::lower_bound(b, e, (int32_t)J);
+        return (int)(it - m->nbr_list.data());
+    };
+    for (int64_t I = 0; I < N; ++I) {
+        int64_t p = pair_base[I];
+        for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
+            const int J = m->nbr_list[k];
+            if (J < I) continue;
+            pI[p] = (int32_t)I;
+            pJ[p] = J;
+            poff[p * 4 + 0] = nb_offfree[k];
+            poff[p * 4 + 1] = nb_offfix[k];
+            const int k2 = slot_of(J, I);
+            poff[p * 4 + 2] = nb_offfree[k2];
+            poff[p * 4 + 3] = nb_offfix[k2];
+            ++p;
+        }
+    }
+    // contributions, element order within each pair
+    const int nen = m->nen;
+    const int nblk = nen * (nen + 1) / 2;
+    std::vector<int32_t> cptr(npairs + 1, 0);
+    std::vector<int64_t> cpair((size_t)m->E * nblk);
+    for (int64_t e = 0; e < m->E; ++e) {
+        int bl = 0;
+        for (int a = 0; a < nen; ++a)
+            for (int b = a; b < nen; ++b, ++bl) {
+                const int64_t I = m->conn[e * nen + a], J = m->conn[e * nen + b];
+                const int64_t lo = std::min(I, J), hi = std::max(I, J);
+                const int k = slot_of(lo, hi);
+                // pair index: pairs of lo are the neighbours >= lo in order
+                const int kfirst = slot_of(lo, lo);
+                const int64_t p = pair_base[lo] + (k - kfirst);
+                cpair[e * nblk + bl] = p;
+                cptr[p + 1]++;
+            }
+    }
+    for (int64_t p = 0; p < npairs; ++p) cptr[p + 1] += cptr[p];
+    if ((int64_t)m->E * nblk >= (int64_t(1) << 30))
+        return fail(c, FS_ERR_UNSUPPORTED, "too many element blocks (%lld)", (long long)(m->E * nblk));
+    std::vector<int32_t> cfill(cptr.begin(), cptr.end() - 1), clist(cptr[npairs]);
+    for (int64_t e = 0; e < m->E; ++e) {
+        int bl = 0;
+        for (int a = 0; a < nen; ++a)
+            for (int b = a; b < nen; ++b, ++bl) {
+                const int64_t I = m->conn[e * nen + a], J = m->conn[e * nen + b];
+                const int tr = (I > J) ? 1 : 0;
+                const int64_t p = cpair[e * nblk + bl];
+                clist[cfill[p]++] = (int32_t)(((e * nblk + bl) << 1) | tr);
+            }
+    }
+    // upload
+    int s;
+    if ((s = upload(c, B.pair_I, pI.data(), pI.size()))) return s;
+    if ((s = upload(c, B.pair_J, pJ.data(), pJ.size()))) return s;
+    if ((s = upload(c, B.pair_off, poff.data(), poff.size()))) return s;
+    if ((s = upload(c, B.contrib_ptr, cptr.data(), cptr.size()))) return s;
+    if ((s = upload(c, B.contrib, clist.data(), clist.size()))) return s;
+    if ((s = upload(c, B.node_rowstart, rowstart.data(), rowstart.size()))) return s;
+    if ((s = upload(c, B.node_code, code.data(), code.size()))) return s;
+    if (nnz < (int64_t(1) << 31)) {
+        std::vector<int32_t> rp32(nrows + 1);
+        for (int64_t i = 0; i <= nrows; ++i) rp32[i] = (int32_t)B.rowptr[i];
+        if ((s = upload(c, B.rowptr32, rp32.data(), rp32.size()))) return s;
+        if ((s = upload(c, B.colidx32, B.colval.data(), B.colval.size()))) return s;
+    }
+    if ((s = dalloc(c, B.kval, (size_t)nnz * sizeof(double)))) return s;
+    if ((s = dalloc(c, B.mdiag, (size_t)nrows * sizeof(double)))) return s;
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    B.npairs = npairs;
+    B.nrows = nrows;
+    B.nnz = nnz;
+    B.built = true;
+    B.k_ready = B.m_ready = false;
+    return FS_OK;
+}
+
+fs::ScatterPlan plan_of(const Block &B) {
+    fs::ScatterPlan sp;
+    sp.npairs = B.npairs;
+    sp.pair_I = B.pair_I.as<int32_t>();
+    sp.pair_J = B.pair_J.as<int32_t>();
+    sp.pair_off = B.pair_off.as<int32_t>();
+    sp.contrib_ptr = B.contrib_ptr.as<int32_t>();
+    sp.contrib = B.contrib.as<int32_t>();
+    sp.node_rowstart = B.node_rowstart.as<int64_t>();
+    sp.node_code = B.node_code.as<int8_t>();
+    return sp;
+}
+
+// Compute element node-pair blocks into m->Kb (device).
+int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
+    fs_ctx *c = m->ctx;
+    if (!p) return fail(c, FS_ERR_ARG, "stiffness: NULL parameters");
+    if (!m->normals_ready) return fail(c, FS_ERR_STATE, "stiffness: normals not computed (associategeometry!)");
+    if (p->formulation != FS_SHEAR_AVERAGE_B && p->formulation != FS_SHEAR_AVERAGE_K)
+        return fail(c, FS_ERR_ARG, "stiffness: unknown transverse shear formulation %d", p->formulation);
+    if (m->nen == FS_Q4 && p->formulation != FS_SHEAR_AVERAGE_B)
+        return fail(c, FS_ERR_ARG, "stiffness: the Q4 family has a single (MITC) shear formulation");
+    fs::ElemParams ep;
+    std::memset(&ep, 0, sizeof ep);
+    ep.nen = m->nen;
+    ep.composite = p->composite ? 1 : 0;
+    ep.formulation = p->formulation;
+    ep.nelem = (int32_t)m->E;
+    ep.nnodes = m->N;
+    ep.thickness = p->thickness;
+    ep.stab_c = p->stab_c;
+    ep.drill_scale = p->drilling_stiffness_scale;
+    int s;
+    if (!p->composite) {
+        double Dps[9], Dt[4];
+        cm2rm(3, p->Dps, Dps);
+        cm2rm(2, p->Dt, Dt);
+        for (int i = 0; i < 4; ++i) Dt[i] *= 5.0 / 6.0;  // FEMMShellT3FFModule.jl:658-659
+        if (!chol_upper(3, Dps, ep.Lps)) return fail(c, FS_ERR_NUMERIC, "stiffness: Dps is not positive definite");
+        if (!chol_upper(2, Dt, ep.Lt)) return fail(c, FS_ERR_NUMERIC, "stiffness: Dt is not positive definite");
+        if (!(p->thickness > 0.0)) return fail(c, FS_ERR_ARG, "stiffness: thickness must be positive");
+    } else {
+        const int ng = p->ngroups;
+        if (ng < 1 || !p->group_ABDH || !p->group_csys || !p->group_thickness)
+            return fail(c, FS_ERR_ARG, "stiffness: layered shell needs ngroups >= 1 and group arrays");
+        std::vector<double> Labd(36 * ng), Lh(4 * ng), csys(9 * ng), tg(ng);
+        for (int g = 0; g < ng; ++g) {
+            const double *src = p->group_ABDH + 31 * g;
+            double A[9], Bm[9], D[9], H[4], ABD[36];
+            cm2rm(3, src, A);
+            cm2rm(3, src + 9, Bm);
+            cm2rm(3, src + 18, D);
+            cm2rm(2, src + 27, H);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    ABD[i * 6 + j] = A[i * 3 + j];
+                    ABD[i * 6 + 3 + j] = Bm[i * 3 + j];
+                    ABD[(3 + i) * 6 + j] = Bm[j * 3 + i];
+                    ABD[(3 + i) * 6 + 3 + j] = D[i * 3 + j];
+                }
+            if (!chol_upper(6, ABD, &Labd[36 * g]))
+                return fail(c, FS_ERR_NUMERIC, "stiffness: laminate [A B; B D] of group %d is not positive definite", g);
+            if (!chol_upper(2, H, &Lh[4 * g]))
+                return fail(c, FS_ERR_NUMERIC, "stiffness: laminate H of group %d is not positive definite", g);
+            cm2rm(3, p->group_csys + 9 * g, &csys[9 * g]);
+            tg[g] = p->group_thickness[g];
+        }
+        if ((s = upload(c, m->g_Labd, Labd.data(), Labd.size()))) return s;
+        if ((s = upload(c, m->g_Lh, Lh.data(), Lh.size()))) return s;
+        if ((s = upload(c, m->g_csys, csys.data(), csys.size()))) return s;
+        if ((s = upload(c, m->g_t, tg.data(), tg.size()))) return s;
+        ep.g_Labd = m->g_Labd.as<double>();
+        ep.g_Lh = m->g_Lh.as<double>();
+        ep.g_csys = m->g_csys.as<double>();
+        ep.g_t = m->g_t.as<double>();
+        if (p->elem_group) {
+            for (int64_t e = 0; e < m->E; ++e)
+                if (p->elem_group[e] < 0 || p->elem_group[e] >= ng)
+                    return fail(c, FS_ERR_ARG, "stiffness: element %lld has group %d", (long long)e, p->elem_group[e]);
+            if ((s = upload(c, m->g_group, p->elem_group, (size_t)m->E))) return s;
+            ep.elem_group = m->g_group.as<int32_t>();
+        }
+    }
+    const int nblk = m->nen * (m->nen + 1) / 2;
+    const size_t kbytes = (size_t)m->E * nblk * 36 * sizeof(double);
+    if (m->Kb.bytes < kbytes && (s = dalloc(c, m->Kb, kbytes))) return s;
+    FS_HIP(c, hipMemsetAsync(m->status.p, 0, sizeof(int), c->stream));
+    hipEvent_t e0, e1;
+    FS_HIP(c, hipEventCreate(&e0));
+    FS_HIP(c, hipEventCreate(&e1));
+    FS_HIP(c, hipEventRecord(e0, c->stream));
+    FS_HIP(c, fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
+                                        m->dconn.as<int32_t>(), m->nx.as<double>(), m->ny.as<double>(),
+                                        m->nz.as<double>(), m->valid.as<uint8_t>(), m->Kb.as<double>(),
+                                        m->status.as<int>(), c->stream));
+    FS_HIP(c, hipEventRecord(e1, c->stream));
+    int st = 0;
+    FS_HIP(c, hipMemcpyAsync(&st, m->status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    FS_HIP(c, hipEventElapsedTime(elem_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return fail(c, FS_ERR_NUMERIC, "Singular metric matrix in _gradN_e! (degenerate quadrilateral)");
+    return FS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fs_version(void) { return "flexshell 0.1.0 (gfx950)"; }
+
+int fs_ctx_create(int device, fs_ctx **out) {
+    if (!out) return FS_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return FS_ERR_DEVICE;
+    if (device < 0 || device >= n) return FS_ERR_ARG;
+    fs_ctx *c = new (std::nothrow) fs_ctx;
+    if (!c) return FS_ERR_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return FS_ERR_DEVICE;
+    }
+    *out = c;
+    return FS_OK;
+}
+
+void fs_ctx_destroy(fs_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *fs_last_error(const fs_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int fs_model_create(fs_ctx *ctx, int nen, int64_t nnodes, const double *xyz, int64_t nelem, const int64_t *conn,
+                    int index_base, fs_model **out) {
+    if (!ctx || !out) return FS_ERR_ARG;
+    *out = nullptr;
+    ctx->err.clear();
+    if (nen != FS_T3 && nen != FS_Q4) return fail(ctx, FS_ERR_ARG, "nen must be 3 or 4, got %d", nen);
+    if (nnodes <= 0 || nelem <= 0 || !xyz || !conn) return fail(ctx, FS_ERR_ARG, "empty mesh or NULL arrays");
+    if (nnodes >= (int64_t(1) << 31) || nelem >= (int64_t(1) << 31))
+        return fail(ctx, FS_ERR_UNSUPPORTED, "mesh too large for 32-bit node/element indices");
+    FS_HIP(ctx, hipSetDevice(ctx->device));
+    fs_model *m = new (std::nothrow) fs_model;
+    if (!m) return fail(ctx, FS_ERR_NOMEM, "model allocation");
+    m->ctx = ctx;
+    m->nen = nen;
+    m->N = nnodes;
+    m->E = nelem;
+    m->conn.resize((size_t)nelem * nen);
+    for (int64_t i = 0; i < nelem * nen; ++i) {
+        const int64_t v = conn[i] - index_base;
+        if (v < 0 || v >= nnodes) {
+            delete m;
+            return fail(ctx, FS_ERR_ARG, "connectivity entry %lld out of range", (long long)i);
+        }
+        m->conn[i] = (int32_t)v;
+    }
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int a = 0; a < nen; ++a)
+            for (int b = a + 1; b < nen; ++b)
+                if (m->conn[e * nen + a] == m->conn[e * nen + b]) {
+                    delete m;
+                    return fail(ctx, FS_ERR_ARG, "element %lld repeats a node", (long long)e);
+                }
+    // node -> element lists in ascending element order
+    m->ne_ptr.assign(nnodes + 1, 0);
+    for (int64_t i = 0; i < nelem * nen; ++i) m->ne_ptr[m->conn[i] + 1]++;
+    for (int64_t n = 0; n < nnodes; ++n) m->ne_ptr[n + 1] += m->ne_ptr[n];
+    m->ne_list.resize(m->ne_ptr[nnodes]);
+    {
+        std::vector<int32_t> fill(m->ne_ptr.begin(), m->ne_ptr.end() - 1);
+        for (int64_t e = 0; e < nelem; ++e)
+            for (int a = 0; a < nen; ++a) m->ne_list[fill[m->conn[e * nen + a]]++] = (int32_t)e;
+    }
+    // node -> neighbour lists (sorted, including the node itself)
+    m->nbr_ptr.assign(nnodes + 1, 0);
+    {
+        std::vector<int32_t> tmp;
+        std::vector<std::vector<int32_t>> lists(nnodes);
+        for (int64_t n = 0; n < nnodes; ++n) {
+            tmp.clear();
+            for (int k = m->ne_ptr[n]; k < m->ne_ptr[n + 1]; ++k) {
+                const int e = m->ne_list[k];
+                for (int a = 0; a < nen; ++a) tmp.push_back(m->conn[(int64_t)e * nen + a]);
+            }
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            // a node in no element has no neighbours: its rows stay empty, as in the COO assembly
+            lists[n] = tmp;
+            m->nbr_ptr[n + 1] = m->nbr_ptr[n] + (int32_t)tmp.size();
+        }
+        m->nbr_list.resize(m->nbr_ptr[nnodes]);
+        for (int64_t n = 0; n < nnodes; ++n) std::copy(lists[n].begin(), lists[n].end(), m->nbr_list.begin() + m->nbr_ptr[n]);
+    }
+    // device geometry (SoA) and connectivity planes
+    int s;
+    std::vector<int32_t> planes((size_t)nelem * nen);
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int a = 0; a < nen; ++a) planes[(size_t)a * nelem + e] = m->conn[e * nen + a];
+    if ((s = upload(ctx, m->x, xyz, nnodes)) || (s = upload(ctx, m->y, xyz + nnodes, nnodes)) ||
+        (s = upload(ctx, m->z, xyz + 2 * nnodes, nnodes)) || (s = upload(ctx, m->dconn, planes.data(), planes.size())) ||
+        (s = upload(ctx, m->dne_ptr, m->ne_ptr.data(), m->ne_ptr.size())) ||
+        (s = upload(ctx, m->dne_list, m->ne_list.data(), m->ne_list.size())) ||
+        (s = dalloc(ctx, m->nx, nnodes * 8)) || (s = dalloc(ctx, m->ny, nnodes * 8)) ||
+        (s = dalloc(ctx, m->nz, nnodes * 8)) || (s = dalloc(ctx, m->valid, nnodes)) ||
+        (s = dalloc(ctx, m->status, sizeof(int)))) {
+        delete m;
+        return s;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        delete m;
+        return fail(ctx, FS_ERR_DEVICE, "upload of the mesh failed");
+    }
+    *out = m;
+    return FS_OK;
+}
+
+void fs_model_destroy(fs_model *m) {
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+}
+
+int fs_model_set_dofs(fs_model *m, const int64_t *dofnums, int64_t nalldofs, int64_t nfreedofs, int index_base) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (!dofnums || nalldofs <= 0 || nfreedofs < 0 || nfreedofs > nalldofs)
+        return fail(c, FS_ERR_ARG, "set_dofs: bad sizes (nalldofs %lld, nfreedofs %lld)", (long long)nalldofs,
+                    (long long)nfreedofs);
+    const int64_t N = m->N;
+    std::vector<int64_t> dof(N * 6);
+    std::vector<uint8_t> seen(nalldofs, 0);
+    for (int64_t I = 0; I < N; ++I)
+        for (int cc = 0; cc < 6; ++cc) {
+            const int64_t d = dofnums[cc * N + I] - index_base;  // column-major N x 6
+            if (d < 0 || d >= nalldofs) return fail(c, FS_ERR_ARG, "dof number out of range at node %lld", (long long)I);
+            if (seen[d]) return fail(c, FS_ERR_ARG, "dof number %lld used twice", (long long)(d + index_base));
+            seen[d] = 1;
+            dof[I * 6 + cc] = d;
+        }
+    if ((int64_t)N * 6 != nalldofs) return fail(c, FS_ERR_ARG, "nalldofs must equal 6 * nnodes");
+    FS_HIP(c, hipSetDevice(c->device));
+    m->dof.swap(dof);
+    m->nall = nalldofs;
+    m->nfree = nfreedofs;
+    m->blk[0].reset();
+    m->blk[1].reset();
+    int s = upload(c, m->node_dof, m->dof.data(), m->dof.size());
+    if (s) return s;
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    m->dofs_ready = true;
+    return FS_OK;
+}
+
+int fs_model_normals_isoparametric(fs_model *m, double threshold_angle_deg, double *normals_out, uint8_t *valid_out) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    FS_HIP(c, hipSetDevice(c->device));
+    const double s = std::sin(threshold_angle_deg / 180 * M_PI);
+    const double ntol = 1 - std::sqrt(1 - s * s);  // FEMMShellT3FFModule.jl:600
+    FS_HIP(c, fs::launch_normals(m->nen, m->N, (int)m->E, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
+                                 m->dconn.as<int32_t>(), m->dne_ptr.as<int32_t>(), m->dne_list.as<int32_t>(), ntol,
+                                 m->nx.as<double>(), m->ny.as<double>(), m->nz.as<double>(), m->valid.as<uint8_t>(),
+                                 c->stream));
+    if (normals_out) {
+        FS_HIP(c, hipMemcpyAsync(normals_out, m->nx.p, m->N * 8, hipMemcpyDeviceToHost, c->stream));
+        FS_HIP(c, hipMemcpyAsync(normals_out + m->N, m->ny.p, m->N * 8, hipMemcpyDeviceToHost, c->stream));
+        FS_HIP(c, hipMemcpyAsync(normals_out + 2 * m->N, m->nz.p, m->N * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (valid_out) FS_HIP(c, hipMemcpyAsync(valid_out, m->valid.p, m->N, hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    m->normals_ready = true;
+    return FS_OK;
+}
+
+int fs_model_normals_upload(fs_model *m, const double *normals, const uint8_t *valid) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (!normals || !valid) return fail(c, FS_ERR_ARG, "normals_upload: NULL arrays");
+    FS_HIP(c, hipSetDevice(c->device));
+    FS_HIP(c, hipMemcpyAsync(m->nx.p, normals, m->N * 8, hipMemcpyHostToDevice, c->stream));
+    FS_HIP(c, hipMemcpyAsync(m->ny.p, normals + m->N, m->N * 8, hipMemcpyHostToDevice, c->stream));
+    FS_HIP(c, hipMemcpyAsync(m->nz.p, normals + 2 * m->N, m->N * 8, hipMemcpyHostToDevice, c->stream));
+    FS_HIP(c, hipMemcpyAsync(m->valid.p, valid, m->N, hipMemcpyHostToDevice, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    m->normals_ready = true;
+    return FS_OK;
+}
+
+int fs_model_pattern(fs_model *m, int block, int64_t *nrows, int64_t *nnz) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (block != FS_BLOCK_ALL && block != FS_BLOCK_FF) return fail(c, FS_ERR_ARG, "unknown block %d", block);
+    FS_HIP(c, hipSetDevice(c->device));
+    int s = build_block(m, block);
+    if (s) return s;
+    if (nrows) *nrows = m->blk[block].nrows;
+    if (nnz) *nnz = m->blk[block].nnz;
+    return FS_OK;
+}
+
+int fs_model_pattern_get(fs_model *m, int block, int64_t *rowptr, int64_t *colval, int index_base) {
+    if (!m) return FS_ERR_ARG;
+    int s = fs_model_pattern(m, block, nullptr, nullptr);
+    if (s) return s;
+    const Block &B = m->blk[block];
+    if (rowptr)
+        for (int64_t i = 0; i <= B.nrows; ++i) rowptr[i] = B.rowptr[i] + index_base;
+    if (colval)
+        for (int64_t i = 0; i < B.nnz; ++i) colval[i] = (int64_t)B.colval[i] + index_base;
+    return FS_OK;
+}
+
+int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *blocks_out) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    FS_HIP(c, hipSetDevice(c->device));
+    float ms = 0.f;
+    int s = element_blocks(m, p, &ms);
+    if (s) return s;
+    m->timing[0] = ms;
+    if (blocks_out) {
+        const int nblk = m->nen * (m->nen + 1) / 2;
+        FS_HIP(c, hipMemcpy(blocks_out, m->Kb.p, (size_t)m->E * nblk * 36 * 8, hipMemcpyDeviceToHost));
+    }
+    return FS_OK;
+}
+
+int fs_model_stiffness(fs_model *m, const fs_shell_params *p, int block, double *nzval_out) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (block != FS_BLOCK_ALL && block != FS_BLOCK_FF) return fail(c, FS_ERR_ARG, "unknown block %d", block);
+    FS_HIP(c, hipSetDevice(c->device));
+    int s = build_block(m, block);
+    if (s) return s;
+    Block &B = m->blk[block];
+    float ms_elem = 0.f;
+    if ((s = element_blocks(m, p, &ms_elem))) return s;
+    hipEvent_t e0, e1;
+    FS_HIP(c, hipEventCreate(&e0));
+    FS_HIP(c, hipEventCreate(&e1));
+    FS_HIP(c, hipEventRecord(e0, c->stream));
+    FS_HIP(c, fs::launch_scatter(plan_of(B), m->Kb.as<double>(), m->nen * (m->nen + 1) / 2, B.kval.as<double>(),
+                                 c->stream));
+    FS_HIP(c, hipEventRecord(e1, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    float ms_sc = 0.f;
+    FS_HIP(c, hipEventElapsedTime(&ms_sc, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    m->timing[0] = ms_elem;
+    m->timing[1] = ms_sc;
+    m->timing[2] = ms_elem + ms_sc;
+    B.k_ready = true;
+    if (nzval_out) FS_HIP(c, hipMemcpy(nzval_out, B.kval.p, (size_t)B.nnz * 8, hipMemcpyDeviceToHost));
+    return FS_OK;
+}
+
+int fs_model_mass(fs_model *m, const fs_mass_params *p, int block, double *diag_out) {
+    if (!m || !p) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (block != FS_BLOCK_ALL && block != FS_BLOCK_FF) return fail(c, FS_ERR_ARG, "unknown block %d", block);
+    FS_HIP(c, hipSetDevice(c->device));
+    int s = build_block(m, block);
+    if (s) return s;
+    Block &B = m->blk[block];
+    const double *ginert = nullptr;
+    const int32_t *ggrp = nullptr;
+    if (p->composite) {
+        if (p->ngroups < 1 || !p->group_inertia) return fail(c, FS_ERR_ARG, "mass: layered shell needs group inertia");
+        if ((s = upload(c, m->g_inertia, p->group_inertia, (size_t)2 * p->ngroups))) return s;
+        ginert = m->g_inertia.as<double>();
+        if (p->elem_group) {
+            for (int64_t e = 0; e < m->E; ++e)
+                if (p->elem_group[e] < 0 || p->elem_group[e] >= p->ngroups)
+                    return fail(c, FS_ERR_ARG, "mass: element %lld has group %d", (long long)e, p->elem_group[e]);
+            if ((s = upload(c, m->m_group, p->elem_group, (size_t)m->E))) return s;
+            ggrp = m->m_group.as<int32_t>();
+        }
+    }
+    FS_HIP(c, fs::launch_mass(m->nen, m->N, (int)m->E, p->composite ? 1 : 0, p->rho, p->thickness, ginert, ggrp,
+                              m->x.as<double>(), m->y.as<double>(), m->z.as<double>(), m->dconn.as<int32_t>(),
+                              m->dne_ptr.as<int32_t>(), m->dne_list.as<int32_t>(), m->node_dof.as<int64_t>(),
+                              B.nrows, B.mdiag.as<double>(), c->stream));
+    if (diag_out)
+        FS_HIP(c, hipMemcpyAsync(diag_out, B.mdiag.p, (size_t)B.nrows * 8, hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    B.m_ready = true;
+    return FS_OK;
+}
+
+int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, const double *V0, const double *Fmag,
+                      const double *g, const int64_t *probe_dofs, int32_t nprobes, double *probe_out, double *U_out,
+                      double *V_out) {
+    if (!m || !p) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    Block &B = m->blk[FS_BLOCK_FF];
+    if (!B.k_ready || !B.m_ready)
+        return fail(c, FS_ERR_STATE, "explicit: FF-block stiffness and mass must be computed first");
+    if (!B.rowptr32.p) return fail(c, FS_ERR_UNSUPPORTED, "explicit: more than 2^31 stored entries");
+    if (p->precision != FS_PREC_FP64) return fail(c, FS_ERR_UNSUPPORTED, "explicit: precision %d", p->precision);
+    if (!U0 || !V0 || !Fmag || !g || p->nsteps < 0 || !(p->dt > 0.0))
+        return fail(c, FS_ERR_ARG, "explicit: bad arguments");
+    const int stride = p->probe_stride > 0 ? p->probe_stride : 1;
+    if (nprobes < 0 || (nprobes > 0 && (!probe_dofs || !probe_out))) return fail(c, FS_ERR_ARG, "explicit: probes");
+    const int64_t n = B.nrows;
+    for (int k = 0; k < nprobes; ++k)
+        if (probe_dofs[k] < 0 || probe_dofs[k] >= n) return fail(c, FS_ERR_ARG, "explicit: probe dof out of range");
+    FS_HIP(c, hipSetDevice(c->device));
+    std::vector<double> M(n), Cd(n), invMC(n);
+    FS_HIP(c, hipMemcpy(M.data(), B.mdiag.p, n * 8, hipMemcpyDeviceToHost));
+    const double dt = p->dt;
+    for (int64_t i = 0; i < n; ++i) {
+        Cd[i] = p->cdamp * M[i];
+        invMC[i] = 1.0 / (M[i] + (dt / 2) * Cd[i]);  // plate_expl_examples.jl:68-69
+    }
+    DevBuf dU[2], dV, dA, dF, dInv, dC, dSlot, dProbe;
+    int s;
+    if ((s = upload(c, dU[0], U0, n)) || (s = dalloc(c, dU[1], n * 8)) || (s = upload(c, dV, V0, n)) ||
+        (s = dalloc(c, dA, n * 8)) || (s = upload(c, dF, Fmag, n)) || (s = upload(c, dInv, invMC.data(), n)) ||
+        (s = upload(c, dC, Cd.data(), n)))
+        return s;
+    const int64_t nrec = p->nsteps / stride + 1;
+    if (nprobes > 0) {
+        std::vector<int32_t> slot(n, -1);
+        for (int k = 0; k < nprobes; ++k) slot[probe_dofs[k]] = k;
+        if ((s = upload(c, dSlot, slot.data(), n)) || (s = dalloc(c, dProbe, (size_t)nrec * nprobes * 8))) return s;
+        for (int k = 0; k < nprobes; ++k) probe_out[k] = U0[probe_dofs[k]];  // peek(0, U0)
+    }
+    const int damped = p->cdamp != 0.0;
+    // rows of ~40-60 entries: 8 lanes per row
+    const int tpr = 8;
+    hipEvent_t e0, e1;
+    FS_HIP(c, hipEventCreate(&e0));
+    FS_HIP(c, hipEventCreate(&e1));
+    FS_HIP(c, hipEventRecord(e0, c->stream));
+    FS_HIP(c, fs::launch_cd_init(n, dt, g[0], dInv.as<double>(), dF.as<double>(), dU[0].as<double>(),
+                                 dV.as<double>(), dA.as<double>(), dU[1].as<double>(), c->stream));
+    int cur = 1;
+    for (int64_t step = 1; step <= p->nsteps; ++step) {
+        const bool rec = nprobes > 0 && (step % stride == 0);
+        FS_HIP(c, fs::launch_cd_step(tpr, n, B.rowptr32.as<int32_t>(), B.colidx32.as<int32_t>(), B.kval.as<double>(),
+                                     dU[cur].as<double>(), dU[1 - cur].as<double>(), dV.as<double>(), dA.as<double>(),
+                                     dInv.as<double>(), dC.as<double>(), dF.as<double>(), g[step], dt, damped,
+                                     rec ? dSlot.as<int32_t>() : nullptr,
+                                     rec ? dProbe.as<double>() + (step / stride) * nprobes : nullptr, c->stream));
+        cur = 1 - cur;
+    }
+    FS_HIP(c, hipEventRecord(e1, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    FS_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    m->timing[3] = ms;
+    // after the last step, dU[1 - cur] holds U_nsteps (cur now indexes the predictor)
+    const int fin = p->nsteps == 0 ? 0 : 1 - cur;
+    if (U_out) FS_HIP(c, hipMemcpy(U_out, dU[fin].p, n * 8, hipMemcpyDeviceToHost));
+    if (V_out) FS_HIP(c, hipMemcpy(V_out, dV.p, n * 8, hipMemcpyDeviceToHost));
+    if (nprobes > 0 && nrec > 1)
+        FS_HIP(c, hipMemcpy(probe_out + nprobes, dProbe.as<double>() + nprobes, (size_t)(nrec - 1) * nprobes * 8,
+                            hipMemcpyDeviceToHost));
+    return FS_OK;
+}
+
+int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *omega_max) {
+    if (!m || !v0 || !omega_max || maxit < 0) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    Block &B = m->blk[FS_BLOCK_FF];
+    if (!B.k_ready || !B.m_ready) return fail(c, FS_ERR_STATE, "omega_max: FF-block K and M must be computed first");
+    if (!B.rowptr32.p) return fail(c, FS_ERR_UNSUPPORTED, "omega_max: more than 2^31 stored entries");
+    FS_HIP(c, hipSetDevice(c->device));
+    const int64_t n = B.nrows;
+    std::vector<double> M(n), invM(n);
+    FS_HIP(c, hipMemcpy(M.data(), B.mdiag.p, n * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) invM[i] = 1.0 / M[i];  // clamp_cyl_transient_examples.jl:47
+    DevBuf dv, dw, dinv, dpart, dss;
+    int s;
+    const double one[4] = {1.0, 1.0, 1.0, 1.0};
+    if ((s = upload(c, dv, v0, n)) || (s = dalloc(c, dw, n * 8)) || (s = upload(c, dinv, invM.data(), n)) ||
+        (s = dalloc(c, dpart, 512 * 8)) || (s = upload(c, dss, one, 4)))
+        return s;
+    double *v = dv.as<double>(), *w = dw.as<double>(), *ss = dss.as<double>();
+    const int32_t *rp = B.rowptr32.as<int32_t>(), *ci = B.colidx32.as<int32_t>();
+    const double *kv = B.kval.as<double>();
+    for (int it = 0; it < maxit; ++it) {
+        FS_HIP(c, fs::launch_spmv(n, rp, ci, kv, v, w, c->stream));                        // w = K v
+        FS_HIP(c, fs::launch_dot(n, 0, w, nullptr, nullptr, dpart.as<double>(), ss, c->stream));
+        FS_HIP(c, fs::launch_scale(n, ss, nullptr, w, w, c->stream));                       // w /= |w|
+        FS_HIP(c, fs::launch_scale(n, ss + 3, dinv.as<double>(), w, v, c->stream));          // v = invM .* w
+        FS_HIP(c, fs::launch_dot(n, 0, v, nullptr, nullptr, dpart.as<double>(), ss + 1, c->stream));
+        FS_HIP(c, fs::launch_scale(n, ss + 1, nullptr, v, v, c->stream));                    // v /= |v|
+    }
+    FS_HIP(c, fs::launch_spmv(n, rp, ci, kv, v, w, c->stream));
+    FS_HIP(c, fs::launch_dot(n, 1, w, v, nullptr, dpart.as<double>(), ss, c->stream));
+    FS_HIP(c, fs::launch_dot(n, 2, v, nullptr, B.mdiag.as<double>(), dpart.as<double>(), ss + 1, c->stream));
+    double h[2];
+    FS_HIP(c, hipMemcpyAsync(h, ss, 16, hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    *omega_max = std::sqrt(h[0] / h[1]);
+    return FS_OK;
+}
+
+int fs_model_last_timing(const fs_model *m, int which, double *ms) {
+    if (!m || !ms || which < 0 || which > 3) return FS_ERR_ARG;
+    *ms = m->timing[which];
+    return FS_OK;
+}
+
+}  // extern "C"

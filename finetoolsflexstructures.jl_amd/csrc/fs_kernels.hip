@@ -1,0 +1,1164 @@
+// fs_kernels.hip -- gfx950 kernels of the shell finite-element hot path.
+//
+// Element stiffness (T3FF / T3FFComp / Q4RS / Q4RSComp)
+// ----------------------------------------------------
+// The reference forms K_e in the element frame from B matrices
+// (add_btdb_ut_only!, FEMMShellT3FFModule.jl:680-706; FEMMShellQ4RSModule.jl:
+// 929-936), then applies two dense 18x18 (24x24) congruences
+// (TransformerQtEQ, :711-730).  Here the same matrix is formed as
+//
+//     K_e = sum_q C_q^T C_q  (+ drilling),   C_q = L_q * B_q * T_ae * T_ga
+//
+// where L_q is the upper Cholesky factor of the (scaled) material matrix at
+// quadrature point q (Dps, Dt; or [A B; B D], H rotated by the layup angle),
+// B_q stacks the membrane, bending and transverse-shear strain-displacement
+// rows, and T_ae * T_ga is the nodal-to-element times global-to-nodal
+// transformation.  C_q has 8 rows (12 for the T3 AVERAGE_K formulation) and
+// 6*nen columns; with exact arithmetic this equals the reference's
+// T_ga' (T_ae' K_loc T_ae + drill) T_ga.  The T3 drilling stiffness (added in
+// the nodal basis, :714-727) becomes a rank-one term kavg * g g' on the
+// rotation block, with g the nodal drilling axis in the global basis; the Q4
+// drilling (FEMMShellQ4RSModule.jl:807-859) is applied to the assembled
+// global K_e exactly as the reference does.
+//
+// One workgroup processes NE elements in phases separated by barriers:
+//   G1  thread per (element, quadrature point): geometry, scaled material
+//       factors, shear B-matrix entries                        -> LDS
+//   G2  thread per (element, qp, node): nodal triad (Rodrigues), C columns
+//       of the node in the element frame, coupling vector d   -> LDS
+//   G3  thread per (element, qp, node): global-basis C columns -> LDS
+//   C   thread per (element, upper node pair): 6x6 block = C_a^T C_b summed
+//       over all rows of all quadrature points, plus drilling
+//   W   blocks staged in LDS and stored contiguously (Kb[e][blk][36]).
+//
+// Deterministic scatter into CSR
+// ------------------------------
+// fs_scatter_kernel sums, for each upper node pair (I, J), the contributing
+// element blocks in ascending element order (the order in which
+// sparse(I, J, V) combines duplicates) and writes the (I, J) and (J, I)
+// 6x6 blocks to their CSR positions.  No atomics: every CSR value is written
+// exactly once and the result is bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include "fs_device.h"
+
+namespace fs {
+
+// ---------------------------------------------------------------------------
+// Small dense helpers (FP64)
+// ---------------------------------------------------------------------------
+
+// _e_g! (FEMMShellT3FFModule.jl:283-305): element basis from the tangents.
+__device__ __forceinline__ void e_g(double EG[3][3], const double J[3][2]) {
+    EG[0][0] = J[0][0];
+    EG[1][0] = J[1][0];
+    EG[2][0] = J[2][0];
+    double n = sqrt(EG[0][0] * EG[0][0] + EG[1][0] * EG[1][0] + EG[2][0] * EG[2][0]);
+    EG[0][0] /= n;
+    EG[1][0] /= n;
+    EG[2][0] /= n;
+    EG[0][2] = -EG[2][0] * J[1][1] + EG[1][0] * J[2][1];
+    EG[1][2] = EG[2][0] * J[0][1] - EG[0][0] * J[2][1];
+    EG[2][2] = -EG[1][0] * J[0][1] + EG[0][0] * J[1][1];
+    n = sqrt(EG[0][2] * EG[0][2] + EG[1][2] * EG[1][2] + EG[2][2] * EG[2][2]);
+    EG[0][2] /= n;
+    EG[1][2] /= n;
+    EG[2][2] /= n;
+    EG[0][1] = -EG[2][2] * EG[1][0] + EG[1][2] * EG[2][0];
+    EG[1][1] = EG[2][2] * EG[0][0] - EG[0][2] * EG[2][0];
+    EG[2][1] = -EG[1][2] * EG[0][0] + EG[0][2] * EG[1][0];
+}
+
+// Layup2ElementAngle (TransformerModule.jl:92-105): cos/sin of the angle
+// between the element frame and the layup reference direction.  The
+// argument of the square root is clamped at zero (the reference would raise
+// a DomainError for a rounding excursion of m above one).
+__device__ __forceinline__ void layup_angle(const double EG[3][3], const double *L, double &m,
+                                            double &n) {
+    double M[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            M[i][j] = EG[0][i] * L[0 * 3 + j] + EG[1][i] * L[1 * 3 + j] + EG[2][i] * L[2 * 3 + j];
+    double c0 = sqrt(M[0][0] * M[0][0] + M[1][0] * M[1][0]);
+    double c1 = sqrt(M[0][1] * M[0][1] + M[1][1] * M[1][1]);
+    double m11 = M[0][0] / c0, m21 = M[1][0] / c0, m12 = M[0][1] / c1, m22 = M[1][1] / c1;
+    double mm = (m11 + m22) / 2;
+    double nn = (m12 - m21) / 2;
+    double sn = (nn >= 0.0) ? 1.0 : -1.0;
+    double arg = 1 - mm * mm;
+    m = mm;
+    n = sn * sqrt(arg > 0.0 ? arg : 0.0);
+}
+
+// Effective material factors of a layered shell at one point:
+//   Lmb = s_mb * Labd * diag(Tb, Tb),  Ls = s_s * Lh * Tt
+// with Tb = plane_stress_Tbar_matrix!(m, n) (CompositeLayupModule.jl:364-378)
+// and Tt = transverse_shear_T_matrix!(m, n) (:463-471), so that
+// Lmb' Lmb = s_mb^2 [Tb' A Tb, Tb' B Tb; Tb' B Tb, Tb' D Tb] and
+// Ls' Ls = s_s^2 Tt' H Tt -- the sA, sB, sD, sH of FEMMShellT3FFCompModule.jl:614-626.
+__device__ __forceinline__ void composite_factors(const double *Labd, const double *Lh, double m, double n,
+                                                  double s_mb, double s_s, double Lmb[36],
+                                                  double Ls[4]) {
+    const double Tb[3][3] = {{m * m, n * n, -(m * n)}, {n * n, m * m, (m * n)},
+                             {2 * (m * n), -2 * (m * n), m * m - n * n}};
+    for (int i = 0; i < 6; ++i)
+        for (int blk = 0; blk < 2; ++blk)
+            for (int j = 0; j < 3; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < 3; ++k) acc += Labd[i * 6 + blk * 3 + k] * Tb[k][j];
+                Lmb[i * 6 + blk * 3 + j] = s_mb * acc;
+            }
+    const double Tt[2][2] = {{m, -n}, {n, m}};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) Ls[i * 2 + j] = s_s * (Lh[i * 2 + 0] * Tt[0][j] + Lh[i * 2 + 1] * Tt[1][j]);
+}
+
+// Homogeneous factors: Lmb = diag(s_m Lps, s_b Lps), Ls = s_s Lt.
+__device__ __forceinline__ void homog_factors(const double *Lps, const double *Lt, double s_m, double s_b,
+                                              double s_s, double Lmb[36], double Ls[4]) {
+    for (int i = 0; i < 36; ++i) Lmb[i] = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            Lmb[i * 6 + j] = s_m * Lps[i * 3 + j];
+            Lmb[(3 + i) * 6 + 3 + j] = s_b * Lps[i * 3 + j];
+        }
+    for (int i = 0; i < 4; ++i) Ls[i] = s_s * Lt[i];
+}
+
+// ---------------------------------------------------------------------------
+// Element kernel
+// ---------------------------------------------------------------------------
+
+template <int NEN, int NQP, int NR>
+struct ElemTraits {
+    static constexpr int K = 6 * NEN;                  // element dofs
+    static constexpr int NBLK = NEN * (NEN + 1) / 2;   // upper node-pair blocks
+    static constexpr int NS = NQP * NR;                // rows of C per element
+    static constexpr int CSTR = NS * K + 2;            // LDS stride of C (padded)
+    static constexpr int NSH = NR - 6;                 // raw shear rows (2 or 6)
+    // geometry record per (element, qp)
+    static constexpr int G_EG = 0;                     // 9
+    static constexpr int G_GN = 9;                     // 2 * NEN  (gradN_e)
+    static constexpr int G_LMB = G_GN + 2 * NEN;       // 36
+    static constexpr int G_LS = G_LMB + 36;            // 4
+    static constexpr int G_BS = G_LS + 4;              // NEN * 3 * NSH: bs[node][comp w,tx,ty][row]
+    static constexpr int GEO = G_BS + NEN * 3 * NSH;
+    // node record per (element, qp, node): A rows 0..1 and coupling vector d
+    static constexpr int NODE = 6 + NR;
+    // per element: drilling vectors, kavg parts, Q4 rotation blocks, kavg
+    static constexpr int M_G = 0;                      // NEN*3
+    static constexpr int M_KP = NEN * 3;               // NEN*2
+    static constexpr int M_KRR = M_KP + NEN * 2;       // NEN*9
+    static constexpr int M_KAVG = M_KRR + NEN * 9;     // 1
+    static constexpr int MISC = M_KAVG + 2;
+    static constexpr int KSTAGE = NBLK * 36;           // staged output per element
+};
+
+template <int NEN, int NQP, int NR, int NE>
+struct ElemSmem {
+    using T = ElemTraits<NEN, NQP, NR>;
+    static constexpr int OFF_C = 0;
+    static constexpr int SZ_C = NE * (T::CSTR > T::KSTAGE ? T::CSTR : T::KSTAGE);
+    static constexpr int OFF_G = SZ_C;
+    static constexpr int SZ_G = NE * NQP * T::GEO;
+    static constexpr int OFF_N = OFF_G + SZ_G;
+    static constexpr int SZ_N = NE * NQP * NEN * T::NODE;
+    static constexpr int OFF_M = OFF_N + SZ_N;
+    static constexpr int SZ_M = NE * T::MISC;
+    static constexpr int TOTAL = OFF_M + SZ_M;  // doubles
+};
+
+// Q4 basis-function derivatives at (xi, eta) (FinEtools FESetQ4 bfundpar).
+__device__ __forceinline__ void q4_gradn(double xi, double eta, double gN[4][2]) {
+    gN[0][0] = -(1. - eta) * 0.25;
+    gN[0][1] = -(1. - xi) * 0.25;
+    gN[1][0] = (1. - eta) * 0.25;
+    gN[1][1] = -(1. + xi) * 0.25;
+    gN[2][0] = (1. + eta) * 0.25;
+    gN[2][1] = (1. + xi) * 0.25;
+    gN[3][0] = -(1. + eta) * 0.25;
+    gN[3][1] = (1. - xi) * 0.25;
+}
+
+// GaussRule(2, 2) abscissa as FinEtools tabulates it.
+#define FS_GAUSS2 0.577350269189626
+
+template <int NEN, int NQP, int NR, int NE, int WG>
+__global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__restrict__ xs,
+                                                  const double *__restrict__ ys,
+                                                  const double *__restrict__ zs,
+                                                  const int32_t *__restrict__ conn,
+                                                  const double *__restrict__ nxs,
+                                                  const double *__restrict__ nys,
+                                                  const double *__restrict__ nzs,
+                                                  const uint8_t *__restrict__ valid,
+                                                  double *__restrict__ Kb, int *__restrict__ status) {
+    using T = ElemTraits<NEN, NQP, NR>;
+    using S = ElemSmem<NEN, NQP, NR, NE>;
+    extern __shared__ double smem[];
+    double *sC = smem + S::OFF_C;
+    double *sG = smem + S::OFF_G;
+    double *sN = smem + S::OFF_N;
+    double *sM = smem + S::OFF_M;
+
+    const int tid = threadIdx.x;
+    const int E = p.nelem;
+    const int e0 = blockIdx.x * NE;
+    const int ne = (E - e0) < NE ? (E - e0) : NE;
+
+    // ---------------- G1: geometry per (element, qp) ----------------
+    if (tid < NE * NQP) {
+        const int le = tid / NQP, q = tid % NQP;
+        if (le < ne) {
+            const int e = e0 + le;
+            double X[NEN][3];
+            for (int a = 0; a < NEN; ++a) {
+                int n = conn[a * E + e];
+                X[a][0] = xs[n];
+                X[a][1] = ys[n];
+                X[a][2] = zs[n];
+            }
+            double *g = sG + (le * NQP + q) * T::GEO;
+            double EG[3][3], gN[NEN][2];
+            double Lmb[36], Ls[4];
+            const int grp = (p.composite && p.elem_group) ? p.elem_group[e] : 0;
+            const double tt = p.composite ? p.g_t[grp] : p.thickness;
+            if constexpr (NEN == 3) {
+                // FEMMShellT3FFModule.jl:269-321,465-484
+                double J0[3][2];
+                for (int j = 0; j < 3; ++j) {
+                    J0[j][0] = X[1][j] - X[0][j];
+                    J0[j][1] = X[2][j] - X[0][j];
+                }
+                e_g(EG, J0);
+                double ec[3][2];
+                ec[0][0] = 0.0;
+                ec[0][1] = 0.0;
+                ec[1][0] = J0[0][0] * EG[0][0] + J0[1][0] * EG[1][0] + J0[2][0] * EG[2][0];
+                ec[1][1] = J0[0][0] * EG[0][1] + J0[1][0] * EG[1][1] + J0[2][0] * EG[2][1];
+                ec[2][0] = J0[0][1] * EG[0][0] + J0[1][1] * EG[1][0] + J0[2][1] * EG[2][0];
+                ec[2][1] = J0[0][1] * EG[0][1] + J0[1][1] * EG[1][1] + J0[2][1] * EG[2][1];
+                const double a = ec[1][0], b = ec[1][1], c = ec[2][0], d = ec[2][1];
+                const double Jd = (a * d - b * c);
+                gN[0][0] = (b - d) / Jd;
+                gN[1][0] = d / Jd;
+                gN[2][0] = -b / Jd;
+                gN[0][1] = (c - a) / Jd;
+                gN[1][1] = -c / Jd;
+                gN[2][1] = a / Jd;
+                const double Ae = Jd / 2;
+                const double h = sqrt(2 * Ae);
+                const double stab = (tt * tt) / (tt * tt + p.stab_c * (h * h));
+                const double wsh = (NR == 12) ? Ae / 3 : Ae;  // AVERAGE_K splits the weight
+                if (!p.composite) {
+                    homog_factors(p.Lps, p.Lt, sqrt(tt * Ae), sqrt((tt * tt * tt) / 12 * Ae),
+                                  sqrt(tt * stab * wsh), Lmb, Ls);
+                } else {
+                    double m, n;
+                    layup_angle(EG, p.g_csys + 9 * grp, m, n);
+                    composite_factors(p.g_Labd + 36 * grp, p.g_Lh + 4 * grp, m, n, sqrt(Ae), sqrt(stab * wsh),
+                                      Lmb, Ls);
+                }
+                // DSG transverse shear entries (_add_Bsmat_o!, :486-537)
+                double bs[3][3][T::NSH];
+                for (int j = 0; j < 3; ++j)
+                    for (int cc = 0; cc < 3; ++cc)
+                        for (int r = 0; r < T::NSH; ++r) bs[j][cc][r] = 0.0;
+                const int ord[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+                for (int o = 0; o < 3; ++o) {
+                    const int s = ord[o][0], pp = ord[o][1], qq = ord[o][2];
+                    const double aa = ec[pp][0] - ec[s][0];
+                    const double bb = ec[pp][1] - ec[s][1];
+                    const double cc = ec[qq][0] - ec[s][0];
+                    const double dd = ec[qq][1] - ec[s][1];
+                    const double mm = (1.0 / 2 / Ae);
+                    const int r0 = (NR == 12) ? 2 * o : 0;
+                    // comps: 0 = w, 1 = theta_x, 2 = theta_y
+                    bs[s][0][r0] += mm * (bb - dd);
+                    bs[s][2][r0] += mm * (Ae);
+                    bs[s][0][r0 + 1] += mm * (cc - aa);
+                    bs[s][1][r0 + 1] += mm * (-Ae);
+                    bs[pp][0][r0] += mm * (dd);
+                    bs[pp][1][r0] += mm * (-bb * dd / 2);
+                    bs[pp][2][r0] += mm * (aa * dd / 2);
+                    bs[pp][0][r0 + 1] += mm * (-cc);
+                    bs[pp][1][r0 + 1] += mm * (bb * cc / 2);
+                    bs[pp][2][r0 + 1] += mm * (-aa * cc / 2);
+                    bs[qq][0][r0] += mm * (-bb);
+                    bs[qq][1][r0] += mm * (bb * dd / 2);
+                    bs[qq][2][r0] += mm * (-bb * cc / 2);
+                    bs[qq][0][r0 + 1] += mm * (aa);
+                    bs[qq][1][r0 + 1] += mm * (-aa * dd / 2);
+                    bs[qq][2][r0 + 1] += mm * (aa * cc / 2);
+                }
+                if (NR == 8) {
+                    for (int j = 0; j < 3; ++j)
+                        for (int cc = 0; cc < 3; ++cc)
+                            for (int r = 0; r < 2; ++r) bs[j][cc][r] *= (1.0 / 3);
+                }
+                for (int j = 0; j < 3; ++j)
+                    for (int cc = 0; cc < 3; ++cc)
+                        for (int r = 0; r < T::NSH; ++r) g[T::G_BS + (j * 3 + cc) * T::NSH + r] = bs[j][cc][r];
+            } else {
+                // Q4RS at Gauss point q (FEMMShellQ4RSModule.jl:919-936)
+                const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+                const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+                double gNp[4][2];
+                q4_gradn(xi, eta, gNp);
+                double J[3][2];
+                for (int i = 0; i < 3; ++i)
+                    for (int k = 0; k < 2; ++k) {
+                        double acc = 0.0;
+                        for (int a = 0; a < 4; ++a) acc += X[a][i] * gNp[a][k];
+                        J[i][k] = acc;
+                    }
+                double cr0 = -J[2][0] * J[1][1] + J[1][0] * J[2][1];
+                double cr1 = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+                double cr2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
+                const double Jac = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+                e_g(EG, J);
+                double cen[3] = {0.0, 0.0, 0.0};
+                for (int a = 0; a < 4; ++a)
+                    for (int k = 0; k < 3; ++k) cen[k] += X[a][k];
+                for (int k = 0; k < 3; ++k) cen[k] /= 4;
+                double ec[4][2];
+                for (int a = 0; a < 4; ++a)
+                    for (int k = 0; k < 2; ++k) {
+                        double dd = 0.0;
+                        for (int m = 0; m < 3; ++m) dd += (X[a][m] - cen[m]) * EG[m][k];
+                        ec[a][k] = dd;
+                    }
+                double G00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
+                double G01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
+                double G11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
+                double detG = G00 * G11 - G01 * G01;
+                if (detG == 0.0) status[0] = 1;
+                double i00 = G11 / detG, i01 = -G01 / detG, i11 = G00 / detG;
+                for (int a = 0; a < 4; ++a) {
+                    double t0 = i00 * gNp[a][0] + i01 * gNp[a][1];
+                    double t1 = i01 * gNp[a][0] + i11 * gNp[a][1];
+                    double gg[3];
+                    for (int i = 0; i < 3; ++i) gg[i] = J[i][0] * t0 + J[i][1] * t1;
+                    gN[a][0] = EG[0][0] * gg[0] + EG[1][0] * gg[1] + EG[2][0] * gg[2];
+                    gN[a][1] = EG[0][1] * gg[0] + EG[1][1] * gg[1] + EG[2][1] * gg[2];
+                }
+                // _quad_diameter (:861-870)
+                double maxd = 0.0;
+                for (int i = 1; i < 4; ++i) {
+                    double dx = X[i][0] - X[0][0], dy = X[i][1] - X[0][1], dz = X[i][2] - X[0][2];
+                    double dist = (dx * dx + dy * dy + dz * dz);
+                    maxd = dist > maxd ? dist : maxd;
+                }
+                const double h = sqrt(maxd);
+                const double stab = (tt * tt) / (tt * tt + p.stab_c * (h * h));
+                if (!p.composite) {
+                    homog_factors(p.Lps, p.Lt, sqrt(tt * Jac), sqrt((tt * tt * tt / 12.0) * Jac),
+                                  sqrt(tt * stab * Jac), Lmb, Ls);
+                } else {
+                    double m, n;
+                    layup_angle(EG, p.g_csys + 9 * grp, m, n);
+                    composite_factors(p.g_Labd + 36 * grp, p.g_Lh + 4 * grp, m, n, sqrt(Jac), sqrt(stab * Jac),
+                                      Lmb, Ls);
+                }
+                // MITC transverse shear (_Bsmat, :753-804)
+                const double X1 = ec[0][0], Y1 = ec[0][1], X2 = ec[1][0], Y2 = ec[1][1];
+                const double X3 = ec[2][0], Y3 = ec[2][1], X4 = ec[3][0], Y4 = ec[3][1];
+                const double r = xi, s = eta;
+                const double J11 = (X1 * (s - 1) / 4 - X2 * (s - 1) / 4 + X3 * (s + 1) / 4 - X4 * (s + 1) / 4);
+                const double J21 = (Y1 * (s - 1) / 4 - Y2 * (s - 1) / 4 + Y3 * (s + 1) / 4 - Y4 * (s + 1) / 4);
+                const double J12 = (X1 * (r - 1) / 4 - X2 * (r + 1) / 4 + X3 * (r + 1) / 4 - X4 * (r - 1) / 4);
+                const double J22 = (Y1 * (r - 1) / 4 - Y2 * (r + 1) / 4 + Y3 * (r + 1) / 4 - Y4 * (r - 1) / 4);
+                const double A = sqrt(J11 * J11 + J21 * J21);
+                const double B = sqrt(J12 * J12 + J22 * J22);
+                const double ca = J11 / A, sa = J21 / A, cb = J12 / B, sb = J22 / B;
+                const double detJ = J11 * J22 - J12 * J21;
+                const double Ax = X1 - X2 - X3 + X4, Ay = Y1 - Y2 - Y3 + Y4;
+                const double Bx = X1 - X2 + X3 - X4, By = Y1 - Y2 + Y3 - Y4;
+                const double Cx = X1 + X2 - X3 - X4, Cy = Y1 + Y2 - Y3 - Y4;
+                const double P = sqrt((Ax + Bx * s) * (Ax + Bx * s) + (Ay + By * s) * (Ay + By * s));
+                const double Q = sqrt((Bx * r + Cx) * (Bx * r + Cx) + (By * r + Cy) * (By * r + Cy));
+                const double i16 = 1.0 / (16 * detJ), i32 = 1.0 / (32 * detJ);
+                double *bs = g + T::G_BS;  // [node][comp][row]
+                // node 1
+                bs[0 * 6 + 0 * 2 + 0] = (sa * (r + 1) * P - sb * (s + 1) * Q) * i16;
+                bs[0 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
+                bs[0 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
+                bs[0 * 6 + 0 * 2 + 1] = (-ca * (r + 1) * P + cb * (s + 1) * Q) * i16;
+                bs[0 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
+                bs[0 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
+                // node 2
+                bs[1 * 6 + 0 * 2 + 0] = (-sa * (r - 1) * P + sb * (s + 1) * Q) * i16;
+                bs[1 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
+                bs[1 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
+                bs[1 * 6 + 0 * 2 + 1] = (ca * (r - 1) * P - cb * (s + 1) * Q) * i16;
+                bs[1 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
+                bs[1 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
+                // node 3
+                bs[2 * 6 + 0 * 2 + 0] = (sa * (r - 1) * P - sb * (s - 1) * Q) * i16;
+                bs[2 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
+                bs[2 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
+                bs[2 * 6 + 0 * 2 + 1] = (-ca * (r - 1) * P + cb * (s - 1) * Q) * i16;
+                bs[2 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
+                bs[2 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
+                // node 4
+                bs[3 * 6 + 0 * 2 + 0] = (-sa * (r + 1) * P + sb * (s - 1) * Q) * i16;
+                bs[3 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
+                bs[3 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
+                bs[3 * 6 + 0 * 2 + 1] = (ca * (r + 1) * P - cb * (s - 1) * Q) * i16;
+                bs[3 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
+                bs[3 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
+            }
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) g[T::G_EG + i * 3 + j] = EG[i][j];
+            for (int a = 0; a < NEN; ++a) {
+                g[T::G_GN + 2 * a] = gN[a][0];
+                g[T::G_GN + 2 * a + 1] = gN[a][1];
+            }
+            for (int i = 0; i < 36; ++i) g[T::G_LMB + i] = Lmb[i];
+            for (int i = 0; i < 4; ++i) g[T::G_LS + i] = Ls[i];
+        }
+    }
+    __syncthreads();
+
+    // ---------------- G2/G3: per (element, qp, node) ----------------
+    const bool gthread = tid < NE * NQP * NEN;
+    const int le2 = tid / (NQP * NEN);
+    const int q2 = (tid / NEN) % NQP;
+    const int j2 = tid % NEN;
+    const bool gact = gthread && (le2 < ne);
+    double A[3][3];
+    double cu[NR], cv[NR], cw[NR], cx[NR], cy[NR];
+    double gNj0 = 0.0, gNj1 = 0.0;
+    double EGl[3][3];
+    if (gact) {
+        const int e = e0 + le2;
+        const double *g = sG + (le2 * NQP + q2) * T::GEO;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) EGl[i][j] = g[T::G_EG + i * 3 + j];
+        const int node = conn[j2 * E + e];
+        // _NodalTriadsE (:355-388)
+        double nk_e[3];
+        if (valid[node]) {
+            const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
+            for (int i = 0; i < 3; ++i) nk_e[i] = EGl[0][i] * n0 + EGl[1][i] * n1 + EGl[2][i] * n2;
+        } else {
+            nk_e[0] = 0.0;
+            nk_e[1] = 0.0;
+            nk_e[2] = 1.0;
+        }
+        const double r0 = -nk_e[1], r1 = nk_e[0];
+        const double na = sqrt(r0 * r0 + r1 * r1);
+        if (na > 1.0e-12) {
+            // rotmat3! for r = (r0, r1, 0)
+            const double u0 = r0 / na, u1 = r1 / na;
+            double sa, ca;
+            sincos(na, &sa, &ca);
+            const double f = sa / na;
+            const double uu[3] = {u0, u1, 0.0};
+            const double Sk[3][3] = {{0.0, 0.0, r1}, {0.0, 0.0, -r0}, {-r1, r0, 0.0}};
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const double aa = uu[i] * uu[j];
+                    A[i][j] = ca * (((i == j) ? 1.0 : 0.0) - aa) + f * Sk[i][j] + aa;
+                }
+        } else {
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) A[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+        gNj0 = g[T::G_GN + 2 * j2];
+        gNj1 = g[T::G_GN + 2 * j2 + 1];
+        const double *Lmb = g + T::G_LMB;
+        const double *Ls = g + T::G_LS;
+        const double *bs = g + T::G_BS + j2 * 3 * T::NSH;
+        // element-frame C columns of this node: [Lmb * (Bm; Bb) ; Ls * Bs]
+        for (int s = 0; s < 6; ++s) {
+            cu[s] = Lmb[s * 6 + 0] * gNj0 + Lmb[s * 6 + 2] * gNj1;
+            cv[s] = Lmb[s * 6 + 1] * gNj1 + Lmb[s * 6 + 2] * gNj0;
+            cw[s] = 0.0;
+            cx[s] = -Lmb[s * 6 + 4] * gNj1 - Lmb[s * 6 + 5] * gNj0;
+            cy[s] = Lmb[s * 6 + 3] * gNj0 + Lmb[s * 6 + 5] * gNj1;
+        }
+        for (int o = 0; o < T::NSH / 2; ++o)
+            for (int rr = 0; rr < 2; ++rr) {
+                const int s = 6 + 2 * o + rr;
+                cu[s] = 0.0;
+                cv[s] = 0.0;
+                cw[s] = Ls[rr * 2 + 0] * bs[0 * T::NSH + 2 * o] + Ls[rr * 2 + 1] * bs[0 * T::NSH + 2 * o + 1];
+                cx[s] = Ls[rr * 2 + 0] * bs[1 * T::NSH + 2 * o] + Ls[rr * 2 + 1] * bs[1 * T::NSH + 2 * o + 1];
+                cy[s] = Ls[rr * 2 + 0] * bs[2 * T::NSH + 2 * o] + Ls[rr * 2 + 1] * bs[2 * T::NSH + 2 * o + 1];
+            }
+        // coupling vector d = m1 * cx + m2 * cy (_transfmat_a_to_e!, :451-460)
+        const double m1 = (1 / A[2][2]) * A[0][2];
+        const double m2 = (1 / A[2][2]) * A[1][2];
+        double *nd = sN + ((le2 * NQP + q2) * NEN + j2) * T::NODE;
+        for (int k = 0; k < 3; ++k) {
+            nd[k] = A[0][k];
+            nd[3 + k] = A[1][k];
+        }
+        for (int s = 0; s < NR; ++s) nd[6 + s] = m1 * cx[s] + m2 * cy[s];
+    }
+    __syncthreads();
+    if (gact) {
+        const double *ndb = sN + ((le2 * NQP + q2) * NEN) * T::NODE;
+        // Q = A' * EG' : global -> nodal block (_TransfmatGToA, :398-419)
+        double Q[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int gg = 0; gg < 3; ++gg) Q[k][gg] = A[0][k] * EGl[gg][0] + A[1][k] * EGl[gg][1] + A[2][k] * EGl[gg][2];
+        double cgt[3][NR], cgr[3][NR];
+        for (int gg = 0; gg < 3; ++gg)
+            for (int s = 0; s < NR; ++s) {
+                cgt[gg][s] = 0.0;
+                cgr[gg][s] = 0.0;
+            }
+        // translations: cn_k = sum_rw c_rw A[rw][k] + sum_i a3(i,j,k) d_i
+        for (int k = 0; k < 3; ++k) {
+            double a3[NEN];
+            for (int i = 0; i < NEN; ++i)
+                a3[i] = 1.0 / 2 * (ndb[i * T::NODE + 3 + k] * gNj0 - ndb[i * T::NODE + k] * gNj1);
+            for (int s = 0; s < NR; ++s) {
+                double cn = cu[s] * A[0][k] + cv[s] * A[1][k] + cw[s] * A[2][k];
+                for (int i = 0; i < NEN; ++i) cn += a3[i] * ndb[i * T::NODE + 6 + s];
+                for (int gg = 0; gg < 3; ++gg) cgt[gg][s] += cn * Q[k][gg];
+            }
+        }
+        // rotations: nodal theta_1,2 from element theta_x,y; drilling column zero
+        double kp[2] = {0.0, 0.0};
+        for (int l = 0; l < 2; ++l) {
+            const double r2a = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
+            const double r2b = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
+            for (int s = 0; s < NR; ++s) {
+                const double cn = cx[s] * r2a + cy[s] * r2b;
+                kp[l] += cn * cn;
+                for (int gg = 0; gg < 3; ++gg) cgr[gg][s] += cn * Q[l][gg];
+            }
+        }
+        double *cdst = sC + le2 * T::CSTR + (q2 * NR) * T::K + 6 * j2;
+        for (int s = 0; s < NR; ++s) {
+            for (int gg = 0; gg < 3; ++gg) {
+                cdst[s * T::K + gg] = cgt[gg][s];
+                cdst[s * T::K + 3 + gg] = cgr[gg][s];
+            }
+        }
+        if (NQP == 1) {
+            double *ms = sM + le2 * T::MISC;
+            for (int gg = 0; gg < 3; ++gg) ms[T::M_G + 3 * j2 + gg] = Q[2][gg];
+            ms[T::M_KP + 2 * j2] = kp[0];
+            ms[T::M_KP + 2 * j2 + 1] = kp[1];
+        }
+    }
+    __syncthreads();
+
+    // ---------------- C: node-pair blocks ----------------
+    const bool cthread = tid < NE * T::NBLK;
+    const int le3 = tid / T::NBLK;
+    const int blk = tid % T::NBLK;
+    int ba = 0, bb = 0;
+    {
+        int cnt = 0;
+        for (int a = 0; a < NEN; ++a)
+            for (int b = a; b < NEN; ++b) {
+                if (cnt == blk) {
+                    ba = a;
+                    bb = b;
+                }
+                ++cnt;
+            }
+    }
+    const bool cact = cthread && (le3 < ne);
+    double acc[6][6];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) acc[r][c] = 0.0;
+    if (cact) {
+        const double *cb = sC + le3 * T::CSTR;
+        for (int s = 0; s < T::NS; ++s) {
+            double va[6], vb[6];
+            for (int r = 0; r < 6; ++r) {
+                va[r] = cb[s * T::K + 6 * ba + r];
+                vb[r] = cb[s * T::K + 6 * bb + r];
+            }
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) acc[r][c] += va[r] * vb[c];
+        }
+        if (NEN == 3 && ba == bb) {
+            // T3 drilling in the nodal basis (:714-727) as a global rank-one term
+            const int e = e0 + le3;
+            const int node = conn[ba * E + e];
+            const double *ms = sM + le3 * T::MISC;
+            const double kavg = (((((ms[T::M_KP + 0] + ms[T::M_KP + 2]) + ms[T::M_KP + 4]) + ms[T::M_KP + 1]) +
+                                   ms[T::M_KP + 3]) + ms[T::M_KP + 5]) / 6 * p.drill_scale;
+            if (valid[node]) {
+                const double *gv = ms + T::M_G + 3 * ba;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * gv[r] * gv[c];
+            }
+        }
+        if (NEN == 4 && ba == bb) {
+            double *ms = sM + le3 * T::MISC + T::M_KRR + 9 * ba;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) ms[r * 3 + c] = acc[3 + r][3 + c];
+        }
+    }
+    if constexpr (NEN == 4) {
+        // Q4 drilling (_drilling_penalty_kavg, _add_drilling_stiffness!, :807-859)
+        __syncthreads();
+        if (cact && ba == bb && p.drill_scale != 0.0) {
+            const int e = e0 + le3;
+            const double *kr = sM + le3 * T::MISC + T::M_KRR;
+            double sum = 0.0;
+            int cnt = 0;
+            for (int k = 0; k < 4; ++k) {
+                const int node = conn[k * E + e];
+                if (!valid[node]) continue;
+                const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
+                const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+                if (nn == 0.0) continue;
+                const double u[3] = {n0 / nn, n1 / nn, n2 / nn};
+                // tr(P K P) with P = I - u u'
+                double Pm[3][3], tmp[3][3];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) Pm[a][b] = ((a == b) ? 1.0 : 0.0) - u[a] * u[b];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        tmp[a][b] = Pm[a][0] * kr[9 * k + 0 * 3 + b] + Pm[a][1] * kr[9 * k + 1 * 3 + b] +
+                                    Pm[a][2] * kr[9 * k + 2 * 3 + b];
+                double tr = 0.0;
+                for (int a = 0; a < 3; ++a) tr += tmp[a][0] * Pm[0][a] + tmp[a][1] * Pm[1][a] + tmp[a][2] * Pm[2][a];
+                tr = tr / 2.0;
+                sum += tr > 0.0 ? tr : 0.0;
+                ++cnt;
+            }
+            const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * p.drill_scale;
+            const int node = conn[ba * E + e];
+            if (kavg != 0.0 && valid[node]) {
+                const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
+                if (sqrt(n0 * n0 + n1 * n1 + n2 * n2) != 0.0) {
+                    const double nv[3] = {n0, n1, n2};
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (nv[r] * nv[c]);
+                }
+            }
+        }
+    }
+    __syncthreads();  // every read of sC is done: reuse it as the output stage
+    if (cact) {
+        double *ks = sC + (le3 * T::NBLK + blk) * 36;
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) ks[r * 6 + c] = acc[r][c];
+    }
+    __syncthreads();
+    {
+        const int total = ne * T::KSTAGE;  // contiguous in Kb
+        double *dst = Kb + (size_t)e0 * T::KSTAGE;
+        for (int i = tid; i < total; i += WG) dst[i] = sC[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scatter: node-pair blocks -> CSR values
+// ---------------------------------------------------------------------------
+//
+// Thread (pair p, side, r): side 0 writes row r of block (I, J) into CSR row
+// dof(I, r); side 1 (I != J) writes row r of block (J, I) = column r of
+// (I, J) into CSR row dof(J, r).  Contributions are summed in ascending
+// element order.
+__global__ __launch_bounds__(256) void scatter_kernel(ScatterPlan sp, const double *__restrict__ Kb,
+                                                      int nblk, double *__restrict__ vals) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = gid / 12;
+    const int sub = (int)(gid % 12);
+    if (p >= sp.npairs) return;
+    const int side = sub / 6, r = sub % 6;
+    const int I = sp.pair_I[p], J = sp.pair_J[p];
+    if (side == 1 && I == J) return;
+    const int rowNode = side == 0 ? I : J;
+    const int colNode = side == 0 ? J : I;
+    const int64_t rs = sp.node_rowstart[(int64_t)rowNode * 6 + r];
+    if (rs < 0) return;
+    const int offFree = sp.pair_off[p * 4 + side * 2 + 0];
+    const int offFix = sp.pair_off[p * 4 + side * 2 + 1];
+    const int cb = sp.contrib_ptr[p], ce = sp.contrib_ptr[p + 1];
+    double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = cb; k < ce; ++k) {
+        const int cv = sp.contrib[k];
+        const int64_t bi = (int64_t)(cv >> 1);
+        // stored block is (I,J) when not transposed; (J,I) when transposed
+        const bool tr = (cv & 1) != 0;
+        // element of the (rowNode, colNode) block at [r][c]:
+        //   side 0, !tr : B[r][c]      side 0, tr : B[c][r]
+        //   side 1, !tr : B[c][r]      side 1, tr : B[r][c]
+        const double *B = Kb + bi * 36;
+        const bool rowmajor = (side == 0) != tr;
+        for (int c = 0; c < 6; ++c) sum[c] += rowmajor ? B[r * 6 + c] : B[c * 6 + r];
+    }
+    const int8_t *code = sp.node_code + (int64_t)colNode * 6;
+    for (int c = 0; c < 6; ++c) {
+        const int cd = code[c];
+        if (cd < 0) continue;
+        const int64_t pos = (cd < 8) ? rs + offFree + cd : rs + offFix + (cd - 8);
+        vals[pos] = sum[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Nodal normals (associategeometry!)
+// ---------------------------------------------------------------------------
+//
+// Thread per node: the incident elements are visited in ascending order
+// (node_elem lists), reproducing the element-order accumulation of
+// FEMMShellT3FFModule.jl:579-587 / FEMMShellQ4RSModule.jl:487-495, then the
+// normalisation and the validity pass (:597-611 / :508-520).
+template <int NEN>
+__global__ __launch_bounds__(256) void normals_kernel(int64_t N, int E, const double *__restrict__ xs,
+                                                      const double *__restrict__ ys,
+                                                      const double *__restrict__ zs,
+                                                      const int32_t *__restrict__ conn,
+                                                      const int32_t *__restrict__ ne_ptr,
+                                                      const int32_t *__restrict__ ne_list, double ntol,
+                                                      double *__restrict__ nx, double *__restrict__ ny,
+                                                      double *__restrict__ nz, uint8_t *__restrict__ valid) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int b = ne_ptr[I], en = ne_ptr[I + 1];
+    double nrm[3];
+    for (int pass = 0; pass < 2; ++pass) {
+        uint8_t ok = 1;
+        for (int k = b; k < en; ++k) {
+            const int e = ne_list[k];
+            double X[NEN][3];
+            int loc = 0;
+            for (int a = 0; a < NEN; ++a) {
+                const int n = conn[a * E + e];
+                if (n == I) loc = a;
+                X[a][0] = xs[n];
+                X[a][1] = ys[n];
+                X[a][2] = zs[n];
+            }
+            double J[3][2], EG[3][3];
+            double w = 1.0;
+            if (NEN == 3) {
+                for (int j = 0; j < 3; ++j) {
+                    J[j][0] = X[1][j] - X[0][j];
+                    J[j][1] = X[2][j] - X[0][j];
+                }
+            } else {
+                const double npc[4][2] = {{-1.0, -1.0}, {1.0, -1.0}, {1.0, 1.0}, {-1.0, 1.0}};
+                double gNp[4][2];
+                q4_gradn(npc[loc][0], npc[loc][1], gNp);
+                for (int i = 0; i < 3; ++i)
+                    for (int kk = 0; kk < 2; ++kk) {
+                        double s = 0.0;
+                        for (int a = 0; a < 4; ++a) s += X[a][i] * gNp[a][kk];
+                        J[i][kk] = s;
+                    }
+                const double c0 = -J[2][0] * J[1][1] + J[1][0] * J[2][1];
+                const double c1 = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+                const double c2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
+                w = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+            }
+            e_g(EG, J);
+            if (pass == 0) {
+                if (NEN == 3) {
+                    acc[0] += EG[0][2];
+                    acc[1] += EG[1][2];
+                    acc[2] += EG[2][2];
+                } else {
+                    acc[0] += w * EG[0][2];
+                    acc[1] += w * EG[1][2];
+                    acc[2] += w * EG[2][2];
+                }
+            } else {
+                const double nd = nrm[0] * EG[0][2] + nrm[1] * EG[1][2] + nrm[2] * EG[2][2];
+                if (nd < 1 - ntol) ok = 0;
+            }
+        }
+        if (pass == 0) {
+            const double nn = sqrt(acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2]);
+            for (int k = 0; k < 3; ++k) nrm[k] = nn > 0.0 ? acc[k] / nn : acc[k];
+        } else {
+            nx[I] = nrm[0];
+            ny[I] = nrm[1];
+            nz[I] = nrm[2];
+            valid[I] = ok;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lumped mass
+// ---------------------------------------------------------------------------
+//
+// Thread per node: sums the element masses of the incident elements in
+// ascending element order.  T3: rho t Ae / 3 and rho t^3/12 Ae / 3
+// (FEMMShellT3FFModule.jl:790-791); Q4: 2x2 Gauss sums divided by 4
+// (FEMMShellQ4RSModule.jl:994-1002); layered: laminate inertia per group.
+struct MassParams {
+    int composite;
+    double rho, t;
+    const double *g_inertia;  // ngroups x 2
+    const int32_t *elem_group;
+};
+
+template <int NEN>
+__global__ __launch_bounds__(256) void mass_kernel(int64_t N, int E, MassParams mp, const double *__restrict__ xs,
+                                                   const double *__restrict__ ys, const double *__restrict__ zs,
+                                                   const int32_t *__restrict__ conn,
+                                                   const int32_t *__restrict__ ne_ptr,
+                                                   const int32_t *__restrict__ ne_list,
+                                                   const int64_t *__restrict__ node_dof, int64_t nrows,
+                                                   double *__restrict__ diag) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    double tm = 0.0, rm = 0.0;
+    bool first = true;
+    for (int k = ne_ptr[I]; k < ne_ptr[I + 1]; ++k) {
+        const int e = ne_list[k];
+        double X[NEN][3];
+        for (int a = 0; a < NEN; ++a) {
+            const int n = conn[a * E + e];
+            X[a][0] = xs[n];
+            X[a][1] = ys[n];
+            X[a][2] = zs[n];
+        }
+        double m0 = 0.0, m1 = 0.0;
+        if (mp.composite) {
+            const int g = mp.elem_group ? mp.elem_group[e] : 0;
+            m0 = mp.g_inertia[2 * g];
+            m1 = mp.g_inertia[2 * g + 1];
+        }
+        double te, re;
+        if (NEN == 3) {
+            double J0[3][2], EG[3][3];
+            for (int j = 0; j < 3; ++j) {
+                J0[j][0] = X[1][j] - X[0][j];
+                J0[j][1] = X[2][j] - X[0][j];
+            }
+            e_g(EG, J0);
+            const double a = J0[0][0] * EG[0][0] + J0[1][0] * EG[1][0] + J0[2][0] * EG[2][0];
+            const double bq = J0[0][0] * EG[0][1] + J0[1][0] * EG[1][1] + J0[2][0] * EG[2][1];
+            const double c = J0[0][1] * EG[0][0] + J0[1][1] * EG[1][0] + J0[2][1] * EG[2][0];
+            const double d = J0[0][1] * EG[0][1] + J0[1][1] * EG[1][1] + J0[2][1] * EG[2][1];
+            const double Ae = (a * d - bq * c) / 2;
+            if (!mp.composite) {
+                te = mp.rho * (mp.t * Ae) / 3;
+                re = mp.rho * (mp.t * mp.t * mp.t / 12 * Ae) / 3;
+            } else {
+                te = m0 * (Ae / 3);
+                re = m1 * (Ae / 3);
+            }
+        } else {
+            double sa = 0.0, sb = 0.0;
+            for (int q = 0; q < 4; ++q) {
+                const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+                const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+                double gNp[4][2], J[3][2];
+                q4_gradn(xi, eta, gNp);
+                for (int i = 0; i < 3; ++i)
+                    for (int kk = 0; kk < 2; ++kk) {
+                        double s = 0.0;
+                        for (int a = 0; a < 4; ++a) s += X[a][i] * gNp[a][kk];
+                        J[i][kk] = s;
+                    }
+                const double c0 = -J[2][0] * J[1][1] + J[1][0] * J[2][1];
+                const double c1 = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+                const double c2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
+                const double Jac = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+                if (!mp.composite) {
+                    sa += mp.rho * mp.t * Jac * 1.0;
+                    sb += mp.rho * (mp.t * mp.t * mp.t) / 12 * Jac * 1.0;
+                } else {
+                    sa += m0 * Jac * 1.0;
+                    sb += m1 * Jac * 1.0;
+                }
+            }
+            te = sa / 4;
+            re = sb / 4;
+        }
+        if (first) {
+            tm = te;
+            rm = re;
+            first = false;
+        } else {
+            tm += te;
+            rm += re;
+        }
+    }
+    for (int c = 0; c < 6; ++c) {
+        const int64_t d = node_dof[I * 6 + c];
+        if (d >= 0 && d < nrows) diag[d] = c < 3 ? tm : rm;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Explicit central difference
+// ---------------------------------------------------------------------------
+//
+// The reference step (plate_expl_examples.jl:83-93) is
+//   U += dt V + dt^2/2 A;  F = g(t) Fmag;  E = K U;
+//   F -= E + C (V + dt/2 A);  V += dt/2 A;  A = invMC F;  V += dt/2 A
+// The fused form here keeps the displacement one update ahead: the kernel of
+// step n reads U_n (already advanced), finishes the step for its row and
+// writes U_{n+1} = U_n + dt V_n + dt^2/2 A_n into the other U buffer, so one
+// launch per step reads K once and every state vector once.
+//
+// cd_init_kernel: A_0 = invMC g_0 Fmag;  U_1 = U_0 + dt V_0 + dt^2/2 A_0.
+__global__ __launch_bounds__(256) void cd_init_kernel(int64_t n, double dt, double g0,
+                                                      const double *__restrict__ invMC,
+                                                      const double *__restrict__ Fmag, const double *__restrict__ U0,
+                                                      const double *__restrict__ V, double *__restrict__ A,
+                                                      double *__restrict__ U1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = invMC[i] * (g0 * Fmag[i]);
+    A[i] = a;
+    const double dt2_2 = ((dt * dt) / 2);
+    U1[i] = U0[i] + (dt * V[i] + dt2_2 * a);
+}
+
+// One step for a group of TPR lanes per row (TPR in {4, 8, 16, 32, 64}).
+template <int TPR>
+__global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ colidx,
+                                                      const double *__restrict__ kval, const double *__restrict__ Ucur,
+                                                      double *__restrict__ Unext, double *__restrict__ V,
+                                                      double *__restrict__ A, const double *__restrict__ invMC,
+                                                      const double *__restrict__ Cd, const double *__restrict__ Fmag,
+                                                      double gstep, double dt, int damped,
+                                                      const int32_t *__restrict__ probe_slot,
+                                                      double *__restrict__ probe_dst) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gt / TPR;
+    const int lane = (int)(gt % TPR);
+    const bool active = row < n;
+    double e = 0.0;
+    if (active) {
+        const int b = rowptr[row], en = rowptr[row + 1];
+        for (int k = b + lane; k < en; k += TPR) e += kval[k] * Ucur[colidx[k]];
+    }
+#pragma unroll
+    for (int off = TPR / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, TPR);
+    if (active && lane == 0) {
+        const double dt_2 = (dt / 2), dt2_2 = ((dt * dt) / 2);
+        const double v = V[row], a = A[row];
+        double f = gstep * Fmag[row];
+        if (damped)
+            f -= e + Cd[row] * (v + dt_2 * a);
+        else
+            f -= e;
+        double vn = v + dt_2 * a;
+        const double an = invMC[row] * f;
+        vn += dt_2 * an;
+        const double u = Ucur[row];
+        V[row] = vn;
+        A[row] = an;
+        Unext[row] = u + (dt * vn + dt2_2 * an);
+        if (probe_slot) {
+            const int ps = probe_slot[row];
+            if (ps >= 0) probe_dst[ps] = u;  // peek(step, U, V, t): U of this step
+        }
+    }
+}
+
+// probes: out[k] = U[probe[k]]
+__global__ void gather_kernel(int n, const int64_t *__restrict__ idx, const double *__restrict__ U,
+                              double *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = U[idx[k]];
+}
+
+// y = K x (power iteration)
+template <int TPR>
+__global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ colidx,
+                                                   const double *__restrict__ kval, const double *__restrict__ x,
+                                                   double *__restrict__ y) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gt / TPR;
+    const int lane = (int)(gt % TPR);
+    double e = 0.0;
+    if (row < n) {
+        const int b = rowptr[row], en = rowptr[row + 1];
+        for (int k = b + lane; k < en; k += TPR) e += kval[k] * x[colidx[k]];
+    }
+#pragma unroll
+    for (int off = TPR / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, TPR);
+    if (row < n && lane == 0) y[row] = e;
+}
+
+// Deterministic reductions for the power iteration: fixed tree per block,
+// then one block sums the block partials in index order.
+// mode 0: sum x*x; mode 1: sum x*y; mode 2: sum x*(d*x)
+__global__ __launch_bounds__(256) void dot_partial_kernel(int64_t n, int mode, const double *__restrict__ x,
+                                                          const double *__restrict__ y,
+                                                          const double *__restrict__ d, double *__restrict__ part) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double xv = x[i];
+        s += mode == 0 ? xv * xv : (mode == 1 ? xv * y[i] : xv * (d[i] * xv));
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void dot_final_kernel(int nb, const double *__restrict__ part,
+                                                        double *__restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// y = (d ? d : 1) * x / sqrt(ss[0])  (normalisation steps of pwr)
+__global__ __launch_bounds__(256) void scale_kernel(int64_t n, const double *__restrict__ ss,
+                                                    const double *__restrict__ d, const double *__restrict__ x,
+                                                    double *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double f = 1.0 / sqrt(ss[0]);
+    y[i] = d ? d[i] * (x[i] * f) : x[i] * f;
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side of this translation unit)
+// ---------------------------------------------------------------------------
+
+template <int NEN, int NQP, int NR, int NE, int WG>
+static hipError_t launch_elem(const ElemParams &p, const double *x, const double *y, const double *z,
+                              const int32_t *conn, const double *nx, const double *ny, const double *nz,
+                              const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
+    using S = ElemSmem<NEN, NQP, NR, NE>;
+    const size_t smem = S::TOTAL * sizeof(double);
+    static_assert(S::TOTAL * sizeof(double) <= 65536, "element kernel LDS exceeds 64 KiB");
+    static_assert(NE * NQP * NEN <= WG && NE * (NEN * (NEN + 1) / 2) <= WG, "workgroup too small");
+    const int nblocks = (p.nelem + NE - 1) / NE;
+    if (nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
+                       nx, ny, nz, valid, Kb, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_element_blocks(const ElemParams &p, const double *x, const double *y, const double *z,
+                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
+                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
+    if (p.nen == 3) {
+        if (p.formulation == 1)
+            return launch_elem<3, 1, 12, 16, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
+        return launch_elem<3, 1, 8, 16, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
+    }
+    return launch_elem<4, 4, 8, 5, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
+}
+
+hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, int nblk, double *vals, hipStream_t st) {
+    if (sp.npairs == 0) return hipSuccess;
+    const int64_t threads = sp.npairs * 12;
+    const int64_t blocks = (threads + 255) / 256;
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, sp, Kb, nblk, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const double *y, const double *z,
+                          const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
+                          double *nx, double *ny, double *nz, uint8_t *valid, hipStream_t st) {
+    if (N == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((N + 255) / 256);
+    if (nen == 3)
+        hipLaunchKernelGGL(normals_kernel<3>, dim3(blocks), dim3(256), 0, st, N, E, x, y, z, conn, ne_ptr, ne_list,
+                           ntol, nx, ny, nz, valid);
+    else
+        hipLaunchKernelGGL(normals_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, x, y, z, conn, ne_ptr, ne_list,
+                           ntol, nx, ny, nz, valid);
+    return hipGetLastError();
+}
+
+hipError_t launch_mass(int nen, int64_t N, int E, int composite, double rho, double t, const double *g_inertia,
+                       const int32_t *elem_group, const double *x, const double *y, const double *z,
+                       const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, const int64_t *node_dof,
+                       int64_t nrows, double *diag, hipStream_t st) {
+    if (N == 0) return hipSuccess;
+    MassParams mp{composite, rho, t, g_inertia, elem_group};
+    const unsigned blocks = (unsigned)((N + 255) / 256);
+    if (nen == 3)
+        hipLaunchKernelGGL(mass_kernel<3>, dim3(blocks), dim3(256), 0, st, N, E, mp, x, y, z, conn, ne_ptr, ne_list,
+                           node_dof, nrows, diag);
+    else
+        hipLaunchKernelGGL(mass_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, mp, x, y, z, conn, ne_ptr, ne_list,
+                           node_dof, nrows, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_cd_init(int64_t n, double dt, double g0, const double *invMC, const double *Fmag, const double *U0,
+                          const double *V, double *A, double *U1, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(cd_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, dt, g0, invMC, Fmag,
+                       U0, V, A, U1);
+    return hipGetLastError();
+}
+
+hipError_t launch_cd_step(int tpr, int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval,
+                          const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
+                          const double *Cd, const double *Fmag, double gstep, double dt, int damped,
+                          const int32_t *probe_slot, double *probe_dst, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int64_t threads = n * tpr;
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+#define FS_CD_CASE(T)                                                                                           \
+    case T:                                                                                                     \
+        hipLaunchKernelGGL(cd_step_kernel<T>, dim3(blocks), dim3(256), 0, st, n, rowptr, colidx, kval, Ucur,     \
+                           Unext, V, A, invMC, Cd, Fmag, gstep, dt, damped, probe_slot, probe_dst);           \
+        break;
+    switch (tpr) {
+        FS_CD_CASE(4)
+        FS_CD_CASE(8)
+        FS_CD_CASE(16)
+        FS_CD_CASE(32)
+        FS_CD_CASE(64)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef FS_CD_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(int n, const int64_t *idx, const double *U, double *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, idx, U, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_spmv(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval, const double *x,
+                       double *y, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int64_t threads = n * 8;
+    hipLaunchKernelGGL(spmv_kernel<8>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, n, rowptr, colidx,
+                       kval, x, y);
+    return hipGetLastError();
+}
+
+static constexpr int kDotBlocks = 512;
+
+hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, const double *d, double *part,
+                      double *out, hipStream_t st) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(256), 0, st, n, mode, x, y, d, part);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, st, kDotBlocks, part, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(int64_t n, const double *ss, const double *d, const double *x, double *y, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, ss, d, x, y);
+    return hipGetLastError();
+}
+
+}  // namespace fs

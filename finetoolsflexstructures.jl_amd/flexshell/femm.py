@@ -1,0 +1,374 @@
+"""Shell FEMMs and assemblers: the drop-in surface of the reference's API.
+
+Mirrors FEMMShellT3FFModule, FEMMShellQ4RSModule, FEMMShellT3FFCompModule and
+FEMMShellQ4RSCompModule (make, associategeometry!, stiffness, mass) and the
+assembler protocol of AssemblyModule.jl / FinEtools (SysmatAssemblerSparseSymm,
+SysmatAssemblerSparseCSRSymm, SysmatAssemblerFFBlock, SysmatAssemblerSparseDiag).
+Every numeric result comes from libflexshell on the GPU; this module only
+marshals arrays across the C ABI.
+
+Julia's `!` suffix is dropped (associategeometry! -> associategeometry).  The
+stabilisation closure stab_fun(t, h) = t^2 / (t^2 + c h^2) is given by its
+coefficient c (`stab_c`), which covers every form the reference uses.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from .material import shell_material_stiffness
+
+__TRANSV_SHEAR_FORMULATION_AVERAGE_B = 0
+__TRANSV_SHEAR_FORMULATION_AVERAGE_K = 1
+AVERAGE_B = 0
+AVERAGE_K = 1
+
+
+# ---------------------------------------------------------------------------
+# assemblers
+# ---------------------------------------------------------------------------
+
+class SysmatAssemblerSparseSymm:
+    """All-dof sparse result (FinEtools default assembler); returned as CSR."""
+    block = _lib.FS_BLOCK_ALL
+    diag_only = False
+
+
+class SysmatAssemblerSparseCSRSymm(SysmatAssemblerSparseSymm):
+    """AssemblyModule.jl:20-53: all dofs, CSR result."""
+
+
+class SysmatAssemblerSparseDiag:
+    """Diagonal (lumped) result over all dofs."""
+    block = _lib.FS_BLOCK_ALL
+    diag_only = True
+
+
+class SysmatAssemblerFFBlock:
+    """Free-free block (FinEtools SysmatAssemblerFFBlock(nfreedofs) or
+    SysmatAssemblerFFBlock(SysmatAssemblerSparseDiag(), nf, nf))."""
+    block = _lib.FS_BLOCK_FF
+
+    def __init__(self, *args):
+        self.diag_only = any(isinstance(a, SysmatAssemblerSparseDiag) for a in args)
+
+
+class SysmatAssemblerHIP:
+    """Keeps the result resident on the device; returns a DeviceMatrix."""
+
+    def __init__(self, block=_lib.FS_BLOCK_FF):
+        self.block = block
+        self.diag_only = False
+
+
+class DeviceMatrix:
+    """Handle to a matrix resident in HBM (the CSR stiffness or the lumped
+    mass of one block of a model)."""
+
+    def __init__(self, femm, block, kind):
+        self.femm = femm
+        self.block = block
+        self.kind = kind
+
+    @property
+    def model(self):
+        return self.femm._model
+
+
+# ---------------------------------------------------------------------------
+# FEMMs
+# ---------------------------------------------------------------------------
+
+class _ShellFEMM:
+    nen = None
+    default_stab_c = None
+    composite = False
+
+    def __init__(self, conn, material, thickness=None, stab_c=None, mcsys=None, ctx=None):
+        self.conn = np.ascontiguousarray(np.asarray(conn, dtype=np.int64))
+        if self.conn.shape[1] != self.nen:
+            raise ValueError(f"{type(self).__name__} needs {self.nen}-node elements")
+        self.material = material
+        self.thickness = thickness
+        self.stab_c = self.default_stab_c if stab_c is None else float(stab_c)
+        self.mcsys = mcsys
+        self.transv_shear_formulation = AVERAGE_B
+        self.drilling_stiffness_scale = 1.0
+        self.threshold_angle = 30.0
+        self._ctx = ctx
+        self._model = None
+        self._geom_key = None
+        self._dofs_key = None
+        self._associatedgeometry = False
+        self._normals = None
+        self._normal_valid = None
+        self._patterns = {}
+
+    # -- device model management --
+    def _context(self):
+        if self._ctx is None:
+            self._ctx = _lib.default_context()
+        return self._ctx
+
+    def _ensure_model(self, geom0):
+        geom0 = np.ascontiguousarray(np.asarray(geom0, dtype=np.float64))
+        key = (geom0.shape, hash(geom0.tobytes()))
+        if self._model is None or key != self._geom_key:
+            if self._model is not None:
+                self._model.close()
+            self._model = _lib.Model(self._context(), self.nen, geom0, self.conn)
+            self._geom_key = key
+            self._dofs_key = None
+            self._patterns = {}
+        return self._model
+
+    def _ensure_dofs(self, dchi):
+        key = (dchi.nalldofs, dchi.nfreedofs, hash(dchi.dofnums.tobytes()))
+        if key != self._dofs_key:
+            self._model.set_dofs(dchi.dofnums, dchi.nalldofs, dchi.nfreedofs)
+            self._dofs_key = key
+            self._patterns = {}
+            if self._normals is not None:
+                self._model.normals_upload(self._normals, self._normal_valid)
+
+    def _pattern(self, block):
+        if block not in self._patterns:
+            self._patterns[block] = self._model.pattern(block)
+        return self._patterns[block]
+
+    # -- associategeometry! --
+    def associategeometry(self, geom0):
+        """Nodal normals (FEMMShellT3FFModule.jl:570-616, FEMMShellQ4RSModule.jl:472-525)."""
+        m = self._ensure_model(geom0)
+        csys_normal = self._csys_normal_function()
+        if csys_normal is None:
+            self._normals, self._normal_valid = m.normals_isoparametric(self.threshold_angle)
+        else:
+            self._normals, self._normal_valid = self._host_csys_normals(np.asarray(geom0, dtype=float), csys_normal)
+            m.normals_upload(self._normals, self._normal_valid)
+        self._associatedgeometry = True
+        return self
+
+    def _csys_normal_function(self):
+        return None if self.mcsys is None else (lambda X, J, e, q: np.asarray(self.mcsys(X, J, e, q))[:, 2])
+
+    def _host_csys_normals(self, xyz, normal_of):
+        """Normals from a coordinate-system closure, which cannot cross the ABI
+        (the `mcsys` / `layup.csys` branch of associategeometry!)."""
+        conn = self.conn
+        N = xyz.shape[0]
+        nrm = np.zeros((N, 3))
+        valid = np.ones(N, dtype=bool)
+        s = np.sin(self.threshold_angle / 180 * np.pi)
+        ntol = 1 - np.sqrt(1 - s * s)
+        if self.nen == 3:
+            for el, (i, j, k) in enumerate(conn):
+                J0 = np.stack([xyz[j] - xyz[i], xyz[k] - xyz[i]], axis=1)
+                for n in (i, j, k):
+                    nrm[n] += normal_of(xyz[n], J0, el, 0)
+            nn = np.linalg.norm(nrm, axis=1)
+            nrm[nn > 0] /= nn[nn > 0, None]
+            for el, (i, j, k) in enumerate(conn):
+                J0 = np.stack([xyz[j] - xyz[i], xyz[k] - xyz[i]], axis=1)
+                e3 = _element_normal(J0)
+                for n in (i, j, k):
+                    if nrm[n] @ e3 < 1 - ntol:
+                        valid[n] = False
+        else:
+            npc = [(-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)]
+            for el, c in enumerate(conn):
+                X = xyz[c]
+                for jj in range(4):
+                    J = _q4_jac(X, *npc[jj])
+                    Jac = np.linalg.norm(np.cross(J[:, 0], J[:, 1]))
+                    loc = _q4_N(*npc[jj]) @ X
+                    nrm[c[jj]] += Jac * normal_of(loc, J, el, jj)
+            nn = np.linalg.norm(nrm, axis=1)
+            nrm[nn > 0] /= nn[nn > 0, None]
+            for el, c in enumerate(conn):
+                X = xyz[c]
+                for jj in range(4):
+                    J = _q4_jac(X, *npc[jj])
+                    loc = _q4_N(*npc[jj]) @ X
+                    if nrm[c[jj]] @ normal_of(loc, J, el, jj) < 1 - ntol:
+                        valid[c[jj]] = False
+        return nrm, valid
+
+    # -- parameters --
+    def _shell_params(self):
+        p = _lib.ShellParams()
+        p.composite = 0
+        p.formulation = int(self.transv_shear_formulation)
+        p.thickness = float(self.thickness)
+        p.stab_c = float(self.stab_c)
+        p.drilling_stiffness_scale = float(self.drilling_stiffness_scale)
+        Dps, Dt = shell_material_stiffness(self.material)
+        p.Dps[:] = list(np.asarray(Dps).T.ravel())
+        p.Dt[:] = list(np.asarray(Dt).T.ravel())
+        self._keep = ()
+        return p
+
+    def _mass_params(self):
+        p = _lib.MassParams()
+        p.composite = 0
+        p.rho = float(self.material.rho)
+        p.thickness = float(self.thickness)
+        self._mkeep = ()
+        return p
+
+    # -- stiffness / mass --
+    def stiffness(self, assembler, geom0, u1=None, Rfield1=None, dchi=None):
+        """stiffness(femm, assembler, geom0, u1, Rfield1, dchi); u1 and Rfield1
+        are ignored by the linear shells, as in the reference."""
+        if dchi is None:  # stiffness(femm, geom0, u1, Rfield1, dchi) form
+            assembler, geom0, dchi = SysmatAssemblerSparseSymm(), assembler, Rfield1
+        assert self._associatedgeometry, "associategeometry must be called first"
+        m = self._ensure_model(geom0)
+        self._ensure_dofs(dchi)
+        block = assembler.block
+        nrows, rp, cv = self._pattern(block)
+        vals = m.stiffness(self._shell_params(), block, len(cv))
+        if isinstance(assembler, SysmatAssemblerHIP):
+            return DeviceMatrix(self, block, "K")
+        return sp.csr_matrix((vals, cv, rp), shape=(nrows, nrows))
+
+    def element_matrices(self, geom0, dchi=None):
+        """Element matrices (nelem x k x k, global basis): the elmat that the
+        reference passes to assemble!."""
+        assert self._associatedgeometry, "associategeometry must be called first"
+        m = self._ensure_model(geom0)
+        blocks = m.element_blocks(self._shell_params())
+        nen = self.nen
+        K = np.zeros((blocks.shape[0], 6 * nen, 6 * nen))
+        b = 0
+        for a in range(nen):
+            for c in range(a, nen):
+                K[:, 6 * a:6 * a + 6, 6 * c:6 * c + 6] = blocks[:, b]
+                if a != c:
+                    K[:, 6 * c:6 * c + 6, 6 * a:6 * a + 6] = np.transpose(blocks[:, b], (0, 2, 1))
+                b += 1
+        return K
+
+    def mass(self, assembler, geom0, dchi=None):
+        """Lumped mass (mass(femm, assembler, geom0, dchi))."""
+        if dchi is None:
+            assembler, geom0, dchi = SysmatAssemblerSparseSymm(), assembler, geom0
+        assert self._associatedgeometry, "associategeometry must be called first"
+        m = self._ensure_model(geom0)
+        self._ensure_dofs(dchi)
+        block = assembler.block
+        nrows = dchi.nfreedofs if block == _lib.FS_BLOCK_FF else dchi.nalldofs
+        d = m.mass(self._mass_params(), block, nrows)
+        if isinstance(assembler, SysmatAssemblerHIP):
+            return DeviceMatrix(self, block, "M")
+        return sp.diags(d, format="csr")
+
+    def num_normals(self):
+        v = self._normal_valid
+        return len(v), int(np.count_nonzero(~v))
+
+
+def _element_normal(J0):
+    e1 = J0[:, 0] / np.linalg.norm(J0[:, 0])
+    n = np.cross(e1, J0[:, 1])
+    return n / np.linalg.norm(n)
+
+
+def _q4_N(xi, eta):
+    return 0.25 * np.array([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)])
+
+
+def _q4_jac(X, xi, eta):
+    gN = 0.25 * np.array([[-(1 - eta), -(1 - xi)], [(1 - eta), -(1 + xi)], [(1 + eta), (1 + xi)],
+                          [-(1 + eta), (1 - xi)]])
+    return X.T @ gN
+
+
+class FEMMShellT3FF(_ShellFEMM):
+    """FEMMShellT3FFModule.FEMMShellT3FF (FEMMShellT3FFModule.jl:101-112)."""
+    nen = 3
+    default_stab_c = 5 / 12 / 1.5  # _stab_fun, :36
+
+
+class FEMMShellQ4RS(_ShellFEMM):
+    """FEMMShellQ4RSModule.FEMMShellQ4RS (FEMMShellQ4RSModule.jl:75-85)."""
+    nen = 4
+    default_stab_c = 0.1  # _stab_fun, :32
+
+
+class _CompositeMixin:
+    composite = True
+
+    def __init__(self, conn, layup, stab_c=None, ctx=None, layup_groups=None):
+        """layup: a CompositeLayup for all elements, or None with layup_groups =
+        [(layup, element_indices), ...] (the layup_groups field of the reference)."""
+        if layup_groups is None:
+            layup_groups = [(layup, np.arange(np.asarray(conn).shape[0]))]
+        super().__init__(conn, None, thickness=None, stab_c=stab_c, ctx=ctx)
+        self.layup_groups = layup_groups
+        E = self.conn.shape[0]
+        self._group = np.full(E, -1, dtype=np.int32)
+        for g, (_, eset) in enumerate(layup_groups):
+            self._group[np.asarray(eset)] = g
+        if (self._group < 0).any():
+            raise ValueError("every element must belong to a layup group")
+
+    def _csys_normal_function(self):
+        groups = self._group
+        lay = [lg[0] for lg in self.layup_groups]
+        return lambda X, J, e, q: lay[groups[e]].csys[:, 2]
+
+    def _shell_params(self):
+        p = _lib.ShellParams()
+        p.composite = 1
+        p.formulation = int(self.transv_shear_formulation)
+        p.stab_c = float(self.stab_c)
+        p.drilling_stiffness_scale = float(self.drilling_stiffness_scale)
+        ng = len(self.layup_groups)
+        abdh = np.zeros((ng, 31))
+        csys = np.zeros((ng, 9))
+        tg = np.zeros(ng)
+        for g, (lay, _) in enumerate(self.layup_groups):
+            A, B, D = lay.laminate_stiffnesses()
+            H = lay.laminate_transverse_stiffness()
+            abdh[g] = np.concatenate([A.T.ravel(), B.T.ravel(), D.T.ravel(), H.T.ravel()])
+            csys[g] = lay.csys.T.ravel()
+            tg[g] = lay.thickness()
+        self._keep = (abdh, csys, tg)  # alive for the duration of the call
+        p.ngroups = ng
+        p.group_ABDH = _lib.ptr(abdh)
+        p.group_csys = _lib.ptr(csys)
+        p.group_thickness = _lib.ptr(tg)
+        p.elem_group = _lib.ptr(self._group, _lib.C.c_int32)
+        return p
+
+    def _mass_params(self):
+        p = _lib.MassParams()
+        p.composite = 1
+        ng = len(self.layup_groups)
+        inert = np.array([lay.laminate_inertia() for lay, _ in self.layup_groups], dtype=float).reshape(ng, 2)
+        self._mkeep = (inert,)
+        p.ngroups = ng
+        p.group_inertia = _lib.ptr(inert)
+        p.elem_group = _lib.ptr(self._group, _lib.C.c_int32)
+        return p
+
+
+class FEMMShellT3FFComp(_CompositeMixin, FEMMShellT3FF):
+    """FEMMShellT3FFCompModule.FEMMShellT3FFComp (FEMMShellT3FFCompModule.jl:63-77)."""
+
+
+class FEMMShellQ4RSComp(_CompositeMixin, FEMMShellQ4RS):
+    """FEMMShellQ4RSCompModule.FEMMShellQ4RSComp (FEMMShellQ4RSCompModule.jl:88-98)."""
+
+
+def make(conn, material_or_layup, thickness=None, stab_c=None, mcsys=None, ctx=None):
+    """formul.make(IntegDomain(fes, rule, thickness), material[, stab_fun]) for the
+    element family of `conn` (3 -> T3FF, 4 -> Q4RS); a CompositeLayup selects
+    the layered FEMMs."""
+    from .material import CompositeLayup
+    nen = np.asarray(conn).shape[1]
+    if isinstance(material_or_layup, CompositeLayup):
+        cls = FEMMShellT3FFComp if nen == 3 else FEMMShellQ4RSComp
+        return cls(conn, material_or_layup, stab_c=stab_c, ctx=ctx)
+    cls = FEMMShellT3FF if nen == 3 else FEMMShellQ4RS
+    return cls(conn, material_or_layup, thickness=thickness, stab_c=stab_c, mcsys=mcsys, ctx=ctx)

@@ -1,0 +1,239 @@
+/*
+ * flexshell.h -- C ABI of the MI355X shell finite-element library.
+ *
+ * The library computes, on one gfx950 GPU, the hot path of the shell FEMMs of
+ * FinEtoolsFlexStructures.jl (v3.6.4): nodal normals, element stiffness of the
+ * T3FF / T3FFComp / Q4RS / Q4RSComp shells scattered into a CSR global matrix,
+ * the lumped mass, and the explicit central-difference loop built on them.
+ *
+ * Every entry point is a plain C function taking plain pointers and sizes; no
+ * C++ or GPU types cross the boundary.  Host arrays are read during the call
+ * and copied to the device; nothing is retained after return.  Every function
+ * returns FS_OK (0) or a negative status; fs_last_error() then holds a
+ * message.  Calls never unwind across the ABI.  Contexts are independent and
+ * may be used from different threads; one context must not be used from two
+ * threads at once.  All arrays use the memory order of the reference's Julia
+ * data: N x 3 / N x 6 node arrays are column-major (x[], y[], z[] planes),
+ * connectivity is E x nen row-major (Vector{NTuple{nen,Int}}).
+ *
+ * The reference interfaces each entry point replaces are cited per function
+ * (paths relative to the FinEtoolsFlexStructures.jl repository root).
+ */
+#ifndef FLEXSHELL_H
+#define FLEXSHELL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define FS_OK 0
+#define FS_ERR_ARG (-1)         /* invalid argument (sizes, indices, NULL) */
+#define FS_ERR_STATE (-2)       /* call sequence violated (e.g. no normals) */
+#define FS_ERR_DEVICE (-3)      /* HIP runtime error */
+#define FS_ERR_NOMEM (-4)       /* device or host allocation failed */
+#define FS_ERR_NUMERIC (-5)     /* singular metric, non-positive material */
+#define FS_ERR_UNSUPPORTED (-6) /* valid input outside what is implemented */
+
+/* ---- element families ---- */
+#define FS_T3 3 /* three-node flat facet: T3FF / T3FFComp */
+#define FS_Q4 4 /* four-node quadrilateral: Q4RS / Q4RSComp */
+
+/* ---- matrix blocks (which rows/columns are assembled) ---- */
+#define FS_BLOCK_ALL 0 /* all dofs, nalldofs x nalldofs (SysmatAssemblerSparse*) */
+#define FS_BLOCK_FF 1  /* free-free block, nfreedofs x nfreedofs (SysmatAssemblerFFBlock) */
+
+/* ---- transverse shear formulations of the T3 family ---- */
+#define FS_SHEAR_AVERAGE_B 0 /* __TRANSV_SHEAR_FORMULATION_AVERAGE_B (default) */
+#define FS_SHEAR_AVERAGE_K 1 /* __TRANSV_SHEAR_FORMULATION_AVERAGE_K */
+
+/* ---- explicit loop precision ---- */
+#define FS_PREC_FP64 0  /* all products in FP64 */
+#define FS_PREC_MIXED 1 /* membrane+bending internal force in FP32; state, mass,
+                           shear and drilling stabilisation terms in FP64 */
+
+typedef struct fs_ctx fs_ctx;
+typedef struct fs_model fs_model;
+
+/*
+ * Material and formulation parameters of one stiffness evaluation.
+ * Replaces the FEMM fields read at the start of stiffness():
+ * FEMMShellT3FFModule.jl:657-663 (material -> Dps, Dt; drilling scale;
+ * formulation; stab_fun), FEMMShellQ4RSModule.jl:901-907, and for layered
+ * shells the per-group laminate data FEMMShellT3FFCompModule.jl:600-605,
+ * FEMMShellQ4RSCompModule.jl:903-907.
+ *
+ * stab_fun(t, h) crosses the ABI as its coefficient c in t^2/(t^2 + c h^2):
+ * the defaults are 5/12/1.5 (T3, FEMMShellT3FFModule.jl:36) and 0.1 (Q4,
+ * FEMMShellQ4RSModule.jl:32).
+ */
+typedef struct fs_shell_params {
+    int32_t composite;        /* 0: homogeneous (Dps, Dt, thickness); 1: layered */
+    int32_t formulation;      /* FS_SHEAR_AVERAGE_B / _K (T3 only) */
+    double thickness;         /* homogeneous: constant otherdimension */
+    double stab_c;            /* Lyly stabilisation coefficient c */
+    double drilling_stiffness_scale;
+    double Dps[9];            /* homogeneous plane-stress 3x3, column-major */
+    double Dt[4];             /* homogeneous transverse shear 2x2 (the 5/6 factor of
+                                 FEMMShellT3FFModule.jl:658-659 is applied inside) */
+    int32_t ngroups;          /* layered: number of layup groups */
+    int32_t reserved;
+    const double *group_ABDH;      /* ngroups x 31: A(9) B(9) D(9) H(4), column-major */
+    const double *group_csys;      /* ngroups x 9: layup csys matrix (column-major) */
+    const double *group_thickness; /* ngroups: thickness(layup) */
+    const int32_t *elem_group;     /* nelem 0-based group ids, NULL = all group 0 */
+} fs_shell_params;
+
+/*
+ * Lumped-mass parameters.  Replaces mass() FEMMShellT3FFModule.jl:772-791,
+ * FEMMShellQ4RSModule.jl:981-1002 (rho, thickness) and the composite forms
+ * FEMMShellT3FFCompModule.jl:737, FEMMShellQ4RSCompModule.jl:1002
+ * (laminate_inertia! per group).
+ */
+typedef struct fs_mass_params {
+    int32_t composite;
+    int32_t ngroups;
+    double rho;                  /* homogeneous mass density */
+    double thickness;            /* homogeneous thickness */
+    const double *group_inertia; /* ngroups x 2: (mass density, moment of inertia density) */
+    const int32_t *elem_group;   /* nelem, NULL = all group 0 */
+} fs_mass_params;
+
+/*
+ * Central-difference loop parameters.  Replaces _cd_loop!
+ * (examples/shells/dynamics/homogeneous/explicit/plate_expl_examples.jl:61-94).
+ * The force closure force!(F, t) crosses the ABI as F(t_n) = g[n] * Fmag with
+ * the caller sampling g at t_n = t_{n-1} + dt, n = 0..nsteps.
+ */
+typedef struct fs_cd_params {
+    double dt;          /* time step, after the tend/dt adjustment of :72-75 */
+    double cdamp;       /* C = cdamp * diag(M)  (ksi * 2 * omegad); 0 = undamped */
+    int64_t nsteps;
+    int32_t precision;  /* FS_PREC_FP64 or FS_PREC_MIXED */
+    int32_t probe_stride; /* record probes every probe_stride steps (>= 1) */
+} fs_cd_params;
+
+/* ---------------------------------------------------------------------- */
+/* Context                                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device` (one stream per context). */
+int fs_ctx_create(int device, fs_ctx **out);
+void fs_ctx_destroy(fs_ctx *ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char *fs_last_error(const fs_ctx *ctx);
+/* Library version string. */
+const char *fs_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Model (mesh + dofs + device-resident matrices)                          */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * Upload a shell mesh.  Replaces the data gathered by gathervalues_asmat! /
+ * fes.conn in the element loops (FEMMShellT3FFModule.jl:671,
+ * FEMMShellQ4RSModule.jl:915).  xyz: nnodes x 3 column-major (geom0.values);
+ * conn: nelem x nen row-major, node numbers with the given index_base (1 for
+ * Julia).  nen = FS_T3 or FS_Q4.
+ */
+int fs_model_create(fs_ctx *ctx, int nen, int64_t nnodes, const double *xyz, int64_t nelem,
+                    const int64_t *conn, int index_base, fs_model **out);
+void fs_model_destroy(fs_model *m);
+
+/*
+ * Upload the equation numbers (NodalField dchi.dofnums after numberdofs!):
+ * nnodes x 6 column-major, numbers with index_base, free dofs numbered
+ * 1..nfreedofs and fixed dofs after them.  The dofs of one node must be
+ * consecutive within the free and within the fixed numbers (as numberdofs!
+ * produces, with or without a node permutation).
+ */
+int fs_model_set_dofs(fs_model *m, const int64_t *dofnums, int64_t nalldofs, int64_t nfreedofs,
+                      int index_base);
+
+/*
+ * Isoparametric nodal normals on the device: associategeometry! of
+ * FEMMShellT3FFModule.jl:570-616 (nen = 3) or FEMMShellQ4RSModule.jl:472-525
+ * (nen = 4).  normals_out (nnodes x 3 column-major) and valid_out (nnodes)
+ * may be NULL; the normals stay resident for the stiffness call.
+ */
+int fs_model_normals_isoparametric(fs_model *m, double threshold_angle_deg, double *normals_out,
+                                   uint8_t *valid_out);
+
+/*
+ * Upload normals computed on the host: the mcsys / layup.csys paths of
+ * associategeometry! (FEMMShellT3FFModule.jl:584 with a user CSys,
+ * FEMMShellT3FFCompModule.jl:509, FEMMShellQ4RSCompModule.jl:471).
+ */
+int fs_model_normals_upload(fs_model *m, const double *normals, const uint8_t *valid);
+
+/*
+ * Build the CSR sparsity pattern and the element-to-CSR scatter plan for a
+ * block.  The pattern is the structural union of all element dof pairs
+ * (zeros retained), as FinEtools' COO assembly followed by
+ * SparseMatricesCSR.sparsecsr produces it (AssemblyModule.jl:47-52).
+ */
+int fs_model_pattern(fs_model *m, int block, int64_t *nrows, int64_t *nnz);
+/* Copy the pattern out: rowptr (nrows+1), colval (nnz), with index_base. */
+int fs_model_pattern_get(fs_model *m, int block, int64_t *rowptr, int64_t *colval, int index_base);
+
+/*
+ * Stiffness: stiffness(femm, assembler, geom0, u1, Rfield1, dchi) of
+ * FEMMShellT3FFModule.jl:635-736, FEMMShellT3FFCompModule.jl:561-689,
+ * FEMMShellQ4RSModule.jl:877-947, FEMMShellQ4RSCompModule.jl:861-958 with the
+ * assembly of AssemblyModule.jl:20-53 (startassembly!/assemble!/makematrix!).
+ * Builds the block's pattern if needed, computes the CSR values on the
+ * device (kept resident for fs_model_explicit) and, if nzval_out is not
+ * NULL, copies them to the host (nnz doubles, in pattern order).
+ */
+int fs_model_stiffness(fs_model *m, const fs_shell_params *p, int block, double *nzval_out);
+
+/*
+ * Lumped mass: mass(femm, SysmatAssemblerSparseDiag or FFBlock, geom0, dchi)
+ * of FEMMShellT3FFModule.jl:757-811 and the other three FEMMs.  Writes the
+ * diagonal (nrows of the block) to diag_out if not NULL; kept resident.
+ */
+int fs_model_mass(fs_model *m, const fs_mass_params *p, int block, double *diag_out);
+
+/*
+ * Explicit central-difference loop on the resident FF-block stiffness and
+ * mass (_cd_loop!, plate_expl_examples.jl:61-94; ThreadedSparseCSR.bmul! at
+ * :87 is the CSR product).  U0, V0, Fmag: nfreedofs; g: nsteps+1 samples.
+ * probe_out receives U[probe_dofs] (0-based) at steps 0, s, 2s, ... (s =
+ * probe_stride): (nsteps/s + 1) x nprobes doubles.  U_out/V_out (may be NULL)
+ * receive the final state.
+ */
+int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, const double *V0,
+                      const double *Fmag, const double *g, const int64_t *probe_dofs, int32_t nprobes,
+                      double *probe_out, double *U_out, double *V_out);
+
+/*
+ * Largest generalized eigenvalue omega_max of (K_ff, M_ff) by power
+ * iteration on the device, the form of pwr (examples/shells/dynamics/
+ * homogeneous/explicit/clamp_cyl_transient_examples.jl:45-60) from the start
+ * vector v0 (nfreedofs; the reference draws rand()).
+ */
+int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *omega_max);
+
+/*
+ * Element matrices before assembly: the elmat that stiffness() hands to
+ * assemble!(assembler, elmat, dofnums, dofnums) (FEMMShellT3FFModule.jl:733,
+ * FEMMShellQ4RSModule.jl:944), for the generic-assembler path and for tests.
+ * blocks_out: nelem x nblk x 36 doubles, nblk = nen(nen+1)/2 upper node-pair
+ * blocks (a, b), a <= b, in the order (0,0),(0,1),..,(1,1),..; each block is
+ * 6 x 6 row-major (rows: dofs of local node a, columns: dofs of node b).
+ */
+int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *blocks_out);
+
+/* Device time (ms) of the last fs_model_stiffness / fs_model_explicit call,
+ * measured with HIP events on the context stream: which = 0 element kernel,
+ * 1 scatter kernel, 2 whole stiffness, 3 explicit loop. */
+int fs_model_last_timing(const fs_model *m, int which, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLEXSHELL_H */

@@ -1,0 +1,129 @@
+"""Reference test scenarios restated as plain arrays (meshes, BCs, loads),
+shared by the oracle pin tests and the GPU parity tests.  Values quoted from
+the reference's test files are cited next to them."""
+import numpy as np
+
+from flexshell import fe
+from flexshell.material import cartesian_csys
+from oracle import oracle as O
+
+mm = 1e-3
+MPa = 1e6
+GPa = 1e9
+
+# test/test_shell_dynamics.jl:113-128 (elastic modes 7..14)
+FV12_REF = [1.572130183778014, 2.2424585076387427, 2.8079394352847316, 3.883763676656034, 4.039123204140305,
+            6.787320617260535, 6.920636670319986, 7.127888889722697]
+# test/test_shell_statics.jl:98-105 and test/test_q4rs_shell_statics.jl:94-101
+SCORDELIS_T3 = dict(zip([4, 8, 10, 12, 16, 24], [66.54771615057949, 85.54615143134853, 89.85075281481419,
+                                                  92.50616661644985, 95.40469210310079, 97.65376880486126]))
+SCORDELIS_Q4 = dict(zip([4, 8, 10, 12, 16, 24], [97.88098976068304, 98.766679964194, 99.1305903792941,
+                                                  99.33597700449404, 99.52904556664862, 99.63694998474682]))
+# test/test_composite_shell_statics.jl:97, test/test_composite_shell_statics_q4rs.jl:97
+BARBERO_T3 = 0.22004349767718365
+BARBERO_Q4 = 0.2278585006007462
+# test/test_composite_shell_dynamics.jl:432-442
+NAYAK_REF = np.array([0.04571652264814249, 0.10096323319412286, 0.11467782865238098, 0.16264669289730896,
+                      0.18683613787902217, 0.20655080934826295, 0.23954509827380233, 0.2476225295798577,
+                      0.2718218525817157])
+
+
+def fv12(n=8):
+    """NAFEMS FV12 free plate (test/test_shell_dynamics.jl:26-73)."""
+    L = 10.0
+    xy, conn = fe.T3block(L, L, n, n)
+    xy[:, 0] -= L / 2
+    xy[:, 1] -= L / 2
+    xyz = fe.xyz3(xy)
+    dch = fe.DofField(xyz.shape[0]).numberdofs()
+    return dict(nen=3, xyz=xyz, conn=conn, dchi=dch, E=200e9, nu=0.3, rho=8000.0, t=0.05, stab_c=0.2)
+
+
+def scordelis(nen, n):
+    """Scordelis-Lo roof (test/test_shell_statics.jl:14-89, test/test_q4rs_shell_statics.jl:14-86)."""
+    R, L = 25.0, 50.0
+    tol = R / n / 1000
+    if nen == 3:
+        xy, conn = fe.T3block(40 / 360 * 2 * np.pi, L / 2, n, n)
+    else:
+        xy, conn = fe.Q4block(40 / 360 * 2 * np.pi, L / 2, n, n)
+    a, y = xy[:, 0].copy(), xy[:, 1].copy()
+    xyz = np.stack([R * np.sin(a), y, R * (np.cos(a) - 1)], axis=1)
+    dch = fe.DofField(xyz.shape[0])
+    dch.setebc(fe.selectnode(xyz, [-np.inf, np.inf, 0, 0, -np.inf, np.inf], tol), [1, 3, 5])
+    dch.setebc(fe.selectnode(xyz, [-np.inf, np.inf, L / 2, L / 2, -np.inf, np.inf], tol), [2, 4, 6])
+    dch.setebc(fe.selectnode(xyz, [0, 0, -np.inf, np.inf, -np.inf, np.inf], tol), [1, 5, 6])
+    dch.numberdofs()
+    s40 = np.sin(40 / 360 * 2 * np.pi) * 25
+    nl = fe.selectnode(xyz, [s40, s40, L / 2, L / 2, -np.inf, np.inf], tol)[0]
+    F = fe.surface_load(xyz, conn, [0, 0, -90, 0, 0, 0], dch.nalldofs, dch.dofnums)
+    return dict(nen=nen, xyz=xyz, conn=conn, dchi=dch, E=4.32e8, nu=0.0, rho=0.0, t=0.25, stab_c=0.2, F=F,
+                probe=nl)
+
+
+def barbero(nen):
+    """Barbero Example 3.1 cross-ply [0/90] plate (test/test_composite_shell_statics.jl:13-98)."""
+    ax = ay = 2000 * mm
+    nx = ny = 8
+    E1, E2, G12, nu12, G23 = 133860 * MPa, 7706 * MPa, 4306 * MPa, 0.301, 2760 * MPa
+    G13 = G12
+    th = 10 * mm
+    tol = ax / nx / 100
+    D6 = O.lamina_D6(E1, E2, nu12, G12, G13, G23)
+    if nen == 3:
+        xy, conn = fe.T3block(ax, ay, nx, ny)
+    else:
+        xy, conn = fe.Q4block(ax, ay, nx, ny)
+    xyz = fe.xyz3(xy)
+    dch = fe.DofField(xyz.shape[0])
+    dch.setebc(fe.selectnode(xyz, [ax, ax, 0, 0, -np.inf, np.inf], tol), [1, 2, 3])
+    dch.setebc(fe.selectnode(xyz, [0, 0, 0, 0, -np.inf, np.inf], tol), [2])
+    dch.setebc(fe.boundary_nodes(conn), [3])
+    dch.numberdofs()
+    bed = fe.boundary_edges(conn)
+
+    def edges_at(y):
+        sel = set(fe.selectnode(xyz, [-np.inf, np.inf, y, y, 0, 0], tol).tolist())
+        return np.array([e for e in bed if e[0] in sel and e[1] in sel])
+
+    F = fe.edge_load(xyz, edges_at(0.0), [0, 0.1 * MPa * th, 0, 0, 0, 0], dch.nalldofs, dch.dofnums)
+    F += fe.edge_load(xyz, edges_at(ay), [0, -0.1 * MPa * th, 0, 0, 0, 0], dch.nalldofs, dch.dofnums)
+    plies = [(D6, th / 2, 0.0, 0.0), (D6, th / 2, 90.0, 0.0)]
+    return dict(nen=nen, xyz=xyz, conn=conn, dchi=dch, plies=plies, csys=np.eye(3), F=F,
+                lamina=(E1, E2, nu12, G12, G13, G23), th=th)
+
+
+def nayak(nplies, axes):
+    """Nayak 4.3 orthotropic plate (test/test_composite_shell_dynamics.jl:341-443)."""
+    ax = ay = 100 * mm
+    nx = ny = 9
+    E1, E2, G12, G13, nu12, G23 = 143.52 * GPa, 75.38 * GPa, 42.03 * GPa, 25.56 * GPa, 0.44, 42.65 * GPa
+    rho, C11 = 1500.0, 159.85 * GPa
+    th = ax / 10
+    tol = ax / nx / 100
+    D6 = O.lamina_D6(E1, E2, nu12, G12, G13, G23)
+    xy, conn = fe.T3block(ax, ay, nx, ny)
+    xyz = fe.xyz3(xy)
+    dch = fe.DofField(xyz.shape[0])
+    dch.setebc(fe.selectnode(xyz, [ax, ax, 0, 0, -np.inf, np.inf], tol), [1, 2, 3])
+    dch.setebc(fe.selectnode(xyz, [0, 0, 0, 0, -np.inf, np.inf], tol), [2])
+    dch.setebc(fe.boundary_nodes(conn), [1, 2, 3])
+    dch.numberdofs()
+    plies = [(D6, th / nplies, 0.0, rho)] * nplies
+    return dict(nen=3, xyz=xyz, conn=conn, dchi=dch, plies=plies, csys=cartesian_csys(axes), th=th, rho=rho,
+                C11=C11, lamina=(E1, E2, nu12, G12, G13, G23))
+
+
+def distorted_mesh(nen, nx, ny, seed, warp=0.0, jitter=0.2):
+    """Small structured mesh with jittered nodes and an optional out-of-plane
+    warp (curved surface), for element-level comparisons."""
+    rng = np.random.default_rng(seed)
+    if nen == 3:
+        xy, conn = fe.T3block(1.0, 0.7, nx, ny)
+    else:
+        xy, conn = fe.Q4block(1.0, 0.7, nx, ny)
+    h = min(1.0 / nx, 0.7 / ny)
+    xy = xy + rng.uniform(-jitter * h, jitter * h, xy.shape)
+    z = warp * (xy[:, 0] ** 2 - 0.5 * xy[:, 1] ** 2 + 0.3 * xy[:, 0] * xy[:, 1])
+    xyz = np.stack([xy[:, 0], xy[:, 1], z], axis=1)
+    return xyz, conn

@@ -1,0 +1,249 @@
+"""Pin the CPU oracle against the reference's own test values, known answers
+and closed forms (CPU only).  A mistake in the oracle -- a dropped term, a
+wrong sign or index, a transposed operand -- moves at least one of these."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+import shell_cases as SC
+from flexshell import fe
+from oracle import oracle as O
+
+
+def _homog_K(case, nrows=None):
+    Dps, Dt = O.shell_D(O.iso_D6(case["E"], case["nu"]))
+    nrm, val = O.normals(case["nen"], case["xyz"], case["conn"])
+    Ke = O.element_matrices(case["nen"], case["xyz"], case["conn"], nrm, val, Dps=Dps, Dt=Dt, t=case["t"],
+                            stab_c=case["stab_c"])
+    d = case["dchi"]
+    return O.assemble_csr(case["nen"], case["conn"], d.dofnums, nrows or d.nalldofs, Ke)
+
+
+def _layered_K(case, nrows=None):
+    A, B, D, H, inert = O.laminate(case["plies"])
+    nrm, val = O.csys_normals(case["nen"], case["xyz"], case["conn"], case["csys"])
+    Ke = O.element_matrices(case["nen"], case["xyz"], case["conn"], nrm, val,
+                            groups=[(A, B, D, H, case["csys"], case["th"])])
+    d = case["dchi"]
+    return O.assemble_csr(case["nen"], case["conn"], d.dofnums, nrows or d.nalldofs, Ke), inert
+
+
+def test_fv12_frequencies():
+    """T3FF stiffness + lumped mass: the 8 elastic FV12 frequencies of
+    test/test_shell_dynamics.jl:113-132 (reference tolerance rtol 1e-6)."""
+    c = SC.fv12()
+    K = _homog_K(c)
+    tm, rm = O.element_masses(3, c["xyz"], c["conn"], rho=c["rho"], t=c["t"])
+    d = c["dchi"]
+    M = O.lumped_diag(3, c["conn"], d.dofnums, d.nalldofs, tm, rm)
+    sh = (0.5 * 2 * np.pi) ** 2
+    w = sla.eigsh(K + sh * sp.diags(M), k=14, M=sp.diags(M), sigma=0, which="LM", return_eigenvectors=False)
+    fs = np.sqrt(np.abs(np.sort(w) - sh)) / (2 * np.pi)
+    assert np.all(fs[:6] < 1e-4)  # six rigid-body modes
+    np.testing.assert_allclose(fs[6:], SC.FV12_REF, rtol=1e-9)
+
+
+@pytest.mark.parametrize("nen,n", [(3, 4), (3, 8), (3, 12), (4, 4), (4, 8), (4, 12)])
+def test_scordelis_lo(nen, n):
+    """T3FF / Q4RS on a curved roof with isoparametric nodal normals
+    (test/test_shell_statics.jl:97-108, test/test_q4rs_shell_statics.jl:92-110)."""
+    c = SC.scordelis(nen, n)
+    K = _homog_K(c)
+    d = c["dchi"]
+    nf = d.nfreedofs
+    U = sla.spsolve(K[:nf, :nf].tocsc(), c["F"][:nf])
+    w = d.scattersysvec(U)[c["probe"], 2]
+    ref = (SC.SCORDELIS_T3 if nen == 3 else SC.SCORDELIS_Q4)[n]
+    assert abs(w / -0.3024 * 100 - ref) / ref < 1e-8
+
+
+@pytest.mark.parametrize("nen", [3, 4])
+def test_barbero_layered_statics(nen):
+    """T3FFComp / Q4RSComp with membrane-bending coupling (B != 0) under edge
+    loads (test/test_composite_shell_statics.jl:97, ..._q4rs.jl:97, `≈`)."""
+    c = SC.barbero(nen)
+    K, _ = _layered_K(c)
+    d = c["dchi"]
+    nf = d.nfreedofs
+    U = sla.spsolve(K[:nf, :nf].tocsc(), c["F"][:nf])
+    wmax = d.scattersysvec(U)[:, 2].max() / SC.mm
+    ref = SC.BARBERO_T3 if nen == 3 else SC.BARBERO_Q4
+    assert abs(wmax - ref) / ref < 1e-10
+
+
+@pytest.mark.parametrize("nplies,axes", [(10, (1, 2, 3)), (3, (2, -1, 3)), (5, (-1, -2, 3)), (4, (-2, 1, 3))])
+def test_nayak_layered_frequencies(nplies, axes):
+    """T3FFComp + laminate inertia with rotated layup csys: the reference's own
+    abs 1e-13 check (test/test_composite_shell_dynamics.jl:432-443)."""
+    c = SC.nayak(nplies, axes)
+    K, (md, mi) = _layered_K(c)
+    tm, rm = O.element_masses(3, c["xyz"], c["conn"], m0=md, m1=mi)
+    d = c["dchi"]
+    M = O.lumped_diag(3, c["conn"], d.dofnums, d.nalldofs, tm, rm)
+    nf = d.nfreedofs
+    w = sl.eigh(K[:nf, :nf].toarray(), np.diag(M[:nf]), eigvals_only=True, subset_by_index=[0, 8])
+    nd = c["th"] * np.sqrt(c["rho"] / c["C11"]) * np.sqrt(w)
+    assert np.linalg.norm(SC.NAYAK_REF - nd) < 1.0e-13
+
+
+def test_layered_equals_homogeneous_for_isotropic_ply():
+    """T3FFComp with one isotropic ply equals T3FF (A = t Dps, D = t^3/12 Dps,
+    H = 5/6 t G for the Vinson-Sierakowski model)."""
+    xyz, conn = SC.distorted_mesh(3, 4, 3, seed=1, warp=0.3)
+    E, nu, t = 70e9, 0.3, 0.01
+    D6 = O.iso_D6(E, nu)
+    Dps, Dt = O.shell_D(D6)
+    nrm, val = O.normals(3, xyz, conn)
+    K1 = O.element_matrices(3, xyz, conn, nrm, val, Dps=Dps, Dt=Dt, t=t)
+    A, B, D, H, _ = O.laminate([(D6, t, 0.0, 0.0)])
+    K2 = O.element_matrices(3, xyz, conn, nrm, val, groups=[(A, B, D, H, np.eye(3), t)])
+    assert np.linalg.norm(K1 - K2) < 1e-13 * np.linalg.norm(K1)
+
+
+def test_layup_known_answers():
+    """test/test_composite_layup.jl:321-328 (isotropic ply, any angle),
+    :431-451 (Barbero Ex 5.7), :594 (cross-ply B = 0)."""
+    E, nu, t = 200e9, 0.3, 0.005
+    Atrue = E / (1 - nu ** 2) * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]]) * t
+    for ang in (0.0, 47.0, 90.0, 129.0, 180.0):
+        A, B, D, H, _ = O.laminate([(O.iso_D6(E, nu), t, ang, 0.0)])
+        assert np.linalg.norm(A - Atrue) < 1e-15 * np.linalg.norm(Atrue)
+        np.testing.assert_allclose(H, 5 / 6 * t * E / 2 / (1 + nu) * np.eye(2), rtol=1e-14, atol=1e-6)
+    MPa = 1e6
+    E1, E2, G12, nu12 = 67192 * MPa, 12139 * MPa, 7638 * MPa, 0.365
+    D6 = O.ortho_D6(E1, E2, E2, nu12, nu12, nu12, G12, G12, G12)
+    Dps, _ = O.ply_matrices(D6)
+    ref = np.array([[68849.10243290635, 4540.006665496835, 0.0], [4540.006665496834, 12438.374426018723, 0.0],
+                    [0.0, 0.0, 7637.999999999999]])
+    assert np.linalg.norm(Dps / MPa - ref) < 1e-15 * E2
+    A, _, _, _, _ = O.laminate([(D6, 1.0, 45.0, 0.0)])
+    refA = np.array([[30229.872547479692, 14953.872547479687, 14102.68200172191],
+                     [14953.872547479687, 30229.872547479685, 14102.682001721907],
+                     [14102.682001721909, 14102.682001721909, 18051.865881982852]])
+    assert np.linalg.norm(A / MPa - refA) < 1e-15 * E2
+    D6l = O.lamina_D6(E1, E2, nu12, G12, G12, G12)
+    plies = [(D6l, 0.001, a, 0.0) for a in (0, 90, 0, 90, 0)]
+    _, B, _, _, _ = O.laminate(plies)
+    assert np.linalg.norm(B) < 1e-15 * E2
+
+
+def test_assembly_matches_dense_sum():
+    """COO -> CSR with duplicate summation against a brute-force dense sum,
+    with fixed dofs interleaved (FF and ALL blocks)."""
+    rng = np.random.default_rng(7)
+    xy, conn = fe.T3block(1.0, 1.0, 3, 2)
+    N = xy.shape[0]
+    d = fe.DofField(N)
+    d.setebc([0, 5], [1, 3, 6])
+    d.numberdofs(perm=rng.permutation(N))
+    Ke = rng.standard_normal((conn.shape[0], 18, 18))
+    dense = np.zeros((d.nalldofs, d.nalldofs))
+    for e in range(conn.shape[0]):
+        dofs = d.dofnums[conn[e]].ravel()
+        dense[np.ix_(dofs, dofs)] += Ke[e]
+    for nrows in (d.nalldofs, d.nfreedofs):
+        K = O.assemble_csr(3, conn, d.dofnums, nrows, Ke)
+        np.testing.assert_allclose(K.toarray(), dense[:nrows, :nrows], rtol=0, atol=1e-12)
+        # structural union of element dof pairs, zeros retained
+        pattern = np.zeros((nrows, nrows), dtype=bool)
+        for e in range(conn.shape[0]):
+            dofs = d.dofnums[conn[e]].ravel()
+            dofs = dofs[dofs < nrows]
+            pattern[np.ix_(dofs, dofs)] = True
+        assert K.nnz == pattern.sum()
+
+
+@pytest.mark.parametrize("nen", [3, 4])
+def test_element_rigid_body_null_space(nen):
+    """A free flat element has the rigid translations and the rigid rotations
+    about in-plane axes in its null space.  The in-plane rotation (about z)
+    is resisted by the artificial drilling stiffness only (kavg on the drilling
+    dof, FEMMShellT3FFModule.jl:714-727; FEMMShellQ4RSModule.jl:840-859), so its
+    energy is bounded by that penalty, which is of bending scale."""
+    xyz, conn = SC.distorted_mesh(nen, 1, 1, seed=3, warp=0.0)
+    conn = conn[:1]
+    used = np.unique(conn)
+    nrm, val = O.normals(nen, xyz, conn)
+    Dps, Dt = O.shell_D(O.iso_D6(70e9, 0.33))
+    Ke = O.element_matrices(nen, xyz, conn, nrm, val, Dps=Dps, Dt=Dt, t=0.01)[0]
+    X = xyz[conn[0]]
+    modes = []
+    for k in range(3):
+        u = np.zeros((nen, 6))
+        u[:, k] = 1.0
+        modes.append(u.ravel())
+        th = np.zeros(3)
+        th[k] = 1.0
+        r = np.zeros((nen, 6))
+        r[:, :3] = np.cross(th, X)
+        r[:, 3:] = th
+        modes.append(r.ravel())
+    for i, m in enumerate(modes):
+        r = np.linalg.norm(Ke @ m) / (np.linalg.norm(Ke) * np.linalg.norm(m))
+        if i == 5:  # rotation about z
+            assert r < 1e-4
+        else:
+            assert r < 1e-9
+    assert used.size == nen
+
+
+def test_normals_flat_and_crease():
+    """Flat plate: normals +z, all valid.  Folded plate (90 degree crease):
+    nodes on the fold are marked invalid (threshold 30 degrees)."""
+    xy, conn = fe.T3block(1.0, 1.0, 4, 4)
+    xyz = fe.xyz3(xy)
+    n, v = O.normals(3, xyz, conn)
+    np.testing.assert_allclose(n, np.tile([0, 0, 1.0], (xyz.shape[0], 1)), atol=1e-15)
+    assert v.all()
+    fold = xyz.copy()
+    m = fold[:, 0] > 0.5
+    fold[m, 2] = fold[m, 0] - 0.5
+    fold[m, 0] = 0.5
+    for nen in (3, 4):
+        c = conn if nen == 3 else fe.Q4block(1.0, 1.0, 4, 4)[1]
+        n, v = O.normals(nen, fold, c)
+        on_fold = np.isclose(fold[:, 0], 0.5) & np.isclose(fold[:, 2], 0.0)
+        assert (~v[on_fold]).all() and v[~on_fold].all()
+
+
+@pytest.mark.parametrize("nen", [3, 4])
+def test_lumped_mass_total(nen):
+    """Translational lumped mass sums to rho t area; rotary to rho t^3/12 area."""
+    xyz, conn = SC.distorted_mesh(nen, 5, 4, seed=2, warp=0.0)
+    rho, t = 2700.0, 0.02
+    tm, rm = O.element_masses(nen, xyz, conn, rho=rho, t=t)
+    # area of the jittered (flat) mesh, independently: triangles, or quads split in two
+    tri = conn if nen == 3 else np.concatenate([conn[:, [0, 1, 2]], conn[:, [0, 2, 3]]])
+    X = xyz[tri]
+    area = 0.5 * np.linalg.norm(np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]), axis=1).sum()
+    assert abs(tm.sum() * nen - rho * t * area) < 1e-12 * rho * t * area
+    assert abs(rm.sum() * nen - rho * t ** 3 / 12 * area) < 1e-12 * rho * t ** 3 / 12 * area
+
+
+def test_cd_loop_single_dof_closed_form():
+    """Undamped CD recurrence of one dof from an initial displacement.  The
+    reference starts from A0 = invMC * F(0) (plate_expl_examples.jl:80-81),
+    i.e. without the elastic force, so u1 = u0 and thereafter
+    u_{n+1} = 2 u_n - u_{n-1} - (omega dt)^2 u_n, whose solution is
+    u_n = u0 cos(n theta - theta/2) / cos(theta/2), cos(theta) = 1 - (omega dt)^2/2."""
+    k, m, u0 = 4.0e6, 2.0, 1e-3
+    omega = np.sqrt(k / m)
+    dt = 0.5 * 2 / omega
+    nsteps = 1000
+    U, V, pr = O.cd_loop(sp.csr_matrix([[k]]), np.array([m]), 0.0, dt, nsteps, np.zeros(1), np.zeros(nsteps + 1),
+                         np.array([u0]), np.zeros(1), probes=[0])
+    th = np.arccos(1 - (omega * dt) ** 2 / 2)
+    n = np.arange(nsteps + 1)
+    np.testing.assert_allclose(pr[:, 0], u0 * np.cos(n * th - th / 2) / np.cos(th / 2), rtol=0, atol=1e-12)
+
+
+def test_pwr_largest_eigenvalue():
+    n = 200
+    kd = np.ones(n)
+    kd[17] = 10.0
+    md = np.full(n, 2.0)
+    v0 = np.random.default_rng(0).random(n)
+    om = O.pwr(sp.diags(kd).tocsr(), md, v0, maxit=60)
+    assert abs(om - np.sqrt(10.0 / 2.0)) < 1e-12
