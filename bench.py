@@ -1,0 +1,185 @@
+"""Benchmark of the shell hot path on one MI355X.
+
+Workload (single large Q4RS model, FP64, plain form): cylindrical panel R 1 m,
+L 2 m, 60 degree arc, t 0.005 m, E 70e9, nu 0.33, rho 2700, Q4 512 x 512
+(262,144 elements, 263,169 nodes), interior nodes jittered +-10% h (seed 2),
+straight edges simply supported.  Timed region: `--steps` FP64 assemblies of
+the free-free stiffness into CSR (pattern and scatter plan prebuilt, inputs
+resident in HBM); then the central-difference loop (`--cd-steps`, default
+1000) at 0.9 x the stable dt from seeded Gaussian initial velocities.
+
+Prints one JSON line.  value = FP64 elements assembled per second;
+dof_steps_per_s reports the explicit loop.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "finetoolsflexstructures.jl_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def assembly_bytes(E, N, nen, nnz):
+    """Algorithmic bytes of one FP64 assembly into CSR values (pattern and
+    slot map prebuilt): conn + slot map + coords + normals + valid + values."""
+    k = 6 * nen
+    return E * (4 * nen + 4 * k * k) + N * (24 + 24 + 1) + nnz * 8
+
+
+def cd_step_bytes(n, nnz):
+    """Algorithmic bytes of one fused FP64 CD step (int32 column indices):
+    K values + indices, row pointers, u gather, U V A invM Fmag reads, U V A writes."""
+    return nnz * 12 + (n + 1) * 4 + n * 72
+
+
+def cpu_baseline(case, nsample, K_host, Mdiag, dt, cd_steps_cpu):
+    """Time the oracle (C restatement, OpenMP) on a bounded sample of the same
+    workload: element matrices + COO assembly for nsample elements, and
+    cd_steps_cpu central-difference steps on the full K."""
+    import numpy as np
+
+    from oracle import oracle as O
+    xyz, conn, d = case["xyz"], case["conn"], case["dchi"]
+    mat = case["material"]
+    Dps, Dt = O.shell_D(O.iso_D6(mat.E, mat.nu))
+    nrm, val = O.normals(4, xyz, conn)
+    sub = conn[:nsample]
+    t0 = time.perf_counter()
+    Ke = O.element_matrices(4, xyz, sub, nrm, val, Dps=Dps, Dt=Dt, t=case["t"])
+    O.assemble_csr(4, sub, d.dofnums, d.nfreedofs, Ke)
+    t_asm = time.perf_counter() - t0
+    n = d.nfreedofs
+    V0 = np.zeros(n)
+    t0 = time.perf_counter()
+    O.cd_loop(K_host, Mdiag, 0.0, dt, cd_steps_cpu, np.zeros(n), np.zeros(cd_steps_cpu + 1), np.zeros(n), V0)
+    t_cd = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nsample / t_asm, "unit": "elements/s", "cores": cores, "kind": "port",
+            "sample": f"{nsample} elements of the workload (element matrices + COO->CSR assembly), "
+                      f"oracle C restatement with OpenMP",
+            "dof_steps_per_s": n * cd_steps_cpu / t_cd,
+            "cd_sample": f"{cd_steps_cpu} CD steps on the full free-free K ({n} dofs), OpenMP CSR SpMV"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cd-steps", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=20000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(shard design only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from flexshell import _lib, explicit, femm as FM, workloads as W
+
+    t_setup = time.perf_counter()
+    case = W.cylinder_panel_c2(n=args.n)
+    d = case["dchi"]
+    ctx = _lib.default_context()
+    f = FM.FEMMShellQ4RS(case["conn"], case["material"], thickness=case["t"], ctx=ctx)
+    f.associategeometry(case["xyz"])
+    nf = d.nfreedofs
+    f._ensure_dofs(d)
+    nrows, rp, cv = f._pattern(_lib.FS_BLOCK_FF)
+    nnz = len(cv)
+    params = f._shell_params()
+    model = f._model
+    E, N = case["conn"].shape[0], case["xyz"].shape[0]
+    print(f"[bench] {case['name']}: E={E} N={N} nfree={nf} nnz={nnz} setup {time.perf_counter() - t_setup:.1f}s",
+          file=sys.stderr, flush=True)
+
+    def assemble():
+        ctx.check(ctx.lib.fs_model_stiffness(model.h, _lib.C.byref(params), _lib.FS_BLOCK_FF, None),
+                  "fs_model_stiffness")
+
+    for _ in range(args.warmup):
+        assemble()
+    elem_ms, scat_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        assemble()
+        elem_ms.append(model.timing(0))
+        scat_ms.append(model.timing(1))
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    dev_ms = float(np.mean(elem_ms) + np.mean(scat_ms))
+    print(f"[bench] assembly: wall {ms_per_step:.3f} ms/assembly, device elem {np.mean(elem_ms):.3f} ms "
+          f"+ scatter {np.mean(scat_ms):.3f} ms", file=sys.stderr, flush=True)
+
+    # explicit loop on the resident K and M
+    Kd = f.stiffness(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), case["xyz"], None, None, d)
+    Mdiag = f.mass(FM.SysmatAssemblerFFBlock(FM.SysmatAssemblerSparseDiag(), nf, nf), case["xyz"], d).diagonal()
+    rng = np.random.default_rng(3)
+    om = explicit.omega_max(f, rng.random(nf), maxit=30)
+    dt = explicit.stable_dt(om)
+    V0 = W.gaussian_velocity(d, 1e-3, seed=3)
+    ncd = args.cd_steps
+    explicit.run(model, 0.0, np.zeros(nf), V0, dt, min(ncd, 20), np.zeros(nf), np.zeros(min(ncd, 20) + 1))
+    t0 = time.perf_counter()
+    r = explicit.run(model, 0.0, np.zeros(nf), V0, dt, ncd, np.zeros(nf), np.zeros(ncd + 1), [0], 100)
+    cd_wall = time.perf_counter() - t0
+    cd_ms = r["ms"]
+    dof_steps = nf * ncd / (cd_ms / 1e3)
+    print(f"[bench] CD: {ncd} steps, device {cd_ms:.2f} ms ({cd_ms / ncd * 1e3:.1f} us/step), wall {cd_wall:.3f}s, "
+          f"omega_max {om:.4e}, dt {dt:.3e}", file=sys.stderr, flush=True)
+    assert np.all(np.isfinite(r["U"])), "non-finite state in the explicit loop"
+
+    b_asm = assembly_bytes(E, N, 4, nnz)
+    b_cd = cd_step_bytes(nf, nnz)
+    ach = b_asm / (dev_ms / 1e3) / 1e9
+    ach_cd = b_cd / (cd_ms / ncd / 1e3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("assembly_bytes_per_launch")
+
+    out = {
+        "metric": "FP64 shell elements assembled/s + DOF-steps/s; p50/p99 latency vs plain GPU",
+        "value": E / (dev_ms / 1e3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded mesh jitter and initial velocities)",
+        "config": {"workload": case["name"] + " Q4RS cylindrical panel R1 L2 60deg t0.005, FF-block K into CSR "
+                                              "+ central-difference loop",
+                   "elements": E, "nodes": N, "free_dofs": nf, "nnz": nnz, "cd_steps": ncd},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "assembly (elem_kernel + scatter_kernel)",
+                     "algorithmic_bytes": b_asm, "device_ms": dev_ms},
+        "dof_steps_per_s": dof_steps,
+        "cd_us_per_step": cd_ms / ncd * 1e3,
+        "roofline_cd": {"bound": "hbm", "achieved": ach_cd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach_cd / HBM_PEAK_GBS, "kernel": "cd_step_kernel",
+                        "algorithmic_bytes": b_cd},
+        "kernels_ms": {"elem_kernel": float(np.mean(elem_ms)), "scatter_kernel": float(np.mean(scat_ms))},
+    }
+    if not args.no_cpu:
+        import scipy.sparse as sp
+        Kh = sp.csr_matrix((np.zeros(nnz), cv, rp), shape=(nf, nf))
+        Kh.data = f._model.stiffness(params, _lib.FS_BLOCK_FF, nnz)
+        out["cpu_baseline"] = cpu_baseline(case, min(args.cpu_sample, E), Kh, Mdiag, dt, 20)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -589,7 +589,7 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
             if (valid[node]) {
                 const double *gv = ms + T::M_G + 3 * ba;
                 for (int r = 0; r < 3; ++r)
-                    for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * gv[r] * gv[c];
+                    for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (gv[r] * gv[c]);
             }
         }
         if (NEN == 4 && ba == bb) {

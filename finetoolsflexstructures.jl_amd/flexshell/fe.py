@@ -30,20 +30,18 @@ def T3blockx(xs, ys, orientation="a"):
     nL, nW = len(xs) - 1, len(ys) - 1
     X, Y = np.meshgrid(xs, ys)  # row j, column i -> node j*(nL+1)+i
     xy = np.stack([X.ravel(), Y.ravel()], axis=1)
+    # element order: i (along x) outer, j inner; two triangles per rectangle
+    I, J = np.meshgrid(np.arange(nL), np.arange(nW), indexing="ij")
+    f = (J * (nL + 1) + I).ravel().astype(np.int64)
     conn = np.zeros((2 * nL * nW, 3), dtype=np.int64)
-    gc = 0
-    for i in range(nL):
-        for j in range(nW):
-            f = j * (nL + 1) + i
-            if orientation == "a":
-                conn[gc] = (f, f + 1, f + (nL + 1))
-                conn[gc + 1] = (f + 1, f + (nL + 1) + 1, f + (nL + 1))
-            elif orientation == "b":
-                conn[gc] = (f, f + 1, f + (nL + 1) + 1)
-                conn[gc + 1] = (f, f + (nL + 1) + 1, f + (nL + 1))
-            else:
-                raise ValueError(f"Cannot handle orientation : {orientation}")
-            gc += 2
+    if orientation == "a":
+        conn[0::2] = np.stack([f, f + 1, f + (nL + 1)], axis=1)
+        conn[1::2] = np.stack([f + 1, f + (nL + 1) + 1, f + (nL + 1)], axis=1)
+    elif orientation == "b":
+        conn[0::2] = np.stack([f, f + 1, f + (nL + 1) + 1], axis=1)
+        conn[1::2] = np.stack([f, f + (nL + 1) + 1, f + (nL + 1)], axis=1)
+    else:
+        raise ValueError(f"Cannot handle orientation : {orientation}")
     return xy, conn
 
 
@@ -58,12 +56,9 @@ def Q4blockx(xs, ys):
     nL, nW = len(xs) - 1, len(ys) - 1
     X, Y = np.meshgrid(xs, ys)
     xy = np.stack([X.ravel(), Y.ravel()], axis=1)
-    conn = np.zeros((nL * nW, 4), dtype=np.int64)
-    f = 0
-    for i in range(nL):
-        for j in range(nW):
-            conn[f] = (j * (nL + 1) + i, j * (nL + 1) + i + 1, (j + 1) * (nL + 1) + i + 1, (j + 1) * (nL + 1) + i)
-            f += 1
+    I, J = np.meshgrid(np.arange(nL), np.arange(nW), indexing="ij")
+    f = (J * (nL + 1) + I).ravel().astype(np.int64)
+    conn = np.stack([f, f + 1, f + (nL + 1) + 1, f + (nL + 1)], axis=1)
     return xy, conn
 
 
@@ -132,19 +127,14 @@ class DofField:
 
     def numberdofs(self, perm=None):
         order = np.arange(self.N) if perm is None else np.asarray(perm)
-        dn = np.full((self.N, self.ndofn), -1, dtype=np.int64)
-        k = 0
-        for i in order:
-            for j in range(self.ndofn):
-                if not self.is_fixed[i, j]:
-                    dn[i, j] = k
-                    k += 1
-        self.nfreedofs = k
-        for i in order:
-            for j in range(self.ndofn):
-                if self.is_fixed[i, j]:
-                    dn[i, j] = k
-                    k += 1
+        fixed = self.is_fixed[order]  # nodes in numbering order, node-major
+        dn_ord = np.empty(fixed.shape, dtype=np.int64)
+        free = ~fixed
+        self.nfreedofs = int(free.sum())
+        dn_ord[free] = np.arange(self.nfreedofs)
+        dn_ord[fixed] = self.nfreedofs + np.arange(int(fixed.sum()))
+        dn = np.empty_like(dn_ord)
+        dn[order] = dn_ord
         self.dofnums = dn
         return self
 

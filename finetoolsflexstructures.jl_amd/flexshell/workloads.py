@@ -1,0 +1,96 @@
+"""Seeded synthetic models of the benchmark configurations.
+
+plate_c1          flat square plate 1 x 1 m, t 0.01, steel, T3 32 x 32 (2048
+                  triangles), all edges clamped.
+cylinder_panel_c2 cylindrical panel R 1 m, L 2 m, 60 degree arc, t 0.005,
+                  E 70e9, nu 0.33, rho 2700, Q4 n x n (512 x 512 = 262,144
+                  elements), interior nodes jittered by U(+-10% h) in the
+                  surface parameters (seed 2), straight edges simply supported
+                  (u = v = w = 0, rotations free).
+plate_c4          flat plate 2 x 1 m, t 0.004, steel, T3 1024 x 512
+                  (1,048,576 triangles), all edges clamped, Gaussian pressure
+                  pulse (sigma 0.1 m, 1e4 Pa peak, centre seeded, 1e-4 s).
+"""
+import numpy as np
+
+from . import fe
+from .material import MatDeforElastIso
+
+STEEL = dict(E=210e9, nu=0.3, rho=7850.0)
+
+
+def _edges_fixed(dchi, nodes, comps):
+    dchi.setebc(nodes, comps)
+    return dchi
+
+
+def plate_c1(n=32, L=1.0, t=0.01):
+    xy, conn = fe.T3block(L, L, n, n)
+    xyz = fe.xyz3(xy)
+    d = fe.DofField(xyz.shape[0])
+    tol = L / n / 100
+    bn = np.unique(np.concatenate([fe.selectnode(xyz, [0, 0, -np.inf, np.inf, -np.inf, np.inf], tol),
+                                   fe.selectnode(xyz, [L, L, -np.inf, np.inf, -np.inf, np.inf], tol),
+                                   fe.selectnode(xyz, [-np.inf, np.inf, 0, 0, -np.inf, np.inf], tol),
+                                   fe.selectnode(xyz, [-np.inf, np.inf, L, L, -np.inf, np.inf], tol)]))
+    d.setebc(bn, [1, 2, 3, 4, 5, 6]).numberdofs()
+    mat = MatDeforElastIso(STEEL["E"], STEEL["nu"], STEEL["rho"])
+    return dict(nen=3, xyz=xyz, conn=conn, dchi=d, material=mat, t=t, name="plate_t3_32x32")
+
+
+def cylinder_panel_c2(n=512, seed=2, R=1.0, L=2.0, arc_deg=60.0, t=0.005, E=70e9, nu=0.33, rho=2700.0,
+                      jitter=0.1):
+    arc = arc_deg / 180 * np.pi
+    ab, conn = fe.Q4block(arc, L, n, n)
+    rng = np.random.default_rng(seed)
+    ha, hy = arc / n, L / n
+    a, y = ab[:, 0].copy(), ab[:, 1].copy()
+    tol = 1e-9
+    interior = (a > tol) & (a < arc - tol) & (y > tol) & (y < L - tol)
+    a[interior] += rng.uniform(-jitter * ha, jitter * ha, interior.sum())
+    y[interior] += rng.uniform(-jitter * hy, jitter * hy, interior.sum())
+    xyz = np.stack([R * np.sin(a), y, R * np.cos(a)], axis=1)
+    d = fe.DofField(xyz.shape[0])
+    straight = np.nonzero((ab[:, 0] < tol) | (ab[:, 0] > arc - tol))[0]
+    d.setebc(straight, [1, 2, 3]).numberdofs()
+    mat = MatDeforElastIso(E, nu, rho)
+    return dict(nen=4, xyz=xyz, conn=conn, dchi=d, material=mat, t=t, name=f"cylinder_panel_q4_{n}x{n}")
+
+
+def plate_c4(nx=1024, ny=512, Lx=2.0, Ly=1.0, t=0.004, seed=5):
+    xy, conn = fe.T3block(Lx, Ly, nx, ny)
+    xyz = fe.xyz3(xy)
+    d = fe.DofField(xyz.shape[0])
+    tol = min(Lx / nx, Ly / ny) / 100
+    bn = np.nonzero((xyz[:, 0] < tol) | (xyz[:, 0] > Lx - tol) | (xyz[:, 1] < tol) | (xyz[:, 1] > Ly - tol))[0]
+    d.setebc(bn, [1, 2, 3, 4, 5, 6]).numberdofs()
+    rng = np.random.default_rng(seed)
+    x0 = np.array([rng.uniform(0.25 * Lx, 0.75 * Lx), rng.uniform(0.25 * Ly, 0.75 * Ly), 0.0])
+    mat = MatDeforElastIso(STEEL["E"], STEEL["nu"], STEEL["rho"])
+    return dict(nen=3, xyz=xyz, conn=conn, dchi=d, material=mat, t=t, x0=x0, name=f"plate_t3_{nx}x{ny}")
+
+
+def pressure_load_t3(xyz, conn, dchi, pressure_at):
+    """Consistent nodal loads of a normal pressure p(x) on a flat T3 mesh in the
+    xy plane, integrated with the 3-point rule (TriRule(3)); vectorized form
+    of distribloads for this case.  Returns the free-dof vector."""
+    X = xyz[conn]                       # E x 3 x 3
+    area = 0.5 * np.linalg.norm(np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]), axis=1)
+    pts = np.array([[2 / 3, 1 / 6, 1 / 6], [1 / 6, 2 / 3, 1 / 6], [1 / 6, 1 / 6, 2 / 3]])  # N at the 3 points
+    F = np.zeros(dchi.nalldofs)
+    fe_ = np.zeros((conn.shape[0], 3))
+    for q in range(3):
+        xq = np.einsum("a,eak->ek", pts[q], X)
+        p = pressure_at(xq)
+        fe_ += np.outer(p * area / 3, pts[q])
+    np.add.at(F, dchi.dofnums[conn, 2].ravel(), fe_.ravel())
+    return F[:dchi.nfreedofs]
+
+
+def gaussian_velocity(dchi, sigma, seed):
+    """Seeded N(0, sigma) velocities on the translational free dofs."""
+    rng = np.random.default_rng(seed)
+    V = np.zeros(dchi.nfreedofs)
+    trans = dchi.dofnums[:, :3][~dchi.is_fixed[:, :3]]
+    V[trans] = rng.normal(0.0, sigma, trans.size)
+    return V

@@ -23,6 +23,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define NDOF 6
 
@@ -787,68 +790,116 @@ void or_element_masses(int nen, int64_t E, const double *xyz, const int64_t *con
 /* (FinEtools SysmatAssemblerSparse -> sparse(I,J,V); AssemblyModule.jl:47-52) */
 /* ------------------------------------------------------------------------ */
 
-typedef struct { int64_t row, col, seq; double val; } coo_t;
-
-static int coo_cmp(const void *pa, const void *pb) {
-    const coo_t *a = (const coo_t *)pa, *b = (const coo_t *)pb;
-    if (a->row != b->row) return a->row < b->row ? -1 : 1;
-    if (a->col != b->col) return a->col < b->col ? -1 : 1;
-    if (a->seq != b->seq) return a->seq < b->seq ? -1 : 1;
-    return 0;
-}
-
 /*
  * Assemble E element matrices (E x k x k, row-major, k = 6 nen) into a CSR
  * matrix of size nrows x nrows.  dofnums: N x 6 row-major, 0-based global
  * equation numbers.  Entries whose row or column is >= nrows are dropped
  * (FFBlock when nrows = nfreedofs; all dofs when nrows = nalldofs).
+ * The COO stream is the one assemble! appends (element order; within an
+ * element column-major: for j in cols, for i in rows).  As in Julia's
+ * sparse(I, J, V): a stable counting sort by row, then duplicates combined
+ * in input order, then each row's columns sorted.
  * Call with nzval == NULL to get nnz (rowptr is filled).  rowptr has
  * nrows + 1 entries; colval/nzval nnz entries.  Returns nnz, or -1 on OOM.
  */
 int64_t or_assemble_csr(int nen, int64_t E, const int64_t *conn, const int64_t *dofnums, int64_t nrows,
                         const double *Ke, int64_t *rowptr, int64_t *colval, double *nzval) {
     int k = NDOF * nen;
-    int64_t cap = E * (int64_t)k * k;
-    coo_t *c = (coo_t *)malloc(sizeof(coo_t) * (cap > 0 ? cap : 1));
-    if (!c) return -1;
-    int64_t m = 0, seq = 0;
+    int64_t *cnt = (int64_t *)calloc(nrows + 1, sizeof(int64_t));
+    if (!cnt) return -1;
+    /* pass 1: entries per row */
     for (int64_t el = 0; el < E; ++el) {
         int64_t dn[24];
         for (int a = 0; a < nen; ++a)
             for (int d = 0; d < NDOF; ++d) dn[a * NDOF + d] = dofnums[conn[el * nen + a] * NDOF + d];
-        /* assemble! appends column-major: for j in cols, for i in rows */
         for (int j = 0; j < k; ++j)
-            for (int i = 0; i < k; ++i) {
-                if (dn[i] < nrows && dn[j] < nrows) {
-                    c[m].row = dn[i];
-                    c[m].col = dn[j];
-                    c[m].seq = seq;
-                    c[m].val = Ke ? Ke[el * k * k + i * k + j] : 0.0;
-                    ++m;
-                }
-                ++seq;
+            if (dn[j] < nrows)
+                for (int i = 0; i < k; ++i)
+                    if (dn[i] < nrows) cnt[dn[i] + 1]++;
+    }
+    for (int64_t r = 0; r < nrows; ++r) cnt[r + 1] += cnt[r];
+    int64_t m = cnt[nrows];
+    int64_t *bc = (int64_t *)malloc(sizeof(int64_t) * (m > 0 ? m : 1));
+    double *bv = (double *)malloc(sizeof(double) * (m > 0 ? m : 1));
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (nrows + 1));
+    if (!bc || !bv || !fill) {
+        free(cnt); free(bc); free(bv); free(fill);
+        return -1;
+    }
+    memcpy(fill, cnt, sizeof(int64_t) * (nrows + 1));
+    /* pass 2: stable bucket by row (input order kept within a row) */
+    for (int64_t el = 0; el < E; ++el) {
+        int64_t dn[24];
+        for (int a = 0; a < nen; ++a)
+            for (int d = 0; d < NDOF; ++d) dn[a * NDOF + d] = dofnums[conn[el * nen + a] * NDOF + d];
+        for (int j = 0; j < k; ++j)
+            if (dn[j] < nrows)
+                for (int i = 0; i < k; ++i)
+                    if (dn[i] < nrows) {
+                        int64_t q = fill[dn[i]]++;
+                        bc[q] = dn[j];
+                        bv[q] = Ke ? Ke[el * k * k + i * k + j] : 0.0;
+                    }
+    }
+    /* pass 3: per row, combine duplicates in input order, then sort columns */
+    int64_t *rlen = (int64_t *)calloc(nrows + 1, sizeof(int64_t));
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    int64_t *marks = (int64_t *)malloc(sizeof(int64_t) * (size_t)nth * (nrows > 0 ? nrows : 1));
+    if (!rlen || !marks) {
+        free(cnt); free(bc); free(bv); free(fill); free(rlen); free(marks);
+        return -1;
+    }
+    for (int64_t i = 0; i < (int64_t)nth * nrows; ++i) marks[i] = -1;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        int64_t *mk = marks + (int64_t)tid * nrows;
+        int64_t b = cnt[r], e = cnt[r + 1], w = b;
+        for (int64_t q = b; q < e; ++q) {
+            int64_t c = bc[q];
+            if (mk[c] >= b && mk[c] < w && bc[mk[c]] == c) {
+                bv[mk[c]] = bv[mk[c]] + bv[q];
+            } else {
+                mk[c] = w;
+                bc[w] = c;
+                bv[w] = bv[q];
+                ++w;
             }
-    }
-    qsort(c, m, sizeof(coo_t), coo_cmp);
-    for (int64_t r = 0; r <= nrows; ++r) rowptr[r] = 0;
-    int64_t nnz = 0;
-    for (int64_t i = 0; i < m;) {
-        int64_t j = i;
-        double acc = c[i].val;
-        while (j + 1 < m && c[j + 1].row == c[i].row && c[j + 1].col == c[i].col) {
-            ++j;
-            acc = acc + c[j].val;
         }
-        if (nzval) {
-            colval[nnz] = c[i].col;
-            nzval[nnz] = acc;
+        /* insertion sort of the compacted row by column */
+        for (int64_t q = b + 1; q < w; ++q) {
+            int64_t c = bc[q];
+            double v = bv[q];
+            int64_t p = q - 1;
+            while (p >= b && bc[p] > c) {
+                bc[p + 1] = bc[p];
+                bv[p + 1] = bv[p];
+                --p;
+            }
+            bc[p + 1] = c;
+            bv[p + 1] = v;
         }
-        rowptr[c[i].row + 1]++;
-        ++nnz;
-        i = j + 1;
+        for (int64_t q = b; q < w; ++q) mk[bc[q]] = -1;
+        rlen[r + 1] = w - b;
     }
-    for (int64_t r = 0; r < nrows; ++r) rowptr[r + 1] += rowptr[r];
-    free(c);
+    rowptr[0] = 0;
+    for (int64_t r = 0; r < nrows; ++r) rowptr[r + 1] = rowptr[r] + rlen[r + 1];
+    int64_t nnz = rowptr[nrows];
+    if (nzval) {
+#pragma omp parallel for schedule(static)
+        for (int64_t r = 0; r < nrows; ++r) {
+            int64_t len = rowptr[r + 1] - rowptr[r];
+            memcpy(colval + rowptr[r], bc + cnt[r], sizeof(int64_t) * len);
+            memcpy(nzval + rowptr[r], bv + cnt[r], sizeof(double) * len);
+        }
+    }
+    free(cnt); free(bc); free(bv); free(fill); free(rlen); free(marks);
     return nnz;
 }
 
