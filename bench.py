@@ -66,6 +66,23 @@ def cpu_baseline(case, nsample, K_host, Mdiag, dt, cd_steps_cpu):
             "cd_sample": f"{cd_steps_cpu} CD steps on the full free-free K ({n} dofs), OpenMP CSR SpMV"}
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes (a plain copy kernel of known size):
+    the byte counters of this dispatch calibrate the profiled traffic."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return 2 * nbytes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,7 +92,12 @@ def main():
     ap.add_argument("--cd-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: 3 assemblies, 50 CD steps, then a 1 GiB torch "
+                         "device copy that calibrates the byte counters")
     args = ap.parse_args()
+    if args.profile:
+        args.steps, args.warmup, args.cd_steps, args.no_cpu = 3, 1, 50, True
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(shard design only); run with --gpus 1", file=sys.stderr)
@@ -178,6 +200,8 @@ def main():
         Kh = sp.csr_matrix((np.zeros(nnz), cv, rp), shape=(nf, nf))
         Kh.data = f._model.stiffness(params, _lib.FS_BLOCK_FF, nnz)
         out["cpu_baseline"] = cpu_baseline(case, min(args.cpu_sample, E), Kh, Mdiag, dt, 20)
+    if args.profile:
+        out["copy_probe_bytes"] = copy_probe(1 << 30)
     print(json.dumps(out), flush=True)
 
 

@@ -35,17 +35,25 @@ struct ElemParams {
     const int32_t *elem_group;  // nelem or nullptr
 };
 
-// Scatter plan of one block (ALL or FF) -- see fs_scatter_kernel.
+// Scatter plan of one block (ALL or FF) -- see scatter_kernel.
+// Node I owns the CSR rows dof(I, r); all of them share one column list made
+// of the runs of I's neighbour nodes J (sorted).  For each neighbour slot k in
+// [nbr_ptr[I], nbr_ptr[I+1]): nbr_pair[k] = (upper pair index << 1) | side
+// (side 1: I is the pair's second node), nbr_off[2k], nbr_off[2k+1] = offset
+// of J's free / fixed column run within I's rows (-1 = none).
+// node_code[J*6+c]: 0..5 offset of comp c in J's free run, 8..13 in the fixed
+// run, -1 not in the block.  node_rowstart[I*6+r]: CSR position of row
+// dof(I, r), -1 if that row is not in the block.
 struct ScatterPlan {
-    int64_t npairs;            // upper node pairs (I <= J)
-    const int32_t *pair_I;     // npairs
-    const int32_t *pair_J;     // npairs
-    const int32_t *pair_off;   // npairs x 4: offset of J's free run / fixed run in
-                               // I's rows, then of I's runs in J's rows (-1 = none)
-    const int32_t *contrib_ptr;  // npairs + 1
-    const int32_t *contrib;      // (block index << 1) | transposed
-    const int64_t *node_rowstart; // N x 6: CSR position of row dof(I,r), -1 = not in block
-    const int8_t *node_code;     // N x 6: 0..5 free-run offset, 8..13 fixed-run, -1 excluded
+    int64_t nnodes;
+    const int32_t *nbr_ptr;       // N + 1
+    const int32_t *nbr_node;      // nbr_ptr[N]: neighbour node J
+    const int32_t *nbr_pair;      // nbr_ptr[N]
+    const int32_t *nbr_off;       // 2 * nbr_ptr[N]
+    const int8_t *node_code;      // N x 6
+    const int32_t *node_rowstart; // N x 6
+    const int32_t *contrib_ptr;   // npairs + 1
+    const int32_t *contrib;       // (block index << 1) | transposed, element order per pair
 };
 
 }  // namespace fs

@@ -23,7 +23,7 @@ namespace fs {
 hipError_t launch_element_blocks(const ElemParams &p, const double *x, const double *y, const double *z,
                                  const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                  const uint8_t *valid, double *Kb, int *status, hipStream_t st);
-hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, int nblk, double *vals, hipStream_t st);
+hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals, hipStream_t st);
 hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const double *y, const double *z,
                           const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
                           double *nx, double *ny, double *nz, uint8_t *valid, hipStream_t st);
@@ -72,11 +72,12 @@ struct Block {
     bool built = false;
     int64_t nrows = 0;
     int64_t nnz = 0;
+    int64_t nnodes = 0;
     std::vector<int64_t> rowptr;   // nrows + 1
     std::vector<int32_t> colval;   // nnz
     // scatter plan (device)
     int64_t npairs = 0;
-    DevBuf pair_I, pair_J, pair_off, contrib_ptr, contrib, node_rowstart, node_code;
+    DevBuf nbr_ptr, nbr_node, nbr_pair, nbr_off, node_code, node_rowstart, contrib_ptr, contrib;
     // device CSR (int32 for the time loop) and values
     DevBuf rowptr32, colidx32, kval, mdiag;
     bool k_ready = false, m_ready = false;
@@ -85,8 +86,8 @@ struct Block {
         nrows = nnz = npairs = 0;
         rowptr.clear();
         colval.clear();
-        for (DevBuf *b : {&pair_I, &pair_J, &pair_off, &contrib_ptr, &contrib, &node_rowstart, &node_code, &rowptr32,
-                          &colidx32, &kval, &mdiag})
+        for (DevBuf *b : {&nbr_ptr, &nbr_node, &nbr_pair, &nbr_off, &node_code, &node_rowstart, &contrib_ptr,
+                          &contrib, &rowptr32, &colidx32, &kval, &mdiag})
             b->release();
     }
 };
@@ -282,14 +283,9 @@ int build_block(fs_model *m, int which) {
             }
         }
     }
-    // node row starts
-    std::vector<int64_t> rowstart(N * 6, -1);
-    for (int64_t I = 0; I < N; ++I)
-        for (int cc = 0; cc < 6; ++cc) {
-            const int64_t d = m->dof[I * 6 + cc];
-            if (d < nrows) rowstart[I * 6 + cc] = B.rowptr[d];
-        }
-    // upper pairs and their offsets
+    if (nnz >= (int64_t(1) << 31))
+        return fail(c, FS_ERR_UNSUPPORTED, "block with %lld stored entries exceeds 32-bit positions", (long long)nnz);
+    // upper node pairs: the neighbours J >= I of each node, in order
     std::vector<int64_t> pair_base(N + 1, 0);
     for (int64_t I = 0; I < N; ++I) {
         int cnt = 0;
@@ -298,31 +294,36 @@ int build_block(fs_model *m, int which) {
         pair_base[I + 1] = pair_base[I] + cnt;
     }
     const int64_t npairs = pair_base[N];
-    std::vector<int32_t> pI(npairs), pJ(npairs), poff(npairs * 4);
+    if (npairs >= (int64_t(1) << 27)) return fail(c, FS_ERR_UNSUPPORTED, "too many node pairs (%lld)", (long long)npairs);
     auto slot_of = [&](int64_t I, int64_t J) -> int {
-        const int32_t *b = m->nbr_list.data() + m->nbr_ptr[I];
-        const int32_t *e = m->nbr_list.data() + m->nbr_ptr[I + 1];
-        const int32_t *it = std::lower_bound(b, e, (int32_t)J);
-        return (int)(it - m->nbr_list.data());
+        const int32_t *b0 = m->nbr_list.data() + m->nbr_ptr[I];
+        const int32_t *e0 = m->nbr_list.data() + m->nbr_ptr[I + 1];
+        return (int)(std::lower_bound(b0, e0, (int32_t)J) - m->nbr_list.data());
     };
+    auto pair_index = [&](int64_t lo, int64_t hi) -> int64_t {
+        return pair_base[lo] + (slot_of(lo, hi) - slot_of(lo, lo));
+    };
+    // neighbour tables: pair, side, run offsets; node row starts
+    const size_t nnb = m->nbr_list.size();
+    std::vector<int32_t> nb_pair(nnb), nb_off(2 * nnb), rowstart(N * 6, -1);
     for (int64_t I = 0; I < N; ++I) {
-        int64_t p = pair_base[I];
         for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
             const int J = m->nbr_list[k];
-            if (J < I) continue;
-            pI[p] = (int32_t)I;
-            pJ[p] = J;
-            poff[p * 4 + 0] = nb_offfree[k];
-            poff[p * 4 + 1] = nb_offfix[k];
-            const int k2 = slot_of(J, I);
-            poff[p * 4 + 2] = nb_offfree[k2];
-            poff[p * 4 + 3] = nb_offfix[k2];
-            ++p;
+            const int64_t p = I <= J ? pair_index(I, J) : pair_index(J, I);
+            nb_pair[k] = (int32_t)((p << 1) | (I > J ? 1 : 0));
+            nb_off[2 * k] = nb_offfree[k];
+            nb_off[2 * k + 1] = nb_offfix[k];
+        }
+        for (int cc = 0; cc < 6; ++cc) {
+            const int64_t d = m->dof[I * 6 + cc];
+            if (d < nrows) rowstart[I * 6 + cc] = (int32_t)B.rowptr[d];
         }
     }
     // contributions, element order within each pair
     const int nen = m->nen;
     const int nblk = nen * (nen + 1) / 2;
+    if ((int64_t)m->E * nblk >= (int64_t(1) << 30))
+        return fail(c, FS_ERR_UNSUPPORTED, "too many element blocks (%lld)", (long long)(m->E * nblk));
     std::vector<int32_t> cptr(npairs + 1, 0);
     std::vector<int64_t> cpair((size_t)m->E * nblk);
     for (int64_t e = 0; e < m->E; ++e) {
@@ -330,18 +331,12 @@ int build_block(fs_model *m, int which) {
         for (int a = 0; a < nen; ++a)
             for (int b = a; b < nen; ++b, ++bl) {
                 const int64_t I = m->conn[e * nen + a], J = m->conn[e * nen + b];
-                const int64_t lo = std::min(I, J), hi = std::max(I, J);
-                const int k = slot_of(lo, hi);
-                // pair index: pairs of lo are the neighbours >= lo in order
-                const int kfirst = slot_of(lo, lo);
-                const int64_t p = pair_base[lo] + (k - kfirst);
+                const int64_t p = pair_index(std::min(I, J), std::max(I, J));
                 cpair[e * nblk + bl] = p;
                 cptr[p + 1]++;
             }
     }
     for (int64_t p = 0; p < npairs; ++p) cptr[p + 1] += cptr[p];
-    if ((int64_t)m->E * nblk >= (int64_t(1) << 30))
-        return fail(c, FS_ERR_UNSUPPORTED, "too many element blocks (%lld)", (long long)(m->E * nblk));
     std::vector<int32_t> cfill(cptr.begin(), cptr.end() - 1), clist(cptr[npairs]);
     for (int64_t e = 0; e < m->E; ++e) {
         int bl = 0;
@@ -355,23 +350,23 @@ int build_block(fs_model *m, int which) {
     }
     // upload
     int s;
-    if ((s = upload(c, B.pair_I, pI.data(), pI.size()))) return s;
-    if ((s = upload(c, B.pair_J, pJ.data(), pJ.size()))) return s;
-    if ((s = upload(c, B.pair_off, poff.data(), poff.size()))) return s;
+    std::vector<int32_t> rp32(nrows + 1);
+    for (int64_t i = 0; i <= nrows; ++i) rp32[i] = (int32_t)B.rowptr[i];
+    if ((s = upload(c, B.nbr_ptr, m->nbr_ptr.data(), m->nbr_ptr.size()))) return s;
+    if ((s = upload(c, B.nbr_node, m->nbr_list.data(), m->nbr_list.size()))) return s;
+    if ((s = upload(c, B.nbr_pair, nb_pair.data(), nb_pair.size()))) return s;
+    if ((s = upload(c, B.nbr_off, nb_off.data(), nb_off.size()))) return s;
+    if ((s = upload(c, B.node_code, code.data(), code.size()))) return s;
+    if ((s = upload(c, B.node_rowstart, rowstart.data(), rowstart.size()))) return s;
     if ((s = upload(c, B.contrib_ptr, cptr.data(), cptr.size()))) return s;
     if ((s = upload(c, B.contrib, clist.data(), clist.size()))) return s;
-    if ((s = upload(c, B.node_rowstart, rowstart.data(), rowstart.size()))) return s;
-    if ((s = upload(c, B.node_code, code.data(), code.size()))) return s;
-    if (nnz < (int64_t(1) << 31)) {
-        std::vector<int32_t> rp32(nrows + 1);
-        for (int64_t i = 0; i <= nrows; ++i) rp32[i] = (int32_t)B.rowptr[i];
-        if ((s = upload(c, B.rowptr32, rp32.data(), rp32.size()))) return s;
-        if ((s = upload(c, B.colidx32, B.colval.data(), B.colval.size()))) return s;
-    }
+    if ((s = upload(c, B.rowptr32, rp32.data(), rp32.size()))) return s;
+    if ((s = upload(c, B.colidx32, B.colval.data(), B.colval.size()))) return s;
     if ((s = dalloc(c, B.kval, (size_t)nnz * sizeof(double)))) return s;
     if ((s = dalloc(c, B.mdiag, (size_t)nrows * sizeof(double)))) return s;
     FS_HIP(c, hipStreamSynchronize(c->stream));
     B.npairs = npairs;
+    B.nnodes = N;
     B.nrows = nrows;
     B.nnz = nnz;
     B.built = true;
@@ -381,14 +376,15 @@ int build_block(fs_model *m, int which) {
 
 fs::ScatterPlan plan_of(const Block &B) {
     fs::ScatterPlan sp;
-    sp.npairs = B.npairs;
-    sp.pair_I = B.pair_I.as<int32_t>();
-    sp.pair_J = B.pair_J.as<int32_t>();
-    sp.pair_off = B.pair_off.as<int32_t>();
+    sp.nnodes = B.nnodes;
+    sp.nbr_ptr = B.nbr_ptr.as<int32_t>();
+    sp.nbr_node = B.nbr_node.as<int32_t>();
+    sp.nbr_pair = B.nbr_pair.as<int32_t>();
+    sp.nbr_off = B.nbr_off.as<int32_t>();
+    sp.node_code = B.node_code.as<int8_t>();
+    sp.node_rowstart = B.node_rowstart.as<int32_t>();
     sp.contrib_ptr = B.contrib_ptr.as<int32_t>();
     sp.contrib = B.contrib.as<int32_t>();
-    sp.node_rowstart = B.node_rowstart.as<int64_t>();
-    sp.node_code = B.node_code.as<int8_t>();
     return sp;
 }
 
@@ -734,8 +730,7 @@ int fs_model_stiffness(fs_model *m, const fs_shell_params *p, int block, double 
     FS_HIP(c, hipEventCreate(&e0));
     FS_HIP(c, hipEventCreate(&e1));
     FS_HIP(c, hipEventRecord(e0, c->stream));
-    FS_HIP(c, fs::launch_scatter(plan_of(B), m->Kb.as<double>(), m->nen * (m->nen + 1) / 2, B.kval.as<double>(),
-                                 c->stream));
+    FS_HIP(c, fs::launch_scatter(plan_of(B), m->Kb.as<double>(), B.kval.as<double>(), c->stream));
     FS_HIP(c, hipEventRecord(e1, c->stream));
     FS_HIP(c, hipStreamSynchronize(c->stream));
     float ms_sc = 0.f;

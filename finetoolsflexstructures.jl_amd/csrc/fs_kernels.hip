@@ -40,6 +40,8 @@
 // exactly once and the result is bitwise reproducible.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "fs_device.h"
 
 namespace fs {
@@ -654,48 +656,436 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
 }
 
 // ---------------------------------------------------------------------------
+// Q4 element kernel, Gauss points processed in sequence
+// ---------------------------------------------------------------------------
+//
+// Same mathematics as elem_kernel<4, 4, 8>; the LDS holds the C matrix of one
+// Gauss point at a time (8 x 24) and the K blocks accumulate in registers of
+// the (element, node pair) threads across the four Gauss points, so an
+// element needs ~3.7 KB of LDS instead of ~11 KB and three times as many
+// elements are resident per CU.  Phase G1 (thread per element and Gauss
+// point) stores a compact geometry record: E_G, gradN_e, the material scale
+// factors (and the layup angle for layered shells) and the MITC shear row
+// entries; the material factors are rebuilt per node thread.
+template <int NE, int WG>
+__global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const double *__restrict__ xs,
+                                                         const double *__restrict__ ys,
+                                                         const double *__restrict__ zs,
+                                                         const int32_t *__restrict__ conn,
+                                                         const double *__restrict__ nxs,
+                                                         const double *__restrict__ nys,
+                                                         const double *__restrict__ nzs,
+                                                         const uint8_t *__restrict__ valid,
+                                                         double *__restrict__ Kb, int *__restrict__ status) {
+    constexpr int NEN = 4, NQP = 4, NR = 8, K = 24, NBLK = 10;
+    constexpr int GEO = 47;             // EG 9 | gN 8 | s_m s_b s_s m n | bs 24 (+1 pad)
+    constexpr int G_EG = 0, G_GN = 9, G_SC = 17, G_BS = 22;
+    constexpr int CSTR = NR * K + 2;    // one Gauss point of C, padded
+    constexpr int NODE = 6 + NR;
+    constexpr int OFF_G = 0;
+    constexpr int OFF_C = OFF_G + NE * NQP * GEO;
+    constexpr int OFF_N = OFF_C + NE * CSTR;
+    constexpr int OFF_M = OFF_N + NE * NEN * NODE;
+    constexpr int MISC = 4 * 9 + 1;
+    static_assert(NE * NQP * GEO + NE * CSTR >= NE * NBLK * 36, "output stage must fit in the G+C region");
+    extern __shared__ double smem[];
+    double *sG = smem + OFF_G;
+    double *sC = smem + OFF_C;
+    double *sN = smem + OFF_N;
+    double *sM = smem + OFF_M;
+
+    const int tid = threadIdx.x;
+    const int E = p.nelem;
+    const int e0 = blockIdx.x * NE;
+    const int ne = (E - e0) < NE ? (E - e0) : NE;
+
+    // ---------------- G1: geometry per (element, Gauss point) ----------------
+    if (tid < NE * NQP) {
+        const int le = tid / NQP, q = tid % NQP;
+        if (le < ne) {
+            const int e = e0 + le;
+            double X[4][3];
+            for (int a = 0; a < 4; ++a) {
+                const int n = conn[a * E + e];
+                X[a][0] = xs[n];
+                X[a][1] = ys[n];
+                X[a][2] = zs[n];
+            }
+            const int grp = (p.composite && p.elem_group) ? p.elem_group[e] : 0;
+            const double tt = p.composite ? p.g_t[grp] : p.thickness;
+            double *g = sG + (le * NQP + q) * GEO;
+            const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+            const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+            double gNp[4][2];
+            q4_gradn(xi, eta, gNp);
+            double J[3][2];
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 2; ++k) {
+                    double acc = 0.0;
+                    for (int a = 0; a < 4; ++a) acc += X[a][i] * gNp[a][k];
+                    J[i][k] = acc;
+                }
+            const double cr0 = -J[2][0] * J[1][1] + J[1][0] * J[2][1];
+            const double cr1 = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+            const double cr2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
+            const double Jac = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+            double EG[3][3];
+            e_g(EG, J);
+            double cen[3] = {0.0, 0.0, 0.0};
+            for (int a = 0; a < 4; ++a)
+                for (int k = 0; k < 3; ++k) cen[k] += X[a][k];
+            for (int k = 0; k < 3; ++k) cen[k] /= 4;
+            double ec[4][2];
+            for (int a = 0; a < 4; ++a)
+                for (int k = 0; k < 2; ++k) {
+                    double dd = 0.0;
+                    for (int m = 0; m < 3; ++m) dd += (X[a][m] - cen[m]) * EG[m][k];
+                    ec[a][k] = dd;
+                }
+            const double G00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
+            const double G01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
+            const double G11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
+            const double detG = G00 * G11 - G01 * G01;
+            if (detG == 0.0) status[0] = 1;
+            const double i00 = G11 / detG, i01 = -G01 / detG, i11 = G00 / detG;
+            for (int a = 0; a < 4; ++a) {
+                const double t0 = i00 * gNp[a][0] + i01 * gNp[a][1];
+                const double t1 = i01 * gNp[a][0] + i11 * gNp[a][1];
+                double gg[3];
+                for (int i = 0; i < 3; ++i) gg[i] = J[i][0] * t0 + J[i][1] * t1;
+                g[G_GN + 2 * a] = EG[0][0] * gg[0] + EG[1][0] * gg[1] + EG[2][0] * gg[2];
+                g[G_GN + 2 * a + 1] = EG[0][1] * gg[0] + EG[1][1] * gg[1] + EG[2][1] * gg[2];
+            }
+            double maxd = 0.0;
+            for (int i = 1; i < 4; ++i) {
+                const double dx = X[i][0] - X[0][0], dy = X[i][1] - X[0][1], dz = X[i][2] - X[0][2];
+                const double dist = (dx * dx + dy * dy + dz * dz);
+                maxd = dist > maxd ? dist : maxd;
+            }
+            const double h = sqrt(maxd);
+            const double stab = (tt * tt) / (tt * tt + p.stab_c * (h * h));
+            if (!p.composite) {
+                g[G_SC + 0] = sqrt(tt * Jac);
+                g[G_SC + 1] = sqrt((tt * tt * tt / 12.0) * Jac);
+                g[G_SC + 2] = sqrt(tt * stab * Jac);
+                g[G_SC + 3] = 0.0;
+                g[G_SC + 4] = 0.0;
+            } else {
+                double m, n;
+                layup_angle(EG, p.g_csys + 9 * grp, m, n);
+                g[G_SC + 0] = sqrt(Jac);
+                g[G_SC + 1] = 0.0;
+                g[G_SC + 2] = sqrt(stab * Jac);
+                g[G_SC + 3] = m;
+                g[G_SC + 4] = n;
+            }
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) g[G_EG + i * 3 + j] = EG[i][j];
+            // MITC transverse shear entries (_Bsmat, FEMMShellQ4RSModule.jl:753-804)
+            const double X1 = ec[0][0], Y1 = ec[0][1], X2 = ec[1][0], Y2 = ec[1][1];
+            const double X3 = ec[2][0], Y3 = ec[2][1], X4 = ec[3][0], Y4 = ec[3][1];
+            const double r = xi, s = eta;
+            const double J11 = (X1 * (s - 1) / 4 - X2 * (s - 1) / 4 + X3 * (s + 1) / 4 - X4 * (s + 1) / 4);
+            const double J21 = (Y1 * (s - 1) / 4 - Y2 * (s - 1) / 4 + Y3 * (s + 1) / 4 - Y4 * (s + 1) / 4);
+            const double J12 = (X1 * (r - 1) / 4 - X2 * (r + 1) / 4 + X3 * (r + 1) / 4 - X4 * (r - 1) / 4);
+            const double J22 = (Y1 * (r - 1) / 4 - Y2 * (r + 1) / 4 + Y3 * (r + 1) / 4 - Y4 * (r - 1) / 4);
+            const double A = sqrt(J11 * J11 + J21 * J21);
+            const double B = sqrt(J12 * J12 + J22 * J22);
+            const double ca = J11 / A, sa = J21 / A, cb = J12 / B, sb = J22 / B;
+            const double detJ = J11 * J22 - J12 * J21;
+            const double Ax = X1 - X2 - X3 + X4, Ay = Y1 - Y2 - Y3 + Y4;
+            const double Bx = X1 - X2 + X3 - X4, By = Y1 - Y2 + Y3 - Y4;
+            const double Cx = X1 + X2 - X3 - X4, Cy = Y1 + Y2 - Y3 - Y4;
+            const double P = sqrt((Ax + Bx * s) * (Ax + Bx * s) + (Ay + By * s) * (Ay + By * s));
+            const double Q = sqrt((Bx * r + Cx) * (Bx * r + Cx) + (By * r + Cy) * (By * r + Cy));
+            const double i16 = 1.0 / (16 * detJ), i32 = 1.0 / (32 * detJ);
+            double *bs = g + G_BS;  // [node][comp w,tx,ty][row]
+            bs[0 * 6 + 0 * 2 + 0] = (sa * (r + 1) * P - sb * (s + 1) * Q) * i16;
+            bs[0 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
+            bs[0 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
+            bs[0 * 6 + 0 * 2 + 1] = (-ca * (r + 1) * P + cb * (s + 1) * Q) * i16;
+            bs[0 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
+            bs[0 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
+            bs[1 * 6 + 0 * 2 + 0] = (-sa * (r - 1) * P + sb * (s + 1) * Q) * i16;
+            bs[1 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
+            bs[1 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
+            bs[1 * 6 + 0 * 2 + 1] = (ca * (r - 1) * P - cb * (s + 1) * Q) * i16;
+            bs[1 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
+            bs[1 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
+            bs[2 * 6 + 0 * 2 + 0] = (sa * (r - 1) * P - sb * (s - 1) * Q) * i16;
+            bs[2 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
+            bs[2 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
+            bs[2 * 6 + 0 * 2 + 1] = (-ca * (r - 1) * P + cb * (s - 1) * Q) * i16;
+            bs[2 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
+            bs[2 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
+            bs[3 * 6 + 0 * 2 + 0] = (-sa * (r + 1) * P + sb * (s - 1) * Q) * i16;
+            bs[3 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
+            bs[3 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
+            bs[3 * 6 + 0 * 2 + 1] = (ca * (r + 1) * P - cb * (s - 1) * Q) * i16;
+            bs[3 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
+            bs[3 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
+        }
+    }
+
+    // thread roles
+    const bool gact = tid < NE * NEN && (tid / NEN) < ne;
+    const int le2 = tid / NEN, j2 = tid % NEN;
+    int node2 = 0;
+    uint8_t valid2 = 0;
+    double n20 = 0.0, n21 = 0.0, n22 = 0.0;
+    if (gact) {
+        node2 = conn[j2 * E + e0 + le2];
+        valid2 = valid[node2];
+        n20 = nxs[node2];
+        n21 = nys[node2];
+        n22 = nzs[node2];
+    }
+    const bool cact = tid < NE * NBLK && (tid / NBLK) < ne;
+    const int le3 = tid / NBLK, blk = tid % NBLK;
+    int ba = 0, bb = 0;
+    {
+        int cnt = 0;
+        for (int a = 0; a < NEN; ++a)
+            for (int b = a; b < NEN; ++b) {
+                if (cnt == blk) {
+                    ba = a;
+                    bb = b;
+                }
+                ++cnt;
+            }
+    }
+    double acc[6][6];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) acc[r][c] = 0.0;
+    __syncthreads();
+
+    for (int q = 0; q < NQP; ++q) {
+        // ---------------- G2: triad and element-frame columns of node j ----------------
+        double A[3][3];
+        double cu[NR], cv[NR], cw[NR], cx[NR], cy[NR];
+        double gNj0 = 0.0, gNj1 = 0.0;
+        const double *g = sG + (le2 * NQP + q) * GEO;
+        if (gact) {
+            double EG[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) EG[i][j] = g[G_EG + i * 3 + j];
+            double nk_e[3];
+            if (valid2) {
+                for (int i = 0; i < 3; ++i) nk_e[i] = EG[0][i] * n20 + EG[1][i] * n21 + EG[2][i] * n22;
+            } else {
+                nk_e[0] = 0.0;
+                nk_e[1] = 0.0;
+                nk_e[2] = 1.0;
+            }
+            const double r0 = -nk_e[1], r1 = nk_e[0];
+            const double na = sqrt(r0 * r0 + r1 * r1);
+            if (na > 1.0e-12) {
+                const double u0 = r0 / na, u1 = r1 / na;
+                double sa, ca;
+                sincos(na, &sa, &ca);
+                const double f = sa / na;
+                const double uu[3] = {u0, u1, 0.0};
+                const double Sk[3][3] = {{0.0, 0.0, r1}, {0.0, 0.0, -r0}, {-r1, r0, 0.0}};
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        const double aa = uu[i] * uu[j];
+                        A[i][j] = ca * (((i == j) ? 1.0 : 0.0) - aa) + f * Sk[i][j] + aa;
+                    }
+            } else {
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) A[i][j] = (i == j) ? 1.0 : 0.0;
+            }
+            gNj0 = g[G_GN + 2 * j2];
+            gNj1 = g[G_GN + 2 * j2 + 1];
+            double Lmb[36], Ls[4];
+            if (!p.composite) {
+                homog_factors(p.Lps, p.Lt, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], Lmb, Ls);
+            } else {
+                const int grp = p.elem_group ? p.elem_group[e0 + le2] : 0;
+                composite_factors(p.g_Labd + 36 * grp, p.g_Lh + 4 * grp, g[G_SC + 3], g[G_SC + 4], g[G_SC + 0],
+                                  g[G_SC + 2], Lmb, Ls);
+            }
+            const double *bs = g + G_BS + j2 * 6;
+            for (int s = 0; s < 6; ++s) {
+                cu[s] = Lmb[s * 6 + 0] * gNj0 + Lmb[s * 6 + 2] * gNj1;
+                cv[s] = Lmb[s * 6 + 1] * gNj1 + Lmb[s * 6 + 2] * gNj0;
+                cw[s] = 0.0;
+                cx[s] = -Lmb[s * 6 + 4] * gNj1 - Lmb[s * 6 + 5] * gNj0;
+                cy[s] = Lmb[s * 6 + 3] * gNj0 + Lmb[s * 6 + 5] * gNj1;
+            }
+            for (int rr = 0; rr < 2; ++rr) {
+                cu[6 + rr] = 0.0;
+                cv[6 + rr] = 0.0;
+                cw[6 + rr] = Ls[rr * 2 + 0] * bs[0 * 2 + 0] + Ls[rr * 2 + 1] * bs[0 * 2 + 1];
+                cx[6 + rr] = Ls[rr * 2 + 0] * bs[1 * 2 + 0] + Ls[rr * 2 + 1] * bs[1 * 2 + 1];
+                cy[6 + rr] = Ls[rr * 2 + 0] * bs[2 * 2 + 0] + Ls[rr * 2 + 1] * bs[2 * 2 + 1];
+            }
+            const double m1 = (1 / A[2][2]) * A[0][2];
+            const double m2 = (1 / A[2][2]) * A[1][2];
+            double *nd = sN + (le2 * NEN + j2) * NODE;
+            for (int k = 0; k < 3; ++k) {
+                nd[k] = A[0][k];
+                nd[3 + k] = A[1][k];
+            }
+            for (int s = 0; s < NR; ++s) nd[6 + s] = m1 * cx[s] + m2 * cy[s];
+        }
+        __syncthreads();
+        // ---------------- G3: global-basis C columns of node j ----------------
+        if (gact) {
+            const double *ndb = sN + (le2 * NEN) * NODE;
+            double Q[3][3];
+            for (int k = 0; k < 3; ++k)
+                for (int gg = 0; gg < 3; ++gg)
+                    Q[k][gg] = A[0][k] * g[G_EG + gg * 3 + 0] + A[1][k] * g[G_EG + gg * 3 + 1] +
+                               A[2][k] * g[G_EG + gg * 3 + 2];
+            double cgt[3][NR], cgr[3][NR];
+            for (int gg = 0; gg < 3; ++gg)
+                for (int s = 0; s < NR; ++s) {
+                    cgt[gg][s] = 0.0;
+                    cgr[gg][s] = 0.0;
+                }
+            for (int k = 0; k < 3; ++k) {
+                double a3[NEN];
+                for (int i = 0; i < NEN; ++i)
+                    a3[i] = 1.0 / 2 * (ndb[i * NODE + 3 + k] * gNj0 - ndb[i * NODE + k] * gNj1);
+                for (int s = 0; s < NR; ++s) {
+                    double cn = cu[s] * A[0][k] + cv[s] * A[1][k] + cw[s] * A[2][k];
+                    for (int i = 0; i < NEN; ++i) cn += a3[i] * ndb[i * NODE + 6 + s];
+                    for (int gg = 0; gg < 3; ++gg) cgt[gg][s] += cn * Q[k][gg];
+                }
+            }
+            for (int l = 0; l < 2; ++l) {
+                const double r2a = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
+                const double r2b = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
+                for (int s = 0; s < NR; ++s) {
+                    const double cn = cx[s] * r2a + cy[s] * r2b;
+                    for (int gg = 0; gg < 3; ++gg) cgr[gg][s] += cn * Q[l][gg];
+                }
+            }
+            double *cdst = sC + le2 * CSTR + 6 * j2;
+            for (int s = 0; s < NR; ++s)
+                for (int gg = 0; gg < 3; ++gg) {
+                    cdst[s * K + gg] = cgt[gg][s];
+                    cdst[s * K + 3 + gg] = cgr[gg][s];
+                }
+        }
+        __syncthreads();
+        // ---------------- C: accumulate node-pair blocks ----------------
+        if (cact) {
+            const double *cbp = sC + le3 * CSTR;
+            for (int s = 0; s < NR; ++s) {
+                double va[6], vb[6];
+                for (int r = 0; r < 6; ++r) {
+                    va[r] = cbp[s * K + 6 * ba + r];
+                    vb[r] = cbp[s * K + 6 * bb + r];
+                }
+                for (int r = 0; r < 6; ++r)
+                    for (int c = 0; c < 6; ++c) acc[r][c] += va[r] * vb[c];
+            }
+        }
+    }
+    // ---------------- Q4 drilling (FEMMShellQ4RSModule.jl:807-859) ----------------
+    if (cact && ba == bb) {
+        double *ms = sM + le3 * MISC + 9 * ba;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) ms[r * 3 + c] = acc[3 + r][3 + c];
+    }
+    __syncthreads();
+    if (cact && ba == bb && p.drill_scale != 0.0) {
+        const int e = e0 + le3;
+        const double *kr = sM + le3 * MISC;
+        double sum = 0.0;
+        int cnt = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int node = conn[k * E + e];
+            if (!valid[node]) continue;
+            const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
+            const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+            if (nn == 0.0) continue;
+            const double u[3] = {n0 / nn, n1 / nn, n2 / nn};
+            double Pm[3][3], tmp[3][3];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) Pm[a][b] = ((a == b) ? 1.0 : 0.0) - u[a] * u[b];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    tmp[a][b] = Pm[a][0] * kr[9 * k + 0 * 3 + b] + Pm[a][1] * kr[9 * k + 1 * 3 + b] +
+                                Pm[a][2] * kr[9 * k + 2 * 3 + b];
+            double tr = 0.0;
+            for (int a = 0; a < 3; ++a) tr += tmp[a][0] * Pm[0][a] + tmp[a][1] * Pm[1][a] + tmp[a][2] * Pm[2][a];
+            tr = tr / 2.0;
+            sum += tr > 0.0 ? tr : 0.0;
+            ++cnt;
+        }
+        const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * p.drill_scale;
+        const int node = conn[ba * E + e];
+        if (kavg != 0.0 && valid[node]) {
+            const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
+            if (sqrt(n0 * n0 + n1 * n1 + n2 * n2) != 0.0) {
+                const double nv[3] = {n0, n1, n2};
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (nv[r] * nv[c]);
+            }
+        }
+    }
+    // ---------------- stage and store ----------------
+    if (cact) {
+        double *ks = smem + (le3 * NBLK + blk) * 36;  // G and C regions are dead here
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) ks[r * 6 + c] = acc[r][c];
+    }
+    __syncthreads();
+    {
+        const int total = ne * NBLK * 36;
+        double *dst = Kb + (size_t)e0 * NBLK * 36;
+        for (int i = tid; i < total; i += WG) dst[i] = smem[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Scatter: node-pair blocks -> CSR values
 // ---------------------------------------------------------------------------
 //
-// Thread (pair p, side, r): side 0 writes row r of block (I, J) into CSR row
-// dof(I, r); side 1 (I != J) writes row r of block (J, I) = column r of
-// (I, J) into CSR row dof(J, r).  Contributions are summed in ascending
-// element order.
+// Thread (node I, row component r, neighbour slot k): sums row r of the 6x6
+// block K_IJ over the pair's contributing element blocks in ascending element
+// order (a block stored as (J, I) is read transposed) and writes the six
+// values into CSR row dof(I, r) at J's column run.  The 16 threads of one
+// (I, r) cover the neighbours in CSR column order, so their 48-byte runs are
+// adjacent and the row is written with coalesced stores.  Every CSR value is
+// written exactly once; no atomics.
+#define FS_SC_NB 16  // neighbour slots per (node, row); more neighbours loop
+
 __global__ __launch_bounds__(256) void scatter_kernel(ScatterPlan sp, const double *__restrict__ Kb,
-                                                      int nblk, double *__restrict__ vals) {
+                                                      double *__restrict__ vals) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t p = gid / 12;
-    const int sub = (int)(gid % 12);
-    if (p >= sp.npairs) return;
-    const int side = sub / 6, r = sub % 6;
-    const int I = sp.pair_I[p], J = sp.pair_J[p];
-    if (side == 1 && I == J) return;
-    const int rowNode = side == 0 ? I : J;
-    const int colNode = side == 0 ? J : I;
-    const int64_t rs = sp.node_rowstart[(int64_t)rowNode * 6 + r];
+    const int64_t I = gid / (6 * FS_SC_NB);
+    if (I >= sp.nnodes) return;
+    const int r = (int)((gid / FS_SC_NB) % 6);
+    const int kk = (int)(gid % FS_SC_NB);
+    const int rs = sp.node_rowstart[I * 6 + r];
     if (rs < 0) return;
-    const int offFree = sp.pair_off[p * 4 + side * 2 + 0];
-    const int offFix = sp.pair_off[p * 4 + side * 2 + 1];
-    const int cb = sp.contrib_ptr[p], ce = sp.contrib_ptr[p + 1];
-    double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int k = cb; k < ce; ++k) {
-        const int cv = sp.contrib[k];
-        const int64_t bi = (int64_t)(cv >> 1);
-        // stored block is (I,J) when not transposed; (J,I) when transposed
-        const bool tr = (cv & 1) != 0;
-        // element of the (rowNode, colNode) block at [r][c]:
-        //   side 0, !tr : B[r][c]      side 0, tr : B[c][r]
-        //   side 1, !tr : B[c][r]      side 1, tr : B[r][c]
-        const double *B = Kb + bi * 36;
-        const bool rowmajor = (side == 0) != tr;
-        for (int c = 0; c < 6; ++c) sum[c] += rowmajor ? B[r * 6 + c] : B[c * 6 + r];
-    }
-    const int8_t *code = sp.node_code + (int64_t)colNode * 6;
-    for (int c = 0; c < 6; ++c) {
-        const int cd = code[c];
-        if (cd < 0) continue;
-        const int64_t pos = (cd < 8) ? rs + offFree + cd : rs + offFix + (cd - 8);
-        vals[pos] = sum[c];
+    const int nb1 = sp.nbr_ptr[I + 1];
+    for (int k = sp.nbr_ptr[I] + kk; k < nb1; k += FS_SC_NB) {
+        const int pr = sp.nbr_pair[k];
+        const int p = pr >> 1;
+        const bool side1 = (pr & 1) != 0;
+        double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const int ce = sp.contrib_ptr[p + 1];
+        for (int q = sp.contrib_ptr[p]; q < ce; ++q) {
+            const int cv = sp.contrib[q];
+            const double *B = Kb + (int64_t)(cv >> 1) * 36;
+            if (side1 == ((cv & 1) != 0)) {
+                for (int c = 0; c < 6; ++c) sum[c] += B[r * 6 + c];
+            } else {
+                for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
+            }
+        }
+        const int8_t *code = sp.node_code + (int64_t)sp.nbr_node[k] * 6;
+        const int of = sp.nbr_off[2 * k], ox = sp.nbr_off[2 * k + 1];
+        for (int c = 0; c < 6; ++c) {
+            const int cd = code[c];
+            if (cd < 0) continue;
+            vals[rs + (cd < 8 ? of + cd : ox + cd - 8)] = sum[c];
+        }
     }
 }
 
@@ -1049,6 +1439,27 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
     return hipGetLastError();
 }
 
+template <int NE, int WG>
+static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
+                                const int32_t *conn, const double *nx, const double *ny, const double *nz,
+                                const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
+    constexpr int total = NE * 4 * 47 + NE * (8 * 24 + 2) + NE * 4 * 14 + NE * 37;
+    const size_t smem = total * sizeof(double);
+    static_assert(NE * 10 <= WG, "workgroup too small");
+    static bool attr = false;
+    if (!attr && smem > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    const int nblocks = (p.nelem + NE - 1) / NE;
+    if (nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn, nx, ny,
+                       nz, valid, Kb, status);
+    return hipGetLastError();
+}
+
 hipError_t launch_element_blocks(const ElemParams &p, const double *x, const double *y, const double *z,
                                  const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                  const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
@@ -1057,14 +1468,14 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
             return launch_elem<3, 1, 12, 16, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
         return launch_elem<3, 1, 8, 16, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
     }
-    return launch_elem<4, 4, 8, 5, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
+    if (std::getenv("FS_Q4_ALLQP")) return launch_elem<4, 4, 8, 5, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
+    return launch_q4_seq<16, 192>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
 }
 
-hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, int nblk, double *vals, hipStream_t st) {
-    if (sp.npairs == 0) return hipSuccess;
-    const int64_t threads = sp.npairs * 12;
-    const int64_t blocks = (threads + 255) / 256;
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, sp, Kb, nblk, vals);
+hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals, hipStream_t st) {
+    if (sp.nnodes == 0) return hipSuccess;
+    const int64_t threads = sp.nnodes * 6 * FS_SC_NB;
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sp, Kb, vals);
     return hipGetLastError();
 }
 
