@@ -66,6 +66,56 @@ def cpu_baseline(case, nsample, K_host, Mdiag, dt, cd_steps_cpu):
             "cd_sample": f"{cd_steps_cpu} CD steps on the full free-free K ({n} dofs), OpenMP CSR SpMV"}
 
 
+def bench_batch(nmodels, nsteps):
+    """256 independent T3 plates (seeded): the plain path (one model per launch
+    sequence, FP64, no graph) against the batched path (all models in one
+    launch per step, hipGraph-captured loop).  Both are timed with HIP events
+    around the step loops; per-model dt from the device power iteration."""
+    import numpy as np
+
+    from flexshell import _lib, batch as B, explicit, femm as FM, workloads as W
+    ctx = _lib.default_context()
+    models = W.batch_c3(nmodels=nmodels)
+    nm = len(models)
+    V0s = [W.gaussian_velocity(m["dchi"], 1e-3, seed=100 + k) for k, m in enumerate(models)]
+    zeros = [np.zeros(m["dchi"].nfreedofs) for m in models]
+    g = np.ones((nsteps + 1, nm))
+    dts, plain_cd_ms, plain_asm_ms, U_plain = [], 0.0, 0.0, {}
+    keep = {0, nm // 2, nm - 1}
+    for k, m in enumerate(models):
+        f = FM.FEMMShellT3FF(m["conn"], m["material"], thickness=m["t"], ctx=ctx)
+        f.associategeometry(m["xyz"])
+        d = m["dchi"]
+        nf = d.nfreedofs
+        f.stiffness(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), m["xyz"], None, None, d)
+        plain_asm_ms += f._model.timing(2)
+        f.mass(FM.SysmatAssemblerFFBlock(FM.SysmatAssemblerSparseDiag(), nf, nf), m["xyz"], d)
+        dts.append(explicit.stable_dt(explicit.omega_max(f, np.random.default_rng(k).random(nf), maxit=30)))
+        r = explicit.run(f._model, 0.0, zeros[k], V0s[k], dts[k], nsteps, zeros[k], g[:, k])
+        plain_cd_ms += r["ms"]
+        if k in keep:
+            U_plain[k] = r["U"]
+        f._model.close()
+    bt = B.ShellBatch(models, ctx=ctx)
+    bt.assemble_device()
+    batch_asm_ms = bt.model.timing(2)
+    bt.mass()
+    nfree = bt.nfree
+    E = int(sum(m["conn"].shape[0] for m in models))
+    r = bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g, nsteps)
+    same = all(np.array_equal(r["U"][bt.model_slice(k)], U_plain[k]) for k in keep)
+    bt.close()
+    plain = nfree * nsteps / (plain_cd_ms / 1e3)
+    batched = nfree * nsteps / (r["ms"] / 1e3)
+    return {"models": nm, "elements": E, "free_dofs": nfree, "steps": nsteps,
+            "plain_dof_steps_per_s": plain, "batched_dof_steps_per_s": batched,
+            "batched_over_plain": batched / plain,
+            "plain_elements_per_s": E / (plain_asm_ms / 1e3), "batched_elements_per_s": E / (batch_asm_ms / 1e3),
+            "bitwise_equal_to_plain": bool(same),
+            "note": "plain = one model per launch sequence (FP64, no graph); batched = all models per launch, "
+                    "hipGraph step loop, FP64"}
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of nbytes (a plain copy kernel of known size):
     the byte counters of this dispatch calibrate the profiled traffic."""
@@ -92,12 +142,15 @@ def main():
     ap.add_argument("--cd-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-models", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: 3 assemblies, 50 CD steps, then a 1 GiB torch "
                          "device copy that calibrates the byte counters")
     args = ap.parse_args()
     if args.profile:
-        args.steps, args.warmup, args.cd_steps, args.no_cpu = 3, 1, 50, True
+        args.steps, args.warmup, args.cd_steps, args.no_cpu, args.no_batch = 3, 1, 50, True, True
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(shard design only); run with --gpus 1", file=sys.stderr)
@@ -200,6 +253,10 @@ def main():
         Kh = sp.csr_matrix((np.zeros(nnz), cv, rp), shape=(nf, nf))
         Kh.data = f._model.stiffness(params, _lib.FS_BLOCK_FF, nnz)
         out["cpu_baseline"] = cpu_baseline(case, min(args.cpu_sample, E), Kh, Mdiag, dt, 20)
+    if not args.no_batch:
+        t0 = time.perf_counter()
+        out["batch"] = bench_batch(args.batch_models, args.batch_steps)
+        print(f"[bench] batch section {time.perf_counter() - t0:.1f}s: {out['batch']}", file=sys.stderr, flush=True)
     if args.profile:
         out["copy_probe_bytes"] = copy_probe(1 << 30)
     print(json.dumps(out), flush=True)

@@ -33,6 +33,9 @@ struct ElemParams {
     const double *g_csys; // ngroups x 9: layup csys, row-major [i][j]
     const double *g_t;    // ngroups: layup thickness
     const int32_t *elem_group;  // nelem or nullptr
+    // homogeneous material per group (ngroups x 16): chol(Dps) 9 | chol(5/6 Dt) 4 |
+    // thickness | stab_c | drilling scale
+    const double *g_homog;
 };
 
 // Scatter plan of one block (ALL or FF) -- see scatter_kernel.

@@ -27,16 +27,18 @@ hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals,
 hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const double *y, const double *z,
                           const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
                           double *nx, double *ny, double *nz, uint8_t *valid, hipStream_t st);
-hipError_t launch_mass(int nen, int64_t N, int E, int composite, double rho, double t, const double *g_inertia,
+hipError_t launch_mass(int nen, int64_t N, int E, int composite, const double *g_rho_t, const double *g_inertia,
                        const int32_t *elem_group, const double *x, const double *y, const double *z,
                        const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, const int64_t *node_dof,
                        int64_t nrows, double *diag, hipStream_t st);
 hipError_t launch_cd_init(int64_t n, double dt, double g0, const double *invMC, const double *Fmag, const double *U0,
-                          const double *V, double *A, double *U1, hipStream_t st);
+                          const double *V, double *A, double *U1, const int32_t *row_group, const double *group_dt,
+                          const double *gtab, hipStream_t st);
 hipError_t launch_cd_step(int tpr, int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval,
                           const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
                           const double *Cd, const double *Fmag, double gstep, double dt, int damped,
-                          const int32_t *probe_slot, double *probe_dst, hipStream_t st);
+                          const int32_t *probe_slot, double *probe_dst, const int32_t *row_group,
+                          const double *group_dt, const double *gtab, hipStream_t st);
 hipError_t launch_spmv(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval, const double *x,
                        double *y, hipStream_t st);
 hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, const double *d, double *part,
@@ -108,7 +110,7 @@ struct fs_model {
     int64_t nall = 0, nfree = 0;
     Block blk[2];
     // group data for the last stiffness/mass call
-    DevBuf g_Labd, g_Lh, g_csys, g_t, g_group, g_inertia, m_group;
+    DevBuf g_Labd, g_Lh, g_csys, g_t, g_group, g_inertia, m_group, g_homog, g_rho_t;
     double timing[4] = {0, 0, 0, 0};
 };
 
@@ -407,15 +409,36 @@ int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
     ep.thickness = p->thickness;
     ep.stab_c = p->stab_c;
     ep.drill_scale = p->drilling_stiffness_scale;
-    int s;
+    int s = 0;
     if (!p->composite) {
-        double Dps[9], Dt[4];
-        cm2rm(3, p->Dps, Dps);
-        cm2rm(2, p->Dt, Dt);
-        for (int i = 0; i < 4; ++i) Dt[i] *= 5.0 / 6.0;  // FEMMShellT3FFModule.jl:658-659
-        if (!chol_upper(3, Dps, ep.Lps)) return fail(c, FS_ERR_NUMERIC, "stiffness: Dps is not positive definite");
-        if (!chol_upper(2, Dt, ep.Lt)) return fail(c, FS_ERR_NUMERIC, "stiffness: Dt is not positive definite");
-        if (!(p->thickness > 0.0)) return fail(c, FS_ERR_ARG, "stiffness: thickness must be positive");
+        // one material record per group: chol(Dps), chol(5/6 Dt), t, stab_c, drilling scale
+        const bool grouped = p->group_homog != nullptr;
+        const int ng = grouped ? p->ngroups : 1;
+        if (grouped && ng < 1) return fail(c, FS_ERR_ARG, "stiffness: group_homog needs ngroups >= 1");
+        std::vector<double> rec(16 * (size_t)ng);
+        for (int g = 0; g < ng; ++g) {
+            const double *src = grouped ? p->group_homog + 16 * g : nullptr;
+            double Dps[9], Dt[4];
+            cm2rm(3, grouped ? src : p->Dps, Dps);
+            cm2rm(2, grouped ? src + 9 : p->Dt, Dt);
+            for (int i = 0; i < 4; ++i) Dt[i] *= 5.0 / 6.0;  // FEMMShellT3FFModule.jl:658-659
+            double *r = &rec[16 * g];
+            if (!chol_upper(3, Dps, r)) return fail(c, FS_ERR_NUMERIC, "stiffness: Dps of group %d is not positive definite", g);
+            if (!chol_upper(2, Dt, r + 9)) return fail(c, FS_ERR_NUMERIC, "stiffness: Dt of group %d is not positive definite", g);
+            r[13] = grouped ? src[13] : p->thickness;
+            r[14] = grouped ? src[14] : p->stab_c;
+            r[15] = grouped ? src[15] : p->drilling_stiffness_scale;
+            if (!(r[13] > 0.0)) return fail(c, FS_ERR_ARG, "stiffness: thickness of group %d must be positive", g);
+        }
+        if ((s = upload(c, m->g_homog, rec.data(), rec.size()))) return s;
+        ep.g_homog = m->g_homog.as<double>();
+        if (grouped && p->elem_group) {
+            for (int64_t e = 0; e < m->E; ++e)
+                if (p->elem_group[e] < 0 || p->elem_group[e] >= ng)
+                    return fail(c, FS_ERR_ARG, "stiffness: element %lld has group %d", (long long)e, p->elem_group[e]);
+            if ((s = upload(c, m->g_group, p->elem_group, (size_t)m->E))) return s;
+            ep.elem_group = m->g_group.as<int32_t>();
+        }
     } else {
         const int ng = p->ngroups;
         if (ng < 1 || !p->group_ABDH || !p->group_csys || !p->group_thickness)
@@ -460,6 +483,7 @@ int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
     }
     const int nblk = m->nen * (m->nen + 1) / 2;
     const size_t kbytes = (size_t)m->E * nblk * 36 * sizeof(double);
+    (void)0;
     if (m->Kb.bytes < kbytes && (s = dalloc(c, m->Kb, kbytes))) return s;
     FS_HIP(c, hipMemsetAsync(m->status.p, 0, sizeof(int), c->stream));
     hipEvent_t e0, e1;
@@ -756,7 +780,26 @@ int fs_model_mass(fs_model *m, const fs_mass_params *p, int block, double *diag_
     Block &B = m->blk[block];
     const double *ginert = nullptr;
     const int32_t *ggrp = nullptr;
-    if (p->composite) {
+    const double *grt = nullptr;
+    if (!p->composite) {
+        const bool grouped = p->group_rho_t != nullptr;
+        const int ng = grouped ? p->ngroups : 1;
+        if (grouped && ng < 1) return fail(c, FS_ERR_ARG, "mass: group_rho_t needs ngroups >= 1");
+        std::vector<double> rt(2 * (size_t)ng);
+        for (int g = 0; g < ng; ++g) {
+            rt[2 * g] = grouped ? p->group_rho_t[2 * g] : p->rho;
+            rt[2 * g + 1] = grouped ? p->group_rho_t[2 * g + 1] : p->thickness;
+        }
+        if ((s = upload(c, m->g_rho_t, rt.data(), rt.size()))) return s;
+        grt = m->g_rho_t.as<double>();
+        if (grouped && p->elem_group) {
+            for (int64_t e = 0; e < m->E; ++e)
+                if (p->elem_group[e] < 0 || p->elem_group[e] >= ng)
+                    return fail(c, FS_ERR_ARG, "mass: element %lld has group %d", (long long)e, p->elem_group[e]);
+            if ((s = upload(c, m->m_group, p->elem_group, (size_t)m->E))) return s;
+            ggrp = m->m_group.as<int32_t>();
+        }
+    } else {
         if (p->ngroups < 1 || !p->group_inertia) return fail(c, FS_ERR_ARG, "mass: layered shell needs group inertia");
         if ((s = upload(c, m->g_inertia, p->group_inertia, (size_t)2 * p->ngroups))) return s;
         ginert = m->g_inertia.as<double>();
@@ -768,7 +811,7 @@ int fs_model_mass(fs_model *m, const fs_mass_params *p, int block, double *diag_
             ggrp = m->m_group.as<int32_t>();
         }
     }
-    FS_HIP(c, fs::launch_mass(m->nen, m->N, (int)m->E, p->composite ? 1 : 0, p->rho, p->thickness, ginert, ggrp,
+    FS_HIP(c, fs::launch_mass(m->nen, m->N, (int)m->E, p->composite ? 1 : 0, grt, ginert, ggrp,
                               m->x.as<double>(), m->y.as<double>(), m->z.as<double>(), m->dconn.as<int32_t>(),
                               m->dne_ptr.as<int32_t>(), m->dne_list.as<int32_t>(), m->node_dof.as<int64_t>(),
                               B.nrows, B.mdiag.as<double>(), c->stream));
@@ -788,29 +831,46 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     Block &B = m->blk[FS_BLOCK_FF];
     if (!B.k_ready || !B.m_ready)
         return fail(c, FS_ERR_STATE, "explicit: FF-block stiffness and mass must be computed first");
-    if (!B.rowptr32.p) return fail(c, FS_ERR_UNSUPPORTED, "explicit: more than 2^31 stored entries");
     if (p->precision != FS_PREC_FP64) return fail(c, FS_ERR_UNSUPPORTED, "explicit: precision %d", p->precision);
-    if (!U0 || !V0 || !Fmag || !g || p->nsteps < 0 || !(p->dt > 0.0))
+    const bool batch = p->ngroups > 0;
+    if (!U0 || !V0 || !Fmag || !g || p->nsteps < 0 || (!batch && !(p->dt > 0.0)))
         return fail(c, FS_ERR_ARG, "explicit: bad arguments");
+    if (batch && (!p->row_group || !p->group_dt || !p->group_cdamp))
+        return fail(c, FS_ERR_ARG, "explicit: a batch needs row_group, group_dt and group_cdamp");
     const int stride = p->probe_stride > 0 ? p->probe_stride : 1;
     if (nprobes < 0 || (nprobes > 0 && (!probe_dofs || !probe_out))) return fail(c, FS_ERR_ARG, "explicit: probes");
     const int64_t n = B.nrows;
     for (int k = 0; k < nprobes; ++k)
         if (probe_dofs[k] < 0 || probe_dofs[k] >= n) return fail(c, FS_ERR_ARG, "explicit: probe dof out of range");
+    const int ng = batch ? p->ngroups : 1;
+    if (batch) {
+        for (int64_t i = 0; i < n; ++i)
+            if (p->row_group[i] < 0 || p->row_group[i] >= ng) return fail(c, FS_ERR_ARG, "explicit: row_group");
+        for (int gi = 0; gi < ng; ++gi)
+            if (!(p->group_dt[gi] > 0.0)) return fail(c, FS_ERR_ARG, "explicit: group_dt must be positive");
+    }
     FS_HIP(c, hipSetDevice(c->device));
     std::vector<double> M(n), Cd(n), invMC(n);
     FS_HIP(c, hipMemcpy(M.data(), B.mdiag.p, n * 8, hipMemcpyDeviceToHost));
-    const double dt = p->dt;
     for (int64_t i = 0; i < n; ++i) {
-        Cd[i] = p->cdamp * M[i];
+        const double dt = batch ? p->group_dt[p->row_group[i]] : p->dt;
+        const double cd = batch ? p->group_cdamp[p->row_group[i]] : p->cdamp;
+        Cd[i] = cd * M[i];
         invMC[i] = 1.0 / (M[i] + (dt / 2) * Cd[i]);  // plate_expl_examples.jl:68-69
     }
-    DevBuf dU[2], dV, dA, dF, dInv, dC, dSlot, dProbe;
+    bool damped = false;
+    for (int gi = 0; gi < ng; ++gi) damped |= (batch ? p->group_cdamp[gi] : p->cdamp) != 0.0;
+    DevBuf dU[2], dV, dA, dF, dInv, dC, dSlot, dProbe, dRG, dGdt, dG;
     int s;
     if ((s = upload(c, dU[0], U0, n)) || (s = dalloc(c, dU[1], n * 8)) || (s = upload(c, dV, V0, n)) ||
         (s = dalloc(c, dA, n * 8)) || (s = upload(c, dF, Fmag, n)) || (s = upload(c, dInv, invMC.data(), n)) ||
         (s = upload(c, dC, Cd.data(), n)))
         return s;
+    if (batch) {
+        if ((s = upload(c, dRG, p->row_group, n)) || (s = upload(c, dGdt, p->group_dt, ng)) ||
+            (s = upload(c, dG, g, (size_t)(p->nsteps + 1) * ng)))
+            return s;
+    }
     const int64_t nrec = p->nsteps / stride + 1;
     if (nprobes > 0) {
         std::vector<int32_t> slot(n, -1);
@@ -818,35 +878,64 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
         if ((s = upload(c, dSlot, slot.data(), n)) || (s = dalloc(c, dProbe, (size_t)nrec * nprobes * 8))) return s;
         for (int k = 0; k < nprobes; ++k) probe_out[k] = U0[probe_dofs[k]];  // peek(0, U0)
     }
-    const int damped = p->cdamp != 0.0;
-    // rows of ~40-60 entries: 8 lanes per row
+    // rows of the shell stencils hold ~40-60 entries: 8 lanes per row
     const int tpr = 8;
+    const int32_t *rg = batch ? dRG.as<int32_t>() : nullptr;
+    const double *gdt = batch ? dGdt.as<double>() : nullptr;
+    auto enqueue = [&](hipStream_t st) -> hipError_t {
+        hipError_t e = fs::launch_cd_init(n, p->dt, batch ? 0.0 : g[0], dInv.as<double>(), dF.as<double>(),
+                                          dU[0].as<double>(), dV.as<double>(), dA.as<double>(), dU[1].as<double>(),
+                                          rg, gdt, batch ? dG.as<double>() : nullptr, st);
+        if (e != hipSuccess) return e;
+        int cur = 1;
+        for (int64_t step = 1; step <= p->nsteps; ++step) {
+            const bool rec = nprobes > 0 && (step % stride == 0);
+            e = fs::launch_cd_step(tpr, n, B.rowptr32.as<int32_t>(), B.colidx32.as<int32_t>(), B.kval.as<double>(),
+                                   dU[cur].as<double>(), dU[1 - cur].as<double>(), dV.as<double>(), dA.as<double>(),
+                                   dInv.as<double>(), dC.as<double>(), dF.as<double>(), batch ? 0.0 : g[step],
+                                   p->dt, damped ? 1 : 0, rec ? dSlot.as<int32_t>() : nullptr,
+                                   rec ? dProbe.as<double>() + (step / stride) * nprobes : nullptr, rg, gdt,
+                                   batch ? dG.as<double>() + step * ng : nullptr, st);
+            if (e != hipSuccess) return e;
+            cur = 1 - cur;
+        }
+        return hipSuccess;
+    };
     hipEvent_t e0, e1;
     FS_HIP(c, hipEventCreate(&e0));
     FS_HIP(c, hipEventCreate(&e1));
-    FS_HIP(c, hipEventRecord(e0, c->stream));
-    FS_HIP(c, fs::launch_cd_init(n, dt, g[0], dInv.as<double>(), dF.as<double>(), dU[0].as<double>(),
-                                 dV.as<double>(), dA.as<double>(), dU[1].as<double>(), c->stream));
-    int cur = 1;
-    for (int64_t step = 1; step <= p->nsteps; ++step) {
-        const bool rec = nprobes > 0 && (step % stride == 0);
-        FS_HIP(c, fs::launch_cd_step(tpr, n, B.rowptr32.as<int32_t>(), B.colidx32.as<int32_t>(), B.kval.as<double>(),
-                                     dU[cur].as<double>(), dU[1 - cur].as<double>(), dV.as<double>(), dA.as<double>(),
-                                     dInv.as<double>(), dC.as<double>(), dF.as<double>(), g[step], dt, damped,
-                                     rec ? dSlot.as<int32_t>() : nullptr,
-                                     rec ? dProbe.as<double>() + (step / stride) * nprobes : nullptr, c->stream));
-        cur = 1 - cur;
+    if (p->use_graph) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        FS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = enqueue(c->stream);
+        hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        if (ce != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(c, FS_ERR_DEVICE, "explicit: graph capture failed: %s",
+                        hipGetErrorString(ce != hipSuccess ? ce : ee));
+        }
+        FS_HIP(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        FS_HIP(c, hipEventRecord(e0, c->stream));
+        FS_HIP(c, hipGraphLaunch(exec, c->stream));
+        FS_HIP(c, hipEventRecord(e1, c->stream));
+        FS_HIP(c, hipStreamSynchronize(c->stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    } else {
+        FS_HIP(c, hipEventRecord(e0, c->stream));
+        FS_HIP(c, enqueue(c->stream));
+        FS_HIP(c, hipEventRecord(e1, c->stream));
+        FS_HIP(c, hipStreamSynchronize(c->stream));
     }
-    FS_HIP(c, hipEventRecord(e1, c->stream));
-    FS_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     FS_HIP(c, hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     m->timing[3] = ms;
-    // after the last step, dU[1 - cur] holds U_nsteps (cur now indexes the predictor)
-    const int fin = p->nsteps == 0 ? 0 : 1 - cur;
-    if (U_out) FS_HIP(c, hipMemcpy(U_out, dU[fin].p, n * 8, hipMemcpyDeviceToHost));
+    // after the last step the buffer not indexed by the loop's final `cur` holds U_nsteps
+    const int fin = (p->nsteps % 2 == 1) ? 1 : 0;
+    if (U_out) FS_HIP(c, hipMemcpy(U_out, dU[p->nsteps == 0 ? 0 : fin].p, n * 8, hipMemcpyDeviceToHost));
     if (V_out) FS_HIP(c, hipMemcpy(V_out, dV.p, n * 8, hipMemcpyDeviceToHost));
     if (nprobes > 0 && nrec > 1)
         FS_HIP(c, hipMemcpy(probe_out + nprobes, dProbe.as<double>() + nprobes, (size_t)(nrec - 1) * nprobes * 8,

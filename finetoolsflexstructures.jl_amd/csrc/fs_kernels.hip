@@ -223,8 +223,10 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
             double *g = sG + (le * NQP + q) * T::GEO;
             double EG[3][3], gN[NEN][2];
             double Lmb[36], Ls[4];
-            const int grp = (p.composite && p.elem_group) ? p.elem_group[e] : 0;
-            const double tt = p.composite ? p.g_t[grp] : p.thickness;
+            const int grp = p.elem_group ? p.elem_group[e] : 0;
+            const double *hrec = p.composite ? nullptr : p.g_homog + 16 * grp;
+            const double tt = p.composite ? p.g_t[grp] : hrec[13];
+            const double stab_c = p.composite ? p.stab_c : hrec[14];
             if constexpr (NEN == 3) {
                 // FEMMShellT3FFModule.jl:269-321,465-484
                 double J0[3][2];
@@ -250,10 +252,10 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
                 gN[2][1] = a / Jd;
                 const double Ae = Jd / 2;
                 const double h = sqrt(2 * Ae);
-                const double stab = (tt * tt) / (tt * tt + p.stab_c * (h * h));
+                const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
                 const double wsh = (NR == 12) ? Ae / 3 : Ae;  // AVERAGE_K splits the weight
                 if (!p.composite) {
-                    homog_factors(p.Lps, p.Lt, sqrt(tt * Ae), sqrt((tt * tt * tt) / 12 * Ae),
+                    homog_factors(hrec, hrec + 9, sqrt(tt * Ae), sqrt((tt * tt * tt) / 12 * Ae),
                                   sqrt(tt * stab * wsh), Lmb, Ls);
                 } else {
                     double m, n;
@@ -352,9 +354,9 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
                     maxd = dist > maxd ? dist : maxd;
                 }
                 const double h = sqrt(maxd);
-                const double stab = (tt * tt) / (tt * tt + p.stab_c * (h * h));
+                const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
                 if (!p.composite) {
-                    homog_factors(p.Lps, p.Lt, sqrt(tt * Jac), sqrt((tt * tt * tt / 12.0) * Jac),
+                    homog_factors(hrec, hrec + 9, sqrt(tt * Jac), sqrt((tt * tt * tt / 12.0) * Jac),
                                   sqrt(tt * stab * Jac), Lmb, Ls);
                 } else {
                     double m, n;
@@ -586,8 +588,10 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
             const int e = e0 + le3;
             const int node = conn[ba * E + e];
             const double *ms = sM + le3 * T::MISC;
+            const int grp = p.elem_group ? p.elem_group[e] : 0;
+            const double dscale = p.composite ? p.drill_scale : p.g_homog[16 * grp + 15];
             const double kavg = (((((ms[T::M_KP + 0] + ms[T::M_KP + 2]) + ms[T::M_KP + 4]) + ms[T::M_KP + 1]) +
-                                   ms[T::M_KP + 3]) + ms[T::M_KP + 5]) / 6 * p.drill_scale;
+                                   ms[T::M_KP + 3]) + ms[T::M_KP + 5]) / 6 * dscale;
             if (valid[node]) {
                 const double *gv = ms + T::M_G + 3 * ba;
                 for (int r = 0; r < 3; ++r)
@@ -603,7 +607,9 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
     if constexpr (NEN == 4) {
         // Q4 drilling (_drilling_penalty_kavg, _add_drilling_stiffness!, :807-859)
         __syncthreads();
-        if (cact && ba == bb && p.drill_scale != 0.0) {
+        const int grpd = cact ? (p.elem_group ? p.elem_group[e0 + le3] : 0) : 0;
+        const double dscale = p.composite ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
+        if (cact && ba == bb && dscale != 0.0) {
             const int e = e0 + le3;
             const double *kr = sM + le3 * T::MISC + T::M_KRR;
             double sum = 0.0;
@@ -629,7 +635,7 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
                 sum += tr > 0.0 ? tr : 0.0;
                 ++cnt;
             }
-            const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * p.drill_scale;
+            const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * dscale;
             const int node = conn[ba * E + e];
             if (kavg != 0.0 && valid[node]) {
                 const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
@@ -711,8 +717,10 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
                 X[a][1] = ys[n];
                 X[a][2] = zs[n];
             }
-            const int grp = (p.composite && p.elem_group) ? p.elem_group[e] : 0;
-            const double tt = p.composite ? p.g_t[grp] : p.thickness;
+            const int grp = p.elem_group ? p.elem_group[e] : 0;
+            const double *hrec = p.composite ? nullptr : p.g_homog + 16 * grp;
+            const double tt = p.composite ? p.g_t[grp] : hrec[13];
+            const double stab_c = p.composite ? p.stab_c : hrec[14];
             double *g = sG + (le * NQP + q) * GEO;
             const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
             const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
@@ -763,7 +771,7 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
                 maxd = dist > maxd ? dist : maxd;
             }
             const double h = sqrt(maxd);
-            const double stab = (tt * tt) / (tt * tt + p.stab_c * (h * h));
+            const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
             if (!p.composite) {
                 g[G_SC + 0] = sqrt(tt * Jac);
                 g[G_SC + 1] = sqrt((tt * tt * tt / 12.0) * Jac);
@@ -898,10 +906,11 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
             gNj0 = g[G_GN + 2 * j2];
             gNj1 = g[G_GN + 2 * j2 + 1];
             double Lmb[36], Ls[4];
+            const int grp = p.elem_group ? p.elem_group[e0 + le2] : 0;
             if (!p.composite) {
-                homog_factors(p.Lps, p.Lt, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], Lmb, Ls);
+                const double *hrec = p.g_homog + 16 * grp;
+                homog_factors(hrec, hrec + 9, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], Lmb, Ls);
             } else {
-                const int grp = p.elem_group ? p.elem_group[e0 + le2] : 0;
                 composite_factors(p.g_Labd + 36 * grp, p.g_Lh + 4 * grp, g[G_SC + 3], g[G_SC + 4], g[G_SC + 0],
                                   g[G_SC + 2], Lmb, Ls);
             }
@@ -991,7 +1000,9 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
             for (int c = 0; c < 3; ++c) ms[r * 3 + c] = acc[3 + r][3 + c];
     }
     __syncthreads();
-    if (cact && ba == bb && p.drill_scale != 0.0) {
+    const int grpd = cact ? (p.elem_group ? p.elem_group[e0 + le3] : 0) : 0;
+    const double dscale = p.composite ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
+    if (cact && ba == bb && dscale != 0.0) {
         const int e = e0 + le3;
         const double *kr = sM + le3 * MISC;
         double sum = 0.0;
@@ -1016,7 +1027,7 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
             sum += tr > 0.0 ? tr : 0.0;
             ++cnt;
         }
-        const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * p.drill_scale;
+        const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * dscale;
         const int node = conn[ba * E + e];
         if (kavg != 0.0 && valid[node]) {
             const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
@@ -1184,8 +1195,8 @@ __global__ __launch_bounds__(256) void normals_kernel(int64_t N, int E, const do
 // (FEMMShellQ4RSModule.jl:994-1002); layered: laminate inertia per group.
 struct MassParams {
     int composite;
-    double rho, t;
-    const double *g_inertia;  // ngroups x 2
+    const double *g_rho_t;    // homogeneous: ngroups x 2 (rho, thickness)
+    const double *g_inertia;  // layered: ngroups x 2
     const int32_t *elem_group;
 };
 
@@ -1210,11 +1221,14 @@ __global__ __launch_bounds__(256) void mass_kernel(int64_t N, int E, MassParams 
             X[a][1] = ys[n];
             X[a][2] = zs[n];
         }
-        double m0 = 0.0, m1 = 0.0;
+        double m0 = 0.0, m1 = 0.0, rho = 0.0, tt = 0.0;
+        const int g = mp.elem_group ? mp.elem_group[e] : 0;
         if (mp.composite) {
-            const int g = mp.elem_group ? mp.elem_group[e] : 0;
             m0 = mp.g_inertia[2 * g];
             m1 = mp.g_inertia[2 * g + 1];
+        } else {
+            rho = mp.g_rho_t[2 * g];
+            tt = mp.g_rho_t[2 * g + 1];
         }
         double te, re;
         if (NEN == 3) {
@@ -1230,8 +1244,8 @@ __global__ __launch_bounds__(256) void mass_kernel(int64_t N, int E, MassParams 
             const double d = J0[0][1] * EG[0][1] + J0[1][1] * EG[1][1] + J0[2][1] * EG[2][1];
             const double Ae = (a * d - bq * c) / 2;
             if (!mp.composite) {
-                te = mp.rho * (mp.t * Ae) / 3;
-                re = mp.rho * (mp.t * mp.t * mp.t / 12 * Ae) / 3;
+                te = rho * (tt * Ae) / 3;
+                re = rho * (tt * tt * tt / 12 * Ae) / 3;
             } else {
                 te = m0 * (Ae / 3);
                 re = m1 * (Ae / 3);
@@ -1254,8 +1268,8 @@ __global__ __launch_bounds__(256) void mass_kernel(int64_t N, int E, MassParams 
                 const double c2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
                 const double Jac = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
                 if (!mp.composite) {
-                    sa += mp.rho * mp.t * Jac * 1.0;
-                    sb += mp.rho * (mp.t * mp.t * mp.t) / 12 * Jac * 1.0;
+                    sa += rho * tt * Jac * 1.0;
+                    sb += rho * (tt * tt * tt) / 12 * Jac * 1.0;
                 } else {
                     sa += m0 * Jac * 1.0;
                     sb += m1 * Jac * 1.0;
@@ -1296,13 +1310,20 @@ __global__ __launch_bounds__(256) void cd_init_kernel(int64_t n, double dt, doub
                                                       const double *__restrict__ invMC,
                                                       const double *__restrict__ Fmag, const double *__restrict__ U0,
                                                       const double *__restrict__ V, double *__restrict__ A,
-                                                      double *__restrict__ U1) {
+                                                      double *__restrict__ U1, const int32_t *__restrict__ row_group,
+                                                      const double *__restrict__ group_dt,
+                                                      const double *__restrict__ gtab) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double a = invMC[i] * (g0 * Fmag[i]);
+    double dtr = dt, gr = g0;
+    if (row_group) {
+        dtr = group_dt[row_group[i]];
+        gr = gtab[row_group[i]];
+    }
+    const double a = invMC[i] * (gr * Fmag[i]);
     A[i] = a;
-    const double dt2_2 = ((dt * dt) / 2);
-    U1[i] = U0[i] + (dt * V[i] + dt2_2 * a);
+    const double dt2_2 = ((dtr * dtr) / 2);
+    U1[i] = U0[i] + (dtr * V[i] + dt2_2 * a);
 }
 
 // One step for a group of TPR lanes per row (TPR in {4, 8, 16, 32, 64}).
@@ -1315,7 +1336,10 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
                                                       const double *__restrict__ Cd, const double *__restrict__ Fmag,
                                                       double gstep, double dt, int damped,
                                                       const int32_t *__restrict__ probe_slot,
-                                                      double *__restrict__ probe_dst) {
+                                                      double *__restrict__ probe_dst,
+                                                      const int32_t *__restrict__ row_group,
+                                                      const double *__restrict__ group_dt,
+                                                      const double *__restrict__ gtab) {
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t row = gt / TPR;
     const int lane = (int)(gt % TPR);
@@ -1328,9 +1352,15 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, TPR);
     if (active && lane == 0) {
-        const double dt_2 = (dt / 2), dt2_2 = ((dt * dt) / 2);
+        double dtr = dt, gr = gstep;
+        if (row_group) {  // batch: the row's model has its own dt and load history
+            const int gi = row_group[row];
+            dtr = group_dt[gi];
+            gr = gtab[gi];
+        }
+        const double dt_2 = (dtr / 2), dt2_2 = ((dtr * dtr) / 2);
         const double v = V[row], a = A[row];
-        double f = gstep * Fmag[row];
+        double f = gr * Fmag[row];
         if (damped)
             f -= e + Cd[row] * (v + dt_2 * a);
         else
@@ -1341,7 +1371,7 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
         const double u = Ucur[row];
         V[row] = vn;
         A[row] = an;
-        Unext[row] = u + (dt * vn + dt2_2 * an);
+        Unext[row] = u + (dtr * vn + dt2_2 * an);
         if (probe_slot) {
             const int ps = probe_slot[row];
             if (ps >= 0) probe_dst[ps] = u;  // peek(step, U, V, t): U of this step
@@ -1468,7 +1498,6 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
             return launch_elem<3, 1, 12, 16, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
         return launch_elem<3, 1, 8, 16, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
     }
-    if (std::getenv("FS_Q4_ALLQP")) return launch_elem<4, 4, 8, 5, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
     return launch_q4_seq<16, 192>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
 }
 
@@ -1493,12 +1522,12 @@ hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const doub
     return hipGetLastError();
 }
 
-hipError_t launch_mass(int nen, int64_t N, int E, int composite, double rho, double t, const double *g_inertia,
+hipError_t launch_mass(int nen, int64_t N, int E, int composite, const double *g_rho_t, const double *g_inertia,
                        const int32_t *elem_group, const double *x, const double *y, const double *z,
                        const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, const int64_t *node_dof,
                        int64_t nrows, double *diag, hipStream_t st) {
     if (N == 0) return hipSuccess;
-    MassParams mp{composite, rho, t, g_inertia, elem_group};
+    MassParams mp{composite, g_rho_t, g_inertia, elem_group};
     const unsigned blocks = (unsigned)((N + 255) / 256);
     if (nen == 3)
         hipLaunchKernelGGL(mass_kernel<3>, dim3(blocks), dim3(256), 0, st, N, E, mp, x, y, z, conn, ne_ptr, ne_list,
@@ -1510,24 +1539,27 @@ hipError_t launch_mass(int nen, int64_t N, int E, int composite, double rho, dou
 }
 
 hipError_t launch_cd_init(int64_t n, double dt, double g0, const double *invMC, const double *Fmag, const double *U0,
-                          const double *V, double *A, double *U1, hipStream_t st) {
+                          const double *V, double *A, double *U1, const int32_t *row_group, const double *group_dt,
+                          const double *gtab, hipStream_t st) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(cd_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, dt, g0, invMC, Fmag,
-                       U0, V, A, U1);
+                       U0, V, A, U1, row_group, group_dt, gtab);
     return hipGetLastError();
 }
 
 hipError_t launch_cd_step(int tpr, int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval,
                           const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
                           const double *Cd, const double *Fmag, double gstep, double dt, int damped,
-                          const int32_t *probe_slot, double *probe_dst, hipStream_t st) {
+                          const int32_t *probe_slot, double *probe_dst, const int32_t *row_group,
+                          const double *group_dt, const double *gtab, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const int64_t threads = n * tpr;
     const unsigned blocks = (unsigned)((threads + 255) / 256);
 #define FS_CD_CASE(T)                                                                                           \
     case T:                                                                                                     \
         hipLaunchKernelGGL(cd_step_kernel<T>, dim3(blocks), dim3(256), 0, st, n, rowptr, colidx, kval, Ucur,     \
-                           Unext, V, A, invMC, Cd, Fmag, gstep, dt, damped, probe_slot, probe_dst);           \
+                           Unext, V, A, invMC, Cd, Fmag, gstep, dt, damped, probe_slot, probe_dst, row_group,    \
+                           group_dt, gtab);                                                                    \
         break;
     switch (tpr) {
         FS_CD_CASE(4)

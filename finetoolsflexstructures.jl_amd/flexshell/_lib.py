@@ -51,6 +51,7 @@ class ShellParams(C.Structure):
         ("group_csys", _dp),
         ("group_thickness", _dp),
         ("elem_group", _i32p),
+        ("group_homog", _dp),
     ]
 
 
@@ -62,6 +63,7 @@ class MassParams(C.Structure):
         ("thickness", C.c_double),
         ("group_inertia", _dp),
         ("elem_group", _i32p),
+        ("group_rho_t", _dp),
     ]
 
 
@@ -72,6 +74,11 @@ class CDParams(C.Structure):
         ("nsteps", C.c_int64),
         ("precision", C.c_int32),
         ("probe_stride", C.c_int32),
+        ("ngroups", C.c_int32),
+        ("use_graph", C.c_int32),
+        ("row_group", _i32p),
+        ("group_dt", _dp),
+        ("group_cdamp", _dp),
     ]
 
 

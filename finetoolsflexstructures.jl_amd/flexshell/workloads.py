@@ -94,3 +94,45 @@ def gaussian_velocity(dchi, sigma, seed):
     trans = dchi.dofnums[:, :3][~dchi.is_fixed[:, :3]]
     V[trans] = rng.normal(0.0, sigma, trans.size)
     return V
+
+
+def batch_c3(nmodels=256, seed=4, nmin=8, nmax=128):
+    """256 independent T3 plate models: n per side U{nmin..nmax} (2 n^2
+    triangles), aspect ratio U(1, 4), thickness log-U(0.002, 0.05), E
+    log-U(1e9, 3e11), nu U(0.2, 0.4), rho U(1000, 8000), interior node jitter
+    +-10% h, flat or cylindrically curved (seeded), each edge clamped or
+    simply supported (u = v = w = 0) by seed."""
+    rng = np.random.default_rng(seed)
+    models = []
+    for k in range(nmodels):
+        n = int(rng.integers(nmin, nmax + 1))
+        aspect = rng.uniform(1.0, 4.0)
+        Ly = 1.0
+        Lx = aspect * Ly
+        t = float(np.exp(rng.uniform(np.log(0.002), np.log(0.05))))
+        E = float(np.exp(rng.uniform(np.log(1e9), np.log(3e11))))
+        nu = float(rng.uniform(0.2, 0.4))
+        rho = float(rng.uniform(1000.0, 8000.0))
+        curved = bool(rng.random() < 0.5)
+        R = float(rng.uniform(2.0, 10.0) * Lx)
+        edges = rng.integers(0, 2, 4)  # 0 clamped, 1 simply supported: x=0, x=Lx, y=0, y=Ly
+        xy, conn = fe.T3block(Lx, Ly, n, n)
+        hx, hy = Lx / n, Ly / n
+        tol = 1e-9
+        interior = (xy[:, 0] > tol) & (xy[:, 0] < Lx - tol) & (xy[:, 1] > tol) & (xy[:, 1] < Ly - tol)
+        xy = xy.copy()
+        xy[interior, 0] += rng.uniform(-0.1 * hx, 0.1 * hx, interior.sum())
+        xy[interior, 1] += rng.uniform(-0.1 * hy, 0.1 * hy, interior.sum())
+        if curved:
+            a = (xy[:, 0] - Lx / 2) / R
+            xyz = np.stack([R * np.sin(a), xy[:, 1], R * (np.cos(a) - 1)], axis=1)
+        else:
+            xyz = fe.xyz3(xy)
+        d = fe.DofField(xyz.shape[0])
+        sel = [xy[:, 0] < tol, xy[:, 0] > Lx - tol, xy[:, 1] < tol, xy[:, 1] > Ly - tol]
+        for s_, kind in zip(sel, edges):
+            d.setebc(np.nonzero(s_)[0], [1, 2, 3, 4, 5, 6] if kind == 0 else [1, 2, 3])
+        d.numberdofs()
+        models.append(dict(xyz=xyz, conn=conn, dchi=d, material=MatDeforElastIso(E, nu, rho), t=t,
+                           name=f"plate{k}_n{n}"))
+    return models

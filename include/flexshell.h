@@ -85,6 +85,11 @@ typedef struct fs_shell_params {
     const double *group_csys;      /* ngroups x 9: layup csys matrix (column-major) */
     const double *group_thickness; /* ngroups: thickness(layup) */
     const int32_t *elem_group;     /* nelem 0-based group ids, NULL = all group 0 */
+    /* Homogeneous shells with several materials (a batch of independent
+     * models): if not NULL, ngroups x 16 doubles per group -- Dps(9) Dt(4)
+     * (column-major), thickness, stab_c, drilling_stiffness_scale -- replace
+     * the single-material fields above for the elements of each group. */
+    const double *group_homog;
 } fs_shell_params;
 
 /*
@@ -100,6 +105,7 @@ typedef struct fs_mass_params {
     double thickness;            /* homogeneous thickness */
     const double *group_inertia; /* ngroups x 2: (mass density, moment of inertia density) */
     const int32_t *elem_group;   /* nelem, NULL = all group 0 */
+    const double *group_rho_t;   /* homogeneous groups: ngroups x 2 (rho, thickness), or NULL */
 } fs_mass_params;
 
 /*
@@ -114,6 +120,15 @@ typedef struct fs_cd_params {
     int64_t nsteps;
     int32_t precision;  /* FS_PREC_FP64 or FS_PREC_MIXED */
     int32_t probe_stride; /* record probes every probe_stride steps (>= 1) */
+    /* Batches of independent models stepped together: with ngroups > 0 each
+     * free row r belongs to model row_group[r], which has its own time step
+     * group_dt[g] and damping group_cdamp[g]; g then holds (nsteps+1) x
+     * ngroups samples (g[n * ngroups + model]) and dt/cdamp above are unused. */
+    int32_t ngroups;
+    int32_t use_graph;  /* 1: capture the step loop in a hipGraph and launch it once */
+    const int32_t *row_group;
+    const double *group_dt;
+    const double *group_cdamp;
 } fs_cd_params;
 
 /* ---------------------------------------------------------------------- */
