@@ -104,16 +104,132 @@ def bench_batch(nmodels, nsteps):
     E = int(sum(m["conn"].shape[0] for m in models))
     r = bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g, nsteps)
     same = all(np.array_equal(r["U"][bt.model_slice(k)], U_plain[k]) for k in keep)
+    bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g[:11], 10,
+                precision=_lib.FS_PREC_MIXED)  # builds the mixed operator
+    rm = bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g, nsteps,
+                     precision=_lib.FS_PREC_MIXED)
+    errs = [float(np.abs(rm["U"][bt.model_slice(k)] - r["U"][bt.model_slice(k)]).max()
+                  / np.abs(r["U"][bt.model_slice(k)]).max()) for k in range(nm)]
     bt.close()
     plain = nfree * nsteps / (plain_cd_ms / 1e3)
     batched = nfree * nsteps / (r["ms"] / 1e3)
+    batched_mixed = nfree * nsteps / (rm["ms"] / 1e3)
     return {"models": nm, "elements": E, "free_dofs": nfree, "steps": nsteps,
             "plain_dof_steps_per_s": plain, "batched_dof_steps_per_s": batched,
             "batched_over_plain": batched / plain,
+            "batched_mixed_dof_steps_per_s": batched_mixed, "batched_mixed_over_plain": batched_mixed / plain,
+            "batched_mixed_max_rel_err_final_U": max(errs),
             "plain_elements_per_s": E / (plain_asm_ms / 1e3), "batched_elements_per_s": E / (batch_asm_ms / 1e3),
             "bitwise_equal_to_plain": bool(same),
             "note": "plain = one model per launch sequence (FP64, no graph); batched = all models per launch, "
-                    "hipGraph step loop, FP64"}
+                    "hipGraph step loop, FP64; batched_mixed = the same with FP32 membrane+bending stiffness "
+                    "(FP64 shear, drilling, mass, state); error = max over models of max|dU| / max|U_fp64|"}
+
+
+def mixed_step_bytes(N, E, nen, kmb_pairs, nsr, nnb, fmag_frac):
+    """Algorithmic bytes of one mixed-precision step: FP32 K_mb slices, the
+    neighbour lists, FP64 shear factor rows, drilling blocks, incidence and
+    connectivity, and the FP64 state over 6 N positions (u gathered once;
+    U V A invMC read, Fmag read on the steps with a nonzero load factor
+    (fraction fmag_frac), U V A written)."""
+    n6 = 6 * N
+    return (kmb_pairs * 8 + nnb * 4 + (N + 1) * 4 + E * nsr * 6 * nen * 8 + N * 48 + E * nen * 4 * 2
+            + (N + 1) * 4 + n6 * (8 + 32 + 8 * fmag_frac + 24))
+
+
+def bench_mixed(nsteps, nx=1024, ny=512):
+    """Flat steel plate 2 x 1 m, t 0.004, T3 nx x ny, clamped, Gaussian
+    pressure pulse (sigma 0.1 m, 1e4 Pa peak, centre seeded, 1e-4 s): nsteps
+    central-difference steps at 0.9 of the stable dt in FP64 (CSR) and mixed
+    precision; trajectory error of the mixed run against the FP64 run (which
+    the tests pin to the oracle) over 64 probe dofs every 10 steps."""
+    import numpy as np
+
+    from flexshell import _lib, explicit, femm as FM, workloads as W
+    t0 = time.perf_counter()
+    ctx = _lib.default_context()
+    c = W.plate_c4(nx=nx, ny=ny)
+    d = c["dchi"]
+    x0 = c["x0"]
+    F = W.pressure_load_t3(c["xyz"], c["conn"], d,
+                           lambda X: 1e4 * np.exp(-np.sum((X - x0) ** 2, axis=1) / (2 * 0.1 ** 2)))
+    f = FM.FEMMShellT3FF(c["conn"], c["material"], thickness=c["t"], ctx=ctx)
+    f.associategeometry(c["xyz"])
+    nf = d.nfreedofs
+    f.stiffness(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), c["xyz"], None, None, d)
+    f.mass(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), c["xyz"], d)
+    dt = explicit.stable_dt(explicit.omega_max(f, np.random.default_rng(7).random(nf), maxit=30))
+    g = (explicit.sample_times(nsteps, dt) <= 1e-4).astype(float)
+    wd = d.dofnums[:, 2][d.dofnums[:, 2] < nf]
+    probes = wd[np.linspace(0, len(wd) - 1, 64).astype(int)]
+    z = np.zeros(nf)
+    print(f"[bench] mixed: setup {time.perf_counter() - t0:.1f}s, {nf} free dofs, dt {dt:.3e}", file=sys.stderr,
+          flush=True)
+    runs = {}
+    for name, prec in (("fp64", _lib.FS_PREC_FP64), ("mixed", _lib.FS_PREC_MIXED),
+                       ("mixed_f32", _lib.FS_PREC_MIXED_F32)):
+        explicit.run(f._model, 0.0, z, z, dt, 10, F, g[:11], precision=prec)  # warm (and build the operator)
+        r = explicit.run(f._model, 0.0, z, z, dt, nsteps, F, g, probes, 10, precision=prec)
+        runs[name] = r
+        print(f"[bench] mixed: {name} {r['ms'] / nsteps * 1e3:.1f} us/step", file=sys.stderr, flush=True)
+    ref = runs["fp64"]["probes"]
+    E, N = c["conn"].shape[0], c["xyz"].shape[0]
+    out = {"workload": f"plate_t3_{nx}x{ny} steel t0.004 clamped, Gaussian pressure pulse", "free_dofs": nf,
+           "steps": nsteps, "dt": dt}
+    for name, r in runs.items():
+        out[f"{name}_dof_steps_per_s"] = nf * nsteps / (r["ms"] / 1e3)
+        out[f"{name}_us_per_step"] = r["ms"] / nsteps * 1e3
+        if name != "fp64":
+            out[f"{name}_max_rel_traj_err"] = float(np.abs(r["probes"] - ref).max() / np.abs(ref).max())
+    out["mixed_over_fp64"] = out["mixed_dof_steps_per_s"] / out["fp64_dof_steps_per_s"]
+    kmb_pairs, nsr, nnb, _ = f._model.mixed_info()
+    b_mx = mixed_step_bytes(N, E, 3, kmb_pairs, nsr, nnb, float(np.mean(g[1:] != 0)))
+    ach = b_mx / (runs["mixed"]["ms"] / nsteps / 1e3) / 1e9
+    out["roofline_mixed"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": ach / HBM_PEAK_GBS, "kernel": "cd_mixed_step_kernel",
+                             "algorithmic_bytes": b_mx, "bytes_per_dof": b_mx / nf}
+    f._model.close()
+    return out
+
+
+def bench_serve(nreq, rate, slots, plain=True, steps=500):
+    """Served layered-shell workload (seed 6): Poisson arrivals at `rate`
+    requests/s of assembly / assembly + modal matrices / explicit requests on
+    T3FFComp or Q4RSComp plates of 1e3..5e5 elements (log-uniform), 3..8
+    orthotropic plies.  The request stream is replayed through the server
+    with `slots` concurrent slots (continuous batching, admission by HBM
+    footprint) and, for comparison, through one slot (the plain GPU path: one
+    request at a time in arrival order).  Meshes are generated before the
+    clock starts; latency is arrival -> result on the host."""
+    from flexshell import serve as S, workloads as W
+    t0 = time.perf_counter()
+    specs = W.served_c5(nreq=nreq, seed=6, rate=rate, explicit_steps=steps)
+    models = [W.c5_model(s) for s in specs]
+    print(f"[bench] serve: {nreq} requests generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def stream():
+        return [S.Request(s["rid"], s["kind"], m, steps=s["steps"], arrival=s["arrival"], seed=s["rid"])
+                for s, m in zip(specs, models)]
+
+    out = {"requests": nreq, "rate_req_s": rate, "explicit_steps": steps,
+           "elements_total": int(sum(m["conn"].shape[0] for m in models))}
+    runs = [("served", slots)] + ([("plain", 1)] if plain else [])
+    for name, ns in runs:
+        reqs = stream()
+        with S.ShellServer(nslots=ns) as srv:
+            # warm the kernels of both element families on every slot before the clock starts
+            warm = [S.Request(-1, "modal", W.c5_model(dict(specs[0], nel=1000.0, nen=nen, kind="modal")))
+                    for nen in (3, 4) for _ in range(ns)]
+            S.replay(srv, warm)
+            summ = S.replay(srv, reqs)
+        out[name] = summ
+        print(f"[bench] serve {name}: {summ}", file=sys.stderr, flush=True)
+        if summ["failed"]:
+            print("[bench] serve failures: " + "; ".join(r.error for r in reqs if r.error)[:2000], file=sys.stderr)
+    if plain:
+        out["p50_served_over_plain"] = out["served"]["p50_ms"] / out["plain"]["p50_ms"]
+        out["p99_served_over_plain"] = out["served"]["p99_ms"] / out["plain"]["p99_ms"]
+    return out
 
 
 def copy_probe(nbytes):
@@ -145,12 +261,21 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-models", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=5000)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-steps", type=int, default=500)
+    ap.add_argument("--no-serve-plain", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: 3 assemblies, 50 CD steps, then a 1 GiB torch "
                          "device copy that calibrates the byte counters")
     args = ap.parse_args()
     if args.profile:
         args.steps, args.warmup, args.cd_steps, args.no_cpu, args.no_batch = 3, 1, 50, True, True
+        args.no_serve = args.no_mixed = True
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(shard design only); run with --gpus 1", file=sys.stderr)
@@ -257,6 +382,13 @@ def main():
         t0 = time.perf_counter()
         out["batch"] = bench_batch(args.batch_models, args.batch_steps)
         print(f"[bench] batch section {time.perf_counter() - t0:.1f}s: {out['batch']}", file=sys.stderr, flush=True)
+    if not args.no_mixed:
+        t0 = time.perf_counter()
+        out["mixed"] = bench_mixed(args.mixed_steps)
+        print(f"[bench] mixed section {time.perf_counter() - t0:.1f}s: {out['mixed']}", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        out["serve"] = bench_serve(args.serve_requests, args.serve_rate, args.serve_slots,
+                                   plain=not args.no_serve_plain, steps=args.serve_steps)
     if args.profile:
         out["copy_probe_bytes"] = copy_probe(1 << 30)
     print(json.dumps(out), flush=True)

@@ -10,6 +10,8 @@
 //                                   row-major 6x6, global basis)
 //   pair tables                     see ScatterPlan
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace fs {
@@ -36,6 +38,43 @@ struct ElemParams {
     // homogeneous material per group (ngroups x 16): chol(Dps) 9 | chol(5/6 Dt) 4 |
     // thickness | stab_c | drilling scale
     const double *g_homog;
+    // Split output for the mixed-precision operator (both nullptr: full blocks).
+    // With split_cs set, Kb receives the membrane + bending part only, the
+    // transverse-shear rows of C (global basis, already scaled by the material
+    // factor, weight and stabilisation) go to split_cs[e][nsr][6 nen], and the
+    // drilling term of each element node goes to split_dr[e][a][9] (3 x 3
+    // rotation block, global basis, row-major; zeros where the reference adds
+    // nothing).  nsr = 2 (T3 AVERAGE_B), 6 (T3 AVERAGE_K), 8 (Q4: 2 per Gauss
+    // point, Gauss point major).
+    double *split_cs;
+    double *split_dr;
+};
+
+// Mixed-precision operator of one model for the explicit step, on node-major
+// dof positions R = 6 i + c (all dofs; fixed positions carry invMC = 0 and
+// stay at zero).
+//   kmb    membrane + bending stiffness, FP32, SELL layout with 64-row slices:
+//          slice g (rows 64g..64g+63) holds nsl_g neighbour slots; the value
+//          pair (2 c2, 2 c2 + 1) of slot s for row R sits at
+//          kmb[goff[g] + (3 s + c2) * 64 + (R & 63)] (float2 units).  Slot s of
+//          row R refers to the s-th neighbour of node i = R / 6 (nbr lists,
+//          sorted, including i); slots past that node's count are zero.
+//   cs     transverse-shear factor rows per element (FP64), K_s = sum C^T C
+//   dr     drilling 3 x 3 rotation block per node, FP64, symmetric storage
+//          (00, 01, 02, 11, 12, 22)
+struct MixedOp {
+    int64_t nrows;            // 6 N
+    int32_t nnodes;
+    int32_t nelem;
+    const int64_t *goff;      // nslices + 1
+    const float2 *kmb;
+    const int32_t *nbr_ptr;   // N + 1
+    const int32_t *nbr_node;
+    const double *dr;         // N x 6
+    const int32_t *ne_ptr;    // node -> incident elements (ascending)
+    const int32_t *ne_list;
+    const int32_t *conn;      // conn[a * E + e]
+    const double *cs;         // E x nsr x 6 nen
 };
 
 // Scatter plan of one block (ALL or FF) -- see scatter_kernel.

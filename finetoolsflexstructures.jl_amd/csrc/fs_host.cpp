@@ -44,6 +44,15 @@ hipError_t launch_spmv(int64_t n, const int32_t *rowptr, const int32_t *colidx, 
 hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, const double *d, double *part,
                       double *out, hipStream_t st);
 hipError_t launch_scale(int64_t n, const double *ss, const double *d, const double *x, double *y, hipStream_t st);
+hipError_t launch_mixed_build(const ScatterPlan &sp, const double *Kb, const int64_t *goff, float2 *kmb,
+                              hipStream_t st);
+hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, const int32_t *ne_list,
+                              const int32_t *conn, const double *dre, double *dr, hipStream_t st);
+hipError_t launch_cd_mixed_step(int nen, int nsr, bool f32prod, const MixedOp &op, const double *Ucur, double *Unext,
+                                double *V, double *A, const double *invMC, const double *Cd, const double *Fmag,
+                                double gstep, double dt, int damped, int forced, const int32_t *probe_slot,
+                                double *probe_dst, const int32_t *row_group, const double *group_dt,
+                                const double *gtab, hipStream_t st);
 }  // namespace fs
 
 struct fs_ctx {
@@ -96,6 +105,21 @@ struct Block {
 
 }  // namespace
 
+// Mixed-precision operator resident for the explicit loop (see fs::MixedOp).
+struct MixedStore {
+    bool built = false;
+    uint64_t kgen = 0;   // element-parameter generation it was built from
+    int nsr = 0;
+    int64_t nslices = 0;
+    int64_t kmb_pairs = 0;  // float2 values
+    DevBuf goff, kmb, cs, dr;
+    float build_ms = 0.f;
+    void reset() {
+        built = false;
+        for (DevBuf *b : {&goff, &kmb, &cs, &dr}) b->release();
+    }
+};
+
 struct fs_model {
     fs_ctx *ctx = nullptr;
     int nen = 0;
@@ -111,7 +135,12 @@ struct fs_model {
     Block blk[2];
     // group data for the last stiffness/mass call
     DevBuf g_Labd, g_Lh, g_csys, g_t, g_group, g_inertia, m_group, g_homog, g_rho_t;
-    double timing[4] = {0, 0, 0, 0};
+    double timing[5] = {0, 0, 0, 0, 0};
+    // parameters of the last element-stiffness launch (device pointers into
+    // the group buffers above) and their generation
+    fs::ElemParams last_ep{};
+    uint64_t kgen = 0;
+    MixedStore mixed;
 };
 
 namespace {
@@ -149,6 +178,15 @@ int upload(fs_ctx *c, DevBuf &b, const T *src, size_t n) {
     int s = dalloc(c, b, n * sizeof(T));
     if (s) return s;
     if (n) FS_HIP(c, hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return FS_OK;
+}
+
+// Device-to-host copy ordered on the context's own stream, so that contexts
+// sharing a device never serialise on the null stream.
+int d2h(fs_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return FS_OK;
+    FS_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
     return FS_OK;
 }
 
@@ -502,7 +540,86 @@ int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) return fail(c, FS_ERR_NUMERIC, "Singular metric matrix in _gradN_e! (degenerate quadrilateral)");
+    m->last_ep = ep;
+    ++m->kgen;
     return FS_OK;
+}
+
+// Build the mixed-precision operator from the parameters of the last
+// stiffness evaluation: split element kernel (membrane + bending blocks,
+// shear factor rows, drilling blocks), then the FP32 slices of K_mb and the
+// per-node drilling blocks.
+int build_mixed(fs_model *m) {
+    fs_ctx *c = m->ctx;
+    Block &B = m->blk[FS_BLOCK_FF];
+    MixedStore &X = m->mixed;
+    X.reset();
+    const int nen = m->nen;
+    const int nblk = nen * (nen + 1) / 2;
+    const int K = 6 * nen;
+    const int nsr = nen == 3 ? (m->last_ep.formulation == FS_SHEAR_AVERAGE_K ? 6 : 2) : 8;
+    const int64_t N = m->N, E = m->E, nrows = 6 * N;
+    // slices of 64 rows: slot count = largest neighbour count of the slice's nodes
+    const int64_t ns = (nrows + 63) / 64;
+    std::vector<int64_t> goff(ns + 1, 0);
+    for (int64_t g = 0; g < ns; ++g) {
+        const int64_t i0 = (64 * g) / 6, i1 = std::min<int64_t>(N - 1, (64 * g + 63) / 6);
+        int mx = 0;
+        for (int64_t i = i0; i <= i1; ++i) mx = std::max(mx, m->nbr_ptr[i + 1] - m->nbr_ptr[i]);
+        goff[g + 1] = goff[g] + (int64_t)mx * 3 * 64;
+    }
+    int s;
+    DevBuf kbm, dre;
+    if ((s = dalloc(c, kbm, (size_t)E * nblk * 36 * 8)) || (s = dalloc(c, dre, (size_t)E * nen * 9 * 8)) ||
+        (s = dalloc(c, X.cs, (size_t)E * nsr * K * 8)) || (s = dalloc(c, X.dr, (size_t)N * 6 * 8)) ||
+        (s = upload(c, X.goff, goff.data(), goff.size())) ||
+        (s = dalloc(c, X.kmb, (size_t)goff[ns] * sizeof(float2))))
+        return s;
+    fs::ElemParams ep = m->last_ep;
+    ep.split_cs = X.cs.as<double>();
+    ep.split_dr = dre.as<double>();
+    hipEvent_t e0, e1;
+    FS_HIP(c, hipEventCreate(&e0));
+    FS_HIP(c, hipEventCreate(&e1));
+    FS_HIP(c, hipEventRecord(e0, c->stream));
+    FS_HIP(c, hipMemsetAsync(X.kmb.p, 0, (size_t)goff[ns] * sizeof(float2), c->stream));
+    FS_HIP(c, fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
+                                        m->dconn.as<int32_t>(), m->nx.as<double>(), m->ny.as<double>(),
+                                        m->nz.as<double>(), m->valid.as<uint8_t>(), kbm.as<double>(),
+                                        m->status.as<int>(), c->stream));
+    FS_HIP(c, fs::launch_mixed_build(plan_of(B), kbm.as<double>(), X.goff.as<int64_t>(), X.kmb.as<float2>(),
+                                     c->stream));
+    FS_HIP(c, fs::launch_mixed_drill(nen, N, (int)E, m->dne_ptr.as<int32_t>(), m->dne_list.as<int32_t>(),
+                                     m->dconn.as<int32_t>(), dre.as<double>(), X.dr.as<double>(), c->stream));
+    FS_HIP(c, hipEventRecord(e1, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    FS_HIP(c, hipEventElapsedTime(&X.build_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    X.nsr = nsr;
+    X.nslices = ns;
+    X.kmb_pairs = goff[ns];
+    X.kgen = m->kgen;
+    X.built = true;
+    return FS_OK;
+}
+
+fs::MixedOp mixed_op(const fs_model *m) {
+    const Block &B = m->blk[FS_BLOCK_FF];
+    fs::MixedOp op;
+    op.nrows = 6 * m->N;
+    op.nnodes = (int32_t)m->N;
+    op.nelem = (int32_t)m->E;
+    op.goff = m->mixed.goff.as<int64_t>();
+    op.kmb = m->mixed.kmb.as<float2>();
+    op.nbr_ptr = B.nbr_ptr.as<int32_t>();
+    op.nbr_node = B.nbr_node.as<int32_t>();
+    op.dr = m->mixed.dr.as<double>();
+    op.ne_ptr = m->dne_ptr.as<int32_t>();
+    op.ne_list = m->dne_list.as<int32_t>();
+    op.conn = m->dconn.as<int32_t>();
+    op.cs = m->mixed.cs.as<double>();
+    return op;
 }
 
 }  // namespace
@@ -520,7 +637,7 @@ int fs_ctx_create(int device, fs_ctx **out) {
     fs_ctx *c = new (std::nothrow) fs_ctx;
     if (!c) return FS_ERR_NOMEM;
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return FS_ERR_DEVICE;
     }
@@ -654,6 +771,7 @@ int fs_model_set_dofs(fs_model *m, const int64_t *dofnums, int64_t nalldofs, int
     m->nfree = nfreedofs;
     m->blk[0].reset();
     m->blk[1].reset();
+    m->mixed.reset();
     int s = upload(c, m->node_dof, m->dof.data(), m->dof.size());
     if (s) return s;
     FS_HIP(c, hipStreamSynchronize(c->stream));
@@ -734,7 +852,7 @@ int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *block
     m->timing[0] = ms;
     if (blocks_out) {
         const int nblk = m->nen * (m->nen + 1) / 2;
-        FS_HIP(c, hipMemcpy(blocks_out, m->Kb.p, (size_t)m->E * nblk * 36 * 8, hipMemcpyDeviceToHost));
+        return d2h(c, blocks_out, m->Kb.p, (size_t)m->E * nblk * 36 * 8);
     }
     return FS_OK;
 }
@@ -765,7 +883,7 @@ int fs_model_stiffness(fs_model *m, const fs_shell_params *p, int block, double 
     m->timing[1] = ms_sc;
     m->timing[2] = ms_elem + ms_sc;
     B.k_ready = true;
-    if (nzval_out) FS_HIP(c, hipMemcpy(nzval_out, B.kval.p, (size_t)B.nnz * 8, hipMemcpyDeviceToHost));
+    if (nzval_out) return d2h(c, nzval_out, B.kval.p, (size_t)B.nnz * 8);
     return FS_OK;
 }
 
@@ -831,7 +949,9 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     Block &B = m->blk[FS_BLOCK_FF];
     if (!B.k_ready || !B.m_ready)
         return fail(c, FS_ERR_STATE, "explicit: FF-block stiffness and mass must be computed first");
-    if (p->precision != FS_PREC_FP64) return fail(c, FS_ERR_UNSUPPORTED, "explicit: precision %d", p->precision);
+    if (p->precision != FS_PREC_FP64 && p->precision != FS_PREC_MIXED && p->precision != FS_PREC_MIXED_F32)
+        return fail(c, FS_ERR_UNSUPPORTED, "explicit: precision %d", p->precision);
+    const bool mixed = p->precision != FS_PREC_FP64;
     const bool batch = p->ngroups > 0;
     if (!U0 || !V0 || !Fmag || !g || p->nsteps < 0 || (!batch && !(p->dt > 0.0)))
         return fail(c, FS_ERR_ARG, "explicit: bad arguments");
@@ -850,52 +970,94 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
             if (!(p->group_dt[gi] > 0.0)) return fail(c, FS_ERR_ARG, "explicit: group_dt must be positive");
     }
     FS_HIP(c, hipSetDevice(c->device));
-    std::vector<double> M(n), Cd(n), invMC(n);
-    FS_HIP(c, hipMemcpy(M.data(), B.mdiag.p, n * 8, hipMemcpyDeviceToHost));
+    int s;
+    double build_ms = 0.0;
+    if (mixed) {
+        if (m->N > (int64_t)INT32_MAX / 6) return fail(c, FS_ERR_UNSUPPORTED, "explicit: model too large for mixed");
+        if (!m->mixed.built || m->mixed.kgen != m->kgen) {
+            if ((s = build_mixed(m))) return s;
+            build_ms = m->mixed.build_ms;
+        }
+    }
+    // State vectors: the free dofs themselves (FP64 CSR path), or node-major
+    // dof positions 6 i + c (mixed path; fixed positions stay zero).
+    const int64_t nv = mixed ? 6 * m->N : n;
+    std::vector<int64_t> pos(n);
+    if (mixed) {
+        for (int64_t I = 0; I < m->N; ++I)
+            for (int cc = 0; cc < 6; ++cc) {
+                const int64_t d = m->dof[I * 6 + cc];
+                if (d < n) pos[d] = 6 * I + cc;
+            }
+    } else {
+        for (int64_t d = 0; d < n; ++d) pos[d] = d;
+    }
+    std::vector<double> M(n), U0v(nv, 0.0), V0v(nv, 0.0), Fv(nv, 0.0), Cd(nv, 0.0), invMC(nv, 0.0);
+    std::vector<int32_t> rgv(batch ? nv : 0, 0);
+    if ((s = d2h(c, M.data(), B.mdiag.p, n * 8))) return s;
     for (int64_t i = 0; i < n; ++i) {
         const double dt = batch ? p->group_dt[p->row_group[i]] : p->dt;
         const double cd = batch ? p->group_cdamp[p->row_group[i]] : p->cdamp;
-        Cd[i] = cd * M[i];
-        invMC[i] = 1.0 / (M[i] + (dt / 2) * Cd[i]);  // plate_expl_examples.jl:68-69
+        const int64_t q = pos[i];
+        Cd[q] = cd * M[i];
+        invMC[q] = 1.0 / (M[i] + (dt / 2) * Cd[q]);  // plate_expl_examples.jl:68-69
+        U0v[q] = U0[i];
+        V0v[q] = V0[i];
+        Fv[q] = Fmag[i];
+        if (batch) rgv[q] = p->row_group[i];
     }
     bool damped = false;
     for (int gi = 0; gi < ng; ++gi) damped |= (batch ? p->group_cdamp[gi] : p->cdamp) != 0.0;
     DevBuf dU[2], dV, dA, dF, dInv, dC, dSlot, dProbe, dRG, dGdt, dG;
-    int s;
-    if ((s = upload(c, dU[0], U0, n)) || (s = dalloc(c, dU[1], n * 8)) || (s = upload(c, dV, V0, n)) ||
-        (s = dalloc(c, dA, n * 8)) || (s = upload(c, dF, Fmag, n)) || (s = upload(c, dInv, invMC.data(), n)) ||
-        (s = upload(c, dC, Cd.data(), n)))
+    if ((s = upload(c, dU[0], U0v.data(), nv)) || (s = dalloc(c, dU[1], nv * 8)) || (s = upload(c, dV, V0v.data(), nv)) ||
+        (s = dalloc(c, dA, nv * 8)) || (s = upload(c, dF, Fv.data(), nv)) || (s = upload(c, dInv, invMC.data(), nv)) ||
+        (s = upload(c, dC, Cd.data(), nv)))
         return s;
     if (batch) {
-        if ((s = upload(c, dRG, p->row_group, n)) || (s = upload(c, dGdt, p->group_dt, ng)) ||
+        if ((s = upload(c, dRG, rgv.data(), nv)) || (s = upload(c, dGdt, p->group_dt, ng)) ||
             (s = upload(c, dG, g, (size_t)(p->nsteps + 1) * ng)))
             return s;
     }
     const int64_t nrec = p->nsteps / stride + 1;
     if (nprobes > 0) {
-        std::vector<int32_t> slot(n, -1);
-        for (int k = 0; k < nprobes; ++k) slot[probe_dofs[k]] = k;
-        if ((s = upload(c, dSlot, slot.data(), n)) || (s = dalloc(c, dProbe, (size_t)nrec * nprobes * 8))) return s;
+        std::vector<int32_t> slot(nv, -1);
+        for (int k = 0; k < nprobes; ++k) slot[pos[probe_dofs[k]]] = k;
+        if ((s = upload(c, dSlot, slot.data(), nv)) || (s = dalloc(c, dProbe, (size_t)nrec * nprobes * 8))) return s;
         for (int k = 0; k < nprobes; ++k) probe_out[k] = U0[probe_dofs[k]];  // peek(0, U0)
     }
+    // steps whose load factor is zero for every model skip the Fmag read (mixed path)
+    std::vector<uint8_t> forced(p->nsteps + 1, 0);
+    for (int64_t step = 0; step <= p->nsteps; ++step)
+        for (int gi = 0; gi < ng && !forced[step]; ++gi) forced[step] = g[step * ng + gi] != 0.0;
     // rows of the shell stencils hold ~40-60 entries: 8 lanes per row
     const int tpr = 8;
     const int32_t *rg = batch ? dRG.as<int32_t>() : nullptr;
     const double *gdt = batch ? dGdt.as<double>() : nullptr;
+    const fs::MixedOp op = mixed ? mixed_op(m) : fs::MixedOp{};
+    const bool f32prod = p->precision == FS_PREC_MIXED_F32;
     auto enqueue = [&](hipStream_t st) -> hipError_t {
-        hipError_t e = fs::launch_cd_init(n, p->dt, batch ? 0.0 : g[0], dInv.as<double>(), dF.as<double>(),
+        hipError_t e = fs::launch_cd_init(nv, p->dt, batch ? 0.0 : g[0], dInv.as<double>(), dF.as<double>(),
                                           dU[0].as<double>(), dV.as<double>(), dA.as<double>(), dU[1].as<double>(),
                                           rg, gdt, batch ? dG.as<double>() : nullptr, st);
         if (e != hipSuccess) return e;
         int cur = 1;
         for (int64_t step = 1; step <= p->nsteps; ++step) {
             const bool rec = nprobes > 0 && (step % stride == 0);
-            e = fs::launch_cd_step(tpr, n, B.rowptr32.as<int32_t>(), B.colidx32.as<int32_t>(), B.kval.as<double>(),
-                                   dU[cur].as<double>(), dU[1 - cur].as<double>(), dV.as<double>(), dA.as<double>(),
-                                   dInv.as<double>(), dC.as<double>(), dF.as<double>(), batch ? 0.0 : g[step],
-                                   p->dt, damped ? 1 : 0, rec ? dSlot.as<int32_t>() : nullptr,
-                                   rec ? dProbe.as<double>() + (step / stride) * nprobes : nullptr, rg, gdt,
-                                   batch ? dG.as<double>() + step * ng : nullptr, st);
+            int32_t *slot = rec ? dSlot.as<int32_t>() : nullptr;
+            double *pdst = rec ? dProbe.as<double>() + (step / stride) * nprobes : nullptr;
+            const double *gt = batch ? dG.as<double>() + step * ng : nullptr;
+            if (mixed)
+                e = fs::launch_cd_mixed_step(m->nen, m->mixed.nsr, f32prod, op, dU[cur].as<double>(),
+                                             dU[1 - cur].as<double>(), dV.as<double>(), dA.as<double>(),
+                                             dInv.as<double>(), dC.as<double>(), dF.as<double>(),
+                                             batch ? 0.0 : g[step], p->dt, damped ? 1 : 0, forced[step] ? 1 : 0,
+                                             slot, pdst, rg, gdt, gt, st);
+            else
+                e = fs::launch_cd_step(tpr, n, B.rowptr32.as<int32_t>(), B.colidx32.as<int32_t>(),
+                                       B.kval.as<double>(), dU[cur].as<double>(), dU[1 - cur].as<double>(),
+                                       dV.as<double>(), dA.as<double>(), dInv.as<double>(), dC.as<double>(),
+                                       dF.as<double>(), batch ? 0.0 : g[step], p->dt, damped ? 1 : 0, slot, pdst, rg,
+                                       gdt, gt, st);
             if (e != hipSuccess) return e;
             cur = 1 - cur;
         }
@@ -933,13 +1095,21 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     m->timing[3] = ms;
+    m->timing[4] = build_ms;
     // after the last step the buffer not indexed by the loop's final `cur` holds U_nsteps
     const int fin = (p->nsteps % 2 == 1) ? 1 : 0;
-    if (U_out) FS_HIP(c, hipMemcpy(U_out, dU[p->nsteps == 0 ? 0 : fin].p, n * 8, hipMemcpyDeviceToHost));
-    if (V_out) FS_HIP(c, hipMemcpy(V_out, dV.p, n * 8, hipMemcpyDeviceToHost));
-    if (nprobes > 0 && nrec > 1)
-        FS_HIP(c, hipMemcpy(probe_out + nprobes, dProbe.as<double>() + nprobes, (size_t)(nrec - 1) * nprobes * 8,
-                            hipMemcpyDeviceToHost));
+    if (U_out || V_out) {
+        std::vector<double> Uv(nv), Vv(nv);
+        if ((s = d2h(c, Uv.data(), dU[p->nsteps == 0 ? 0 : fin].p, nv * 8)) || (s = d2h(c, Vv.data(), dV.p, nv * 8)))
+            return s;
+        for (int64_t i = 0; i < n; ++i) {
+            if (U_out) U_out[i] = Uv[pos[i]];
+            if (V_out) V_out[i] = Vv[pos[i]];
+        }
+    }
+    if (nprobes > 0 && nrec > 1 &&
+        (s = d2h(c, probe_out + nprobes, dProbe.as<double>() + nprobes, (size_t)(nrec - 1) * nprobes * 8)))
+        return s;
     return FS_OK;
 }
 
@@ -953,10 +1123,10 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
     FS_HIP(c, hipSetDevice(c->device));
     const int64_t n = B.nrows;
     std::vector<double> M(n), invM(n);
-    FS_HIP(c, hipMemcpy(M.data(), B.mdiag.p, n * 8, hipMemcpyDeviceToHost));
+    int s;
+    if ((s = d2h(c, M.data(), B.mdiag.p, n * 8))) return s;
     for (int64_t i = 0; i < n; ++i) invM[i] = 1.0 / M[i];  // clamp_cyl_transient_examples.jl:47
     DevBuf dv, dw, dinv, dpart, dss;
-    int s;
     const double one[4] = {1.0, 1.0, 1.0, 1.0};
     if ((s = upload(c, dv, v0, n)) || (s = dalloc(c, dw, n * 8)) || (s = upload(c, dinv, invM.data(), n)) ||
         (s = dalloc(c, dpart, 512 * 8)) || (s = upload(c, dss, one, 4)))
@@ -982,8 +1152,18 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
     return FS_OK;
 }
 
+int fs_model_mixed_info(const fs_model *m, int64_t *info) {
+    if (!m || !info) return FS_ERR_ARG;
+    const MixedStore &X = m->mixed;
+    info[0] = X.built ? X.kmb_pairs : 0;
+    info[1] = X.built ? X.nsr : 0;
+    info[2] = (int64_t)m->nbr_list.size();
+    info[3] = X.built ? X.nslices : 0;
+    return FS_OK;
+}
+
 int fs_model_last_timing(const fs_model *m, int which, double *ms) {
-    if (!m || !ms || which < 0 || which > 3) return FS_ERR_ARG;
+    if (!m || !ms || which < 0 || which > 4) return FS_ERR_ARG;
     *ms = m->timing[which];
     return FS_OK;
 }

@@ -569,12 +569,14 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
             }
     }
     const bool cact = cthread && (le3 < ne);
+    const bool split = p.split_cs != nullptr;
     double acc[6][6];
     for (int r = 0; r < 6; ++r)
         for (int c = 0; c < 6; ++c) acc[r][c] = 0.0;
     if (cact) {
         const double *cb = sC + le3 * T::CSTR;
         for (int s = 0; s < T::NS; ++s) {
+            if (split && (s % NR) >= 6) continue;  // shear rows leave as factors
             double va[6], vb[6];
             for (int r = 0; r < 6; ++r) {
                 va[r] = cb[s * T::K + 6 * ba + r];
@@ -592,8 +594,12 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
             const double dscale = p.composite ? p.drill_scale : p.g_homog[16 * grp + 15];
             const double kavg = (((((ms[T::M_KP + 0] + ms[T::M_KP + 2]) + ms[T::M_KP + 4]) + ms[T::M_KP + 1]) +
                                    ms[T::M_KP + 3]) + ms[T::M_KP + 5]) / 6 * dscale;
-            if (valid[node]) {
-                const double *gv = ms + T::M_G + 3 * ba;
+            const double *gv = ms + T::M_G + 3 * ba;
+            if (split) {
+                double *dd = p.split_dr + ((size_t)e * NEN + ba) * 9;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) dd[r * 3 + c] = valid[node] ? kavg * (gv[r] * gv[c]) : 0.0;
+            } else if (valid[node]) {
                 for (int r = 0; r < 3; ++r)
                     for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (gv[r] * gv[c]);
             }
@@ -645,6 +651,16 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
                         for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (nv[r] * nv[c]);
                 }
             }
+        }
+    }
+    if (split) {
+        // shear rows of C (global basis) of every element of the block
+        constexpr int NSR = NQP * T::NSH;
+        for (int i = tid; i < ne * NSR * T::K; i += WG) {
+            const int le = i / (NSR * T::K), rem = i % (NSR * T::K);
+            const int row = rem / T::K, col = rem % T::K;
+            const int srow = (row / T::NSH) * NR + 6 + row % T::NSH;
+            p.split_cs[(size_t)(e0 + le) * NSR * T::K + rem] = sC[le * T::CSTR + srow * T::K + col];
         }
     }
     __syncthreads();  // every read of sC is done: reuse it as the output stage
@@ -862,9 +878,13 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
                 ++cnt;
             }
     }
+    const bool split = p.split_cs != nullptr;
     double acc[6][6];
+    double accs[3][3];  // split: shear part of the rotation block (drilling penalty input)
     for (int r = 0; r < 6; ++r)
         for (int c = 0; c < 6; ++c) acc[r][c] = 0.0;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) accs[r][c] = 0.0;
     __syncthreads();
 
     for (int q = 0; q < NQP; ++q) {
@@ -988,8 +1008,22 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
                     va[r] = cbp[s * K + 6 * ba + r];
                     vb[r] = cbp[s * K + 6 * bb + r];
                 }
+                if (split && s >= 6) {
+                    if (ba == bb)
+                        for (int r = 0; r < 3; ++r)
+                            for (int c = 0; c < 3; ++c) accs[r][c] += va[3 + r] * vb[3 + c];
+                    continue;
+                }
                 for (int r = 0; r < 6; ++r)
                     for (int c = 0; c < 6; ++c) acc[r][c] += va[r] * vb[c];
+            }
+        }
+        if (split) {
+            // shear rows of this Gauss point (global basis): factor rows 2q, 2q+1
+            for (int i = tid; i < ne * 2 * K; i += WG) {
+                const int le = i / (2 * K), rem = i % (2 * K);
+                const int rr = rem / K, col = rem % K;
+                p.split_cs[(size_t)(e0 + le) * 8 * K + (2 * q + rr) * K + col] = sC[le * CSTR + (6 + rr) * K + col];
             }
         }
     }
@@ -997,11 +1031,15 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
     if (cact && ba == bb) {
         double *ms = sM + le3 * MISC + 9 * ba;
         for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) ms[r * 3 + c] = acc[3 + r][3 + c];
+            for (int c = 0; c < 3; ++c) ms[r * 3 + c] = acc[3 + r][3 + c] + accs[r][c];
     }
     __syncthreads();
     const int grpd = cact ? (p.elem_group ? p.elem_group[e0 + le3] : 0) : 0;
     const double dscale = p.composite ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
+    if (split && cact && ba == bb) {
+        double *dd = p.split_dr + ((size_t)(e0 + le3) * NEN + ba) * 9;
+        for (int i = 0; i < 9; ++i) dd[i] = 0.0;
+    }
     if (cact && ba == bb && dscale != 0.0) {
         const int e = e0 + le3;
         const double *kr = sM + le3 * MISC;
@@ -1033,8 +1071,14 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
             const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
             if (sqrt(n0 * n0 + n1 * n1 + n2 * n2) != 0.0) {
                 const double nv[3] = {n0, n1, n2};
-                for (int r = 0; r < 3; ++r)
-                    for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (nv[r] * nv[c]);
+                if (split) {
+                    double *dd = p.split_dr + ((size_t)e * NEN + ba) * 9;
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) dd[r * 3 + c] = kavg * (nv[r] * nv[c]);
+                } else {
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (nv[r] * nv[c]);
+                }
             }
         }
     }
@@ -1379,6 +1423,178 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
     }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision explicit step (FP32 membrane + bending, FP64 elsewhere)
+// ---------------------------------------------------------------------------
+//
+// K = K_mb + K_s + K_d.  K_mb (membrane, bending and their coupling) is held
+// in FP32 in node-block SELL slices (MixedOp); the transverse-shear part
+// K_s = sum_e C_e^T C_e stays as FP64 element factor rows C_e (2 rows per T3
+// element, 8 per Q4) and the drilling part K_d as an FP64 3 x 3 rotation
+// block per node.  The row of dof position R = 6 i + r is
+//   (K u)_R = sum_slots K_mb[R, slot] . u_j  +  sum_{e ∋ i} C_e[:, (a, r)]' (C_e u_e)
+//             + [r >= 3] D_i[r-3, :] . theta_i
+// with the incident elements visited in ascending order, so the result is
+// deterministic.  K_mb values are the FP64 element-order sums of the element
+// blocks, rounded once to FP32.
+
+// K_mb slices from the split element blocks: thread per (node, row, neighbour
+// slot); the sum over the pair's contributions follows scatter_kernel.
+__global__ __launch_bounds__(256) void mixed_build_kernel(ScatterPlan sp, const double *__restrict__ Kb,
+                                                          const int64_t *__restrict__ goff, float2 *__restrict__ kmb) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t I = gid / (6 * FS_SC_NB);
+    if (I >= sp.nnodes) return;
+    const int r = (int)((gid / FS_SC_NB) % 6);
+    const int kk = (int)(gid % FS_SC_NB);
+    const int64_t R = 6 * I + r;
+    float2 *dst = kmb + goff[R >> 6] + (R & 63);
+    const int nb0 = sp.nbr_ptr[I], nb1 = sp.nbr_ptr[I + 1];
+    for (int k = nb0 + kk; k < nb1; k += FS_SC_NB) {
+        const int pr = sp.nbr_pair[k];
+        const int p = pr >> 1;
+        const bool side1 = (pr & 1) != 0;
+        double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const int ce = sp.contrib_ptr[p + 1];
+        for (int q = sp.contrib_ptr[p]; q < ce; ++q) {
+            const int cv = sp.contrib[q];
+            const double *B = Kb + (int64_t)(cv >> 1) * 36;
+            if (side1 == ((cv & 1) != 0)) {
+                for (int c = 0; c < 6; ++c) sum[c] += B[r * 6 + c];
+            } else {
+                for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
+            }
+        }
+        const int slot = k - nb0;
+        for (int c2 = 0; c2 < 3; ++c2)
+            dst[(3 * slot + c2) * 64] = make_float2((float)sum[2 * c2], (float)sum[2 * c2 + 1]);
+    }
+}
+
+// Drilling block per node: sum of the element-node blocks in element order.
+template <int NEN>
+__global__ __launch_bounds__(256) void mixed_drill_kernel(int64_t N, int E, const int32_t *__restrict__ ne_ptr,
+                                                          const int32_t *__restrict__ ne_list,
+                                                          const int32_t *__restrict__ conn,
+                                                          const double *__restrict__ dre, double *__restrict__ dr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double d[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = ne_ptr[i]; k < ne_ptr[i + 1]; ++k) {
+        const int e = ne_list[k];
+        int a = 0;
+        for (int b = 0; b < NEN; ++b)
+            if (conn[b * E + e] == (int32_t)i) a = b;
+        const double *src = dre + ((int64_t)e * NEN + a) * 9;
+        for (int t = 0; t < 9; ++t) d[t] += src[t];
+    }
+    double *o = dr + i * 6;
+    o[0] = d[0];
+    o[1] = d[1];
+    o[2] = d[2];
+    o[3] = d[4];
+    o[4] = d[5];
+    o[5] = d[8];
+}
+
+// One fused central-difference step on the mixed operator: thread per dof
+// position; the update is the one of cd_step_kernel.  F32PROD: the K_mb
+// products and their sum are formed in FP32 (u rounded to FP32); otherwise
+// the FP32 values are widened and accumulated in FP64.
+template <int NEN, int NSR, bool F32PROD>
+__global__ __launch_bounds__(256) void cd_mixed_step_kernel(
+    MixedOp op, const double *__restrict__ Ucur, double *__restrict__ Unext, double *__restrict__ V,
+    double *__restrict__ A, const double *__restrict__ invMC, const double *__restrict__ Cd,
+    const double *__restrict__ Fmag, double gstep, double dt, int damped, int forced,
+    const int32_t *__restrict__ probe_slot, double *__restrict__ probe_dst, const int32_t *__restrict__ row_group,
+    const double *__restrict__ group_dt, const double *__restrict__ gtab) {
+    constexpr int K = 6 * NEN;
+    const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (R >= op.nrows) return;
+    const int i = (int)(R / 6);
+    const int r = (int)(R - 6 * (int64_t)i);
+    const int64_t g0 = op.goff[R >> 6];
+    const int nsl = (int)((op.goff[(R >> 6) + 1] - g0) / 192);
+    const float2 *kv = op.kmb + g0 + (R & 63);
+    const int nb0 = op.nbr_ptr[i];
+    const int cnt = op.nbr_ptr[i + 1] - nb0;
+    double e = 0.0;
+    float ef = 0.0f;
+    for (int s = 0; s < nsl && s < cnt; ++s) {
+        const float2 k0 = kv[(3 * s + 0) * 64];
+        const float2 k1 = kv[(3 * s + 1) * 64];
+        const float2 k2 = kv[(3 * s + 2) * 64];
+        const double *uj = Ucur + 6 * (int64_t)op.nbr_node[nb0 + s];
+        if constexpr (F32PROD) {
+            ef += k0.x * (float)uj[0] + k0.y * (float)uj[1] + k1.x * (float)uj[2] + k1.y * (float)uj[3] +
+                  k2.x * (float)uj[4] + k2.y * (float)uj[5];
+        } else {
+            e += (double)k0.x * uj[0] + (double)k0.y * uj[1] + (double)k1.x * uj[2] + (double)k1.y * uj[3] +
+                 (double)k2.x * uj[4] + (double)k2.y * uj[5];
+        }
+    }
+    if constexpr (F32PROD) e = (double)ef;
+    // drilling (FP64)
+    if (r >= 3) {
+        const double *d = op.dr + (int64_t)i * 6;
+        const double *th = Ucur + 6 * (int64_t)i + 3;
+        const int rr = r - 3;
+        const double d0 = rr == 0 ? d[0] : (rr == 1 ? d[1] : d[2]);
+        const double d1 = rr == 0 ? d[1] : (rr == 1 ? d[3] : d[4]);
+        const double d2 = rr == 0 ? d[2] : (rr == 1 ? d[4] : d[5]);
+        e += d0 * th[0] + d1 * th[1] + d2 * th[2];
+    }
+    // transverse shear (FP64), incident elements in ascending order
+    const int E = op.nelem;
+    for (int k = op.ne_ptr[i]; k < op.ne_ptr[i + 1]; ++k) {
+        const int el = op.ne_list[k];
+        int nodes[NEN];
+        int a = 0;
+        for (int b = 0; b < NEN; ++b) {
+            nodes[b] = op.conn[b * E + el];
+            if (nodes[b] == i) a = b;
+        }
+        const double *C = op.cs + (int64_t)el * NSR * K;
+        double sv[NSR];
+        for (int q = 0; q < NSR; ++q) sv[q] = 0.0;
+        for (int b = 0; b < NEN; ++b) {
+            const double *ub = Ucur + 6 * (int64_t)nodes[b];
+            double u6[6];
+            for (int c = 0; c < 6; ++c) u6[c] = ub[c];
+            for (int q = 0; q < NSR; ++q) {
+                const double *cq = C + q * K + 6 * b;
+                sv[q] += cq[0] * u6[0] + cq[1] * u6[1] + cq[2] * u6[2] + cq[3] * u6[3] + cq[4] * u6[4] +
+                         cq[5] * u6[5];
+            }
+        }
+        for (int q = 0; q < NSR; ++q) e += C[q * K + 6 * a + r] * sv[q];
+    }
+    double dtr = dt, gr = gstep;
+    if (row_group) {
+        const int gi = row_group[R];
+        dtr = group_dt[gi];
+        gr = gtab[gi];
+    }
+    const double dt_2 = (dtr / 2), dt2_2 = ((dtr * dtr) / 2);
+    const double v = V[R], a0 = A[R];
+    double f = forced ? gr * Fmag[R] : 0.0;
+    if (damped)
+        f -= e + Cd[R] * (v + dt_2 * a0);
+    else
+        f -= e;
+    double vn = v + dt_2 * a0;
+    const double an = invMC[R] * f;
+    vn += dt_2 * an;
+    const double u = Ucur[R];
+    V[R] = vn;
+    A[R] = an;
+    Unext[R] = u + (dtr * vn + dt2_2 * an);
+    if (probe_slot) {
+        const int ps = probe_slot[R];
+        if (ps >= 0) probe_dst[ps] = u;
+    }
+}
+
 // probes: out[k] = U[probe[k]]
 __global__ void gather_kernel(int n, const int64_t *__restrict__ idx, const double *__restrict__ U,
                               double *__restrict__ out) {
@@ -1476,12 +1692,12 @@ static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const doub
     constexpr int total = NE * 4 * 47 + NE * (8 * 24 + 2) + NE * 4 * 14 + NE * 37;
     const size_t smem = total * sizeof(double);
     static_assert(NE * 10 <= WG, "workgroup too small");
-    static bool attr = false;
-    if (!attr && smem > 65536) {
+    // set on every launch: cheap, per device, and free of shared host state
+    // when several contexts launch from different threads
+    if (smem > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
-        attr = true;
     }
     const int nblocks = (p.nelem + NE - 1) / NE;
     if (nblocks == 0) return hipSuccess;
@@ -1602,6 +1818,50 @@ hipError_t launch_scale(int64_t n, const double *ss, const double *d, const doub
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, ss, d, x, y);
     return hipGetLastError();
+}
+
+hipError_t launch_mixed_build(const ScatterPlan &sp, const double *Kb, const int64_t *goff, float2 *kmb,
+                              hipStream_t st) {
+    const int64_t threads = sp.nnodes * 6 * FS_SC_NB;
+    if (threads == 0) return hipSuccess;
+    hipLaunchKernelGGL(mixed_build_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sp, Kb, goff,
+                       kmb);
+    return hipGetLastError();
+}
+
+hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, const int32_t *ne_list,
+                              const int32_t *conn, const double *dre, double *dr, hipStream_t st) {
+    if (N == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((N + 255) / 256);
+    if (nen == 3)
+        hipLaunchKernelGGL(mixed_drill_kernel<3>, dim3(blocks), dim3(256), 0, st, N, E, ne_ptr, ne_list, conn, dre, dr);
+    else
+        hipLaunchKernelGGL(mixed_drill_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, ne_ptr, ne_list, conn, dre, dr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cd_mixed_step(int nen, int nsr, bool f32prod, const MixedOp &op, const double *Ucur, double *Unext,
+                                double *V, double *A, const double *invMC, const double *Cd, const double *Fmag,
+                                double gstep, double dt, int damped, int forced, const int32_t *probe_slot,
+                                double *probe_dst, const int32_t *row_group, const double *group_dt,
+                                const double *gtab, hipStream_t st) {
+    if (op.nrows == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((op.nrows + 255) / 256);
+#define FS_MX_CASE(NEN_, NSR_, F32_)                                                                             \
+    if (nen == NEN_ && nsr == NSR_ && f32prod == F32_) {                                                        \
+        hipLaunchKernelGGL((cd_mixed_step_kernel<NEN_, NSR_, F32_>), dim3(blocks), dim3(256), 0, st, op, Ucur,   \
+                           Unext, V, A, invMC, Cd, Fmag, gstep, dt, damped, forced, probe_slot, probe_dst,       \
+                           row_group, group_dt, gtab);                                                          \
+        return hipGetLastError();                                                                               \
+    }
+    FS_MX_CASE(3, 2, false)
+    FS_MX_CASE(3, 2, true)
+    FS_MX_CASE(3, 6, false)
+    FS_MX_CASE(3, 6, true)
+    FS_MX_CASE(4, 8, false)
+    FS_MX_CASE(4, 8, true)
+#undef FS_MX_CASE
+    return hipErrorInvalidValue;
 }
 
 }  // namespace fs

@@ -18,6 +18,7 @@ FS_SHEAR_AVERAGE_B = 0
 FS_SHEAR_AVERAGE_K = 1
 FS_PREC_FP64 = 0
 FS_PREC_MIXED = 1
+FS_PREC_MIXED_F32 = 2
 
 _STATUS = {
     -1: "invalid argument",
@@ -103,6 +104,7 @@ SIGNATURES = {
                                     _dp, _dp, _dp]),
     "fs_model_omega_max": (C.c_int, [C.c_void_p, _dp, C.c_int32, _dp]),
     "fs_model_last_timing": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "fs_model_mixed_info": (C.c_int, [C.c_void_p, _i64p]),
 }
 
 _lib = None
@@ -230,8 +232,9 @@ class Model:
                                                          ptr(cv, C.c_int64), 0), "fs_model_pattern_get")
         return nr.value, rp, cv
 
-    def stiffness(self, params, block, nnz):
-        out = np.zeros(nnz)
+    def stiffness(self, params, block, nnz=None):
+        """CSR values of the block; nnz None keeps them on the device only."""
+        out = None if nnz is None else np.zeros(nnz)
         self.ctx.check(self.ctx.lib.fs_model_stiffness(self.h, C.byref(params), int(block), ptr(out)),
                        "fs_model_stiffness")
         return out
@@ -243,8 +246,9 @@ class Model:
                        "fs_model_element_blocks")
         return out
 
-    def mass(self, params, block, nrows):
-        out = np.zeros(nrows)
+    def mass(self, params, block, nrows=None):
+        """Lumped-mass diagonal of the block; nrows None keeps it on the device only."""
+        out = None if nrows is None else np.zeros(nrows)
         self.ctx.check(self.ctx.lib.fs_model_mass(self.h, C.byref(params), int(block), ptr(out)), "fs_model_mass")
         return out
 
@@ -252,6 +256,13 @@ class Model:
         v = C.c_double()
         self.ctx.check(self.ctx.lib.fs_model_last_timing(self.h, int(which), C.byref(v)), "fs_model_last_timing")
         return v.value
+
+    def mixed_info(self):
+        """(FP32 value pairs, shear rows per element, neighbour entries, slices)
+        of the mixed-precision operator."""
+        info = np.zeros(4, dtype=np.int64)
+        self.ctx.check(self.ctx.lib.fs_model_mixed_info(self.h, ptr(info, C.c_int64)), "fs_model_mixed_info")
+        return tuple(int(v) for v in info)
 
     def close(self):
         if getattr(self, "h", None):

@@ -121,7 +121,8 @@ class ShellBatch:
         """Free-dof range of model k in the batch vectors."""
         return slice(int(self.free_off[k]), int(self.free_off[k + 1]))
 
-    def explicit(self, dts, cdamps, U0, V0, Fmag, gvals, nsteps, use_graph=True, probe_dofs=(), probe_stride=1):
+    def explicit(self, dts, cdamps, U0, V0, Fmag, gvals, nsteps, use_graph=True, probe_dofs=(), probe_stride=1,
+                 precision=_lib.FS_PREC_FP64):
         """Central-difference loop of all models: dts, cdamps per model; U0,
         V0, Fmag concatenated free-dof vectors; gvals (nsteps+1) x nmodels."""
         nm = len(self.models)
@@ -141,7 +142,7 @@ class ShellBatch:
         V = np.zeros(n)
         p = _lib.CDParams()
         p.nsteps = int(nsteps)
-        p.precision = _lib.FS_PREC_FP64
+        p.precision = int(precision)
         p.probe_stride = stride
         p.ngroups = nm
         p.use_graph = 1 if use_graph else 0
@@ -153,7 +154,7 @@ class ShellBatch:
                                             _lib.ptr(gv), _lib.ptr(probes, C.c_int64) if len(probes) else None,
                                             int(len(probes)), _lib.ptr(pout) if len(probes) else None,
                                             _lib.ptr(U), _lib.ptr(V)), "fs_model_explicit")
-        return {"U": U, "V": V, "probes": pout, "ms": self.model.timing(3)}
+        return {"U": U, "V": V, "probes": pout, "ms": self.model.timing(3), "build_ms": self.model.timing(4)}
 
     def close(self):
         self.model.close()

@@ -72,7 +72,8 @@ def run(model, cdamp, U0, V0, dt, nsteps, Fmag, gvals, probe_dofs=(), probe_stri
                                         _lib.ptr(gvals), _lib.ptr(probes, C.c_int64) if len(probes) else None,
                                         int(len(probes)), _lib.ptr(pout) if len(probes) else None,
                                         _lib.ptr(U), _lib.ptr(V)), "fs_model_explicit")
-    return {"U": U, "V": V, "probes": pout, "dt": dt, "nsteps": nsteps, "ms": model.timing(3)}
+    return {"U": U, "V": V, "probes": pout, "dt": dt, "nsteps": nsteps, "ms": model.timing(3),
+            "build_ms": model.timing(4)}
 
 
 def omega_max(femm, v0, maxit=30):

@@ -225,10 +225,11 @@ class _ShellFEMM:
         m = self._ensure_model(geom0)
         self._ensure_dofs(dchi)
         block = assembler.block
+        if isinstance(assembler, SysmatAssemblerHIP):
+            m.stiffness(self._shell_params(), block)
+            return DeviceMatrix(self, block, "K")
         nrows, rp, cv = self._pattern(block)
         vals = m.stiffness(self._shell_params(), block, len(cv))
-        if isinstance(assembler, SysmatAssemblerHIP):
-            return DeviceMatrix(self, block, "K")
         return sp.csr_matrix((vals, cv, rp), shape=(nrows, nrows))
 
     def element_matrices(self, geom0, dchi=None):
@@ -256,10 +257,11 @@ class _ShellFEMM:
         m = self._ensure_model(geom0)
         self._ensure_dofs(dchi)
         block = assembler.block
+        if isinstance(assembler, SysmatAssemblerHIP):
+            m.mass(self._mass_params(), block)
+            return DeviceMatrix(self, block, "M")
         nrows = dchi.nfreedofs if block == _lib.FS_BLOCK_FF else dchi.nalldofs
         d = m.mass(self._mass_params(), block, nrows)
-        if isinstance(assembler, SysmatAssemblerHIP):
-            return DeviceMatrix(self, block, "M")
         return sp.diags(d, format="csr")
 
     def num_normals(self):
@@ -316,6 +318,48 @@ class _CompositeMixin:
         groups = self._group
         lay = [lg[0] for lg in self.layup_groups]
         return lambda X, J, e, q: lay[groups[e]].csys[:, 2]
+
+    def _host_csys_normals(self, xyz, normal_of):
+        """Constant layup csys per group: the nodal normal is the (normalized)
+        sum of the csys normals of the incident elements' groups
+        (FEMMShellT3FFCompModule.jl:500-521, FEMMShellQ4RSCompModule.jl:462-503),
+        validity against the element normal (T3) or the csys normal itself (Q4)."""
+        conn = self.conn
+        N = xyz.shape[0]
+        n3 = np.array([lg[0].csys[:, 2] for lg in self.layup_groups], dtype=float)[self._group]  # E x 3
+        nrm = np.zeros((N, 3))
+        if self.nen == 3:
+            for a in range(3):
+                np.add.at(nrm, conn[:, a], n3)
+        else:
+            X = xyz[conn]
+            npc = [(-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)]
+            for a in range(4):
+                xi, eta = npc[a]
+                gN = 0.25 * np.array([[-(1 - eta), -(1 - xi)], [(1 - eta), -(1 + xi)], [(1 + eta), (1 + xi)],
+                                      [-(1 + eta), (1 - xi)]])
+                J = np.einsum("eak,al->ekl", X, gN)
+                Jac = np.linalg.norm(np.cross(J[:, :, 0], J[:, :, 1]), axis=1)
+                np.add.at(nrm, conn[:, a], Jac[:, None] * n3)
+        nn = np.linalg.norm(nrm, axis=1)
+        nrm[nn > 0] /= nn[nn > 0, None]
+        valid = np.ones(N, dtype=bool)
+        s = np.sin(self.threshold_angle / 180 * np.pi)
+        ntol = 1 - np.sqrt(1 - s * s)
+        if self.nen == 3:
+            X = xyz[conn]
+            e1 = X[:, 1] - X[:, 0]
+            e1 /= np.linalg.norm(e1, axis=1)[:, None]
+            e3 = np.cross(e1, X[:, 2] - X[:, 0])
+            e3 /= np.linalg.norm(e3, axis=1)[:, None]
+            for a in range(3):
+                bad = np.einsum("ek,ek->e", nrm[conn[:, a]], e3) < 1 - ntol
+                valid[conn[bad, a]] = False
+        else:
+            for a in range(4):
+                bad = np.einsum("ek,ek->e", nrm[conn[:, a]], n3) < 1 - ntol
+                valid[conn[bad, a]] = False
+        return nrm, valid
 
     def _shell_params(self):
         p = _lib.ShellParams()

@@ -136,3 +136,72 @@ def batch_c3(nmodels=256, seed=4, nmin=8, nmax=128):
         models.append(dict(xyz=xyz, conn=conn, dchi=d, material=MatDeforElastIso(E, nu, rho), t=t,
                            name=f"plate{k}_n{n}"))
     return models
+
+
+C5_LAMINA = dict(E1=140e9, E2=10e9, G12=5e9, nu12=0.3, rho=1600.0)
+C5_KINDS = ("assembly", "modal", "explicit")
+
+
+def c5_layup(nplies, angles, ply_t=0.125e-3):
+    """Layered plate of orthotropic plies (E1 140e9, E2 10e9, G12 5e9, nu12
+    0.3, G13 = G12, G23 = E2 / (2 (1 + 0.3)), rho 1600) in the global
+    cartesian frame (csys = I, so the layup normals are the true normals of
+    a flat plate in the xy plane)."""
+    from .material import CompositeLayup, Ply, lamina_material
+    L = C5_LAMINA
+    mat = lamina_material(L["rho"], L["E1"], L["E2"], L["nu12"], L["G12"], L["G12"], L["E2"] / (2 * 1.3))
+    plies = [Ply(f"ply{k}", mat, ply_t, float(a)) for k, a in enumerate(angles[:nplies])]
+    return CompositeLayup("c5", plies, np.eye(3))
+
+
+def served_c5(nreq=1000, seed=6, rate=20.0, emin=1e3, emax=5e5, explicit_steps=500):
+    """Request specs of the served layered-shell workload: Poisson arrivals at
+    `rate` requests/s (seed), kind uniform over assembly / assembly + modal
+    matrices / explicit dynamics, element count log-uniform in [emin, emax],
+    T3FFComp or Q4RSComp, 3..8 plies with angles uniform in {0, 45, -45, 90}
+    and ply thickness 0.125 mm; flat plates Ly = 0.5 m, aspect U(1, 3),
+    interior jitter +-10% h, edges clamped or simply supported (seeded).
+    Explicit requests start from N(0, 1e-3 m/s) translational velocities and
+    run `explicit_steps` steps at 0.9 of the stable dt."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    specs = []
+    t = 0.0
+    for k in range(nreq):
+        t += gaps[k]
+        nel = float(np.exp(rng.uniform(np.log(emin), np.log(emax))))
+        nen = int(rng.choice([3, 4]))
+        aspect = float(rng.uniform(1.0, 3.0))
+        nplies = int(rng.integers(3, 9))
+        angles = [float(a) for a in rng.choice([0.0, 45.0, -45.0, 90.0], 8)]
+        kind = C5_KINDS[int(rng.integers(0, 3))]
+        specs.append(dict(rid=k, arrival=t, kind=kind, nen=nen, nel=nel, aspect=aspect, nplies=nplies,
+                          angles=angles, edges=[int(e) for e in rng.integers(0, 2, 4)],
+                          mesh_seed=int(rng.integers(0, 2 ** 31)), steps=explicit_steps if kind == "explicit" else 0))
+    return specs
+
+
+def c5_model(spec, Ly=0.5):
+    """Mesh, dof field and layup of one served request spec."""
+    nen, aspect = spec["nen"], spec["aspect"]
+    cells = spec["nel"] / (2.0 if nen == 3 else 1.0)
+    ny = max(2, int(round(np.sqrt(cells / aspect))))
+    nx = max(2, int(round(cells / ny)))
+    Lx = aspect * Ly
+    xy, conn = (fe.T3block if nen == 3 else fe.Q4block)(Lx, Ly, nx, ny)
+    rng = np.random.default_rng(spec["mesh_seed"])
+    hx, hy = Lx / nx, Ly / ny
+    tol = 1e-9
+    interior = (xy[:, 0] > tol) & (xy[:, 0] < Lx - tol) & (xy[:, 1] > tol) & (xy[:, 1] < Ly - tol)
+    xy = xy.copy()
+    xy[interior, 0] += rng.uniform(-0.1 * hx, 0.1 * hx, interior.sum())
+    xy[interior, 1] += rng.uniform(-0.1 * hy, 0.1 * hy, interior.sum())
+    xyz = fe.xyz3(xy)
+    d = fe.DofField(xyz.shape[0])
+    sel = [xy[:, 0] < tol, xy[:, 0] > Lx - tol, xy[:, 1] < tol, xy[:, 1] > Ly - tol]
+    for s_, kind in zip(sel, spec["edges"]):
+        d.setebc(np.nonzero(s_)[0], [1, 2, 3, 4, 5, 6] if kind == 0 else [1, 2, 3])
+    d.numberdofs()
+    layup = c5_layup(spec["nplies"], spec["angles"])
+    V0 = gaussian_velocity(d, 1e-3, spec["mesh_seed"] + 1) if spec["kind"] == "explicit" else None
+    return dict(nen=nen, xyz=xyz, conn=conn, dchi=d, layup=layup, V0=V0)

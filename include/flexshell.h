@@ -52,8 +52,11 @@ extern "C" {
 
 /* ---- explicit loop precision ---- */
 #define FS_PREC_FP64 0  /* all products in FP64 */
-#define FS_PREC_MIXED 1 /* membrane+bending internal force in FP32; state, mass,
-                           shear and drilling stabilisation terms in FP64 */
+#define FS_PREC_MIXED 1 /* membrane+bending stiffness held in FP32 (products
+                           widened to and summed in FP64); state, mass, shear
+                           and drilling stabilisation terms in FP64 */
+#define FS_PREC_MIXED_F32 2 /* as FS_PREC_MIXED with the membrane+bending
+                               products and their row sum in FP32 */
 
 typedef struct fs_ctx fs_ctx;
 typedef struct fs_model fs_model;
@@ -118,7 +121,7 @@ typedef struct fs_cd_params {
     double dt;          /* time step, after the tend/dt adjustment of :72-75 */
     double cdamp;       /* C = cdamp * diag(M)  (ksi * 2 * omegad); 0 = undamped */
     int64_t nsteps;
-    int32_t precision;  /* FS_PREC_FP64 or FS_PREC_MIXED */
+    int32_t precision;  /* FS_PREC_FP64, FS_PREC_MIXED or FS_PREC_MIXED_F32 */
     int32_t probe_stride; /* record probes every probe_stride steps (>= 1) */
     /* Batches of independent models stepped together: with ngroups > 0 each
      * free row r belongs to model row_group[r], which has its own time step
@@ -244,8 +247,18 @@ int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *block
 
 /* Device time (ms) of the last fs_model_stiffness / fs_model_explicit call,
  * measured with HIP events on the context stream: which = 0 element kernel,
- * 1 scatter kernel, 2 whole stiffness, 3 explicit loop. */
+ * 1 scatter kernel, 2 whole stiffness, 3 explicit loop, 4 construction of
+ * the mixed-precision operator (0 when the last loop ran in FP64 or reused
+ * an operator built earlier). */
 int fs_model_last_timing(const fs_model *m, int which, double *ms);
+
+/* Storage of the mixed-precision operator built by the last FS_PREC_MIXED /
+ * FS_PREC_MIXED_F32 loop (no counterpart in the reference, whose loop runs on
+ * the FP64 CSR matrix only): info[0] FP32 value pairs of the membrane +
+ * bending slices, info[1] shear factor rows per element, info[2] entries of
+ * the node neighbour lists, info[3] 64-row slices.  Zeros before the first
+ * mixed loop. */
+int fs_model_mixed_info(const fs_model *m, int64_t *info);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,86 @@
+"""Served requests: each result is bitwise the same alone and in a mix of
+concurrently running requests (several slots, overlapping on the device), and
+one served layered request agrees with the oracle."""
+import numpy as np
+import pytest
+
+from flexshell import serve as S, workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(specs):
+    return [S.Request(s["rid"], s["kind"], W.c5_model(s), steps=s["steps"], arrival=s["arrival"], seed=s["rid"])
+            for s in specs]
+
+
+def _same(a, b):
+    if a.keys() != b.keys():
+        return False
+    for k in a:
+        x, y = a[k], b[k]
+        if hasattr(x, "indptr"):
+            if not (np.array_equal(x.indptr, y.indptr) and np.array_equal(x.indices, y.indices)
+                    and np.array_equal(x.data, y.data)):
+                return False
+        elif isinstance(x, np.ndarray):
+            if not np.array_equal(x, y):
+                return False
+        elif k != "cd_ms" and x != y:
+            return False
+    return True
+
+
+def test_served_results_identical_alone_and_in_mix(ctx):
+    print("[gpu] serve: alone vs mix", flush=True)
+    specs = W.served_c5(nreq=12, seed=6, rate=400.0, emin=1e3, emax=3e4, explicit_steps=40)
+    alone = []
+    for r in _requests(specs):
+        alone.append(S.execute(r, ctx))
+    mixed = _requests(specs)
+    with S.ShellServer(nslots=4, hbm_budget=8 << 30) as srv:
+        summary = S.replay(srv, mixed)
+        assert summary["failed"] == 0, [r.error for r in mixed if r.error]
+        assert srv.max_concurrent >= 2
+    for r, a in zip(mixed, alone):
+        assert _same(r.result, a), (r.rid, r.kind)
+    print(f"[gpu] serve summary {summary}", flush=True)
+
+
+def test_served_admission_respects_budget(ctx):
+    print("[gpu] serve: admission by footprint", flush=True)
+    specs = W.served_c5(nreq=6, seed=7, rate=1000.0, emin=2e3, emax=4e3, explicit_steps=10)
+    reqs = _requests(specs)
+    fp = sorted(r.footprint for r in reqs)
+    budget = fp[0] + fp[1] - 1  # no two requests fit together
+    assert budget >= fp[-1]
+    with S.ShellServer(nslots=3, hbm_budget=budget) as srv:
+        summary = S.replay(srv, reqs)
+        assert summary["failed"] == 0
+        assert srv.max_concurrent == 1
+        assert srv.max_inflight_bytes <= budget
+
+
+def test_served_layered_request_matches_oracle(ctx):
+    print("[gpu] serve: layered modal request vs oracle", flush=True)
+    spec = dict(W.served_c5(nreq=3, seed=8)[0], kind="modal", nel=1500.0, nen=4, nplies=5,
+                angles=[0.0, 45.0, -45.0, 90.0, 45.0, 0.0, 0.0, 0.0], steps=0)
+    m = W.c5_model(spec)
+    r = S.Request(0, "modal", m)
+    res = S.execute(r, ctx)
+    L = W.C5_LAMINA
+    D6 = O.lamina_D6(L["E1"], L["E2"], L["nu12"], L["G12"], L["G12"], L["E2"] / 2.6)
+    plies = [(D6, 0.125e-3, a, L["rho"]) for a in spec["angles"][:spec["nplies"]]]
+    A, B, D, H, (md, mi) = O.laminate(plies)
+    d = m["dchi"]
+    nrm, val = O.csys_normals(4, m["xyz"], m["conn"], np.eye(3))
+    Ke = O.element_matrices(4, m["xyz"], m["conn"], nrm, val, groups=[(A, B, D, H, np.eye(3), 5 * 0.125e-3)])
+    nf = d.nfreedofs
+    Kff = O.assemble_csr(4, m["conn"], d.dofnums, nf, Ke)
+    K = res["K"]
+    assert np.array_equal(K.indptr, Kff.indptr) and np.array_equal(K.indices, Kff.indices)
+    assert np.linalg.norm(K.data - Kff.data) <= 1e-12 * np.linalg.norm(Kff.data)
+    tm, rm = O.element_masses(4, m["xyz"], m["conn"], m0=md, m1=mi)
+    Mo = O.lumped_diag(4, m["conn"], d.dofnums, nf, tm, rm)
+    assert np.allclose(res["M"], Mo, rtol=1e-13, atol=0)
