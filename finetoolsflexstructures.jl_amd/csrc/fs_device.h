@@ -77,7 +77,7 @@ struct MixedOp {
     const double *cs;         // E x nsr x 6 nen
 };
 
-// Scatter plan of one block (ALL or FF) -- see scatter_kernel.
+// Scatter plan of one block (ALL or FF) -- see scatter_pair_kernel.
 // Node I owns the CSR rows dof(I, r); all of them share one column list made
 // of the runs of I's neighbour nodes J (sorted).  For each neighbour slot k in
 // [nbr_ptr[I], nbr_ptr[I+1]): nbr_pair[k] = (upper pair index << 1) | side
@@ -96,6 +96,11 @@ struct ScatterPlan {
     const int32_t *node_rowstart; // N x 6
     const int32_t *contrib_ptr;   // npairs + 1
     const int32_t *contrib;       // (block index << 1) | transposed, element order per pair
+    // upper pairs (I <= J): nodes and the neighbour slots kI (J in I's list)
+    // and kJ (I in J's list), for the pair-driven scatter
+    int64_t npairs;
+    const int32_t *pair_node;     // 2 * npairs: I, J
+    const int32_t *pair_slot;     // 2 * npairs: kI, kJ
 };
 
 }  // namespace fs

@@ -22,7 +22,9 @@
 namespace fs {
 hipError_t launch_element_blocks(const ElemParams &p, const double *x, const double *y, const double *z,
                                  const int32_t *conn, const double *nx, const double *ny, const double *nz,
-                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st);
+                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st, int variant);
+int element_variants(int nen);
+void element_variant_shape(int nen, int variant, int *ne, int *wg);
 hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals, hipStream_t st);
 hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const double *y, const double *z,
                           const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
@@ -59,6 +61,11 @@ struct fs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    // element-kernel launch shape per (family T3/Q4, shear formulation):
+    // -1 = not tuned yet (the autotuner times every shape on the first
+    // stiffness evaluation of that kind and keeps the fastest)
+    int elem_variant[2][2] = {{-1, -1}, {-1, -1}};
+    bool autotune = true;
 };
 
 namespace {
@@ -88,7 +95,8 @@ struct Block {
     std::vector<int32_t> colval;   // nnz
     // scatter plan (device)
     int64_t npairs = 0;
-    DevBuf nbr_ptr, nbr_node, nbr_pair, nbr_off, node_code, node_rowstart, contrib_ptr, contrib;
+    DevBuf nbr_ptr, nbr_node, nbr_pair, nbr_off, node_code, node_rowstart, contrib_ptr, contrib, pair_node,
+        pair_slot;
     // device CSR (int32 for the time loop) and values
     DevBuf rowptr32, colidx32, kval, mdiag;
     bool k_ready = false, m_ready = false;
@@ -98,7 +106,7 @@ struct Block {
         rowptr.clear();
         colval.clear();
         for (DevBuf *b : {&nbr_ptr, &nbr_node, &nbr_pair, &nbr_off, &node_code, &node_rowstart, &contrib_ptr,
-                          &contrib, &rowptr32, &colidx32, &kval, &mdiag})
+                          &contrib, &pair_node, &pair_slot, &rowptr32, &colidx32, &kval, &mdiag})
             b->release();
     }
 };
@@ -346,6 +354,7 @@ int build_block(fs_model *m, int which) {
     // neighbour tables: pair, side, run offsets; node row starts
     const size_t nnb = m->nbr_list.size();
     std::vector<int32_t> nb_pair(nnb), nb_off(2 * nnb), rowstart(N * 6, -1);
+    std::vector<int32_t> pnode(2 * npairs), pslot(2 * npairs);
     for (int64_t I = 0; I < N; ++I) {
         for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
             const int J = m->nbr_list[k];
@@ -353,6 +362,12 @@ int build_block(fs_model *m, int which) {
             nb_pair[k] = (int32_t)((p << 1) | (I > J ? 1 : 0));
             nb_off[2 * k] = nb_offfree[k];
             nb_off[2 * k + 1] = nb_offfix[k];
+            if (I <= J) {
+                pnode[2 * p] = (int32_t)I;
+                pnode[2 * p + 1] = J;
+                pslot[2 * p] = k;
+                pslot[2 * p + 1] = slot_of(J, I);
+            }
         }
         for (int cc = 0; cc < 6; ++cc) {
             const int64_t d = m->dof[I * 6 + cc];
@@ -400,6 +415,8 @@ int build_block(fs_model *m, int which) {
     if ((s = upload(c, B.node_rowstart, rowstart.data(), rowstart.size()))) return s;
     if ((s = upload(c, B.contrib_ptr, cptr.data(), cptr.size()))) return s;
     if ((s = upload(c, B.contrib, clist.data(), clist.size()))) return s;
+    if ((s = upload(c, B.pair_node, pnode.data(), pnode.size()))) return s;
+    if ((s = upload(c, B.pair_slot, pslot.data(), pslot.size()))) return s;
     if ((s = upload(c, B.rowptr32, rp32.data(), rp32.size()))) return s;
     if ((s = upload(c, B.colidx32, B.colval.data(), B.colval.size()))) return s;
     if ((s = dalloc(c, B.kval, (size_t)nnz * sizeof(double)))) return s;
@@ -425,6 +442,9 @@ fs::ScatterPlan plan_of(const Block &B) {
     sp.node_rowstart = B.node_rowstart.as<int32_t>();
     sp.contrib_ptr = B.contrib_ptr.as<int32_t>();
     sp.contrib = B.contrib.as<int32_t>();
+    sp.npairs = B.npairs;
+    sp.pair_node = B.pair_node.as<int32_t>();
+    sp.pair_slot = B.pair_slot.as<int32_t>();
     return sp;
 }
 
@@ -527,11 +547,41 @@ int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
     hipEvent_t e0, e1;
     FS_HIP(c, hipEventCreate(&e0));
     FS_HIP(c, hipEventCreate(&e1));
+    int &variant = c->elem_variant[m->nen == 4 ? 1 : 0][ep.formulation == FS_SHEAR_AVERAGE_K ? 1 : 0];
+    if (variant < 0) {
+        if (!c->autotune) {
+            variant = 0;
+        } else {
+            // autotuner: one warm launch and one timed launch per shape on
+            // this model; every shape computes bitwise the same blocks
+            float best = 0.f;
+            int bestv = 0;
+            for (int v = 0; v < fs::element_variants(m->nen); ++v) {
+                float t = 0.f;
+                for (int rep = 0; rep < 2; ++rep) {
+                    FS_HIP(c, hipEventRecord(e0, c->stream));
+                    FS_HIP(c, fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(),
+                                                        m->z.as<double>(), m->dconn.as<int32_t>(),
+                                                        m->nx.as<double>(), m->ny.as<double>(), m->nz.as<double>(),
+                                                        m->valid.as<uint8_t>(), m->Kb.as<double>(),
+                                                        m->status.as<int>(), c->stream, v));
+                    FS_HIP(c, hipEventRecord(e1, c->stream));
+                    FS_HIP(c, hipEventSynchronize(e1));
+                    FS_HIP(c, hipEventElapsedTime(&t, e0, e1));
+                }
+                if (v == 0 || t < best) {
+                    best = t;
+                    bestv = v;
+                }
+            }
+            variant = bestv;
+        }
+    }
     FS_HIP(c, hipEventRecord(e0, c->stream));
     FS_HIP(c, fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
                                         m->dconn.as<int32_t>(), m->nx.as<double>(), m->ny.as<double>(),
                                         m->nz.as<double>(), m->valid.as<uint8_t>(), m->Kb.as<double>(),
-                                        m->status.as<int>(), c->stream));
+                                        m->status.as<int>(), c->stream, variant));
     FS_HIP(c, hipEventRecord(e1, c->stream));
     int st = 0;
     FS_HIP(c, hipMemcpyAsync(&st, m->status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -586,7 +636,7 @@ int build_mixed(fs_model *m) {
     FS_HIP(c, fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
                                         m->dconn.as<int32_t>(), m->nx.as<double>(), m->ny.as<double>(),
                                         m->nz.as<double>(), m->valid.as<uint8_t>(), kbm.as<double>(),
-                                        m->status.as<int>(), c->stream));
+                                        m->status.as<int>(), c->stream, 0));
     FS_HIP(c, fs::launch_mixed_build(plan_of(B), kbm.as<double>(), X.goff.as<int64_t>(), X.kmb.as<float2>(),
                                      c->stream));
     FS_HIP(c, fs::launch_mixed_drill(nen, N, (int)E, m->dne_ptr.as<int32_t>(), m->dne_list.as<int32_t>(),
@@ -1149,6 +1199,34 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
     FS_HIP(c, hipMemcpyAsync(h, ss, 16, hipMemcpyDeviceToHost, c->stream));
     FS_HIP(c, hipStreamSynchronize(c->stream));
     *omega_max = std::sqrt(h[0] / h[1]);
+    return FS_OK;
+}
+
+int fs_ctx_tuning(const fs_ctx *ctx, int nen, int formulation, int32_t *shape) {
+    if (!ctx || !shape || (nen != FS_T3 && nen != FS_Q4) ||
+        (formulation != FS_SHEAR_AVERAGE_B && formulation != FS_SHEAR_AVERAGE_K))
+        return FS_ERR_ARG;
+    const int v = ctx->elem_variant[nen == FS_Q4 ? 1 : 0][formulation];
+    shape[0] = v;
+    int ne = 0, wg = 0;
+    if (v >= 0) fs::element_variant_shape(nen, v, &ne, &wg);
+    shape[1] = v >= 0 ? ne : 0;
+    shape[2] = v >= 0 ? wg : 0;
+    return FS_OK;
+}
+
+int fs_ctx_set_tuning(fs_ctx *ctx, int nen, int formulation, int32_t variant) {
+    if (!ctx || (nen != FS_T3 && nen != FS_Q4) ||
+        (formulation != FS_SHEAR_AVERAGE_B && formulation != FS_SHEAR_AVERAGE_K) || variant < -2 ||
+        variant >= fs::element_variants(nen))
+        return FS_ERR_ARG;
+    if (variant == -2) {
+        ctx->autotune = false;
+        ctx->elem_variant[nen == FS_Q4 ? 1 : 0][formulation] = -1;
+        return FS_OK;
+    }
+    ctx->autotune = true;
+    ctx->elem_variant[nen == FS_Q4 ? 1 : 0][formulation] = variant;
     return FS_OK;
 }
 

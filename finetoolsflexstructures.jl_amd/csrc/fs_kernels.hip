@@ -33,8 +33,8 @@
 //
 // Deterministic scatter into CSR
 // ------------------------------
-// fs_scatter_kernel sums, for each upper node pair (I, J), the contributing
-// element blocks in ascending element order (the order in which
+// scatter_pair_kernel sums, for each upper node pair (I, J), the
+// contributing element blocks in ascending element order (the order in which
 // sparse(I, J, V) combines duplicates) and writes the (I, J) and (J, I)
 // 6x6 blocks to their CSR positions.  No atomics: every CSR value is written
 // exactly once and the result is bitwise reproducible.
@@ -127,6 +127,123 @@ __device__ __forceinline__ void homog_factors(const double *Lps, const double *L
     for (int i = 0; i < 4; ++i) Ls[i] = s_s * Lt[i];
 }
 
+// _NodalTriadsE (FEMMShellT3FFModule.jl:355-388): rotation taking the
+// element normal e3 to the nodal normal nk_e (element frame); the rotation
+// vector is r = e3 x nk_e and its length |r| = sin(theta) is used as the
+// angle (rotmat3!), identity below 1e-12.
+__device__ __forceinline__ void nodal_triad(const double nk_e[3], double A[3][3]) {
+    const double r0 = -nk_e[1], r1 = nk_e[0];
+    const double na = sqrt(r0 * r0 + r1 * r1);
+    if (na > 1.0e-12) {
+        const double u0 = r0 / na, u1 = r1 / na;
+        double sa, ca;
+        sincos(na, &sa, &ca);
+        const double f = sa / na;
+        const double uu[3] = {u0, u1, 0.0};
+        const double Sk[3][3] = {{0.0, 0.0, r1}, {0.0, 0.0, -r0}, {-r1, r0, 0.0}};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const double aa = uu[i] * uu[j];
+                A[i][j] = ca * (((i == j) ? 1.0 : 0.0) - aa) + f * Sk[i][j] + aa;
+            }
+    } else {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) A[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+}
+
+// Material factor rows at one quadrature point, produced on demand (the
+// rows of Lmb and Ls of homog_factors / composite_factors, same arithmetic).
+struct MatRows {
+    bool composite;
+    const double *L;  // chol(Dps) (homogeneous) or chol([A B; B D]) (layered)
+    double s_m, s_b;  // membrane / bending scales (layered: s_m = s_mb)
+    double m, n;      // layup angle cosine / sine (layered)
+    double Ls[4];
+};
+
+__device__ __forceinline__ MatRows mat_rows(const ElemParams &p, int e, double sc0, double sc1, double sc2,
+                                            double sc3, double sc4) {
+    MatRows r;
+    const int grp = p.elem_group ? p.elem_group[e] : 0;
+    r.composite = p.composite != 0;
+    if (!r.composite) {
+        const double *hrec = p.g_homog + 16 * grp;
+        r.L = hrec;
+        r.s_m = sc0;
+        r.s_b = sc1;
+        r.m = r.n = 0.0;
+        for (int i = 0; i < 4; ++i) r.Ls[i] = sc2 * hrec[9 + i];
+    } else {
+        r.L = p.g_Labd + 36 * grp;
+        r.s_m = sc0;
+        r.s_b = 0.0;
+        r.m = sc3;
+        r.n = sc4;
+        const double *Lh = p.g_Lh + 4 * grp;
+        const double Tt[2][2] = {{sc3, -sc4}, {sc4, sc3}};
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) r.Ls[i * 2 + j] = sc2 * (Lh[i * 2 + 0] * Tt[0][j] + Lh[i * 2 + 1] * Tt[1][j]);
+    }
+    return r;
+}
+
+__device__ __forceinline__ void lmb_row(const MatRows &mr, int s, double row[6]) {
+    if (!mr.composite) {
+        for (int j = 0; j < 6; ++j) row[j] = 0.0;
+        if (s < 3) {
+            for (int j = 0; j < 3; ++j) row[j] = mr.s_m * mr.L[s * 3 + j];
+        } else {
+            for (int j = 0; j < 3; ++j) row[3 + j] = mr.s_b * mr.L[(s - 3) * 3 + j];
+        }
+    } else {
+        const double m = mr.m, n = mr.n;
+        const double Tb[3][3] = {{m * m, n * n, -(m * n)}, {n * n, m * m, (m * n)},
+                                 {2 * (m * n), -2 * (m * n), m * m - n * n}};
+        for (int blk = 0; blk < 2; ++blk)
+            for (int j = 0; j < 3; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < 3; ++k) acc += mr.L[s * 6 + blk * 3 + k] * Tb[k][j];
+                row[blk * 3 + j] = mr.s_m * acc;
+            }
+    }
+}
+
+// Element-frame columns (u, v, w, theta_x, theta_y) of one node for row s of
+// C: membrane/bending rows from a row of Lmb and the shape-function
+// gradients, shear rows from Ls and the node's shear B entries
+// bs[comp * NSH + row] (comp 0 = w, 1 = theta_x, 2 = theta_y).
+__device__ __forceinline__ void c_row_mb(const double lrow[6], double gN0, double gN1, double &cu, double &cv,
+                                         double &cw, double &cx, double &cy) {
+    cu = lrow[0] * gN0 + lrow[2] * gN1;
+    cv = lrow[1] * gN1 + lrow[2] * gN0;
+    cw = 0.0;
+    cx = -lrow[4] * gN1 - lrow[5] * gN0;
+    cy = lrow[3] * gN0 + lrow[5] * gN1;
+}
+
+template <int NSH>
+__device__ __forceinline__ void c_row_sh(const double Ls[4], int s, const double *bs, double &cu, double &cv,
+                                         double &cw, double &cx, double &cy) {
+    const int o = (s - 6) / 2, rr = (s - 6) % 2;
+    cu = 0.0;
+    cv = 0.0;
+    cw = Ls[rr * 2 + 0] * bs[0 * NSH + 2 * o] + Ls[rr * 2 + 1] * bs[0 * NSH + 2 * o + 1];
+    cx = Ls[rr * 2 + 0] * bs[1 * NSH + 2 * o] + Ls[rr * 2 + 1] * bs[1 * NSH + 2 * o + 1];
+    cy = Ls[rr * 2 + 0] * bs[2 * NSH + 2 * o] + Ls[rr * 2 + 1] * bs[2 * NSH + 2 * o + 1];
+}
+
+__device__ __forceinline__ void c_row_q4(const MatRows &mr, int s, double gN0, double gN1, const double *bs,
+                                         double &cu, double &cv, double &cw, double &cx, double &cy) {
+    if (s < 6) {
+        double lrow[6];
+        lmb_row(mr, s, lrow);
+        c_row_mb(lrow, gN0, gN1, cu, cv, cw, cx, cy);
+    } else {
+        c_row_sh<2>(mr.Ls, s, bs, cu, cv, cw, cx, cy);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Element kernel
 // ---------------------------------------------------------------------------
@@ -185,8 +302,8 @@ __device__ __forceinline__ void q4_gradn(double xi, double eta, double gN[4][2])
 // GaussRule(2, 2) abscissa as FinEtools tabulates it.
 #define FS_GAUSS2 0.577350269189626
 
-template <int NEN, int NQP, int NR, int NE, int WG>
-__global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__restrict__ xs,
+template <int NEN, int NQP, int NR, int NE, int WG, bool SPLIT>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void elem_kernel(ElemParams p, const double *__restrict__ xs,
                                                   const double *__restrict__ ys,
                                                   const double *__restrict__ zs,
                                                   const int32_t *__restrict__ conn,
@@ -425,72 +542,32 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
     __syncthreads();
 
     // ---------------- G2/G3: per (element, qp, node) ----------------
+    // Rows of C are produced one at a time from the material factor rows in
+    // the geometry record, so only the triad stays in registers across the
+    // barrier.
     const bool gthread = tid < NE * NQP * NEN;
     const int le2 = tid / (NQP * NEN);
     const int q2 = (tid / NEN) % NQP;
     const int j2 = tid % NEN;
     const bool gact = gthread && (le2 < ne);
     double A[3][3];
-    double cu[NR], cv[NR], cw[NR], cx[NR], cy[NR];
-    double gNj0 = 0.0, gNj1 = 0.0;
-    double EGl[3][3];
     if (gact) {
         const int e = e0 + le2;
         const double *g = sG + (le2 * NQP + q2) * T::GEO;
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) EGl[i][j] = g[T::G_EG + i * 3 + j];
         const int node = conn[j2 * E + e];
-        // _NodalTriadsE (:355-388)
         double nk_e[3];
         if (valid[node]) {
             const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
-            for (int i = 0; i < 3; ++i) nk_e[i] = EGl[0][i] * n0 + EGl[1][i] * n1 + EGl[2][i] * n2;
+            for (int i = 0; i < 3; ++i)
+                nk_e[i] = g[T::G_EG + 0 * 3 + i] * n0 + g[T::G_EG + 1 * 3 + i] * n1 + g[T::G_EG + 2 * 3 + i] * n2;
         } else {
             nk_e[0] = 0.0;
             nk_e[1] = 0.0;
             nk_e[2] = 1.0;
         }
-        const double r0 = -nk_e[1], r1 = nk_e[0];
-        const double na = sqrt(r0 * r0 + r1 * r1);
-        if (na > 1.0e-12) {
-            // rotmat3! for r = (r0, r1, 0)
-            const double u0 = r0 / na, u1 = r1 / na;
-            double sa, ca;
-            sincos(na, &sa, &ca);
-            const double f = sa / na;
-            const double uu[3] = {u0, u1, 0.0};
-            const double Sk[3][3] = {{0.0, 0.0, r1}, {0.0, 0.0, -r0}, {-r1, r0, 0.0}};
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) {
-                    const double aa = uu[i] * uu[j];
-                    A[i][j] = ca * (((i == j) ? 1.0 : 0.0) - aa) + f * Sk[i][j] + aa;
-                }
-        } else {
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) A[i][j] = (i == j) ? 1.0 : 0.0;
-        }
-        gNj0 = g[T::G_GN + 2 * j2];
-        gNj1 = g[T::G_GN + 2 * j2 + 1];
-        const double *Lmb = g + T::G_LMB;
-        const double *Ls = g + T::G_LS;
+        nodal_triad(nk_e, A);
+        const double gNj0 = g[T::G_GN + 2 * j2], gNj1 = g[T::G_GN + 2 * j2 + 1];
         const double *bs = g + T::G_BS + j2 * 3 * T::NSH;
-        // element-frame C columns of this node: [Lmb * (Bm; Bb) ; Ls * Bs]
-        for (int s = 0; s < 6; ++s) {
-            cu[s] = Lmb[s * 6 + 0] * gNj0 + Lmb[s * 6 + 2] * gNj1;
-            cv[s] = Lmb[s * 6 + 1] * gNj1 + Lmb[s * 6 + 2] * gNj0;
-            cw[s] = 0.0;
-            cx[s] = -Lmb[s * 6 + 4] * gNj1 - Lmb[s * 6 + 5] * gNj0;
-            cy[s] = Lmb[s * 6 + 3] * gNj0 + Lmb[s * 6 + 5] * gNj1;
-        }
-        for (int o = 0; o < T::NSH / 2; ++o)
-            for (int rr = 0; rr < 2; ++rr) {
-                const int s = 6 + 2 * o + rr;
-                cu[s] = 0.0;
-                cv[s] = 0.0;
-                cw[s] = Ls[rr * 2 + 0] * bs[0 * T::NSH + 2 * o] + Ls[rr * 2 + 1] * bs[0 * T::NSH + 2 * o + 1];
-                cx[s] = Ls[rr * 2 + 0] * bs[1 * T::NSH + 2 * o] + Ls[rr * 2 + 1] * bs[1 * T::NSH + 2 * o + 1];
-                cy[s] = Ls[rr * 2 + 0] * bs[2 * T::NSH + 2 * o] + Ls[rr * 2 + 1] * bs[2 * T::NSH + 2 * o + 1];
-            }
         // coupling vector d = m1 * cx + m2 * cy (_transfmat_a_to_e!, :451-460)
         const double m1 = (1 / A[2][2]) * A[0][2];
         const double m2 = (1 / A[2][2]) * A[1][2];
@@ -499,48 +576,60 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
             nd[k] = A[0][k];
             nd[3 + k] = A[1][k];
         }
-        for (int s = 0; s < NR; ++s) nd[6 + s] = m1 * cx[s] + m2 * cy[s];
+        for (int s = 0; s < NR; ++s) {
+            double cu, cv, cw, cx, cy;
+            if (s < 6)
+                c_row_mb(g + T::G_LMB + s * 6, gNj0, gNj1, cu, cv, cw, cx, cy);
+            else
+                c_row_sh<T::NSH>(g + T::G_LS, s, bs, cu, cv, cw, cx, cy);
+            nd[6 + s] = m1 * cx + m2 * cy;
+        }
     }
     __syncthreads();
     if (gact) {
+        const double *g = sG + (le2 * NQP + q2) * T::GEO;
         const double *ndb = sN + ((le2 * NQP + q2) * NEN) * T::NODE;
+        const double gNj0 = g[T::G_GN + 2 * j2], gNj1 = g[T::G_GN + 2 * j2 + 1];
+        const double *bs = g + T::G_BS + j2 * 3 * T::NSH;
         // Q = A' * EG' : global -> nodal block (_TransfmatGToA, :398-419)
         double Q[3][3];
         for (int k = 0; k < 3; ++k)
-            for (int gg = 0; gg < 3; ++gg) Q[k][gg] = A[0][k] * EGl[gg][0] + A[1][k] * EGl[gg][1] + A[2][k] * EGl[gg][2];
-        double cgt[3][NR], cgr[3][NR];
-        for (int gg = 0; gg < 3; ++gg)
-            for (int s = 0; s < NR; ++s) {
-                cgt[gg][s] = 0.0;
-                cgr[gg][s] = 0.0;
-            }
+            for (int gg = 0; gg < 3; ++gg)
+                Q[k][gg] = A[0][k] * g[T::G_EG + gg * 3 + 0] + A[1][k] * g[T::G_EG + gg * 3 + 1] +
+                           A[2][k] * g[T::G_EG + gg * 3 + 2];
         // translations: cn_k = sum_rw c_rw A[rw][k] + sum_i a3(i,j,k) d_i
-        for (int k = 0; k < 3; ++k) {
-            double a3[NEN];
+        double a3[3][NEN];
+        for (int k = 0; k < 3; ++k)
             for (int i = 0; i < NEN; ++i)
-                a3[i] = 1.0 / 2 * (ndb[i * T::NODE + 3 + k] * gNj0 - ndb[i * T::NODE + k] * gNj1);
-            for (int s = 0; s < NR; ++s) {
-                double cn = cu[s] * A[0][k] + cv[s] * A[1][k] + cw[s] * A[2][k];
-                for (int i = 0; i < NEN; ++i) cn += a3[i] * ndb[i * T::NODE + 6 + s];
-                for (int gg = 0; gg < 3; ++gg) cgt[gg][s] += cn * Q[k][gg];
-            }
-        }
+                a3[k][i] = 1.0 / 2 * (ndb[i * T::NODE + 3 + k] * gNj0 - ndb[i * T::NODE + k] * gNj1);
         // rotations: nodal theta_1,2 from element theta_x,y; drilling column zero
-        double kp[2] = {0.0, 0.0};
+        double r2a[2], r2b[2];
         for (int l = 0; l < 2; ++l) {
-            const double r2a = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
-            const double r2b = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
-            for (int s = 0; s < NR; ++s) {
-                const double cn = cx[s] * r2a + cy[s] * r2b;
-                kp[l] += cn * cn;
-                for (int gg = 0; gg < 3; ++gg) cgr[gg][s] += cn * Q[l][gg];
-            }
+            r2a[l] = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
+            r2b[l] = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
         }
+        double kp[2] = {0.0, 0.0};
         double *cdst = sC + le2 * T::CSTR + (q2 * NR) * T::K + 6 * j2;
         for (int s = 0; s < NR; ++s) {
+            double cu, cv, cw, cx, cy;
+            if (s < 6)
+                c_row_mb(g + T::G_LMB + s * 6, gNj0, gNj1, cu, cv, cw, cx, cy);
+            else
+                c_row_sh<T::NSH>(g + T::G_LS, s, bs, cu, cv, cw, cx, cy);
+            double cgt[3] = {0.0, 0.0, 0.0}, cgr[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < 3; ++k) {
+                double cn = cu * A[0][k] + cv * A[1][k] + cw * A[2][k];
+                for (int i = 0; i < NEN; ++i) cn += a3[k][i] * ndb[i * T::NODE + 6 + s];
+                for (int gg = 0; gg < 3; ++gg) cgt[gg] += cn * Q[k][gg];
+            }
+            for (int l = 0; l < 2; ++l) {
+                const double cn = cx * r2a[l] + cy * r2b[l];
+                kp[l] += cn * cn;
+                for (int gg = 0; gg < 3; ++gg) cgr[gg] += cn * Q[l][gg];
+            }
             for (int gg = 0; gg < 3; ++gg) {
-                cdst[s * T::K + gg] = cgt[gg][s];
-                cdst[s * T::K + 3 + gg] = cgr[gg][s];
+                cdst[s * T::K + gg] = cgt[gg];
+                cdst[s * T::K + 3 + gg] = cgr[gg];
             }
         }
         if (NQP == 1) {
@@ -569,7 +658,7 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
             }
     }
     const bool cact = cthread && (le3 < ne);
-    const bool split = p.split_cs != nullptr;
+    constexpr bool split = SPLIT;
     double acc[6][6];
     for (int r = 0; r < 6; ++r)
         for (int c = 0; c < 6; ++c) acc[r][c] = 0.0;
@@ -689,8 +778,8 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemParams p, const double *__
 // point) stores a compact geometry record: E_G, gradN_e, the material scale
 // factors (and the layup angle for layered shells) and the MITC shear row
 // entries; the material factors are rebuilt per node thread.
-template <int NE, int WG>
-__global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const double *__restrict__ xs,
+template <int NE, int WG, bool SPLIT>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_q4_seq_kernel(ElemParams p, const double *__restrict__ xs,
                                                          const double *__restrict__ ys,
                                                          const double *__restrict__ zs,
                                                          const int32_t *__restrict__ conn,
@@ -878,7 +967,7 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
                 ++cnt;
             }
     }
-    const bool split = p.split_cs != nullptr;
+    constexpr bool split = SPLIT;
     double acc[6][6];
     double accs[3][3];  // split: shear part of the rotation block (drilling penalty input)
     for (int r = 0; r < 6; ++r)
@@ -888,67 +977,27 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
     __syncthreads();
 
     for (int q = 0; q < NQP; ++q) {
-        // ---------------- G2: triad and element-frame columns of node j ----------------
+        // ---------------- G2: triad and coupling vector of node j ----------------
+        // Rows of C are produced one at a time (element-frame columns from the
+        // material factor row, then the global-basis columns), so only the
+        // triad and a few scalars stay in registers across the phases.
         double A[3][3];
-        double cu[NR], cv[NR], cw[NR], cx[NR], cy[NR];
-        double gNj0 = 0.0, gNj1 = 0.0;
         const double *g = sG + (le2 * NQP + q) * GEO;
         if (gact) {
-            double EG[3][3];
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) EG[i][j] = g[G_EG + i * 3 + j];
             double nk_e[3];
             if (valid2) {
-                for (int i = 0; i < 3; ++i) nk_e[i] = EG[0][i] * n20 + EG[1][i] * n21 + EG[2][i] * n22;
+                for (int i = 0; i < 3; ++i)
+                    nk_e[i] = g[G_EG + 0 * 3 + i] * n20 + g[G_EG + 1 * 3 + i] * n21 + g[G_EG + 2 * 3 + i] * n22;
             } else {
                 nk_e[0] = 0.0;
                 nk_e[1] = 0.0;
                 nk_e[2] = 1.0;
             }
-            const double r0 = -nk_e[1], r1 = nk_e[0];
-            const double na = sqrt(r0 * r0 + r1 * r1);
-            if (na > 1.0e-12) {
-                const double u0 = r0 / na, u1 = r1 / na;
-                double sa, ca;
-                sincos(na, &sa, &ca);
-                const double f = sa / na;
-                const double uu[3] = {u0, u1, 0.0};
-                const double Sk[3][3] = {{0.0, 0.0, r1}, {0.0, 0.0, -r0}, {-r1, r0, 0.0}};
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j) {
-                        const double aa = uu[i] * uu[j];
-                        A[i][j] = ca * (((i == j) ? 1.0 : 0.0) - aa) + f * Sk[i][j] + aa;
-                    }
-            } else {
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j) A[i][j] = (i == j) ? 1.0 : 0.0;
-            }
-            gNj0 = g[G_GN + 2 * j2];
-            gNj1 = g[G_GN + 2 * j2 + 1];
-            double Lmb[36], Ls[4];
-            const int grp = p.elem_group ? p.elem_group[e0 + le2] : 0;
-            if (!p.composite) {
-                const double *hrec = p.g_homog + 16 * grp;
-                homog_factors(hrec, hrec + 9, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], Lmb, Ls);
-            } else {
-                composite_factors(p.g_Labd + 36 * grp, p.g_Lh + 4 * grp, g[G_SC + 3], g[G_SC + 4], g[G_SC + 0],
-                                  g[G_SC + 2], Lmb, Ls);
-            }
+            nodal_triad(nk_e, A);
+            const MatRows mr = mat_rows(p, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], g[G_SC + 3],
+                                        g[G_SC + 4]);
+            const double gNj0 = g[G_GN + 2 * j2], gNj1 = g[G_GN + 2 * j2 + 1];
             const double *bs = g + G_BS + j2 * 6;
-            for (int s = 0; s < 6; ++s) {
-                cu[s] = Lmb[s * 6 + 0] * gNj0 + Lmb[s * 6 + 2] * gNj1;
-                cv[s] = Lmb[s * 6 + 1] * gNj1 + Lmb[s * 6 + 2] * gNj0;
-                cw[s] = 0.0;
-                cx[s] = -Lmb[s * 6 + 4] * gNj1 - Lmb[s * 6 + 5] * gNj0;
-                cy[s] = Lmb[s * 6 + 3] * gNj0 + Lmb[s * 6 + 5] * gNj1;
-            }
-            for (int rr = 0; rr < 2; ++rr) {
-                cu[6 + rr] = 0.0;
-                cv[6 + rr] = 0.0;
-                cw[6 + rr] = Ls[rr * 2 + 0] * bs[0 * 2 + 0] + Ls[rr * 2 + 1] * bs[0 * 2 + 1];
-                cx[6 + rr] = Ls[rr * 2 + 0] * bs[1 * 2 + 0] + Ls[rr * 2 + 1] * bs[1 * 2 + 1];
-                cy[6 + rr] = Ls[rr * 2 + 0] * bs[2 * 2 + 0] + Ls[rr * 2 + 1] * bs[2 * 2 + 1];
-            }
             const double m1 = (1 / A[2][2]) * A[0][2];
             const double m2 = (1 / A[2][2]) * A[1][2];
             double *nd = sN + (le2 * NEN + j2) * NODE;
@@ -956,47 +1005,53 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
                 nd[k] = A[0][k];
                 nd[3 + k] = A[1][k];
             }
-            for (int s = 0; s < NR; ++s) nd[6 + s] = m1 * cx[s] + m2 * cy[s];
+            for (int s = 0; s < NR; ++s) {
+                double cu, cv, cw, cx, cy;
+                c_row_q4(mr, s, gNj0, gNj1, bs, cu, cv, cw, cx, cy);
+                nd[6 + s] = m1 * cx + m2 * cy;
+            }
         }
         __syncthreads();
         // ---------------- G3: global-basis C columns of node j ----------------
         if (gact) {
             const double *ndb = sN + (le2 * NEN) * NODE;
+            const double gNj0 = g[G_GN + 2 * j2], gNj1 = g[G_GN + 2 * j2 + 1];
             double Q[3][3];
             for (int k = 0; k < 3; ++k)
                 for (int gg = 0; gg < 3; ++gg)
                     Q[k][gg] = A[0][k] * g[G_EG + gg * 3 + 0] + A[1][k] * g[G_EG + gg * 3 + 1] +
                                A[2][k] * g[G_EG + gg * 3 + 2];
-            double cgt[3][NR], cgr[3][NR];
-            for (int gg = 0; gg < 3; ++gg)
-                for (int s = 0; s < NR; ++s) {
-                    cgt[gg][s] = 0.0;
-                    cgr[gg][s] = 0.0;
-                }
-            for (int k = 0; k < 3; ++k) {
-                double a3[NEN];
+            double a3[3][NEN];
+            for (int k = 0; k < 3; ++k)
                 for (int i = 0; i < NEN; ++i)
-                    a3[i] = 1.0 / 2 * (ndb[i * NODE + 3 + k] * gNj0 - ndb[i * NODE + k] * gNj1);
-                for (int s = 0; s < NR; ++s) {
-                    double cn = cu[s] * A[0][k] + cv[s] * A[1][k] + cw[s] * A[2][k];
-                    for (int i = 0; i < NEN; ++i) cn += a3[i] * ndb[i * NODE + 6 + s];
-                    for (int gg = 0; gg < 3; ++gg) cgt[gg][s] += cn * Q[k][gg];
-                }
-            }
+                    a3[k][i] = 1.0 / 2 * (ndb[i * NODE + 3 + k] * gNj0 - ndb[i * NODE + k] * gNj1);
+            double r2a[2], r2b[2];
             for (int l = 0; l < 2; ++l) {
-                const double r2a = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
-                const double r2b = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
-                for (int s = 0; s < NR; ++s) {
-                    const double cn = cx[s] * r2a + cy[s] * r2b;
-                    for (int gg = 0; gg < 3; ++gg) cgr[gg][s] += cn * Q[l][gg];
+                r2a[l] = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
+                r2b[l] = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
+            }
+            const MatRows mr = mat_rows(p, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], g[G_SC + 3],
+                                        g[G_SC + 4]);
+            const double *bs = g + G_BS + j2 * 6;
+            double *cdst = sC + le2 * CSTR + 6 * j2;
+            for (int s = 0; s < NR; ++s) {
+                double cu, cv, cw, cx, cy;
+                c_row_q4(mr, s, gNj0, gNj1, bs, cu, cv, cw, cx, cy);
+                double cgt[3] = {0.0, 0.0, 0.0}, cgr[3] = {0.0, 0.0, 0.0};
+                for (int k = 0; k < 3; ++k) {
+                    double cn = cu * A[0][k] + cv * A[1][k] + cw * A[2][k];
+                    for (int i = 0; i < NEN; ++i) cn += a3[k][i] * ndb[i * NODE + 6 + s];
+                    for (int gg = 0; gg < 3; ++gg) cgt[gg] += cn * Q[k][gg];
+                }
+                for (int l = 0; l < 2; ++l) {
+                    const double cn = cx * r2a[l] + cy * r2b[l];
+                    for (int gg = 0; gg < 3; ++gg) cgr[gg] += cn * Q[l][gg];
+                }
+                for (int gg = 0; gg < 3; ++gg) {
+                    cdst[s * K + gg] = cgt[gg];
+                    cdst[s * K + 3 + gg] = cgr[gg];
                 }
             }
-            double *cdst = sC + le2 * CSTR + 6 * j2;
-            for (int s = 0; s < NR; ++s)
-                for (int gg = 0; gg < 3; ++gg) {
-                    cdst[s * K + gg] = cgt[gg][s];
-                    cdst[s * K + 3 + gg] = cgr[gg][s];
-                }
         }
         __syncthreads();
         // ---------------- C: accumulate node-pair blocks ----------------
@@ -1100,46 +1155,54 @@ __global__ __launch_bounds__(WG) void elem_q4_seq_kernel(ElemParams p, const dou
 // Scatter: node-pair blocks -> CSR values
 // ---------------------------------------------------------------------------
 //
-// Thread (node I, row component r, neighbour slot k): sums row r of the 6x6
-// block K_IJ over the pair's contributing element blocks in ascending element
-// order (a block stored as (J, I) is read transposed) and writes the six
-// values into CSR row dof(I, r) at J's column run.  The 16 threads of one
-// (I, r) cover the neighbours in CSR column order, so their 48-byte runs are
-// adjacent and the row is written with coalesced stores.  Every CSR value is
-// written exactly once; no atomics.
-#define FS_SC_NB 16  // neighbour slots per (node, row); more neighbours loop
+// Neighbour slots per (node, row) thread of the slice builder of the
+// mixed-precision operator; more neighbours loop.
+#define FS_SC_NB 16
 
-__global__ __launch_bounds__(256) void scatter_kernel(ScatterPlan sp, const double *__restrict__ Kb,
-                                                      double *__restrict__ vals) {
+// Pair-driven scatter: thread (upper pair (I, J), row component r) sums row r
+// of K_IJ over the pair's contributing element blocks in ascending element
+// order (a block stored as (J, I) is read transposed) and writes the six values
+// into CSR row dof(I, r) at J's column run and, for I != J, the transposed
+// entries into rows dof(J, c) at I's column run.  Each element block is read
+// once (six threads read its six rows), every CSR value is written exactly
+// once, and K is bitwise symmetric by construction.
+__global__ __launch_bounds__(256) void scatter_pair_kernel(ScatterPlan sp, const double *__restrict__ Kb,
+                                                           double *__restrict__ vals) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t I = gid / (6 * FS_SC_NB);
-    if (I >= sp.nnodes) return;
-    const int r = (int)((gid / FS_SC_NB) % 6);
-    const int kk = (int)(gid % FS_SC_NB);
-    const int rs = sp.node_rowstart[I * 6 + r];
-    if (rs < 0) return;
-    const int nb1 = sp.nbr_ptr[I + 1];
-    for (int k = sp.nbr_ptr[I] + kk; k < nb1; k += FS_SC_NB) {
-        const int pr = sp.nbr_pair[k];
-        const int p = pr >> 1;
-        const bool side1 = (pr & 1) != 0;
-        double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        const int ce = sp.contrib_ptr[p + 1];
-        for (int q = sp.contrib_ptr[p]; q < ce; ++q) {
-            const int cv = sp.contrib[q];
-            const double *B = Kb + (int64_t)(cv >> 1) * 36;
-            if (side1 == ((cv & 1) != 0)) {
-                for (int c = 0; c < 6; ++c) sum[c] += B[r * 6 + c];
-            } else {
-                for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
-            }
+    const int64_t p = gid / 6;
+    if (p >= sp.npairs) return;
+    const int r = (int)(gid - 6 * p);
+    const int I = sp.pair_node[2 * p], J = sp.pair_node[2 * p + 1];
+    const int kI = sp.pair_slot[2 * p], kJ = sp.pair_slot[2 * p + 1];
+    double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int ce = sp.contrib_ptr[p + 1];
+    for (int q = sp.contrib_ptr[p]; q < ce; ++q) {
+        const int cv = sp.contrib[q];
+        const double *B = Kb + (int64_t)(cv >> 1) * 36;
+        if ((cv & 1) == 0) {
+            for (int c = 0; c < 6; ++c) sum[c] += B[r * 6 + c];
+        } else {
+            for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
         }
-        const int8_t *code = sp.node_code + (int64_t)sp.nbr_node[k] * 6;
-        const int of = sp.nbr_off[2 * k], ox = sp.nbr_off[2 * k + 1];
+    }
+    const int rsI = sp.node_rowstart[(int64_t)I * 6 + r];
+    if (rsI >= 0) {
+        const int8_t *codeJ = sp.node_code + (int64_t)J * 6;
+        const int of = sp.nbr_off[2 * kI], ox = sp.nbr_off[2 * kI + 1];
         for (int c = 0; c < 6; ++c) {
-            const int cd = code[c];
+            const int cd = codeJ[c];
             if (cd < 0) continue;
-            vals[rs + (cd < 8 ? of + cd : ox + cd - 8)] = sum[c];
+            vals[rsI + (cd < 8 ? of + cd : ox + cd - 8)] = sum[c];
+        }
+    }
+    if (I != J) {
+        const int cdI = sp.node_code[(int64_t)I * 6 + r];
+        if (cdI >= 0) {
+            const int pos = cdI < 8 ? sp.nbr_off[2 * kJ] + cdI : sp.nbr_off[2 * kJ + 1] + cdI - 8;
+            for (int c = 0; c < 6; ++c) {
+                const int rsJ = sp.node_rowstart[(int64_t)J * 6 + c];
+                if (rsJ >= 0) vals[rsJ + pos] = sum[c];
+            }
         }
     }
 }
@@ -1439,7 +1502,7 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
 // blocks, rounded once to FP32.
 
 // K_mb slices from the split element blocks: thread per (node, row, neighbour
-// slot); the sum over the pair's contributions follows scatter_kernel.
+// slot); the sum over the pair's contributions follows scatter_pair_kernel.
 __global__ __launch_bounds__(256) void mixed_build_kernel(ScatterPlan sp, const double *__restrict__ Kb,
                                                           const int64_t *__restrict__ goff, float2 *__restrict__ kmb) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1670,57 +1733,98 @@ __global__ __launch_bounds__(256) void scale_kernel(int64_t n, const double *__r
 // Launch wrappers (host side of this translation unit)
 // ---------------------------------------------------------------------------
 
-template <int NEN, int NQP, int NR, int NE, int WG>
+template <int NEN, int NQP, int NR, int NE, int WG, bool SPLIT>
 static hipError_t launch_elem(const ElemParams &p, const double *x, const double *y, const double *z,
                               const int32_t *conn, const double *nx, const double *ny, const double *nz,
                               const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
     using S = ElemSmem<NEN, NQP, NR, NE>;
     const size_t smem = S::TOTAL * sizeof(double);
-    static_assert(S::TOTAL * sizeof(double) <= 65536, "element kernel LDS exceeds 64 KiB");
+    static_assert(S::TOTAL * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
     static_assert(NE * NQP * NEN <= WG && NE * (NEN * (NEN + 1) / 2) <= WG, "workgroup too small");
+    // set on every launch: cheap, per device, and free of shared host state
+    // when several contexts launch from different threads
+    if (smem > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)elem_kernel<NEN, NQP, NR, NE, WG, SPLIT>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return e;
+    }
     const int nblocks = (p.nelem + NE - 1) / NE;
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
-                       nx, ny, nz, valid, Kb, status);
+    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG, SPLIT>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z,
+                       conn, nx, ny, nz, valid, Kb, status);
     return hipGetLastError();
 }
 
-template <int NE, int WG>
+template <int NE, int WG, bool SPLIT>
 static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
     constexpr int total = NE * 4 * 47 + NE * (8 * 24 + 2) + NE * 4 * 14 + NE * 37;
     const size_t smem = total * sizeof(double);
     static_assert(NE * 10 <= WG, "workgroup too small");
-    // set on every launch: cheap, per device, and free of shared host state
-    // when several contexts launch from different threads
+    static_assert(total * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
     if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG>,
+        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
     const int nblocks = (p.nelem + NE - 1) / NE;
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn, nx, ny,
-                       nz, valid, Kb, status);
+    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
+                       nx, ny, nz, valid, Kb, status);
     return hipGetLastError();
+}
+
+// Launch shapes (elements per workgroup, workgroup size) the autotuner
+// chooses from; every shape computes bitwise the same blocks.
+int element_variants(int nen) { return nen == 3 ? 3 : 3; }
+
+void element_variant_shape(int nen, int variant, int *ne, int *wg) {
+    static const int t3[3][2] = {{16, 128}, {8, 64}, {32, 256}};
+    static const int q4[3][2] = {{16, 192}, {12, 128}, {6, 64}};
+    const int v = (variant >= 0 && variant < 3) ? variant : 0;
+    *ne = nen == 3 ? t3[v][0] : q4[v][0];
+    *wg = nen == 3 ? t3[v][1] : q4[v][1];
 }
 
 hipError_t launch_element_blocks(const ElemParams &p, const double *x, const double *y, const double *z,
                                  const int32_t *conn, const double *nx, const double *ny, const double *nz,
-                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
-    if (p.nen == 3) {
-        if (p.formulation == 1)
-            return launch_elem<3, 1, 12, 16, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
-        return launch_elem<3, 1, 8, 16, 128>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
+                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st, int variant) {
+#define FS_ARGS p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st
+    const bool split = p.split_cs != nullptr;
+    if (split) {
+        if (p.nen == 3) {
+            if (p.formulation == 1) return launch_elem<3, 1, 12, 16, 128, true>(FS_ARGS);
+            return launch_elem<3, 1, 8, 16, 128, true>(FS_ARGS);
+        }
+        return launch_q4_seq<16, 192, true>(FS_ARGS);
     }
-    return launch_q4_seq<16, 192>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st);
+    if (p.nen == 3) {
+        if (p.formulation == 1) {
+            switch (variant) {
+                case 1: return launch_elem<3, 1, 12, 8, 64, false>(FS_ARGS);
+                case 2: return launch_elem<3, 1, 12, 32, 256, false>(FS_ARGS);
+                default: return launch_elem<3, 1, 12, 16, 128, false>(FS_ARGS);
+            }
+        }
+        switch (variant) {
+            case 1: return launch_elem<3, 1, 8, 8, 64, false>(FS_ARGS);
+            case 2: return launch_elem<3, 1, 8, 32, 256, false>(FS_ARGS);
+            default: return launch_elem<3, 1, 8, 16, 128, false>(FS_ARGS);
+        }
+    }
+    switch (variant) {
+        case 1: return launch_q4_seq<12, 128, false>(FS_ARGS);
+        case 2: return launch_q4_seq<6, 64, false>(FS_ARGS);
+        default: return launch_q4_seq<16, 192, false>(FS_ARGS);
+    }
+#undef FS_ARGS
 }
 
 hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals, hipStream_t st) {
-    if (sp.nnodes == 0) return hipSuccess;
-    const int64_t threads = sp.nnodes * 6 * FS_SC_NB;
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sp, Kb, vals);
+    const int64_t threads = sp.npairs * 6;
+    if (threads == 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_pair_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sp, Kb, vals);
     return hipGetLastError();
 }
 

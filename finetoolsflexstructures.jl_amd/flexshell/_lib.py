@@ -105,6 +105,8 @@ SIGNATURES = {
     "fs_model_omega_max": (C.c_int, [C.c_void_p, _dp, C.c_int32, _dp]),
     "fs_model_last_timing": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "fs_model_mixed_info": (C.c_int, [C.c_void_p, _i64p]),
+    "fs_ctx_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i32p]),
+    "fs_ctx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int32]),
 }
 
 _lib = None
@@ -156,6 +158,17 @@ class Context:
         if st != FS_OK:
             msg = self.lib.fs_last_error(self.h).decode()
             raise FlexShellError(f"{what}: {_STATUS.get(st, st)}: {msg}")
+
+    def tuning(self, nen, formulation=FS_SHEAR_AVERAGE_B):
+        """(variant, elements per workgroup, workgroup size) of the element
+        kernel chosen by the autotuner (variant -1: not tuned yet)."""
+        out = np.zeros(3, dtype=np.int32)
+        self.check(self.lib.fs_ctx_tuning(self.h, int(nen), int(formulation), ptr(out, C.c_int32)), "fs_ctx_tuning")
+        return tuple(int(v) for v in out)
+
+    def set_tuning(self, nen, formulation, variant):
+        self.check(self.lib.fs_ctx_set_tuning(self.h, int(nen), int(formulation), int(variant)),
+                   "fs_ctx_set_tuning")
 
     def close(self):
         if self.h:

@@ -1,0 +1,341 @@
+"""
+    FlexShellHIP
+
+Drop-in GPU path for the shell FEMMs of FinEtoolsFlexStructures
+(`FEMMShellT3FF`, `FEMMShellQ4RS`, `FEMMShellT3FFComp`, `FEMMShellQ4RSComp`)
+on one AMD Instinct MI355X, through the C ABI of `libflexshell.so`
+(`include/flexshell.h`).
+
+The FEMM call sites stay as they are; only the assembler changes:
+
+    associategeometry!(femm, geom0)                      # reference code, unchanged
+    K  = stiffness(femm, SysmatAssemblerHIPCSR(), geom0, u1, Rfield1, dchi)
+    Kf = stiffness(femm, SysmatAssemblerHIPCSR(; block = :ff), geom0, u1, Rfield1, dchi)
+    M  = mass(femm, SysmatAssemblerHIPDiag(; block = :ff), geom0, dchi)
+
+`stiffness` returns the same `SparseMatrixCSR{1,Float64,Int64}` that
+`SysmatAssemblerSparseCSRSymm` produces (`AssemblyModule.jl:20-53`): identical
+pattern (structural union of the element dof pairs, zeros retained), values
+within 1e-12 relative.  The nodal normals computed by the reference
+`associategeometry!` are uploaded, so custom `mcsys` closures keep working.
+
+The explicit loop of `examples/shells/dynamics/homogeneous/explicit/
+plate_expl_examples.jl:61-94` runs on the device with [`cd_loop!`](@ref).
+"""
+module FlexShellHIP
+
+using SparseArrays
+using SparseMatricesCSR
+using FinEtools
+using FinEtools.AssemblyModule: AbstractSysmatAssembler
+using FinEtoolsFlexStructures.FEMMShellT3FFModule: FEMMShellT3FF
+using FinEtoolsFlexStructures.FEMMShellQ4RSModule: FEMMShellQ4RS
+using FinEtoolsFlexStructures.FEMMShellT3FFCompModule: FEMMShellT3FFComp
+using FinEtoolsFlexStructures.FEMMShellQ4RSCompModule: FEMMShellQ4RSComp
+using FinEtoolsFlexStructures.CompositeLayupModule: thickness, laminate_stiffnesses!,
+    laminate_transverse_stiffness!, laminate_inertia!
+import FinEtoolsFlexStructures.FEMMShellT3FFModule
+import FinEtoolsFlexStructures.FEMMShellQ4RSModule
+import FinEtoolsFlexStructures.FEMMShellT3FFCompModule
+import FinEtoolsFlexStructures.FEMMShellQ4RSCompModule
+
+const libflexshell = get(ENV, "FLEXSHELL_LIB", joinpath(@__DIR__, "..", "flexshell", "libflexshell.so"))
+
+const FS_OK = Cint(0)
+const FS_BLOCK_ALL = Cint(0)
+const FS_BLOCK_FF = Cint(1)
+const FS_PREC_FP64 = Int32(0)
+const FS_PREC_MIXED = Int32(1)
+
+# ---------------------------------------------------------------- ABI structs
+# Field order and types follow include/flexshell.h exactly.
+struct ShellParams               # fs_shell_params
+    composite::Int32
+    formulation::Int32
+    thickness::Float64
+    stab_c::Float64
+    drilling_stiffness_scale::Float64
+    Dps::NTuple{9,Float64}       # column-major 3 x 3
+    Dt::NTuple{4,Float64}        # column-major 2 x 2 (without the 5/6 factor)
+    ngroups::Int32
+    reserved::Int32
+    group_ABDH::Ptr{Float64}     # ngroups x 31: A 9 | B 9 | D 9 | H 4, column-major
+    group_csys::Ptr{Float64}     # ngroups x 9, column-major
+    group_thickness::Ptr{Float64}
+    elem_group::Ptr{Int32}
+    group_homog::Ptr{Float64}
+end
+
+struct MassParams                # fs_mass_params
+    composite::Int32
+    ngroups::Int32
+    rho::Float64
+    thickness::Float64
+    group_inertia::Ptr{Float64}  # ngroups x 2
+    elem_group::Ptr{Int32}
+    group_rho_t::Ptr{Float64}
+end
+
+struct CDParams                  # fs_cd_params
+    dt::Float64
+    cdamp::Float64
+    nsteps::Int64
+    precision::Int32
+    probe_stride::Int32
+    ngroups::Int32
+    use_graph::Int32
+    row_group::Ptr{Int32}
+    group_dt::Ptr{Float64}
+    group_cdamp::Ptr{Float64}
+end
+
+# ---------------------------------------------------------------- handles
+mutable struct Context
+    h::Ptr{Cvoid}
+    function Context(device::Integer = 0)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:fs_ctx_create, libflexshell), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, out)
+        st == FS_OK || error("fs_ctx_create(device = $device) failed with status $st (a HIP device is required)")
+        c = new(out[])
+        finalizer(c -> ccall((:fs_ctx_destroy, libflexshell), Cvoid, (Ptr{Cvoid},), c.h), c)
+        return c
+    end
+end
+
+function check(ctx::Context, st::Cint, what::AbstractString)
+    st == FS_OK && return nothing
+    msg = unsafe_string(ccall((:fs_last_error, libflexshell), Cstring, (Ptr{Cvoid},), ctx.h))
+    error("$what: status $st: $msg")   # the reference's error() convention
+end
+
+const _default_ctx = Ref{Union{Nothing,Context}}(nothing)
+default_context() = something(_default_ctx[], (_default_ctx[] = Context(parse(Int, get(ENV, "FLEXSHELL_DEVICE", "0")))))
+
+mutable struct DeviceModel
+    ctx::Context
+    h::Ptr{Cvoid}
+    nen::Int
+    geom_key::UInt
+    dofs_key::UInt
+end
+
+function _destroy!(m::DeviceModel)
+    m.h == C_NULL || ccall((:fs_model_destroy, libflexshell), Cvoid, (Ptr{Cvoid},), m.h)
+    m.h = C_NULL
+end
+
+# ---------------------------------------------------------------- assemblers
+"""
+    SysmatAssemblerHIPCSR(ctx = default_context(); block = :all)
+
+Assembler whose `stiffness` runs on the GPU.  `block = :all` gives the
+all-dof matrix of `SysmatAssemblerSparseCSRSymm`; `block = :ff` the free-free
+block of `SysmatAssemblerFFBlock`.  The device model of each FEMM (mesh,
+dofs, normals, CSR pattern, matrices) stays resident between calls.
+"""
+mutable struct SysmatAssemblerHIPCSR <: AbstractSysmatAssembler
+    ctx::Context
+    block::Symbol
+    models::IdDict{Any,DeviceModel}
+end
+SysmatAssemblerHIPCSR(ctx::Context = default_context(); block::Symbol = :all) =
+    SysmatAssemblerHIPCSR(ctx, block, IdDict{Any,DeviceModel}())
+
+"""
+    SysmatAssemblerHIPDiag(ctx = default_context(); block = :all)
+
+Lumped-mass counterpart of `SysmatAssemblerSparseDiag` /
+`SysmatAssemblerFFBlock(SysmatAssemblerSparseDiag(), nf, nf)`.
+"""
+mutable struct SysmatAssemblerHIPDiag <: AbstractSysmatAssembler
+    ctx::Context
+    block::Symbol
+    models::IdDict{Any,DeviceModel}
+end
+SysmatAssemblerHIPDiag(ctx::Context = default_context(); block::Symbol = :all) =
+    SysmatAssemblerHIPDiag(ctx, block, IdDict{Any,DeviceModel}())
+
+_block(a) = a.block === :ff ? FS_BLOCK_FF : FS_BLOCK_ALL
+
+# ---------------------------------------------------------------- model upload
+_nen(::Union{FEMMShellT3FF,FEMMShellT3FFComp}) = 3
+_nen(::Union{FEMMShellQ4RS,FEMMShellQ4RSComp}) = 4
+
+function _device_model!(a, self, geom0::NodalField{Float64}, dchi::NodalField)
+    ctx = a.ctx
+    nen = _nen(self)
+    X = geom0.values                                   # N x 3, column-major (SoA)
+    gkey = hash(X, hash(self.integdomain.fes.conn))
+    m = get(a.models, self, nothing)
+    if m === nothing || m.geom_key != gkey
+        m === nothing || _destroy!(m)
+        conn = connasarray(self.integdomain.fes)       # E x nen, 1-based
+        connrm = permutedims(conn)                     # row-major E x nen
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve X connrm begin
+            check(ctx, ccall((:fs_model_create, libflexshell), Cint,
+                             (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Cint, Ptr{Ptr{Cvoid}}),
+                             ctx.h, nen, size(X, 1), X, size(conn, 1), connrm, 1, out), "fs_model_create")
+        end
+        m = DeviceModel(ctx, out[], nen, gkey, UInt(0))
+        finalizer(_destroy!, m)
+        a.models[self] = m
+    end
+    dkey = hash(dchi.dofnums, hash(nfreedofs(dchi)))
+    if m.dofs_key != dkey
+        dn = dchi.dofnums                              # N x 6, column-major, 1-based
+        GC.@preserve dn begin
+            check(ctx, ccall((:fs_model_set_dofs, libflexshell), Cint,
+                             (Ptr{Cvoid}, Ptr{Int64}, Int64, Int64, Cint),
+                             m.h, dn, nalldofs(dchi), nfreedofs(dchi), 1), "fs_model_set_dofs")
+        end
+        m.dofs_key = dkey
+    end
+    # normals from the reference associategeometry! (any mcsys closure)
+    nrm = self._normals
+    valid = UInt8.(self._normal_valid)
+    GC.@preserve nrm valid begin
+        check(ctx, ccall((:fs_model_normals_upload, libflexshell), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{UInt8}),
+                         m.h, nrm, valid), "fs_model_normals_upload")
+    end
+    return m
+end
+
+# stab_fun(t, h) = t^2 / (t^2 + c h^2) crosses the ABI as c
+_stab_coefficient(f) = 1 / f(1.0, 1.0) - 1.0
+
+function _thickness(self)
+    # constant thickness (every in-repo use): evaluated at the first element,
+    # as stiffness() does per element (FEMMShellT3FFModule.jl:677)
+    fes = self.integdomain.fes
+    return self.integdomain.otherdimension([1/3 1/3 1/3], fes.conn[1], [1/3 1/3])
+end
+
+function _homog_params(self)
+    Dps, Dt = FEMMShellT3FFModule._shell_material_stiffness(self.material)
+    return ShellParams(0, self.transv_shear_formulation, _thickness(self), _stab_coefficient(self.stab_fun),
+                       self.drilling_stiffness_scale, Tuple(vec(Dps)), Tuple(vec(Dt)), 0, 0,
+                       C_NULL, C_NULL, C_NULL, C_NULL, C_NULL)
+end
+
+function _layup_arrays(self)
+    ng = length(self.layup_groups)
+    abdh = zeros(31, ng); csys = zeros(9, ng); tg = zeros(ng); inert = zeros(2, ng)
+    for (g, (layup, _)) in enumerate(self.layup_groups)
+        A = zeros(3, 3); B = zeros(3, 3); D = zeros(3, 3); H = zeros(2, 2)
+        laminate_stiffnesses!(layup, A, B, D)
+        laminate_transverse_stiffness!(layup, H)
+        abdh[:, g] = vcat(vec(A), vec(B), vec(D), vec(H))
+        csys[:, g] = vec(csmat(layup.csys))
+        tg[g] = thickness(layup)
+        inert[:, g] .= laminate_inertia!(layup)
+    end
+    grp = Int32.(self._layup_group_lookup .- 1)
+    return abdh, csys, tg, inert, grp
+end
+
+# ---------------------------------------------------------------- stiffness
+function _stiffness(self, assembler::SysmatAssemblerHIPCSR, geom0, dchi)
+    @assert self._associatedgeometry == true
+    m = _device_model!(assembler, self, geom0, dchi)
+    ctx, block = assembler.ctx, _block(assembler)
+    nr, nz = Ref{Int64}(0), Ref{Int64}(0)
+    check(ctx, ccall((:fs_model_pattern, libflexshell), Cint, (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Int64}),
+                     m.h, block, nr, nz), "fs_model_pattern")
+    rowptr = Vector{Int64}(undef, nr[] + 1)
+    colval = Vector{Int64}(undef, nz[])
+    nzval = Vector{Float64}(undef, nz[])
+    check(ctx, ccall((:fs_model_pattern_get, libflexshell), Cint,
+                     (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Int64}, Cint), m.h, block, rowptr, colval, 1),
+          "fs_model_pattern_get")
+    if self isa Union{FEMMShellT3FFComp,FEMMShellQ4RSComp}
+        abdh, csys, tg, _, grp = _layup_arrays(self)
+        GC.@preserve abdh csys tg grp begin
+            p = Ref(ShellParams(1, self.transv_shear_formulation, 0.0, _stab_coefficient(self.stab_fun),
+                                self.drilling_stiffness_scale, ntuple(_ -> 0.0, 9), ntuple(_ -> 0.0, 4),
+                                length(tg), 0, pointer(abdh), pointer(csys), pointer(tg), pointer(grp), C_NULL))
+            check(ctx, ccall((:fs_model_stiffness, libflexshell), Cint, (Ptr{Cvoid}, Ptr{ShellParams}, Cint, Ptr{Float64}),
+                             m.h, p, block, nzval), "fs_model_stiffness")
+        end
+    else
+        p = Ref(_homog_params(self))
+        check(ctx, ccall((:fs_model_stiffness, libflexshell), Cint, (Ptr{Cvoid}, Ptr{ShellParams}, Cint, Ptr{Float64}),
+                         m.h, p, block, nzval), "fs_model_stiffness")
+    end
+    n = Int(nr[])
+    return SparseMatrixCSR{1}(n, n, rowptr, colval, nzval)
+end
+
+for (M, F) in ((FEMMShellT3FFModule, FEMMShellT3FF), (FEMMShellQ4RSModule, FEMMShellQ4RS),
+               (FEMMShellT3FFCompModule, FEMMShellT3FFComp), (FEMMShellQ4RSCompModule, FEMMShellQ4RSComp))
+    @eval function $M.stiffness(self::$F, assembler::SysmatAssemblerHIPCSR, geom0::NodalField{Float64},
+                                u1::NodalField{TI}, Rfield1::NodalField{TI}, dchi::NodalField{TI}) where {TI<:Number}
+        return _stiffness(self, assembler, geom0, dchi)
+    end
+    @eval function $M.mass(self::$F, assembler::SysmatAssemblerHIPDiag, geom0::NodalField{Float64},
+                           dchi::NodalField{TI}) where {TI<:Number}
+        return _mass(self, assembler, geom0, dchi)
+    end
+end
+
+# ---------------------------------------------------------------- mass
+function _mass(self, assembler::SysmatAssemblerHIPDiag, geom0, dchi)
+    @assert self._associatedgeometry == true
+    m = _device_model!(assembler, self, geom0, dchi)
+    ctx, block = assembler.ctx, _block(assembler)
+    n = block == FS_BLOCK_FF ? nfreedofs(dchi) : nalldofs(dchi)
+    d = Vector{Float64}(undef, n)
+    if self isa Union{FEMMShellT3FFComp,FEMMShellQ4RSComp}
+        _, _, _, inert, grp = _layup_arrays(self)
+        GC.@preserve inert grp begin
+            p = Ref(MassParams(1, size(inert, 2), 0.0, 0.0, pointer(inert), pointer(grp), C_NULL))
+            check(ctx, ccall((:fs_model_mass, libflexshell), Cint, (Ptr{Cvoid}, Ptr{MassParams}, Cint, Ptr{Float64}),
+                             m.h, p, block, d), "fs_model_mass")
+        end
+    else
+        p = Ref(MassParams(0, 0, massdensity(self.material), _thickness(self), C_NULL, C_NULL, C_NULL))
+        check(ctx, ccall((:fs_model_mass, libflexshell), Cint, (Ptr{Cvoid}, Ptr{MassParams}, Cint, Ptr{Float64}),
+                         m.h, p, block, d), "fs_model_mass")
+    end
+    return spdiagm(0 => d)
+end
+
+# ---------------------------------------------------------------- explicit loop
+"""
+    cd_loop!(femm, assembler, ksi_omega, U0, V0, tend, dt, Fmag, g; probes = Int[], stride = 1, mixed = false)
+
+`_cd_loop!` (plate_expl_examples.jl:61-94) on the FF-block stiffness and mass
+left resident by `stiffness(femm, SysmatAssemblerHIPCSR(; block = :ff), ...)`
+and `mass(femm, SysmatAssemblerHIPDiag(; block = :ff), ...)` (same assembler
+context).  The force closure is `F(t) = g(t) * Fmag`; `peek` becomes the probe
+dofs (1-based) recorded every `stride` steps.  Returns (U, V, probe history).
+"""
+function cd_loop!(femm, assembler, cdamp, U0, V0, tend, dt, Fmag, g; probes = Int[], stride = 1, mixed = false)
+    m = assembler.models[femm]
+    nsteps = Int(round(tend / dt))
+    if nsteps * dt < tend                      # plate_expl_examples.jl:72-75
+        dt = tend / (nsteps + 1)
+    end
+    t = 0.0
+    gv = zeros(nsteps + 1)
+    gv[1] = g(0.0)
+    for n in 1:nsteps
+        t = t + dt
+        gv[n + 1] = g(t)
+    end
+    n = length(U0)
+    U = similar(U0); V = similar(V0)
+    pr = Int64.(probes .- 1)
+    pout = zeros(length(pr), nsteps ÷ stride + 1)
+    p = Ref(CDParams(dt, cdamp, nsteps, mixed ? FS_PREC_MIXED : FS_PREC_FP64, stride, 0, 1, C_NULL, C_NULL, C_NULL))
+    check(m.ctx, ccall((:fs_model_explicit, libflexshell), Cint,
+                       (Ptr{Cvoid}, Ptr{CDParams}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                        Ptr{Int64}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                       m.h, p, U0, V0, Fmag, gv, isempty(pr) ? C_NULL : pr, length(pr),
+                       isempty(pr) ? C_NULL : pout, U, V), "fs_model_explicit")
+    return U, V, permutedims(pout)
+end
+
+export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context
+
+end # module

@@ -252,6 +252,19 @@ int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *block
  * an operator built earlier). */
 int fs_model_last_timing(const fs_model *m, int which, double *ms);
 
+/* Element-kernel launch shape chosen by the autotuner for a family (FS_T3 /
+ * FS_Q4) and shear formulation on this context (no counterpart in the
+ * reference, which has no GPU launch shapes): shape[0] = variant index (-1
+ * before the first stiffness evaluation of that kind), shape[1] = elements
+ * per workgroup, shape[2] = workgroup size.  The autotuner times every shape
+ * once on the first evaluation and keeps the fastest; every shape computes
+ * bitwise the same element blocks. */
+int fs_ctx_tuning(const fs_ctx *ctx, int nen, int formulation, int32_t *shape);
+
+/* Fix the launch shape (variant >= 0), re-arm the autotuner (-1), or switch
+ * it off so that the default shape is used (-2). */
+int fs_ctx_set_tuning(fs_ctx *ctx, int nen, int formulation, int32_t variant);
+
 /* Storage of the mixed-precision operator built by the last FS_PREC_MIXED /
  * FS_PREC_MIXED_F32 loop (no counterpart in the reference, whose loop runs on
  * the FP64 CSR matrix only): info[0] FP32 value pairs of the membrane +
