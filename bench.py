@@ -129,12 +129,12 @@ def bench_batch(nmodels, nsteps):
 def mixed_step_bytes(N, E, nen, kmb_pairs, nsr, nnb, fmag_frac):
     """Algorithmic bytes of one mixed-precision step: FP32 K_mb slices, the
     neighbour lists, FP64 shear factor rows, drilling blocks, incidence and
-    connectivity, and the FP64 state over 6 N positions (u gathered once;
-    U V A invMC read, Fmag read on the steps with a nonzero load factor
-    (fraction fmag_frac), U V A written)."""
+    connectivity, node coordinates, and the FP64 state over 6 N positions
+    (u gathered once; U V A invMC read, Fmag read on the steps with a nonzero
+    load factor (fraction fmag_frac), U V A written)."""
     n6 = 6 * N
     return (kmb_pairs * 8 + nnb * 4 + (N + 1) * 4 + E * nsr * 6 * nen * 8 + N * 48 + E * nen * 4 * 2
-            + (N + 1) * 4 + n6 * (8 + 32 + 8 * fmag_frac + 24))
+            + (N + 1) * 4 + N * 24 + n6 * (8 + 32 + 8 * fmag_frac + 24))
 
 
 def bench_mixed(nsteps, nx=1024, ny=512):

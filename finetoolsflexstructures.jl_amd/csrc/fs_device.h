@@ -53,15 +53,20 @@ struct ElemParams {
 // Mixed-precision operator of one model for the explicit step, on node-major
 // dof positions R = 6 i + c (all dofs; fixed positions carry invMC = 0 and
 // stay at zero).
-//   kmb    membrane + bending stiffness, FP32, SELL layout with 64-row slices:
-//          slice g (rows 64g..64g+63) holds nsl_g neighbour slots; the value
-//          pair (2 c2, 2 c2 + 1) of slot s for row R sits at
-//          kmb[goff[g] + (3 s + c2) * 64 + (R & 63)] (float2 units).  Slot s of
-//          row R refers to the s-th neighbour of node i = R / 6 (nbr lists,
-//          sorted, including i); slots past that node's count are zero.
+//   kmb    membrane + bending stiffness, FP32, in slices of 10 nodes (60
+//          rows, one wavefront per slice, lane = 6 (i mod 10) + c): slice g
+//          holds nsl_g neighbour slots and the value pair (2 c2, 2 c2 + 1) of
+//          slot s for lane l sits at kmb[goff[g] + (3 s + c2) * 60 + l]
+//          (float2 units).  Slot s of a row refers to the s-th neighbour of
+//          its node (nbr lists, sorted, including the node); slots past that
+//          node's count are zero.
 //   cs     transverse-shear factor rows per element (FP64), K_s = sum C^T C
 //   dr     drilling 3 x 3 rotation block per node, FP64, symmetric storage
 //          (00, 01, 02, 11, 12, 22)
+// The membrane + bending product is formed on displacements relative to the
+// rigid motion of the row's node (K_mb annihilates rigid motions), so the
+// FP32 rounding of K_mb multiplies small differences instead of the full
+// displacement.
 struct MixedOp {
     int64_t nrows;            // 6 N
     int32_t nnodes;
@@ -75,6 +80,7 @@ struct MixedOp {
     const int32_t *ne_list;
     const int32_t *conn;      // conn[a * E + e]
     const double *cs;         // E x nsr x 6 nen
+    const double *x, *y, *z;  // node coordinates
 };
 
 // Scatter plan of one block (ALL or FF) -- see scatter_pair_kernel.

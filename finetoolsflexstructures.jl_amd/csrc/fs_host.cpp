@@ -609,14 +609,15 @@ int build_mixed(fs_model *m) {
     const int K = 6 * nen;
     const int nsr = nen == 3 ? (m->last_ep.formulation == FS_SHEAR_AVERAGE_K ? 6 : 2) : 8;
     const int64_t N = m->N, E = m->E, nrows = 6 * N;
-    // slices of 64 rows: slot count = largest neighbour count of the slice's nodes
-    const int64_t ns = (nrows + 63) / 64;
+    // slices of 10 nodes (60 rows): slot count = largest neighbour count of the slice's nodes
+    (void)nrows;
+    const int64_t ns = (N + 9) / 10;
     std::vector<int64_t> goff(ns + 1, 0);
     for (int64_t g = 0; g < ns; ++g) {
-        const int64_t i0 = (64 * g) / 6, i1 = std::min<int64_t>(N - 1, (64 * g + 63) / 6);
+        const int64_t i0 = 10 * g, i1 = std::min<int64_t>(N - 1, 10 * g + 9);
         int mx = 0;
         for (int64_t i = i0; i <= i1; ++i) mx = std::max(mx, m->nbr_ptr[i + 1] - m->nbr_ptr[i]);
-        goff[g + 1] = goff[g] + (int64_t)mx * 3 * 64;
+        goff[g + 1] = goff[g] + (int64_t)mx * 3 * 60;
     }
     int s;
     DevBuf kbm, dre;
@@ -669,6 +670,9 @@ fs::MixedOp mixed_op(const fs_model *m) {
     op.ne_list = m->dne_list.as<int32_t>();
     op.conn = m->dconn.as<int32_t>();
     op.cs = m->mixed.cs.as<double>();
+    op.x = m->x.as<double>();
+    op.y = m->y.as<double>();
+    op.z = m->z.as<double>();
     return op;
 }
 

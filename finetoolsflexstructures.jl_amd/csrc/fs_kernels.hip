@@ -1510,8 +1510,7 @@ __global__ __launch_bounds__(256) void mixed_build_kernel(ScatterPlan sp, const 
     if (I >= sp.nnodes) return;
     const int r = (int)((gid / FS_SC_NB) % 6);
     const int kk = (int)(gid % FS_SC_NB);
-    const int64_t R = 6 * I + r;
-    float2 *dst = kmb + goff[R >> 6] + (R & 63);
+    float2 *dst = kmb + goff[I / 10] + 6 * (I % 10) + r;
     const int nb0 = sp.nbr_ptr[I], nb1 = sp.nbr_ptr[I + 1];
     for (int k = nb0 + kk; k < nb1; k += FS_SC_NB) {
         const int pr = sp.nbr_pair[k];
@@ -1530,7 +1529,7 @@ __global__ __launch_bounds__(256) void mixed_build_kernel(ScatterPlan sp, const 
         }
         const int slot = k - nb0;
         for (int c2 = 0; c2 < 3; ++c2)
-            dst[(3 * slot + c2) * 64] = make_float2((float)sum[2 * c2], (float)sum[2 * c2 + 1]);
+            dst[(3 * slot + c2) * 60] = make_float2((float)sum[2 * c2], (float)sum[2 * c2 + 1]);
     }
 }
 
@@ -1560,10 +1559,19 @@ __global__ __launch_bounds__(256) void mixed_drill_kernel(int64_t N, int E, cons
     o[5] = d[8];
 }
 
-// One fused central-difference step on the mixed operator: thread per dof
-// position; the update is the one of cd_step_kernel.  F32PROD: the K_mb
-// products and their sum are formed in FP32 (u rounded to FP32); otherwise
-// the FP32 values are widened and accumulated in FP64.
+// One fused central-difference step on the mixed operator: one wavefront per
+// slice of 10 nodes, lane 6 (i mod 10) + r for dof r of node i (lanes 60..63
+// idle); the update is the one of cd_step_kernel.
+//   K_mb: the row sum runs over the node's neighbours j on the displacement
+//     relative to the rigid motion of node i, du_j = u_j - t_i - th_i x
+//     (x_j - x_i), dth_j = th_j - th_i; K_mb annihilates rigid motions, so
+//     this equals (K_mb u)_R while the FP32 values multiply O(h) differences.
+//     F32PROD: products and their sum in FP32; otherwise widened to FP64.
+//   K_s: for every incident element e (ascending), lane r forms the partial
+//     sums of s_e = C_e u_e over its own component r; the six lanes of the
+//     node add the partials in a fixed order, and each lane adds
+//     C_e[:, (a, r)]' s_e.  FP64 throughout.
+//   K_d: drilling block times the node's rotations, FP64.
 template <int NEN, int NSR, bool F32PROD>
 __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     MixedOp op, const double *__restrict__ Ucur, double *__restrict__ Unext, double *__restrict__ V,
@@ -1572,65 +1580,76 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     const int32_t *__restrict__ probe_slot, double *__restrict__ probe_dst, const int32_t *__restrict__ row_group,
     const double *__restrict__ group_dt, const double *__restrict__ gtab) {
     constexpr int K = 6 * NEN;
-    const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (R >= op.nrows) return;
-    const int i = (int)(R / 6);
-    const int r = (int)(R - 6 * (int64_t)i);
-    const int64_t g0 = op.goff[R >> 6];
-    const int nsl = (int)((op.goff[(R >> 6) + 1] - g0) / 192);
-    const float2 *kv = op.kmb + g0 + (R & 63);
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g = gid >> 6;  // slice = wavefront
+    const int l = (int)(gid & 63);
+    if (l >= 60) return;
+    const int64_t i64 = 10 * g + l / 6;
+    if (i64 >= op.nnodes) return;
+    const int i = (int)i64;
+    const int r = l % 6;
+    const int64_t R = 6 * i64 + r;
+    const int base = (int)(threadIdx.x & 63) - r;  // lane of the node's component 0
+    const double xi = op.x[i], yi = op.y[i], zi = op.z[i];
+    const double *ui = Ucur + 6 * i64;
+    const double t0 = ui[0], t1 = ui[1], t2 = ui[2], h0 = ui[3], h1 = ui[4], h2 = ui[5];
+    const int64_t g0 = op.goff[g];
+    const int nsl = (int)((op.goff[g + 1] - g0) / 180);
+    const float2 *kv = op.kmb + g0 + l;
     const int nb0 = op.nbr_ptr[i];
     const int cnt = op.nbr_ptr[i + 1] - nb0;
     double e = 0.0;
     float ef = 0.0f;
     for (int s = 0; s < nsl && s < cnt; ++s) {
-        const float2 k0 = kv[(3 * s + 0) * 64];
-        const float2 k1 = kv[(3 * s + 1) * 64];
-        const float2 k2 = kv[(3 * s + 2) * 64];
-        const double *uj = Ucur + 6 * (int64_t)op.nbr_node[nb0 + s];
+        const float2 k0 = kv[(3 * s + 0) * 60];
+        const float2 k1 = kv[(3 * s + 1) * 60];
+        const float2 k2 = kv[(3 * s + 2) * 60];
+        const int j = op.nbr_node[nb0 + s];
+        const double *uj = Ucur + 6 * (int64_t)j;
+        const double dx = op.x[j] - xi, dy = op.y[j] - yi, dz = op.z[j] - zi;
+        // u_j minus the rigid motion of node i evaluated at node j
+        const double d0 = uj[0] - (t0 + (h1 * dz - h2 * dy));
+        const double d1 = uj[1] - (t1 + (h2 * dx - h0 * dz));
+        const double d2 = uj[2] - (t2 + (h0 * dy - h1 * dx));
+        const double d3 = uj[3] - h0, d4 = uj[4] - h1, d5 = uj[5] - h2;
         if constexpr (F32PROD) {
-            ef += k0.x * (float)uj[0] + k0.y * (float)uj[1] + k1.x * (float)uj[2] + k1.y * (float)uj[3] +
-                  k2.x * (float)uj[4] + k2.y * (float)uj[5];
+            ef += k0.x * (float)d0 + k0.y * (float)d1 + k1.x * (float)d2 + k1.y * (float)d3 + k2.x * (float)d4 +
+                  k2.y * (float)d5;
         } else {
-            e += (double)k0.x * uj[0] + (double)k0.y * uj[1] + (double)k1.x * uj[2] + (double)k1.y * uj[3] +
-                 (double)k2.x * uj[4] + (double)k2.y * uj[5];
+            e += (double)k0.x * d0 + (double)k0.y * d1 + (double)k1.x * d2 + (double)k1.y * d3 +
+                 (double)k2.x * d4 + (double)k2.y * d5;
         }
     }
     if constexpr (F32PROD) e = (double)ef;
     // drilling (FP64)
     if (r >= 3) {
-        const double *d = op.dr + (int64_t)i * 6;
-        const double *th = Ucur + 6 * (int64_t)i + 3;
+        const double *d = op.dr + i64 * 6;
         const int rr = r - 3;
         const double d0 = rr == 0 ? d[0] : (rr == 1 ? d[1] : d[2]);
         const double d1 = rr == 0 ? d[1] : (rr == 1 ? d[3] : d[4]);
         const double d2 = rr == 0 ? d[2] : (rr == 1 ? d[4] : d[5]);
-        e += d0 * th[0] + d1 * th[1] + d2 * th[2];
+        e += d0 * h0 + d1 * h1 + d2 * h2;
     }
-    // transverse shear (FP64), incident elements in ascending order
+    // transverse shear (FP64), incident elements in ascending order; the six
+    // lanes of the node share the element products through cross-lane reads
     const int E = op.nelem;
     for (int k = op.ne_ptr[i]; k < op.ne_ptr[i + 1]; ++k) {
         const int el = op.ne_list[k];
-        int nodes[NEN];
-        int a = 0;
-        for (int b = 0; b < NEN; ++b) {
-            nodes[b] = op.conn[b * E + el];
-            if (nodes[b] == i) a = b;
-        }
         const double *C = op.cs + (int64_t)el * NSR * K;
-        double sv[NSR];
-        for (int q = 0; q < NSR; ++q) sv[q] = 0.0;
+        int a = 0;
+        double part[NSR];
+        for (int q = 0; q < NSR; ++q) part[q] = 0.0;
         for (int b = 0; b < NEN; ++b) {
-            const double *ub = Ucur + 6 * (int64_t)nodes[b];
-            double u6[6];
-            for (int c = 0; c < 6; ++c) u6[c] = ub[c];
-            for (int q = 0; q < NSR; ++q) {
-                const double *cq = C + q * K + 6 * b;
-                sv[q] += cq[0] * u6[0] + cq[1] * u6[1] + cq[2] * u6[2] + cq[3] * u6[3] + cq[4] * u6[4] +
-                         cq[5] * u6[5];
-            }
+            const int nb = op.conn[b * E + el];
+            if (nb == i) a = b;
+            const double ub = Ucur[6 * (int64_t)nb + r];
+            for (int q = 0; q < NSR; ++q) part[q] += C[q * K + 6 * b + r] * ub;
         }
-        for (int q = 0; q < NSR; ++q) e += C[q * K + 6 * a + r] * sv[q];
+        for (int q = 0; q < NSR; ++q) {
+            double sq = 0.0;
+            for (int c = 0; c < 6; ++c) sq += __shfl(part[q], base + c, 64);
+            e += C[q * K + 6 * a + r] * sq;
+        }
     }
     double dtr = dt, gr = gstep;
     if (row_group) {
@@ -1648,7 +1667,7 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     double vn = v + dt_2 * a0;
     const double an = invMC[R] * f;
     vn += dt_2 * an;
-    const double u = Ucur[R];
+    const double u = ui[r];
     V[R] = vn;
     A[R] = an;
     Unext[R] = u + (dtr * vn + dt2_2 * an);
@@ -1949,8 +1968,9 @@ hipError_t launch_cd_mixed_step(int nen, int nsr, bool f32prod, const MixedOp &o
                                 double gstep, double dt, int damped, int forced, const int32_t *probe_slot,
                                 double *probe_dst, const int32_t *row_group, const double *group_dt,
                                 const double *gtab, hipStream_t st) {
-    if (op.nrows == 0) return hipSuccess;
-    const unsigned blocks = (unsigned)((op.nrows + 255) / 256);
+    if (op.nnodes == 0) return hipSuccess;
+    const int64_t threads = (int64_t)((op.nnodes + 9) / 10) * 64;  // one wavefront per 10-node slice
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
 #define FS_MX_CASE(NEN_, NSR_, F32_)                                                                             \
     if (nen == NEN_ && nsr == NSR_ && f32prod == F32_) {                                                        \
         hipLaunchKernelGGL((cd_mixed_step_kernel<NEN_, NSR_, F32_>), dim3(blocks), dim3(256), 0, st, op, Ucur,   \
