@@ -270,12 +270,12 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=500)
     ap.add_argument("--no-serve-plain", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: 3 assemblies, 50 CD steps, then a 1 GiB torch "
-                         "device copy that calibrates the byte counters")
+                    help="short run for rocprofv3: 3 assemblies, 50 CD steps, 50 steps of each explicit "
+                         "precision on the plate, then a 1 GiB device copy that calibrates the byte counters")
     args = ap.parse_args()
     if args.profile:
         args.steps, args.warmup, args.cd_steps, args.no_cpu, args.no_batch = 3, 1, 50, True, True
-        args.no_serve = args.no_mixed = True
+        args.no_serve, args.mixed_steps = True, 50
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(shard design only); run with --gpus 1", file=sys.stderr)
@@ -372,6 +372,7 @@ def main():
                         "frac": ach_cd / HBM_PEAK_GBS, "kernel": "cd_step_kernel",
                         "algorithmic_bytes": b_cd},
         "kernels_ms": {"elem_kernel": float(np.mean(elem_ms)), "scatter_kernel": float(np.mean(scat_ms))},
+        "elem_launch_shape": dict(zip(("variant", "elements_per_workgroup", "workgroup"), ctx.tuning(4, 0))),
     }
     if not args.no_cpu:
         import scipy.sparse as sp

@@ -127,6 +127,18 @@ __device__ __forceinline__ void homog_factors(const double *Lps, const double *L
     for (int i = 0; i < 4; ++i) Ls[i] = s_s * Lt[i];
 }
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b runs on XCD b mod 8) and every XCD has its own L2.  Renumber
+// so that each XCD processes one contiguous range of the logical blocks:
+// neighbouring rows / nodes then share an L2, and the gathers of a sparse
+// row (u, coordinates, element factors) hit there instead of HBM.  A
+// bijection for any block count.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
+    const int64_t q = nblocks / 8, r = nblocks % 8;
+    const int64_t x = b % 8, k = b / 8;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // _NodalTriadsE (FEMMShellT3FFModule.jl:355-388): rotation taking the
 // element normal e3 to the nodal normal nk_e (element frame); the rotation
 // vector is r = e3 x nk_e and its length |r| = sin(theta) is used as the
@@ -226,11 +238,13 @@ template <int NSH>
 __device__ __forceinline__ void c_row_sh(const double Ls[4], int s, const double *bs, double &cu, double &cv,
                                          double &cw, double &cx, double &cy) {
     const int o = (s - 6) / 2, rr = (s - 6) % 2;
+    // select the factor row without indexing (keeps Ls in registers)
+    const double l0 = rr ? Ls[2] : Ls[0], l1 = rr ? Ls[3] : Ls[1];
     cu = 0.0;
     cv = 0.0;
-    cw = Ls[rr * 2 + 0] * bs[0 * NSH + 2 * o] + Ls[rr * 2 + 1] * bs[0 * NSH + 2 * o + 1];
-    cx = Ls[rr * 2 + 0] * bs[1 * NSH + 2 * o] + Ls[rr * 2 + 1] * bs[1 * NSH + 2 * o + 1];
-    cy = Ls[rr * 2 + 0] * bs[2 * NSH + 2 * o] + Ls[rr * 2 + 1] * bs[2 * NSH + 2 * o + 1];
+    cw = l0 * bs[0 * NSH + 2 * o] + l1 * bs[0 * NSH + 2 * o + 1];
+    cx = l0 * bs[1 * NSH + 2 * o] + l1 * bs[1 * NSH + 2 * o + 1];
+    cy = l0 * bs[2 * NSH + 2 * o] + l1 * bs[2 * NSH + 2 * o + 1];
 }
 
 __device__ __forceinline__ void c_row_q4(const MatRows &mr, int s, double gN0, double gN1, const double *bs,
@@ -1168,7 +1182,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
 // once, and K is bitwise symmetric by construction.
 __global__ __launch_bounds__(256) void scatter_pair_kernel(ScatterPlan sp, const double *__restrict__ Kb,
                                                            double *__restrict__ vals) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t p = gid / 6;
     if (p >= sp.npairs) return;
     const int r = (int)(gid - 6 * p);
@@ -1447,7 +1461,7 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
                                                       const int32_t *__restrict__ row_group,
                                                       const double *__restrict__ group_dt,
                                                       const double *__restrict__ gtab) {
-    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t row = gt / TPR;
     const int lane = (int)(gt % TPR);
     const bool active = row < n;
@@ -1505,7 +1519,7 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
 // slot); the sum over the pair's contributions follows scatter_pair_kernel.
 __global__ __launch_bounds__(256) void mixed_build_kernel(ScatterPlan sp, const double *__restrict__ Kb,
                                                           const int64_t *__restrict__ goff, float2 *__restrict__ kmb) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t I = gid / (6 * FS_SC_NB);
     if (I >= sp.nnodes) return;
     const int r = (int)((gid / FS_SC_NB) % 6);
@@ -1580,7 +1594,7 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     const int32_t *__restrict__ probe_slot, double *__restrict__ probe_dst, const int32_t *__restrict__ row_group,
     const double *__restrict__ group_dt, const double *__restrict__ gtab) {
     constexpr int K = 6 * NEN;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t g = gid >> 6;  // slice = wavefront
     const int l = (int)(gid & 63);
     if (l >= 60) return;
@@ -1690,7 +1704,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__r
                                                    const int32_t *__restrict__ colidx,
                                                    const double *__restrict__ kval, const double *__restrict__ x,
                                                    double *__restrict__ y) {
-    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t row = gt / TPR;
     const int lane = (int)(gt % TPR);
     double e = 0.0;
