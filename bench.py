@@ -211,8 +211,9 @@ def bench_serve(nreq, rate, slots, plain=True, steps=500):
         return [S.Request(s["rid"], s["kind"], m, steps=s["steps"], arrival=s["arrival"], seed=s["rid"])
                 for s, m in zip(specs, models)]
 
-    out = {"requests": nreq, "rate_req_s": rate, "explicit_steps": steps,
-           "elements_total": int(sum(m["conn"].shape[0] for m in models))}
+    out = {"requests": nreq, "rate_req_s": rate, "explicit_steps": steps, "explicit_precision": "mixed",
+           "elements_total": int(sum(m["conn"].shape[0] for m in models)),
+           "kinds": {k: sum(s["kind"] == k for s in specs) for k in ("assembly", "modal", "explicit")}}
     runs = [("served", slots)] + ([("plain", 1)] if plain else [])
     for name, ns in runs:
         reqs = stream()

@@ -1619,13 +1619,14 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
         const float2 k1 = kv[(3 * s + 1) * 60];
         const float2 k2 = kv[(3 * s + 2) * 60];
         const int j = op.nbr_node[nb0 + s];
-        const double *uj = Ucur + 6 * (int64_t)j;
+        const double2 *uj = reinterpret_cast<const double2 *>(Ucur + 6 * (int64_t)j);  // 16-byte aligned
+        const double2 u01 = uj[0], u23 = uj[1], u45 = uj[2];
         const double dx = op.x[j] - xi, dy = op.y[j] - yi, dz = op.z[j] - zi;
         // u_j minus the rigid motion of node i evaluated at node j
-        const double d0 = uj[0] - (t0 + (h1 * dz - h2 * dy));
-        const double d1 = uj[1] - (t1 + (h2 * dx - h0 * dz));
-        const double d2 = uj[2] - (t2 + (h0 * dy - h1 * dx));
-        const double d3 = uj[3] - h0, d4 = uj[4] - h1, d5 = uj[5] - h2;
+        const double d0 = u01.x - (t0 + (h1 * dz - h2 * dy));
+        const double d1 = u01.y - (t1 + (h2 * dx - h0 * dz));
+        const double d2 = u23.x - (t2 + (h0 * dy - h1 * dx));
+        const double d3 = u23.y - h0, d4 = u45.x - h1, d5 = u45.y - h2;
         if constexpr (F32PROD) {
             ef += k0.x * (float)d0 + k0.y * (float)d1 + k1.x * (float)d2 + k1.y * (float)d3 + k2.x * (float)d4 +
                   k2.y * (float)d5;
@@ -1650,19 +1651,24 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     for (int k = op.ne_ptr[i]; k < op.ne_ptr[i + 1]; ++k) {
         const int el = op.ne_list[k];
         const double *C = op.cs + (int64_t)el * NSR * K;
-        int a = 0;
-        double part[NSR];
-        for (int q = 0; q < NSR; ++q) part[q] = 0.0;
+        double part[NSR], ca[NSR];
+        for (int q = 0; q < NSR; ++q) {
+            part[q] = 0.0;
+            ca[q] = 0.0;
+        }
         for (int b = 0; b < NEN; ++b) {
             const int nb = op.conn[b * E + el];
-            if (nb == i) a = b;
             const double ub = Ucur[6 * (int64_t)nb + r];
-            for (int q = 0; q < NSR; ++q) part[q] += C[q * K + 6 * b + r] * ub;
+            for (int q = 0; q < NSR; ++q) {
+                const double cqb = C[q * K + 6 * b + r];
+                part[q] += cqb * ub;
+                if (nb == i) ca[q] = cqb;  // column (a, r) of the node's own block
+            }
         }
         for (int q = 0; q < NSR; ++q) {
             double sq = 0.0;
             for (int c = 0; c < 6; ++c) sq += __shfl(part[q], base + c, 64);
-            e += C[q * K + 6 * a + r] * sq;
+            e += ca[q] * sq;
         }
     }
     double dtr = dt, gr = gstep;

@@ -29,7 +29,8 @@ _STATUS = {
     -6: "unsupported input",
 }
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libflexshell.so")
+LIB_PATH = os.environ.get("FLEXSHELL_LIB",
+                          os.path.join(os.path.dirname(os.path.abspath(__file__)), "libflexshell.so"))
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)

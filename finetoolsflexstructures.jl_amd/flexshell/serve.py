@@ -50,7 +50,8 @@ class Request:
     CompositeLayup: layered FEMM) or `material` + `t` (homogeneous FEMM);
     explicit jobs also take V0 (free-dof velocities) and `steps`."""
 
-    def __init__(self, rid, kind, model, steps=0, arrival=0.0, omega_iters=30, seed=0):
+    def __init__(self, rid, kind, model, steps=0, arrival=0.0, omega_iters=30, seed=0,
+                 precision=_lib.FS_PREC_MIXED):
         if kind not in (ASSEMBLY, MODAL, EXPLICIT):
             raise ValueError(f"unknown request kind {kind!r}")
         self.rid = rid
@@ -60,6 +61,7 @@ class Request:
         self.arrival = float(arrival)
         self.omega_iters = int(omega_iters)
         self.seed = seed
+        self.precision = int(precision)  # explicit jobs: FS_PREC_FP64, FS_PREC_MIXED or FS_PREC_MIXED_F32
         self.result = None
         self.error = None
         self.t_arrive = self.t_admit = self.t_done = None
@@ -85,7 +87,8 @@ def execute(req, ctx):
     assembly  -> K (all dofs, CSR);
     modal     -> K (free-free CSR) and the lumped mass diagonal M;
     explicit  -> U, V after `steps` central-difference steps at 0.9 of the
-                 stable dt from the device power iteration, and that dt."""
+                 stable dt from the device power iteration (FP64 or mixed
+                 precision, per request), and that dt."""
     m = req.model
     f = make_femm(m, ctx)
     d = m["dchi"]
@@ -107,7 +110,7 @@ def execute(req, ctx):
             dt = explicit.stable_dt(om)
             V0 = m["V0"] if m.get("V0") is not None else np.zeros(nf)
             r = explicit.run(f._model, 0.0, np.zeros(nf), V0, dt, req.steps, np.zeros(nf),
-                             np.zeros(req.steps + 1))
+                             np.zeros(req.steps + 1), precision=req.precision)
             out.update(U=r["U"], V=r["V"], dt=dt, omega_max=om, cd_ms=r["ms"])
         out["elements"] = req.nelem
         out["free_dofs"] = nf
@@ -118,10 +121,21 @@ def execute(req, ctx):
     return out
 
 
+def device_memory(device=0):
+    """(free, total) bytes of a HIP device, from the HIP runtime that
+    libflexshell is linked against."""
+    import ctypes as C
+    _lib.load()
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    if hip.hipSetDevice(int(device)) != 0 or hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        raise _lib.FlexShellError(f"hipMemGetInfo failed on device {device}")
+    return free.value, total.value
+
+
 def default_budget(device=0, fraction=0.9):
     """HBM budget for admission: a fraction of the device memory free now."""
-    import torch
-    free, _total = torch.cuda.mem_get_info(device)
+    free, _total = device_memory(device)
     return int(fraction * free)
 
 
