@@ -77,10 +77,9 @@ struct MixedOp {
     const int32_t *nbr_node;
     const double *dr;         // N x 6
     const int32_t *ne_ptr;    // node -> incident elements (ascending)
-    const int32_t *ne_list;
-    const int32_t *conn;      // conn[a * E + e]
+    const int32_t *ne_pack;   // per incidence: element, its nen nodes (stride 4 for T3, 8 for Q4)
     const double *cs;         // E x nsr x 6 nen
-    const double *x, *y, *z;  // node coordinates
+    const double4 *xyz;       // node coordinates (x, y, z, 0)
 };
 
 // Scatter plan of one block (ALL or FF) -- see scatter_pair_kernel.

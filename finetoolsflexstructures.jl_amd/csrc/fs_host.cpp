@@ -50,6 +50,8 @@ hipError_t launch_mixed_build(const ScatterPlan &sp, const double *Kb, const int
                               hipStream_t st);
 hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, const int32_t *ne_list,
                               const int32_t *conn, const double *dre, double *dr, hipStream_t st);
+hipError_t launch_pack_xyz(int64_t N, const double *x, const double *y, const double *z, double4 *xyz,
+                           hipStream_t st);
 hipError_t launch_cd_mixed_step(int nen, int nsr, bool f32prod, const MixedOp &op, const double *Ucur, double *Unext,
                                 double *V, double *A, const double *invMC, const double *Cd, const double *Fmag,
                                 double gstep, double dt, int damped, int forced, const int32_t *probe_slot,
@@ -120,11 +122,11 @@ struct MixedStore {
     int nsr = 0;
     int64_t nslices = 0;
     int64_t kmb_pairs = 0;  // float2 values
-    DevBuf goff, kmb, cs, dr;
+    DevBuf goff, kmb, cs, dr, ne_pack, xyz;
     float build_ms = 0.f;
     void reset() {
         built = false;
-        for (DevBuf *b : {&goff, &kmb, &cs, &dr}) b->release();
+        for (DevBuf *b : {&goff, &kmb, &cs, &dr, &ne_pack, &xyz}) b->release();
     }
 };
 
@@ -626,6 +628,17 @@ int build_mixed(fs_model *m) {
         (s = upload(c, X.goff, goff.data(), goff.size())) ||
         (s = dalloc(c, X.kmb, (size_t)goff[ns] * sizeof(float2))))
         return s;
+    // incidence records: element and its nodes, in each node's ascending element order
+    const int ps = nen == 3 ? 4 : 8;
+    std::vector<int32_t> pack((size_t)m->ne_list.size() * ps, 0);
+    for (size_t k = 0; k < m->ne_list.size(); ++k) {
+        const int32_t e = m->ne_list[k];
+        pack[k * ps] = e;
+        for (int a = 0; a < nen; ++a) pack[k * ps + 1 + a] = m->conn[(size_t)e * nen + a];
+    }
+    if ((s = upload(c, X.ne_pack, pack.data(), pack.size())) || (s = dalloc(c, X.xyz, (size_t)N * 32))) return s;
+    FS_HIP(c, fs::launch_pack_xyz(N, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(), X.xyz.as<double4>(),
+                                  c->stream));
     fs::ElemParams ep = m->last_ep;
     ep.split_cs = X.cs.as<double>();
     ep.split_dr = dre.as<double>();
@@ -667,12 +680,9 @@ fs::MixedOp mixed_op(const fs_model *m) {
     op.nbr_node = B.nbr_node.as<int32_t>();
     op.dr = m->mixed.dr.as<double>();
     op.ne_ptr = m->dne_ptr.as<int32_t>();
-    op.ne_list = m->dne_list.as<int32_t>();
-    op.conn = m->dconn.as<int32_t>();
+    op.ne_pack = m->mixed.ne_pack.as<int32_t>();
     op.cs = m->mixed.cs.as<double>();
-    op.x = m->x.as<double>();
-    op.y = m->y.as<double>();
-    op.z = m->z.as<double>();
+    op.xyz = m->mixed.xyz.as<double4>();
     return op;
 }
 

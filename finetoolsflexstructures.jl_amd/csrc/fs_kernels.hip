@@ -1547,6 +1547,14 @@ __global__ __launch_bounds__(256) void mixed_build_kernel(ScatterPlan sp, const 
     }
 }
 
+// Coordinates of the mixed operator as one 32-byte record per node.
+__global__ __launch_bounds__(256) void pack_xyz_kernel(int64_t N, const double *__restrict__ x,
+                                                       const double *__restrict__ y, const double *__restrict__ z,
+                                                       double4 *__restrict__ xyz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) xyz[i] = make_double4(x[i], y[i], z[i], 0.0);
+}
+
 // Drilling block per node: sum of the element-node blocks in element order.
 template <int NEN>
 __global__ __launch_bounds__(256) void mixed_drill_kernel(int64_t N, int E, const int32_t *__restrict__ ne_ptr,
@@ -1586,6 +1594,9 @@ __global__ __launch_bounds__(256) void mixed_drill_kernel(int64_t N, int E, cons
 //     node add the partials in a fixed order, and each lane adds
 //     C_e[:, (a, r)]' s_e.  FP64 throughout.
 //   K_d: drilling block times the node's rotations, FP64.
+#ifndef MX_CHUNK
+#define MX_CHUNK 1  // neighbour slots whose loads are issued together (1: highest occupancy, fastest)
+#endif
 template <int NEN, int NSR, bool F32PROD>
 __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     MixedOp op, const double *__restrict__ Ucur, double *__restrict__ Unext, double *__restrict__ V,
@@ -1604,35 +1615,56 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     const int r = l % 6;
     const int64_t R = 6 * i64 + r;
     const int base = (int)(threadIdx.x & 63) - r;  // lane of the node's component 0
-    const double xi = op.x[i], yi = op.y[i], zi = op.z[i];
-    const double *ui = Ucur + 6 * i64;
-    const double t0 = ui[0], t1 = ui[1], t2 = ui[2], h0 = ui[3], h1 = ui[4], h2 = ui[5];
+    const double4 xiv = op.xyz[i];
+    const double2 *ui2 = reinterpret_cast<const double2 *>(Ucur + 6 * i64);  // 16-byte aligned
+    const double2 ut01 = ui2[0], ut2h0 = ui2[1], uh12 = ui2[2];
+    const double t0 = ut01.x, t1 = ut01.y, t2 = ut2h0.x, h0 = ut2h0.y, h1 = uh12.x, h2 = uh12.y;
     const int64_t g0 = op.goff[g];
     const int nsl = (int)((op.goff[g + 1] - g0) / 180);
     const float2 *kv = op.kmb + g0 + l;
     const int nb0 = op.nbr_ptr[i];
-    const int cnt = op.nbr_ptr[i + 1] - nb0;
+    const int cnt = min(op.nbr_ptr[i + 1] - nb0, nsl);
     double e = 0.0;
     float ef = 0.0f;
-    for (int s = 0; s < nsl && s < cnt; ++s) {
-        const float2 k0 = kv[(3 * s + 0) * 60];
-        const float2 k1 = kv[(3 * s + 1) * 60];
-        const float2 k2 = kv[(3 * s + 2) * 60];
-        const int j = op.nbr_node[nb0 + s];
-        const double2 *uj = reinterpret_cast<const double2 *>(Ucur + 6 * (int64_t)j);  // 16-byte aligned
-        const double2 u01 = uj[0], u23 = uj[1], u45 = uj[2];
-        const double dx = op.x[j] - xi, dy = op.y[j] - yi, dz = op.z[j] - zi;
-        // u_j minus the rigid motion of node i evaluated at node j
-        const double d0 = u01.x - (t0 + (h1 * dz - h2 * dy));
-        const double d1 = u01.y - (t1 + (h2 * dx - h0 * dz));
-        const double d2 = u23.x - (t2 + (h0 * dy - h1 * dx));
-        const double d3 = u23.y - h0, d4 = u45.x - h1, d5 = u45.y - h2;
-        if constexpr (F32PROD) {
-            ef += k0.x * (float)d0 + k0.y * (float)d1 + k1.x * (float)d2 + k1.y * (float)d3 + k2.x * (float)d4 +
-                  k2.y * (float)d5;
-        } else {
-            e += (double)k0.x * d0 + (double)k0.y * d1 + (double)k1.x * d2 + (double)k1.y * d3 +
-                 (double)k2.x * d4 + (double)k2.y * d5;
+    // neighbour slots in chunks of MX_CHUNK: all loads of a chunk are issued
+    // before its arithmetic
+    for (int s0 = 0; s0 < cnt; s0 += MX_CHUNK) {
+        int jj[MX_CHUNK];
+#pragma unroll
+        for (int t = 0; t < MX_CHUNK; ++t) jj[t] = (s0 + t < cnt) ? op.nbr_node[nb0 + s0 + t] : i;
+        float2 kk[MX_CHUNK][3];
+        double2 uu[MX_CHUNK][3];
+        double4 xx[MX_CHUNK];
+#pragma unroll
+        for (int t = 0; t < MX_CHUNK; ++t) {
+            const int s = s0 + t;
+            if (s < cnt) {
+                for (int c2 = 0; c2 < 3; ++c2) kk[t][c2] = kv[(3 * s + c2) * 60];
+            } else {
+                for (int c2 = 0; c2 < 3; ++c2) kk[t][c2] = make_float2(0.f, 0.f);
+            }
+            const double2 *uj = reinterpret_cast<const double2 *>(Ucur + 6 * (int64_t)jj[t]);
+            uu[t][0] = uj[0];
+            uu[t][1] = uj[1];
+            uu[t][2] = uj[2];
+            xx[t] = op.xyz[jj[t]];
+        }
+#pragma unroll
+        for (int t = 0; t < MX_CHUNK; ++t) {
+            const double dx = xx[t].x - xiv.x, dy = xx[t].y - xiv.y, dz = xx[t].z - xiv.z;
+            // u_j minus the rigid motion of node i evaluated at node j (zero for padding: j = i)
+            const double d0 = uu[t][0].x - (t0 + (h1 * dz - h2 * dy));
+            const double d1 = uu[t][0].y - (t1 + (h2 * dx - h0 * dz));
+            const double d2 = uu[t][1].x - (t2 + (h0 * dy - h1 * dx));
+            const double d3 = uu[t][1].y - h0, d4 = uu[t][2].x - h1, d5 = uu[t][2].y - h2;
+            const float2 k0 = kk[t][0], k1 = kk[t][1], k2 = kk[t][2];
+            if constexpr (F32PROD) {
+                ef += k0.x * (float)d0 + k0.y * (float)d1 + k1.x * (float)d2 + k1.y * (float)d3 +
+                      k2.x * (float)d4 + k2.y * (float)d5;
+            } else {
+                e += (double)k0.x * d0 + (double)k0.y * d1 + (double)k1.x * d2 + (double)k1.y * d3 +
+                     (double)k2.x * d4 + (double)k2.y * d5;
+            }
         }
     }
     if constexpr (F32PROD) e = (double)ef;
@@ -1646,23 +1678,29 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
         e += d0 * h0 + d1 * h1 + d2 * h2;
     }
     // transverse shear (FP64), incident elements in ascending order; the six
-    // lanes of the node share the element products through cross-lane reads
-    const int E = op.nelem;
+    // lanes of the node share the element products through cross-lane reads.
+    // Incidence records: element id and its nodes (ne_pack, stride 4 or 8).
+    constexpr int PS = NEN == 3 ? 4 : 8;
     for (int k = op.ne_ptr[i]; k < op.ne_ptr[i + 1]; ++k) {
-        const int el = op.ne_list[k];
-        const double *C = op.cs + (int64_t)el * NSR * K;
+        const int4 pk = reinterpret_cast<const int4 *>(op.ne_pack + (int64_t)k * PS)[0];
+        int nodes[NEN];
+        nodes[0] = pk.y;
+        nodes[1] = pk.z;
+        nodes[2] = pk.w;
+        if constexpr (NEN == 4) nodes[3] = op.ne_pack[(int64_t)k * PS + 4];
+        const double *C = op.cs + (int64_t)pk.x * NSR * K;
+        double ub[NEN];
+        for (int b2 = 0; b2 < NEN; ++b2) ub[b2] = Ucur[6 * (int64_t)nodes[b2] + r];
         double part[NSR], ca[NSR];
         for (int q = 0; q < NSR; ++q) {
             part[q] = 0.0;
             ca[q] = 0.0;
         }
-        for (int b = 0; b < NEN; ++b) {
-            const int nb = op.conn[b * E + el];
-            const double ub = Ucur[6 * (int64_t)nb + r];
+        for (int b2 = 0; b2 < NEN; ++b2) {
             for (int q = 0; q < NSR; ++q) {
-                const double cqb = C[q * K + 6 * b + r];
-                part[q] += cqb * ub;
-                if (nb == i) ca[q] = cqb;  // column (a, r) of the node's own block
+                const double cqb = C[q * K + 6 * b2 + r];
+                part[q] += cqb * ub[b2];
+                if (nodes[b2] == i) ca[q] = cqb;  // column (a, r) of the node's own block
             }
         }
         for (int q = 0; q < NSR; ++q) {
@@ -1687,7 +1725,7 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     double vn = v + dt_2 * a0;
     const double an = invMC[R] * f;
     vn += dt_2 * an;
-    const double u = ui[r];
+    const double u = Ucur[R];
     V[R] = vn;
     A[R] = an;
     Unext[R] = u + (dtr * vn + dt2_2 * an);
@@ -1980,6 +2018,13 @@ hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, 
         hipLaunchKernelGGL(mixed_drill_kernel<3>, dim3(blocks), dim3(256), 0, st, N, E, ne_ptr, ne_list, conn, dre, dr);
     else
         hipLaunchKernelGGL(mixed_drill_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, ne_ptr, ne_list, conn, dre, dr);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_xyz(int64_t N, const double *x, const double *y, const double *z, double4 *xyz,
+                           hipStream_t st) {
+    if (N == 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_xyz_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, x, y, z, xyz);
     return hipGetLastError();
 }
 
