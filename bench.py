@@ -342,9 +342,11 @@ def main():
     b_cd = cd_step_bytes(nf, nnz)
     ach = b_asm / (dev_ms / 1e3) / 1e9
     ach_cd = b_cd / (cd_ms / ncd / 1e3) / 1e9
+    # HBM bytes per assembly from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # summary of this workload (512 x 512 only)
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_c2.json")
-    if os.path.exists(tf):
+    tf = os.path.join(ROOT, "profiles", "traffic_q4_cylinder_512x512.json")
+    if args.n == 512 and os.path.exists(tf):
         traffic = json.load(open(tf)).get("assembly_bytes_per_launch")
 
     out = {
