@@ -105,7 +105,8 @@ class ShellBatch:
             return None
         nrows, rp, cv = self.pattern()
         vals = self.model.stiffness(self._shell_params(), _lib.FS_BLOCK_FF, len(cv))
-        return sp.csr_matrix((vals, cv, rp), shape=(nrows, nrows))
+        from .femm import _csr
+        return _csr(vals, cv, rp, nrows)
 
     def assemble_device(self):
         """Assemble the FF-block stiffness into HBM without a host copy."""

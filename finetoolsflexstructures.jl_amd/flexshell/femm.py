@@ -230,7 +230,7 @@ class _ShellFEMM:
             return DeviceMatrix(self, block, "K")
         nrows, rp, cv = self._pattern(block)
         vals = m.stiffness(self._shell_params(), block, len(cv))
-        return sp.csr_matrix((vals, cv, rp), shape=(nrows, nrows))
+        return _csr(vals, cv, rp, nrows)
 
     def element_matrices(self, geom0, dchi=None):
         """Element matrices (nelem x k x k, global basis): the elmat that the
@@ -267,6 +267,16 @@ class _ShellFEMM:
     def num_normals(self):
         v = self._normal_valid
         return len(v), int(np.count_nonzero(~v))
+
+
+def _csr(vals, colval, rowptr, n):
+    """scipy CSR view of library output: the arrays are already a valid,
+    sorted CSR (int64 indices), so they are attached without the copy and
+    index-dtype conversion of the validating constructor."""
+    K = sp.csr_matrix((n, n), dtype=np.float64)
+    K.data, K.indices, K.indptr = vals, colval, rowptr
+    K.has_sorted_indices = True
+    return K
 
 
 def _element_normal(J0):
@@ -327,20 +337,24 @@ class _CompositeMixin:
         conn = self.conn
         N = xyz.shape[0]
         n3 = np.array([lg[0].csys[:, 2] for lg in self.layup_groups], dtype=float)[self._group]  # E x 3
-        nrm = np.zeros((N, 3))
+        # per (element, node) weights in element-major order, summed per node in
+        # that order (the element loop of the reference)
         if self.nen == 3:
-            for a in range(3):
-                np.add.at(nrm, conn[:, a], n3)
+            w = np.ones(conn.shape)
         else:
             X = xyz[conn]
             npc = [(-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)]
+            w = np.empty(conn.shape)
             for a in range(4):
                 xi, eta = npc[a]
                 gN = 0.25 * np.array([[-(1 - eta), -(1 - xi)], [(1 - eta), -(1 + xi)], [(1 + eta), (1 + xi)],
                                       [-(1 + eta), (1 - xi)]])
-                J = np.einsum("eak,al->ekl", X, gN)
-                Jac = np.linalg.norm(np.cross(J[:, :, 0], J[:, :, 1]), axis=1)
-                np.add.at(nrm, conn[:, a], Jac[:, None] * n3)
+                J0 = gN[0, 0] * X[:, 0] + gN[1, 0] * X[:, 1] + gN[2, 0] * X[:, 2] + gN[3, 0] * X[:, 3]
+                J1 = gN[0, 1] * X[:, 0] + gN[1, 1] * X[:, 1] + gN[2, 1] * X[:, 2] + gN[3, 1] * X[:, 3]
+                w[:, a] = np.linalg.norm(np.cross(J0, J1), axis=1)
+        idx = conn.ravel()
+        nrm = np.stack([np.bincount(idx, weights=(w * n3[:, k:k + 1]).ravel(), minlength=N) for k in range(3)],
+                       axis=1)
         nn = np.linalg.norm(nrm, axis=1)
         nrm[nn > 0] /= nn[nn > 0, None]
         valid = np.ones(N, dtype=bool)
@@ -352,13 +366,13 @@ class _CompositeMixin:
             e1 /= np.linalg.norm(e1, axis=1)[:, None]
             e3 = np.cross(e1, X[:, 2] - X[:, 0])
             e3 /= np.linalg.norm(e3, axis=1)[:, None]
-            for a in range(3):
-                bad = np.einsum("ek,ek->e", nrm[conn[:, a]], e3) < 1 - ntol
-                valid[conn[bad, a]] = False
+            ref = e3
         else:
-            for a in range(4):
-                bad = np.einsum("ek,ek->e", nrm[conn[:, a]], n3) < 1 - ntol
-                valid[conn[bad, a]] = False
+            ref = n3
+        for a in range(self.nen):
+            na = nrm[conn[:, a]]
+            bad = (na[:, 0] * ref[:, 0] + na[:, 1] * ref[:, 1] + na[:, 2] * ref[:, 2]) < 1 - ntol
+            valid[conn[bad, a]] = False
         return nrm, valid
 
     def _shell_params(self):

@@ -64,6 +64,7 @@ class Request:
         self.precision = int(precision)  # explicit jobs: FS_PREC_FP64, FS_PREC_MIXED or FS_PREC_MIXED_F32
         self.result = None
         self.error = None
+        self.phase_ms = {}  # wall time of each phase of execute(), ms
         self.t_arrive = self.t_admit = self.t_done = None
         m = model
         d = m["dchi"]
@@ -90,27 +91,43 @@ def execute(req, ctx):
                  stable dt from the device power iteration (FP64 or mixed
                  precision, per request), and that dt."""
     m = req.model
+    phase = {}
+    clock = [time.perf_counter()]
+
+    def mark(name):
+        now = time.perf_counter()
+        phase[name] = phase.get(name, 0.0) + (now - clock[0]) * 1e3
+        clock[0] = now
+
     f = make_femm(m, ctx)
     d = m["dchi"]
     nf = d.nfreedofs
     out = {}
     try:
         f.associategeometry(m["xyz"])
+        mark("geometry")
         if req.kind == ASSEMBLY:
             out["K"] = f.stiffness(FM.SysmatAssemblerSparseCSRSymm(), m["xyz"], None, None, d)
+            mark("stiffness")
         elif req.kind == MODAL:
             out["K"] = f.stiffness(FM.SysmatAssemblerFFBlock(nf), m["xyz"], None, None, d)
+            mark("stiffness")
             out["M"] = f.mass(FM.SysmatAssemblerFFBlock(FM.SysmatAssemblerSparseDiag(), nf, nf), m["xyz"],
                               d).diagonal()
+            mark("mass")
         else:
             f.stiffness(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), m["xyz"], None, None, d)
+            mark("stiffness")
             f.mass(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), m["xyz"], d)
+            mark("mass")
             v0 = np.random.default_rng(req.seed).random(nf)
             om = explicit.omega_max(f, v0, maxit=req.omega_iters)
             dt = explicit.stable_dt(om)
+            mark("omega")
             V0 = m["V0"] if m.get("V0") is not None else np.zeros(nf)
             r = explicit.run(f._model, 0.0, np.zeros(nf), V0, dt, req.steps, np.zeros(nf),
                              np.zeros(req.steps + 1), precision=req.precision)
+            mark("explicit")
             out.update(U=r["U"], V=r["V"], dt=dt, omega_max=om, cd_ms=r["ms"])
         out["elements"] = req.nelem
         out["free_dofs"] = nf
@@ -118,6 +135,8 @@ def execute(req, ctx):
         if f._model is not None:
             f._model.close()
             f._model = None
+        mark("release")
+        req.phase_ms = phase
     return out
 
 
@@ -251,6 +270,14 @@ def summarize(requests, wall, server=None):
         if lk.size:
             out[f"{kind}_p50_ms"] = float(np.percentile(lk, 50))
             out[f"{kind}_p99_ms"] = float(np.percentile(lk, 99))
+    phases = {}
+    for r in ok:
+        for k, v in r.phase_ms.items():
+            phases.setdefault(k, []).append(v)
+    out["phase_mean_ms"] = {k: float(np.mean(v)) for k, v in phases.items()}
+    wait = [r.t_admit - r.t_arrive for r in requests if r.t_admit is not None]
+    if wait:
+        out["queue_wait_p50_ms"] = float(np.percentile(wait, 50) * 1e3)
     if server is not None:
         out.update(slots=len(server.contexts), max_concurrent=server.max_concurrent,
                    max_inflight_gb=server.max_inflight_bytes / 1e9, budget_gb=server.budget / 1e9)
