@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-steps", type=int, default=500)
     ap.add_argument("--no-serve-plain", action="store_true")
     ap.add_argument("--profile", action="store_true",
