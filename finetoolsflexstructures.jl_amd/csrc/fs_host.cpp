@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -59,10 +61,28 @@ hipError_t launch_cd_mixed_step(int nen, int nsr, bool f32prod, const MixedOp &o
                                 const double *gtab, hipStream_t st);
 }  // namespace fs
 
+namespace {
+// Device buffers released by the models of a context are kept for reuse by
+// the same context instead of being returned with hipFree, which waits for
+// the whole device (every other context's work included).  Reuse is safe
+// because every ABI call ends with its stream synchronised, so a released
+// buffer has no pending work.  Shared by the context and every live buffer
+// allocated through it; outlives the context until the last buffer is gone.
+struct BufCache {
+    std::mutex mu;
+    std::multimap<size_t, void *> spare;  // bytes -> block
+    size_t spare_bytes = 0;
+    size_t cap = size_t(4) << 30;         // bytes kept per context at most
+    int refs = 1;                         // the context + live buffers
+    bool alive = true;                    // false once the context is destroyed
+};
+}  // namespace
+
 struct fs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    BufCache *cache = nullptr;
     // element-kernel launch shape per (family T3/Q4, shear formulation):
     // -1 = not tuned yet (the autotuner times every shape on the first
     // stiffness evaluation of that kind and keeps the fastest)
@@ -74,15 +94,34 @@ namespace {
 
 struct DevBuf {
     void *p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;          // size of the block (may exceed the request)
+    BufCache *cache = nullptr; // owner's cache, or nullptr
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (cache) {
+                int left;
+                {
+                    std::lock_guard<std::mutex> lk(cache->mu);
+                    if (cache->alive && cache->spare_bytes + bytes <= cache->cap) {
+                        cache->spare.insert({bytes, p});
+                        cache->spare_bytes += bytes;
+                    } else {
+                        (void)hipFree(p);
+                    }
+                    left = --cache->refs;
+                }
+                if (left == 0) delete cache;
+            } else {
+                (void)hipFree(p);
+            }
+        }
         p = nullptr;
         bytes = 0;
+        cache = nullptr;
     }
     template <class T>
     T *as() const { return static_cast<T *>(p); }
@@ -174,12 +213,40 @@ int fail(fs_ctx *c, int code, const char *fmt, ...) {
 int dalloc(fs_ctx *c, DevBuf &b, size_t bytes) {
     b.release();
     if (bytes == 0) bytes = 8;
+    BufCache *cache = c->cache;
+    {
+        // best fit among the spare blocks, at most 25 % + 1 MiB larger
+        std::lock_guard<std::mutex> lk(cache->mu);
+        auto it = cache->spare.lower_bound(bytes);
+        if (it != cache->spare.end() && it->first <= bytes + bytes / 4 + (size_t(1) << 20)) {
+            b.p = it->second;
+            b.bytes = it->first;
+            cache->spare_bytes -= it->first;
+            cache->spare.erase(it);
+            b.cache = cache;
+            ++cache->refs;
+            return FS_OK;
+        }
+    }
     hipError_t e = hipMalloc(&b.p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        // give the spare blocks back and try once more
+        {
+            std::lock_guard<std::mutex> lk(cache->mu);
+            for (auto &kv : cache->spare) (void)hipFree(kv.second);
+            cache->spare.clear();
+            cache->spare_bytes = 0;
+        }
+        e = hipMalloc(&b.p, bytes);
+    }
     if (e != hipSuccess) {
         b.p = nullptr;
         return fail(c, FS_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
     }
     b.bytes = bytes;
+    std::lock_guard<std::mutex> lk(cache->mu);
+    b.cache = cache;
+    ++cache->refs;
     return FS_OK;
 }
 
@@ -700,8 +767,14 @@ int fs_ctx_create(int device, fs_ctx **out) {
     if (device < 0 || device >= n) return FS_ERR_ARG;
     fs_ctx *c = new (std::nothrow) fs_ctx;
     if (!c) return FS_ERR_NOMEM;
+    c->cache = new (std::nothrow) BufCache;
+    if (!c->cache) {
+        delete c;
+        return FS_ERR_NOMEM;
+    }
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c->cache;
         delete c;
         return FS_ERR_DEVICE;
     }
@@ -712,6 +785,17 @@ int fs_ctx_create(int device, fs_ctx **out) {
 void fs_ctx_destroy(fs_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    int left;
+    {
+        std::lock_guard<std::mutex> lk(ctx->cache->mu);
+        for (auto &kv : ctx->cache->spare) (void)hipFree(kv.second);
+        ctx->cache->spare.clear();
+        ctx->cache->spare_bytes = 0;
+        ctx->cache->alive = false;  // buffers of models still alive are freed when released
+        left = --ctx->cache->refs;
+    }
+    if (left == 0) delete ctx->cache;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
