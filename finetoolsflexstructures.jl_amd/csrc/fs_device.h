@@ -23,7 +23,8 @@ struct ElemParams {
     int32_t nen;          // 3 or 4
     int32_t composite;    // 0 or 1
     int32_t formulation;  // 0 AVERAGE_B, 1 AVERAGE_K (T3)
-    int32_t nelem;
+    int32_t nelem;        // elements of the mesh (connectivity stride)
+    int32_t ebegin, eend; // element range of this launch, [ebegin, eend)
     int64_t nnodes;
     double thickness;     // homogeneous thickness
     double stab_c;
@@ -104,6 +105,7 @@ struct ScatterPlan {
     // upper pairs (I <= J): nodes and the neighbour slots kI (J in I's list)
     // and kJ (I in J's list), for the pair-driven scatter
     int64_t npairs;
+    int64_t pbegin, pend;         // pair range of this launch, [pbegin, pend)
     const int32_t *pair_node;     // 2 * npairs: I, J
     const int32_t *pair_slot;     // 2 * npairs: kI, kJ
 };

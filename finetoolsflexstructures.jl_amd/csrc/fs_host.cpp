@@ -88,6 +88,12 @@ struct fs_ctx {
     // stiffness evaluation of that kind and keeps the fastest)
     int elem_variant[2][2] = {{-1, -1}, {-1, -1}};
     bool autotune = true;
+    // assembly pipeline: the element kernels run in asm_chunks element
+    // ranges on `stream`; the scatter of the node pairs completed by each
+    // range runs on `stream2` (default priority) while the next range computes
+    hipStream_t stream2 = nullptr;
+    int asm_chunks = 1;
+    std::vector<hipEvent_t> ev;  // ordering events of the pipeline (no timing)
 };
 
 namespace {
@@ -136,6 +142,10 @@ struct Block {
     std::vector<int32_t> colval;   // nnz
     // scatter plan (device)
     int64_t npairs = 0;
+    // running maximum over the pairs, in pair order, of the last element
+    // contributing to a pair: the pairs [0, k) with pair_emax[k-1] < e are
+    // complete once the elements [0, e) are (host)
+    std::vector<int32_t> pair_emax;
     DevBuf nbr_ptr, nbr_node, nbr_pair, nbr_off, node_code, node_rowstart, contrib_ptr, contrib, pair_node,
         pair_slot;
     // device CSR (int32 for the time loop) and values
@@ -146,6 +156,7 @@ struct Block {
         nrows = nnz = npairs = 0;
         rowptr.clear();
         colval.clear();
+        pair_emax.clear();
         for (DevBuf *b : {&nbr_ptr, &nbr_node, &nbr_pair, &nbr_off, &node_code, &node_rowstart, &contrib_ptr,
                           &contrib, &pair_node, &pair_slot, &rowptr32, &colidx32, &kval, &mdiag})
             b->release();
@@ -462,6 +473,13 @@ int build_block(fs_model *m, int which) {
     }
     for (int64_t p = 0; p < npairs; ++p) cptr[p + 1] += cptr[p];
     std::vector<int32_t> cfill(cptr.begin(), cptr.end() - 1), clist(cptr[npairs]);
+    B.pair_emax.assign(npairs, -1);
+    for (int64_t e = 0; e < m->E; ++e)
+        for (int bl = 0; bl < nblk; ++bl) {
+            int32_t &x = B.pair_emax[cpair[e * nblk + bl]];
+            x = std::max(x, (int32_t)e);
+        }
+    for (int64_t p = 1; p < npairs; ++p) B.pair_emax[p] = std::max(B.pair_emax[p], B.pair_emax[p - 1]);
     for (int64_t e = 0; e < m->E; ++e) {
         int bl = 0;
         for (int a = 0; a < nen; ++a)
@@ -512,13 +530,25 @@ fs::ScatterPlan plan_of(const Block &B) {
     sp.contrib_ptr = B.contrib_ptr.as<int32_t>();
     sp.contrib = B.contrib.as<int32_t>();
     sp.npairs = B.npairs;
+    sp.pbegin = 0;
+    sp.pend = B.npairs;
     sp.pair_node = B.pair_node.as<int32_t>();
     sp.pair_slot = B.pair_slot.as<int32_t>();
     return sp;
 }
 
-// Compute element node-pair blocks into m->Kb (device).
-int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
+// Element-kernel launch on the model's device arrays.
+hipError_t launch_elements(fs_model *m, const fs::ElemParams &ep, int variant, hipStream_t st) {
+    return fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
+                                     m->dconn.as<int32_t>(), m->nx.as<double>(), m->ny.as<double>(),
+                                     m->nz.as<double>(), m->valid.as<uint8_t>(), m->Kb.as<double>(),
+                                     m->status.as<int>(), st, variant);
+}
+
+// Validate the stiffness parameters, upload the per-group material records,
+// size m->Kb, clear the status flag and pick the element-kernel launch shape
+// (the autotuner times every shape on the first evaluation of a kind).
+int prepare_elements(fs_model *m, const fs_shell_params *p, fs::ElemParams &ep, int *variant_out) {
     fs_ctx *c = m->ctx;
     if (!p) return fail(c, FS_ERR_ARG, "stiffness: NULL parameters");
     if (!m->normals_ready) return fail(c, FS_ERR_STATE, "stiffness: normals not computed (associategeometry!)");
@@ -526,12 +556,13 @@ int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
         return fail(c, FS_ERR_ARG, "stiffness: unknown transverse shear formulation %d", p->formulation);
     if (m->nen == FS_Q4 && p->formulation != FS_SHEAR_AVERAGE_B)
         return fail(c, FS_ERR_ARG, "stiffness: the Q4 family has a single (MITC) shear formulation");
-    fs::ElemParams ep;
     std::memset(&ep, 0, sizeof ep);
     ep.nen = m->nen;
     ep.composite = p->composite ? 1 : 0;
     ep.formulation = p->formulation;
     ep.nelem = (int32_t)m->E;
+    ep.ebegin = 0;
+    ep.eend = (int32_t)m->E;
     ep.nnodes = m->N;
     ep.thickness = p->thickness;
     ep.stab_c = p->stab_c;
@@ -610,47 +641,62 @@ int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
     }
     const int nblk = m->nen * (m->nen + 1) / 2;
     const size_t kbytes = (size_t)m->E * nblk * 36 * sizeof(double);
-    (void)0;
     if (m->Kb.bytes < kbytes && (s = dalloc(c, m->Kb, kbytes))) return s;
     FS_HIP(c, hipMemsetAsync(m->status.p, 0, sizeof(int), c->stream));
+    int &variant = c->elem_variant[m->nen == 4 ? 1 : 0][ep.formulation == FS_SHEAR_AVERAGE_K ? 1 : 0];
+    if (variant < 0 && !c->autotune) variant = 0;
+    if (variant < 0) {
+        // autotuner: one warm launch and one timed launch per shape on
+        // this model; every shape computes bitwise the same blocks
+        hipEvent_t e0, e1;
+        FS_HIP(c, hipEventCreate(&e0));
+        FS_HIP(c, hipEventCreate(&e1));
+        float best = 0.f;
+        int bestv = 0;
+        for (int v = 0; v < fs::element_variants(m->nen); ++v) {
+            float t = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {
+                FS_HIP(c, hipEventRecord(e0, c->stream));
+                FS_HIP(c, launch_elements(m, ep, v, c->stream));
+                FS_HIP(c, hipEventRecord(e1, c->stream));
+                FS_HIP(c, hipEventSynchronize(e1));
+                FS_HIP(c, hipEventElapsedTime(&t, e0, e1));
+            }
+            if (v == 0 || t < best) {
+                best = t;
+                bestv = v;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        variant = bestv;
+    }
+    *variant_out = variant;
+    return FS_OK;
+}
+
+// After the stream has been synchronised: the singular-metric flag of the
+// element kernels, and the bookkeeping of a successful evaluation.
+int finish_elements(fs_model *m, const fs::ElemParams &ep, int status) {
+    if (status)
+        return fail(m->ctx, FS_ERR_NUMERIC, "Singular metric matrix in _gradN_e! (degenerate quadrilateral)");
+    m->last_ep = ep;
+    ++m->kgen;
+    return FS_OK;
+}
+
+// Compute element node-pair blocks into m->Kb (device), timed with events.
+int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
+    fs_ctx *c = m->ctx;
+    fs::ElemParams ep;
+    int variant = 0;
+    int s = prepare_elements(m, p, ep, &variant);
+    if (s) return s;
     hipEvent_t e0, e1;
     FS_HIP(c, hipEventCreate(&e0));
     FS_HIP(c, hipEventCreate(&e1));
-    int &variant = c->elem_variant[m->nen == 4 ? 1 : 0][ep.formulation == FS_SHEAR_AVERAGE_K ? 1 : 0];
-    if (variant < 0) {
-        if (!c->autotune) {
-            variant = 0;
-        } else {
-            // autotuner: one warm launch and one timed launch per shape on
-            // this model; every shape computes bitwise the same blocks
-            float best = 0.f;
-            int bestv = 0;
-            for (int v = 0; v < fs::element_variants(m->nen); ++v) {
-                float t = 0.f;
-                for (int rep = 0; rep < 2; ++rep) {
-                    FS_HIP(c, hipEventRecord(e0, c->stream));
-                    FS_HIP(c, fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(),
-                                                        m->z.as<double>(), m->dconn.as<int32_t>(),
-                                                        m->nx.as<double>(), m->ny.as<double>(), m->nz.as<double>(),
-                                                        m->valid.as<uint8_t>(), m->Kb.as<double>(),
-                                                        m->status.as<int>(), c->stream, v));
-                    FS_HIP(c, hipEventRecord(e1, c->stream));
-                    FS_HIP(c, hipEventSynchronize(e1));
-                    FS_HIP(c, hipEventElapsedTime(&t, e0, e1));
-                }
-                if (v == 0 || t < best) {
-                    best = t;
-                    bestv = v;
-                }
-            }
-            variant = bestv;
-        }
-    }
     FS_HIP(c, hipEventRecord(e0, c->stream));
-    FS_HIP(c, fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
-                                        m->dconn.as<int32_t>(), m->nx.as<double>(), m->ny.as<double>(),
-                                        m->nz.as<double>(), m->valid.as<uint8_t>(), m->Kb.as<double>(),
-                                        m->status.as<int>(), c->stream, variant));
+    FS_HIP(c, launch_elements(m, ep, variant, c->stream));
     FS_HIP(c, hipEventRecord(e1, c->stream));
     int st = 0;
     FS_HIP(c, hipMemcpyAsync(&st, m->status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -658,10 +704,7 @@ int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
     FS_HIP(c, hipEventElapsedTime(elem_ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (st) return fail(c, FS_ERR_NUMERIC, "Singular metric matrix in _gradN_e! (degenerate quadrilateral)");
-    m->last_ep = ep;
-    ++m->kgen;
-    return FS_OK;
+    return finish_elements(m, ep, st);
 }
 
 // Build the mixed-precision operator from the parameters of the last
@@ -796,6 +839,11 @@ void fs_ctx_destroy(fs_ctx *ctx) {
         left = --ctx->cache->refs;
     }
     if (left == 0) delete ctx->cache;
+    if (ctx->stream2) {
+        (void)hipStreamSynchronize(ctx->stream2);
+        (void)hipStreamDestroy(ctx->stream2);
+    }
+    for (hipEvent_t ev : ctx->ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1014,22 +1062,76 @@ int fs_model_stiffness(fs_model *m, const fs_shell_params *p, int block, double 
     int s = build_block(m, block);
     if (s) return s;
     Block &B = m->blk[block];
-    float ms_elem = 0.f;
-    if ((s = element_blocks(m, p, &ms_elem))) return s;
-    hipEvent_t e0, e1;
-    FS_HIP(c, hipEventCreate(&e0));
-    FS_HIP(c, hipEventCreate(&e1));
-    FS_HIP(c, hipEventRecord(e0, c->stream));
-    FS_HIP(c, fs::launch_scatter(plan_of(B), m->Kb.as<double>(), B.kval.as<double>(), c->stream));
-    FS_HIP(c, hipEventRecord(e1, c->stream));
+    fs::ElemParams ep;
+    int variant = 0;
+    if ((s = prepare_elements(m, p, ep, &variant))) return s;
+    // Element ranges of whole workgroups.  After range k the pairs whose
+    // contributing elements all lie in ranges 0..k are complete (pair_emax),
+    // so their scatter is queued on stream2 behind an event and overlaps the
+    // element kernels of the next ranges.  Each pair is still summed by one
+    // thread in ascending element order: the values are bitwise those of the
+    // unpipelined sequence.
+    int ne = 1, wg = 0;
+    fs::element_variant_shape(m->nen, variant, &ne, &wg);
+    const int64_t E = m->E;
+    const int nch = std::max(1, c->asm_chunks);
+    int64_t per = (E + nch - 1) / nch;
+    per = std::max<int64_t>(ne, (per + ne - 1) / ne * ne);
+    const int64_t nranges = E == 0 ? 0 : (E + per - 1) / per;
+    while ((int64_t)c->ev.size() < nranges + 1) {
+        hipEvent_t ev;
+        FS_HIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        c->ev.push_back(ev);
+    }
+    if (!c->stream2) FS_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    const bool piped = nranges > 1;
+    hipEvent_t t0, t1, t2;
+    FS_HIP(c, hipEventCreate(&t0));
+    FS_HIP(c, hipEventCreate(&t1));
+    FS_HIP(c, hipEventCreate(&t2));
+    FS_HIP(c, hipEventRecord(t0, c->stream));
+    fs::ScatterPlan sp = plan_of(B);
+    int64_t pdone = 0;
+    for (int64_t k = 0; k < nranges; ++k) {
+        ep.ebegin = (int32_t)(k * per);
+        ep.eend = (int32_t)std::min(E, (k + 1) * per);
+        FS_HIP(c, launch_elements(m, ep, variant, c->stream));
+        if (!piped) break;
+        int64_t pend = B.npairs;
+        if (ep.eend < E)
+            pend = std::upper_bound(B.pair_emax.begin(), B.pair_emax.end(), ep.eend - 1) - B.pair_emax.begin();
+        if (pend > pdone) {
+            FS_HIP(c, hipEventRecord(c->ev[k], c->stream));
+            FS_HIP(c, hipStreamWaitEvent(c->stream2, c->ev[k], 0));
+            sp.pbegin = pdone;
+            sp.pend = pend;
+            FS_HIP(c, fs::launch_scatter(sp, m->Kb.as<double>(), B.kval.as<double>(), c->stream2));
+            pdone = pend;
+        }
+    }
+    FS_HIP(c, hipEventRecord(t1, c->stream));
+    if (piped) {
+        FS_HIP(c, hipEventRecord(c->ev[nranges], c->stream2));
+        FS_HIP(c, hipStreamWaitEvent(c->stream, c->ev[nranges], 0));
+    } else {
+        FS_HIP(c, fs::launch_scatter(plan_of(B), m->Kb.as<double>(), B.kval.as<double>(), c->stream));
+    }
+    FS_HIP(c, hipEventRecord(t2, c->stream));
+    int st = 0;
+    FS_HIP(c, hipMemcpyAsync(&st, m->status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     FS_HIP(c, hipStreamSynchronize(c->stream));
-    float ms_sc = 0.f;
-    FS_HIP(c, hipEventElapsedTime(&ms_sc, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    float ms_elem = 0.f, ms_all = 0.f;
+    FS_HIP(c, hipEventElapsedTime(&ms_elem, t0, t1));
+    FS_HIP(c, hipEventElapsedTime(&ms_all, t0, t2));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipEventDestroy(t2);
+    if ((s = finish_elements(m, ep, st))) return s;
+    m->last_ep.ebegin = 0;
+    m->last_ep.eend = (int32_t)E;
     m->timing[0] = ms_elem;
-    m->timing[1] = ms_sc;
-    m->timing[2] = ms_elem + ms_sc;
+    m->timing[1] = ms_all - ms_elem;
+    m->timing[2] = ms_all;
     B.k_ready = true;
     if (nzval_out) return d2h(c, nzval_out, B.kval.p, (size_t)B.nnz * 8);
     return FS_OK;
@@ -1325,6 +1427,12 @@ int fs_ctx_set_tuning(fs_ctx *ctx, int nen, int formulation, int32_t variant) {
     }
     ctx->autotune = true;
     ctx->elem_variant[nen == FS_Q4 ? 1 : 0][formulation] = variant;
+    return FS_OK;
+}
+
+int fs_ctx_set_pipeline(fs_ctx *ctx, int ranges) {
+    if (!ctx || ranges < 1 || ranges > 64) return FS_ERR_ARG;
+    ctx->asm_chunks = ranges;
     return FS_OK;
 }
 

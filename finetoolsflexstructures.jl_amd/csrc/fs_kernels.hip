@@ -55,17 +55,17 @@ __device__ __forceinline__ void e_g(double EG[3][3], const double J[3][2]) {
     EG[0][0] = J[0][0];
     EG[1][0] = J[1][0];
     EG[2][0] = J[2][0];
-    double n = sqrt(EG[0][0] * EG[0][0] + EG[1][0] * EG[1][0] + EG[2][0] * EG[2][0]);
-    EG[0][0] /= n;
-    EG[1][0] /= n;
-    EG[2][0] /= n;
+    double n = 1.0 / sqrt(EG[0][0] * EG[0][0] + EG[1][0] * EG[1][0] + EG[2][0] * EG[2][0]);
+    EG[0][0] *= n;
+    EG[1][0] *= n;
+    EG[2][0] *= n;
     EG[0][2] = -EG[2][0] * J[1][1] + EG[1][0] * J[2][1];
     EG[1][2] = EG[2][0] * J[0][1] - EG[0][0] * J[2][1];
     EG[2][2] = -EG[1][0] * J[0][1] + EG[0][0] * J[1][1];
-    n = sqrt(EG[0][2] * EG[0][2] + EG[1][2] * EG[1][2] + EG[2][2] * EG[2][2]);
-    EG[0][2] /= n;
-    EG[1][2] /= n;
-    EG[2][2] /= n;
+    n = 1.0 / sqrt(EG[0][2] * EG[0][2] + EG[1][2] * EG[1][2] + EG[2][2] * EG[2][2]);
+    EG[0][2] *= n;
+    EG[1][2] *= n;
+    EG[2][2] *= n;
     EG[0][1] = -EG[2][2] * EG[1][0] + EG[1][2] * EG[2][0];
     EG[1][1] = EG[2][2] * EG[0][0] - EG[0][2] * EG[2][0];
     EG[2][1] = -EG[1][2] * EG[0][0] + EG[0][2] * EG[1][0];
@@ -147,10 +147,11 @@ __device__ __forceinline__ void nodal_triad(const double nk_e[3], double A[3][3]
     const double r0 = -nk_e[1], r1 = nk_e[0];
     const double na = sqrt(r0 * r0 + r1 * r1);
     if (na > 1.0e-12) {
-        const double u0 = r0 / na, u1 = r1 / na;
+        const double ina = 1.0 / na;
+        const double u0 = r0 * ina, u1 = r1 * ina;
         double sa, ca;
         sincos(na, &sa, &ca);
-        const double f = sa / na;
+        const double f = sa * ina;
         const double uu[3] = {u0, u1, 0.0};
         const double Sk[3][3] = {{0.0, 0.0, r1}, {0.0, 0.0, -r0}, {-r1, r0, 0.0}};
         for (int i = 0; i < 3; ++i)
@@ -336,8 +337,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
 
     const int tid = threadIdx.x;
     const int E = p.nelem;
-    const int e0 = blockIdx.x * NE;
-    const int ne = (E - e0) < NE ? (E - e0) : NE;
+    const int e0 = p.ebegin + blockIdx.x * NE;
+    const int ne = (p.eend - e0) < NE ? (p.eend - e0) : NE;
 
     // ---------------- G1: geometry per (element, qp) ----------------
     if (tid < NE * NQP) {
@@ -811,7 +812,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     constexpr int OFF_C = OFF_G + NE * NQP * GEO;
     constexpr int OFF_N = OFF_C + NE * CSTR;
     constexpr int OFF_M = OFF_N + NE * NEN * NODE;
-    constexpr int MISC = 4 * 9 + 1;
+    constexpr int MISC = 4 * 9 + 4 + 1;  // rotation blocks K_rr of the 4 nodes | trace terms
     static_assert(NE * NQP * GEO + NE * CSTR >= NE * NBLK * 36, "output stage must fit in the G+C region");
     extern __shared__ double smem[];
     double *sG = smem + OFF_G;
@@ -821,8 +822,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
 
     const int tid = threadIdx.x;
     const int E = p.nelem;
-    const int e0 = blockIdx.x * NE;
-    const int ne = (E - e0) < NE ? (E - e0) : NE;
+    const int e0 = p.ebegin + blockIdx.x * NE;
+    const int ne = (p.eend - e0) < NE ? (p.eend - e0) : NE;
 
     // ---------------- G1: geometry per (element, Gauss point) ----------------
     if (tid < NE * NQP) {
@@ -874,7 +875,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             const double G11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
             const double detG = G00 * G11 - G01 * G01;
             if (detG == 0.0) status[0] = 1;
-            const double i00 = G11 / detG, i01 = -G01 / detG, i11 = G00 / detG;
+            const double idetG = 1.0 / detG;
+            const double i00 = G11 * idetG, i01 = -G01 * idetG, i11 = G00 * idetG;
             for (int a = 0; a < 4; ++a) {
                 const double t0 = i00 * gNp[a][0] + i01 * gNp[a][1];
                 const double t1 = i01 * gNp[a][0] + i11 * gNp[a][1];
@@ -918,14 +920,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             const double J22 = (Y1 * (r - 1) / 4 - Y2 * (r + 1) / 4 + Y3 * (r + 1) / 4 - Y4 * (r - 1) / 4);
             const double A = sqrt(J11 * J11 + J21 * J21);
             const double B = sqrt(J12 * J12 + J22 * J22);
-            const double ca = J11 / A, sa = J21 / A, cb = J12 / B, sb = J22 / B;
+            const double iA = 1.0 / A, iB = 1.0 / B;
+            const double ca = J11 * iA, sa = J21 * iA, cb = J12 * iB, sb = J22 * iB;
             const double detJ = J11 * J22 - J12 * J21;
             const double Ax = X1 - X2 - X3 + X4, Ay = Y1 - Y2 - Y3 + Y4;
             const double Bx = X1 - X2 + X3 - X4, By = Y1 - Y2 + Y3 - Y4;
             const double Cx = X1 + X2 - X3 - X4, Cy = Y1 + Y2 - Y3 - Y4;
             const double P = sqrt((Ax + Bx * s) * (Ax + Bx * s) + (Ay + By * s) * (Ay + By * s));
             const double Q = sqrt((Bx * r + Cx) * (Bx * r + Cx) + (By * r + Cy) * (By * r + Cy));
-            const double i16 = 1.0 / (16 * detJ), i32 = 1.0 / (32 * detJ);
+            const double i16 = 1.0 / (16 * detJ), i32 = 0.5 * i16;  // exact: power-of-two scale
             double *bs = g + G_BS;  // [node][comp w,tx,ty][row]
             bs[0 * 6 + 0 * 2 + 0] = (sa * (r + 1) * P - sb * (s + 1) * Q) * i16;
             bs[0 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
@@ -1103,6 +1106,34 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             for (int c = 0; c < 3; ++c) ms[r * 3 + c] = acc[3 + r][3 + c] + accs[r][c];
     }
     __syncthreads();
+    // trace term of each element node, once per (element, node): thread
+    // (element, k) projects node k's rotation block onto the plane normal to
+    // the nodal normal; -1 marks a node the average skips
+    if (tid < NE * NEN && tid / NEN < ne) {
+        const int le = tid / NEN, k = tid % NEN;
+        const int node = conn[k * E + e0 + le];
+        double t = -1.0;
+        if (valid[node]) {
+            const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
+            const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+            if (nn != 0.0) {
+                const double *kr = sM + le * MISC + 9 * k;
+                const double u[3] = {n0 / nn, n1 / nn, n2 / nn};
+                double Pm[3][3], tmp[3][3];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) Pm[a][b] = ((a == b) ? 1.0 : 0.0) - u[a] * u[b];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        tmp[a][b] = Pm[a][0] * kr[0 * 3 + b] + Pm[a][1] * kr[1 * 3 + b] + Pm[a][2] * kr[2 * 3 + b];
+                double tr = 0.0;
+                for (int a = 0; a < 3; ++a) tr += tmp[a][0] * Pm[0][a] + tmp[a][1] * Pm[1][a] + tmp[a][2] * Pm[2][a];
+                tr = tr / 2.0;
+                t = tr > 0.0 ? tr : 0.0;
+            }
+        }
+        sM[le * MISC + 36 + k] = t;
+    }
+    __syncthreads();
     const int grpd = cact ? (p.elem_group ? p.elem_group[e0 + le3] : 0) : 0;
     const double dscale = p.composite ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
     if (split && cact && ba == bb) {
@@ -1111,27 +1142,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     }
     if (cact && ba == bb && dscale != 0.0) {
         const int e = e0 + le3;
-        const double *kr = sM + le3 * MISC;
+        // kavg: mean over the counted nodes, summed in node order
         double sum = 0.0;
         int cnt = 0;
         for (int k = 0; k < 4; ++k) {
-            const int node = conn[k * E + e];
-            if (!valid[node]) continue;
-            const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
-            const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-            if (nn == 0.0) continue;
-            const double u[3] = {n0 / nn, n1 / nn, n2 / nn};
-            double Pm[3][3], tmp[3][3];
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) Pm[a][b] = ((a == b) ? 1.0 : 0.0) - u[a] * u[b];
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b)
-                    tmp[a][b] = Pm[a][0] * kr[9 * k + 0 * 3 + b] + Pm[a][1] * kr[9 * k + 1 * 3 + b] +
-                                Pm[a][2] * kr[9 * k + 2 * 3 + b];
-            double tr = 0.0;
-            for (int a = 0; a < 3; ++a) tr += tmp[a][0] * Pm[0][a] + tmp[a][1] * Pm[1][a] + tmp[a][2] * Pm[2][a];
-            tr = tr / 2.0;
-            sum += tr > 0.0 ? tr : 0.0;
+            const double t = sM[le3 * MISC + 36 + k];
+            if (t < 0.0) continue;
+            sum += t;
             ++cnt;
         }
         const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * dscale;
@@ -1183,9 +1200,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
 __global__ __launch_bounds__(256) void scatter_pair_kernel(ScatterPlan sp, const double *__restrict__ Kb,
                                                            double *__restrict__ vals) {
     const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const int64_t p = gid / 6;
-    if (p >= sp.npairs) return;
-    const int r = (int)(gid - 6 * p);
+    const int64_t p = sp.pbegin + gid / 6;
+    if (p >= sp.pend) return;
+    const int r = (int)(gid % 6);
     const int I = sp.pair_node[2 * p], J = sp.pair_node[2 * p + 1];
     const int kI = sp.pair_slot[2 * p], kJ = sp.pair_slot[2 * p + 1];
     double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1825,7 +1842,8 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
-    const int nblocks = (p.nelem + NE - 1) / NE;
+    if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
+    const int nblocks = (p.eend - p.ebegin + NE - 1) / NE;
     if (nblocks == 0) return hipSuccess;
     hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG, SPLIT>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z,
                        conn, nx, ny, nz, valid, Kb, status);
@@ -1836,7 +1854,7 @@ template <int NE, int WG, bool SPLIT>
 static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
-    constexpr int total = NE * 4 * 47 + NE * (8 * 24 + 2) + NE * 4 * 14 + NE * 37;
+    constexpr int total = NE * 4 * 47 + NE * (8 * 24 + 2) + NE * 4 * 14 + NE * 41;
     const size_t smem = total * sizeof(double);
     static_assert(NE * 10 <= WG, "workgroup too small");
     static_assert(total * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
@@ -1845,7 +1863,8 @@ static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const doub
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
-    const int nblocks = (p.nelem + NE - 1) / NE;
+    if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
+    const int nblocks = (p.eend - p.ebegin + NE - 1) / NE;
     if (nblocks == 0) return hipSuccess;
     hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
                        nx, ny, nz, valid, Kb, status);
@@ -1899,7 +1918,8 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
 }
 
 hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals, hipStream_t st) {
-    const int64_t threads = sp.npairs * 6;
+    if (sp.pbegin < 0 || sp.pend > sp.npairs || sp.pbegin > sp.pend) return hipErrorInvalidValue;
+    const int64_t threads = (sp.pend - sp.pbegin) * 6;
     if (threads == 0) return hipSuccess;
     hipLaunchKernelGGL(scatter_pair_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sp, Kb, vals);
     return hipGetLastError();

@@ -108,6 +108,7 @@ SIGNATURES = {
     "fs_model_mixed_info": (C.c_int, [C.c_void_p, _i64p]),
     "fs_ctx_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i32p]),
     "fs_ctx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int32]),
+    "fs_ctx_set_pipeline": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 _lib = None
@@ -170,6 +171,11 @@ class Context:
     def set_tuning(self, nen, formulation, variant):
         self.check(self.lib.fs_ctx_set_tuning(self.h, int(nen), int(formulation), int(variant)),
                    "fs_ctx_set_tuning")
+
+    def set_pipeline(self, ranges):
+        """Element ranges of the assembly pipeline (1: no overlap of the
+        element kernel and the scatter)."""
+        self.check(self.lib.fs_ctx_set_pipeline(self.h, int(ranges)), "fs_ctx_set_pipeline")
 
     def close(self):
         if self.h:

@@ -246,10 +246,13 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
 int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *blocks_out);
 
 /* Device time (ms) of the last fs_model_stiffness / fs_model_explicit call,
- * measured with HIP events on the context stream: which = 0 element kernel,
- * 1 scatter kernel, 2 whole stiffness, 3 explicit loop, 4 construction of
- * the mixed-precision operator (0 when the last loop ran in FP64 or reused
- * an operator built earlier). */
+ * measured with HIP events on the context stream: which = 0 element kernels
+ * (first launch to last element range done; with the assembly pipeline this
+ * includes the scatter work that overlaps them), 1 the rest of the scatter
+ * (the whole scatter kernel when the pipeline has one range), 2 whole
+ * stiffness (0 + 1), 3 explicit loop, 4 construction of the mixed-precision
+ * operator (0 when the last loop ran in FP64 or reused an operator built
+ * earlier). */
 int fs_model_last_timing(const fs_model *m, int which, double *ms);
 
 /* Element-kernel launch shape chosen by the autotuner for a family (FS_T3 /
@@ -264,6 +267,14 @@ int fs_ctx_tuning(const fs_ctx *ctx, int nen, int formulation, int32_t *shape);
 /* Fix the launch shape (variant >= 0), re-arm the autotuner (-1), or switch
  * it off so that the default shape is used (-2). */
 int fs_ctx_set_tuning(fs_ctx *ctx, int nen, int formulation, int32_t variant);
+
+/* Element ranges of the assembly pipeline of fs_model_stiffness (no
+ * counterpart in the reference, which assembles on the CPU): the element
+ * kernel runs over `ranges` consecutive element ranges, and the scatter of
+ * the node pairs completed by each range runs on a second stream while the
+ * next range computes.  1 = element kernel, then scatter, on one stream.
+ * Every setting gives bitwise the same matrix.  Default 1; 1 <= ranges <= 64. */
+int fs_ctx_set_pipeline(fs_ctx *ctx, int ranges);
 
 /* Storage of the mixed-precision operator built by the last FS_PREC_MIXED /
  * FS_PREC_MIXED_F32 loop (no counterpart in the reference, whose loop runs on
