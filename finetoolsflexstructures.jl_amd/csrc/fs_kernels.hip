@@ -259,6 +259,34 @@ __device__ __forceinline__ void c_row_q4(const MatRows &mr, int s, double gN0, d
     }
 }
 
+// Row of C handled by iteration i of lane half h when the eight Q4 rows of
+// a node are split over two lanes: iterations 0..2 take the membrane rows
+// (h = 0) or the bending rows (h = 1) built from row i of the material
+// factor, iteration 3 the shear row 6 + h.  The two halves follow the same
+// control flow; their values equal those of c_row_q4 for the same row.
+__device__ __forceinline__ int q4_split_row(int i, int h) { return i < 3 ? 3 * h + i : 6 + h; }
+
+__device__ __forceinline__ void c_row_q4_split(const MatRows &mr, int i, int h, double gN0, double gN1,
+                                               const double *bs, double &cu, double &cv, double &cw,
+                                               double &cx, double &cy) {
+    if (i < 3) {
+        double lrow[6];
+        if (!mr.composite) {
+            const double sc = h ? mr.s_b : mr.s_m;
+            for (int j = 0; j < 3; ++j) {
+                const double v = sc * mr.L[i * 3 + j];
+                lrow[j] = h ? 0.0 : v;
+                lrow[3 + j] = h ? v : 0.0;
+            }
+        } else {
+            lmb_row(mr, 3 * h + i, lrow);
+        }
+        c_row_mb(lrow, gN0, gN1, cu, cv, cw, cx, cy);
+    } else {
+        c_row_sh<2>(mr.Ls, 6 + h, bs, cu, cv, cw, cx, cy);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Element kernel
 // ---------------------------------------------------------------------------
@@ -793,7 +821,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
 // point) stores a compact geometry record: E_G, gradN_e, the material scale
 // factors (and the layup angle for layered shells) and the MITC shear row
 // entries; the material factors are rebuilt per node thread.
-template <int NE, int WG, bool SPLIT>
+template <int NE, int WG, bool SPLIT, int RS>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_q4_seq_kernel(ElemParams p, const double *__restrict__ xs,
                                                          const double *__restrict__ ys,
                                                          const double *__restrict__ zs,
@@ -957,9 +985,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         }
     }
 
-    // thread roles
-    const bool gact = tid < NE * NEN && (tid / NEN) < ne;
-    const int le2 = tid / NEN, j2 = tid % NEN;
+    // thread roles: G2/G3 lane (element, node, row half) -- RS lanes share
+    // the eight rows of C of a node
+    static_assert(RS == 1 || RS == 2, "row split");
+    const bool gact = tid < NE * NEN * RS && (tid / (NEN * RS)) < ne;
+    const int le2 = tid / (NEN * RS), j2 = (tid / RS) % NEN, h2 = tid % RS;
     int node2 = 0;
     uint8_t valid2 = 0;
     double n20 = 0.0, n21 = 0.0, n22 = 0.0;
@@ -1018,14 +1048,23 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             const double m1 = (1 / A[2][2]) * A[0][2];
             const double m2 = (1 / A[2][2]) * A[1][2];
             double *nd = sN + (le2 * NEN + j2) * NODE;
-            for (int k = 0; k < 3; ++k) {
-                nd[k] = A[0][k];
-                nd[3 + k] = A[1][k];
-            }
-            for (int s = 0; s < NR; ++s) {
-                double cu, cv, cw, cx, cy;
-                c_row_q4(mr, s, gNj0, gNj1, bs, cu, cv, cw, cx, cy);
-                nd[6 + s] = m1 * cx + m2 * cy;
+            if (h2 == 0)
+                for (int k = 0; k < 3; ++k) {
+                    nd[k] = A[0][k];
+                    nd[3 + k] = A[1][k];
+                }
+            if (RS == 1) {
+                for (int s = 0; s < NR; ++s) {
+                    double cu, cv, cw, cx, cy;
+                    c_row_q4(mr, s, gNj0, gNj1, bs, cu, cv, cw, cx, cy);
+                    nd[6 + s] = m1 * cx + m2 * cy;
+                }
+            } else {
+                for (int i = 0; i < NR / 2; ++i) {
+                    double cu, cv, cw, cx, cy;
+                    c_row_q4_split(mr, i, h2, gNj0, gNj1, bs, cu, cv, cw, cx, cy);
+                    nd[6 + q4_split_row(i, h2)] = m1 * cx + m2 * cy;
+                }
             }
         }
         __syncthreads();
@@ -1051,9 +1090,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                                         g[G_SC + 4]);
             const double *bs = g + G_BS + j2 * 6;
             double *cdst = sC + le2 * CSTR + 6 * j2;
-            for (int s = 0; s < NR; ++s) {
-                double cu, cv, cw, cx, cy;
-                c_row_q4(mr, s, gNj0, gNj1, bs, cu, cv, cw, cx, cy);
+            // global-basis columns of node j for row s of C
+            auto emit = [&](int s, double cu, double cv, double cw, double cx, double cy) {
                 double cgt[3] = {0.0, 0.0, 0.0}, cgr[3] = {0.0, 0.0, 0.0};
                 for (int k = 0; k < 3; ++k) {
                     double cn = cu * A[0][k] + cv * A[1][k] + cw * A[2][k];
@@ -1067,6 +1105,19 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 for (int gg = 0; gg < 3; ++gg) {
                     cdst[s * K + gg] = cgt[gg];
                     cdst[s * K + 3 + gg] = cgr[gg];
+                }
+            };
+            if (RS == 1) {
+                for (int s = 0; s < NR; ++s) {
+                    double cu, cv, cw, cx, cy;
+                    c_row_q4(mr, s, gNj0, gNj1, bs, cu, cv, cw, cx, cy);
+                    emit(s, cu, cv, cw, cx, cy);
+                }
+            } else {
+                for (int i = 0; i < NR / 2; ++i) {
+                    double cu, cv, cw, cx, cy;
+                    c_row_q4_split(mr, i, h2, gNj0, gNj1, bs, cu, cv, cw, cx, cy);
+                    emit(q4_split_row(i, h2), cu, cv, cw, cx, cy);
                 }
             }
         }
@@ -1850,23 +1901,28 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
     return hipGetLastError();
 }
 
-template <int NE, int WG, bool SPLIT>
+// lanes per node in the G2/G3 phases of the Q4 kernel (1 or 2)
+#ifndef FS_Q4_RS
+#define FS_Q4_RS 2
+#endif
+
+template <int NE, int WG, bool SPLIT, int RS>
 static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
     constexpr int total = NE * 4 * 47 + NE * (8 * 24 + 2) + NE * 4 * 14 + NE * 41;
     const size_t smem = total * sizeof(double);
-    static_assert(NE * 10 <= WG, "workgroup too small");
+    static_assert(NE * 10 <= WG && NE * 4 * RS <= WG, "workgroup too small");
     static_assert(total * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
     if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT>,
+        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT, RS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
     if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
     const int nblocks = (p.eend - p.ebegin + NE - 1) / NE;
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
+    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT, RS>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
                        nx, ny, nz, valid, Kb, status);
     return hipGetLastError();
 }
@@ -1893,7 +1949,7 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
             if (p.formulation == 1) return launch_elem<3, 1, 12, 16, 128, true>(FS_ARGS);
             return launch_elem<3, 1, 8, 16, 128, true>(FS_ARGS);
         }
-        return launch_q4_seq<16, 192, true>(FS_ARGS);
+        return launch_q4_seq<16, 192, true, FS_Q4_RS>(FS_ARGS);
     }
     if (p.nen == 3) {
         if (p.formulation == 1) {
@@ -1910,9 +1966,9 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
         }
     }
     switch (variant) {
-        case 1: return launch_q4_seq<12, 128, false>(FS_ARGS);
-        case 2: return launch_q4_seq<6, 64, false>(FS_ARGS);
-        default: return launch_q4_seq<16, 192, false>(FS_ARGS);
+        case 1: return launch_q4_seq<12, 128, false, FS_Q4_RS>(FS_ARGS);
+        case 2: return launch_q4_seq<6, 64, false, FS_Q4_RS>(FS_ARGS);
+        default: return launch_q4_seq<16, 192, false, FS_Q4_RS>(FS_ARGS);
     }
 #undef FS_ARGS
 }
