@@ -175,11 +175,11 @@ struct MatRows {
     double Ls[4];
 };
 
-__device__ __forceinline__ MatRows mat_rows(const ElemParams &p, int e, double sc0, double sc1, double sc2,
-                                            double sc3, double sc4) {
+__device__ __forceinline__ MatRows mat_rows(const ElemParams &p, bool composite, int e, double sc0, double sc1,
+                                            double sc2, double sc3, double sc4) {
     MatRows r;
     const int grp = p.elem_group ? p.elem_group[e] : 0;
-    r.composite = p.composite != 0;
+    r.composite = composite;
     if (!r.composite) {
         const double *hrec = p.g_homog + 16 * grp;
         r.L = hrec;
@@ -821,7 +821,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
 // point) stores a compact geometry record: E_G, gradN_e, the material scale
 // factors (and the layup angle for layered shells) and the MITC shear row
 // entries; the material factors are rebuilt per node thread.
-template <int NE, int WG, bool SPLIT, int RS>
+template <int NE, int WG, bool SPLIT, int RS, bool COMP>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_q4_seq_kernel(ElemParams p, const double *__restrict__ xs,
                                                          const double *__restrict__ ys,
                                                          const double *__restrict__ zs,
@@ -866,9 +866,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 X[a][2] = zs[n];
             }
             const int grp = p.elem_group ? p.elem_group[e] : 0;
-            const double *hrec = p.composite ? nullptr : p.g_homog + 16 * grp;
-            const double tt = p.composite ? p.g_t[grp] : hrec[13];
-            const double stab_c = p.composite ? p.stab_c : hrec[14];
+            const double *hrec = COMP ? nullptr : p.g_homog + 16 * grp;
+            const double tt = COMP ? p.g_t[grp] : hrec[13];
+            const double stab_c = COMP ? p.stab_c : hrec[14];
             double *g = sG + (le * NQP + q) * GEO;
             const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
             const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
@@ -921,7 +921,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             }
             const double h = sqrt(maxd);
             const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
-            if (!p.composite) {
+            if (!COMP) {
                 g[G_SC + 0] = sqrt(tt * Jac);
                 g[G_SC + 1] = sqrt((tt * tt * tt / 12.0) * Jac);
                 g[G_SC + 2] = sqrt(tt * stab * Jac);
@@ -1041,7 +1041,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 nk_e[2] = 1.0;
             }
             nodal_triad(nk_e, A);
-            const MatRows mr = mat_rows(p, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], g[G_SC + 3],
+            const MatRows mr = mat_rows(p, COMP, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], g[G_SC + 3],
                                         g[G_SC + 4]);
             const double gNj0 = g[G_GN + 2 * j2], gNj1 = g[G_GN + 2 * j2 + 1];
             const double *bs = g + G_BS + j2 * 6;
@@ -1086,7 +1086,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 r2a[l] = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
                 r2b[l] = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
             }
-            const MatRows mr = mat_rows(p, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], g[G_SC + 3],
+            const MatRows mr = mat_rows(p, COMP, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], g[G_SC + 3],
                                         g[G_SC + 4]);
             const double *bs = g + G_BS + j2 * 6;
             double *cdst = sC + le2 * CSTR + 6 * j2;
@@ -1186,7 +1186,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     }
     __syncthreads();
     const int grpd = cact ? (p.elem_group ? p.elem_group[e0 + le3] : 0) : 0;
-    const double dscale = p.composite ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
+    const double dscale = COMP ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
     if (split && cact && ba == bb) {
         double *dd = p.split_dr + ((size_t)(e0 + le3) * NEN + ba) * 9;
         for (int i = 0; i < 9; ++i) dd[i] = 0.0;
@@ -1906,7 +1906,7 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
 #define FS_Q4_RS 2
 #endif
 
-template <int NE, int WG, bool SPLIT, int RS>
+template <int NE, int WG, bool SPLIT, int RS, bool COMP>
 static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
@@ -1915,14 +1915,14 @@ static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const doub
     static_assert(NE * 10 <= WG && NE * 4 * RS <= WG, "workgroup too small");
     static_assert(total * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
     if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT, RS>,
+        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
     if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
     const int nblocks = (p.eend - p.ebegin + NE - 1) / NE;
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT, RS>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
+    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
                        nx, ny, nz, valid, Kb, status);
     return hipGetLastError();
 }
@@ -1949,7 +1949,8 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
             if (p.formulation == 1) return launch_elem<3, 1, 12, 16, 128, true>(FS_ARGS);
             return launch_elem<3, 1, 8, 16, 128, true>(FS_ARGS);
         }
-        return launch_q4_seq<16, 192, true, FS_Q4_RS>(FS_ARGS);
+        return p.composite ? launch_q4_seq<16, 192, true, FS_Q4_RS, true>(FS_ARGS)
+                           : launch_q4_seq<16, 192, true, FS_Q4_RS, false>(FS_ARGS);
     }
     if (p.nen == 3) {
         if (p.formulation == 1) {
@@ -1966,9 +1967,12 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
         }
     }
     switch (variant) {
-        case 1: return launch_q4_seq<12, 128, false, FS_Q4_RS>(FS_ARGS);
-        case 2: return launch_q4_seq<6, 64, false, FS_Q4_RS>(FS_ARGS);
-        default: return launch_q4_seq<16, 192, false, FS_Q4_RS>(FS_ARGS);
+        case 1: return p.composite ? launch_q4_seq<12, 128, false, FS_Q4_RS, true>(FS_ARGS)
+                                   : launch_q4_seq<12, 128, false, FS_Q4_RS, false>(FS_ARGS);
+        case 2: return p.composite ? launch_q4_seq<6, 64, false, FS_Q4_RS, true>(FS_ARGS)
+                                   : launch_q4_seq<6, 64, false, FS_Q4_RS, false>(FS_ARGS);
+        default: return p.composite ? launch_q4_seq<16, 192, false, FS_Q4_RS, true>(FS_ARGS)
+                                    : launch_q4_seq<16, 192, false, FS_Q4_RS, false>(FS_ARGS);
     }
 #undef FS_ARGS
 }
