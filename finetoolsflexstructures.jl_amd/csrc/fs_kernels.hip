@@ -345,7 +345,7 @@ __device__ __forceinline__ void q4_gradn(double xi, double eta, double gN[4][2])
 // GaussRule(2, 2) abscissa as FinEtools tabulates it.
 #define FS_GAUSS2 0.577350269189626
 
-template <int NEN, int NQP, int NR, int NE, int WG, bool SPLIT>
+template <int NEN, int NQP, int NR, int NE, int WG, bool SPLIT, int RS>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void elem_kernel(ElemParams p, const double *__restrict__ xs,
                                                   const double *__restrict__ ys,
                                                   const double *__restrict__ zs,
@@ -588,11 +588,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
     // Rows of C are produced one at a time from the material factor rows in
     // the geometry record, so only the triad stays in registers across the
     // barrier.
-    const bool gthread = tid < NE * NQP * NEN;
-    const int le2 = tid / (NQP * NEN);
-    const int q2 = (tid / NEN) % NQP;
-    const int j2 = tid % NEN;
+    // Lane (element, qp, node, half): with RS = 2 the two halves of a node
+    // take rows 3h..3h+2 of the membrane/bending factor and half of the
+    // shear rows each (the same control flow on both halves).
+    static_assert(RS == 1 || RS == 2, "row split");
+    constexpr int RPL = NR / RS;  // rows of C per lane
+    const bool gthread = tid < NE * NQP * NEN * RS;
+    const int le2 = tid / (NQP * NEN * RS);
+    const int q2 = (tid / (NEN * RS)) % NQP;
+    const int j2 = (tid / RS) % NEN;
+    const int h2 = tid % RS;
     const bool gact = gthread && (le2 < ne);
+    auto row_of = [&](int i) { return RS == 1 ? i : (i < 3 ? 3 * h2 + i : 6 + h2 * (T::NSH / 2) + (i - 3)); };
     double A[3][3];
     if (gact) {
         const int e = e0 + le2;
@@ -615,13 +622,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
         const double m1 = (1 / A[2][2]) * A[0][2];
         const double m2 = (1 / A[2][2]) * A[1][2];
         double *nd = sN + ((le2 * NQP + q2) * NEN + j2) * T::NODE;
-        for (int k = 0; k < 3; ++k) {
-            nd[k] = A[0][k];
-            nd[3 + k] = A[1][k];
-        }
-        for (int s = 0; s < NR; ++s) {
+        if (h2 == 0)
+            for (int k = 0; k < 3; ++k) {
+                nd[k] = A[0][k];
+                nd[3 + k] = A[1][k];
+            }
+        for (int i = 0; i < RPL; ++i) {
+            const int s = row_of(i);
             double cu, cv, cw, cx, cy;
-            if (s < 6)
+            if (i < 3 || (RS == 1 && s < 6))
                 c_row_mb(g + T::G_LMB + s * 6, gNj0, gNj1, cu, cv, cw, cx, cy);
             else
                 c_row_sh<T::NSH>(g + T::G_LS, s, bs, cu, cv, cw, cx, cy);
@@ -653,9 +662,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
         }
         double kp[2] = {0.0, 0.0};
         double *cdst = sC + le2 * T::CSTR + (q2 * NR) * T::K + 6 * j2;
-        for (int s = 0; s < NR; ++s) {
+        for (int i = 0; i < RPL; ++i) {
+            const int s = row_of(i);
             double cu, cv, cw, cx, cy;
-            if (s < 6)
+            if (i < 3 || (RS == 1 && s < 6))
                 c_row_mb(g + T::G_LMB + s * 6, gNj0, gNj1, cu, cv, cw, cx, cy);
             else
                 c_row_sh<T::NSH>(g + T::G_LS, s, bs, cu, cv, cw, cx, cy);
@@ -675,7 +685,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
                 cdst[s * T::K + 3 + gg] = cgr[gg];
             }
         }
-        if (NQP == 1) {
+        if (RS == 2)  // both halves of the node: half 0's sum + half 1's sum
+            for (int l = 0; l < 2; ++l) {
+                const double other = __shfl_xor(kp[l], 1);
+                kp[l] = h2 == 0 ? kp[l] + other : other + kp[l];
+            }
+        if (NQP == 1 && h2 == 0) {
             double *ms = sM + le2 * T::MISC;
             for (int gg = 0; gg < 3; ++gg) ms[T::M_G + 3 * j2 + gg] = Q[2][gg];
             ms[T::M_KP + 2 * j2] = kp[0];
@@ -1878,25 +1893,30 @@ __global__ __launch_bounds__(256) void scale_kernel(int64_t n, const double *__r
 // Launch wrappers (host side of this translation unit)
 // ---------------------------------------------------------------------------
 
-template <int NEN, int NQP, int NR, int NE, int WG, bool SPLIT>
+// lanes per node in the G2/G3 phases of the T3 kernel (1 or 2)
+#ifndef FS_T3_RS
+#define FS_T3_RS 2
+#endif
+
+template <int NEN, int NQP, int NR, int NE, int WG, bool SPLIT, int RS = FS_T3_RS>
 static hipError_t launch_elem(const ElemParams &p, const double *x, const double *y, const double *z,
                               const int32_t *conn, const double *nx, const double *ny, const double *nz,
                               const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
     using S = ElemSmem<NEN, NQP, NR, NE>;
     const size_t smem = S::TOTAL * sizeof(double);
     static_assert(S::TOTAL * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
-    static_assert(NE * NQP * NEN <= WG && NE * (NEN * (NEN + 1) / 2) <= WG, "workgroup too small");
+    static_assert(NE * NQP * NEN * RS <= WG && NE * (NEN * (NEN + 1) / 2) <= WG, "workgroup too small");
     // set on every launch: cheap, per device, and free of shared host state
     // when several contexts launch from different threads
     if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)elem_kernel<NEN, NQP, NR, NE, WG, SPLIT>,
+        hipError_t e = hipFuncSetAttribute((const void *)elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
     if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
     const int nblocks = (p.eend - p.ebegin + NE - 1) / NE;
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG, SPLIT>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z,
+    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z,
                        conn, nx, ny, nz, valid, Kb, status);
     return hipGetLastError();
 }
