@@ -396,8 +396,10 @@ def main():
                    "elements": E, "nodes": N, "free_dofs": nf, "nnz": nnz, "cd_steps": ncd},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"assembly: elem_q4_seq_kernel over {args.pipeline} element ranges + "
-                               "scatter_pair_kernel on a second stream, HIP events around the whole",
+                     "kernel": ("assembly: elem_q4_seq_kernel then scatter_pair_kernel on one stream, HIP events "
+                                "around both" if args.pipeline == 1 else
+                                f"assembly: elem_q4_seq_kernel over {args.pipeline} element ranges + "
+                                "scatter_pair_kernel on a second stream, HIP events around the whole"),
                      "algorithmic_bytes": b_asm, "device_ms": dev_ms},
         "dof_steps_per_s": dof_steps,
         "cd_us_per_step": cd_ms / ncd * 1e3,
