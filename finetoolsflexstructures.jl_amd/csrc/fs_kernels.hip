@@ -1531,6 +1531,36 @@ __global__ __launch_bounds__(256) void cd_init_kernel(int64_t n, double dt, doub
 }
 
 // One step for a group of TPR lanes per row (TPR in {4, 8, 16, 32, 64}).
+// Row segment of a CSR product, lane `lane` of TPR: entries b + lane + i*TPR
+// in ascending i, summed in that order.  The column indices and values of a
+// chunk of FS_ROW_UNROLL entries are loaded before any of the gathers they
+// address, so a row costs two dependent memory round trips per chunk instead
+// of two per entry.
+#define FS_ROW_UNROLL 8
+template <int TPR>
+__device__ __forceinline__ double csr_row_dot(int b, int en, int lane, const int32_t *__restrict__ colidx,
+                                              const double *__restrict__ kval, const double *__restrict__ x) {
+    double e = 0.0;
+    for (int base = b + lane; base < en; base += TPR * FS_ROW_UNROLL) {
+        int c[FS_ROW_UNROLL];
+        double v[FS_ROW_UNROLL];
+#pragma unroll
+        for (int t = 0; t < FS_ROW_UNROLL; ++t) {
+            const int k = base + t * TPR;
+            const bool in = k < en;
+            c[t] = in ? colidx[k] : -1;
+            v[t] = in ? kval[k] : 0.0;
+        }
+        double xv[FS_ROW_UNROLL];
+#pragma unroll
+        for (int t = 0; t < FS_ROW_UNROLL; ++t) xv[t] = c[t] >= 0 ? x[c[t]] : 0.0;
+#pragma unroll
+        for (int t = 0; t < FS_ROW_UNROLL; ++t)
+            if (c[t] >= 0) e += v[t] * xv[t];
+    }
+    return e;
+}
+
 template <int TPR>
 __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *__restrict__ rowptr,
                                                       const int32_t *__restrict__ colidx,
@@ -1549,10 +1579,7 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
     const int lane = (int)(gt % TPR);
     const bool active = row < n;
     double e = 0.0;
-    if (active) {
-        const int b = rowptr[row], en = rowptr[row + 1];
-        for (int k = b + lane; k < en; k += TPR) e += kval[k] * Ucur[colidx[k]];
-    }
+    if (active) e = csr_row_dot<TPR>(rowptr[row], rowptr[row + 1], lane, colidx, kval, Ucur);
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, TPR);
     if (active && lane == 0) {
@@ -1835,10 +1862,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__r
     const int64_t row = gt / TPR;
     const int lane = (int)(gt % TPR);
     double e = 0.0;
-    if (row < n) {
-        const int b = rowptr[row], en = rowptr[row + 1];
-        for (int k = b + lane; k < en; k += TPR) e += kval[k] * x[colidx[k]];
-    }
+    if (row < n) e = csr_row_dot<TPR>(rowptr[row], rowptr[row + 1], lane, colidx, kval, x);
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, TPR);
     if (row < n && lane == 0) y[row] = e;
