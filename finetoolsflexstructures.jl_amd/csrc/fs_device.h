@@ -110,4 +110,18 @@ struct ScatterPlan {
     const int32_t *pair_slot;     // 2 * npairs: kI, kJ
 };
 
+// Rows of the free-free CSR matrix for the row products of the time loop and
+// the power iteration.  Each row of a shell matrix is a sequence of runs of
+// consecutive columns, one run per neighbour node (its free dofs), so the
+// products need no per-entry column index:
+//   rowrec[R]  = {first entry of row R, first and end neighbour slot of the
+//                 row's node, 0}
+//   slotrec[k] = {(offset of neighbour k's run within the row) << 4 | run
+//                 length, first column of the run}
+struct RunRows {
+    int64_t n;
+    const int4 *rowrec;
+    const int2 *slotrec;
+};
+
 }  // namespace fs

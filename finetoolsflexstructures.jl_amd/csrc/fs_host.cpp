@@ -38,13 +38,12 @@ hipError_t launch_mass(int nen, int64_t N, int E, int composite, const double *g
 hipError_t launch_cd_init(int64_t n, double dt, double g0, const double *invMC, const double *Fmag, const double *U0,
                           const double *V, double *A, double *U1, const int32_t *row_group, const double *group_dt,
                           const double *gtab, hipStream_t st);
-hipError_t launch_cd_step(int tpr, int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval,
+hipError_t launch_cd_step(int tpr, const RunRows &rows, const double *kval,
                           const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
                           const double *Cd, const double *Fmag, double gstep, double dt, int damped,
                           const int32_t *probe_slot, double *probe_dst, const int32_t *row_group,
                           const double *group_dt, const double *gtab, hipStream_t st);
-hipError_t launch_spmv(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval, const double *x,
-                       double *y, hipStream_t st);
+hipError_t launch_spmv(const RunRows &rows, const double *kval, const double *x, double *y, hipStream_t st);
 hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, const double *d, double *part,
                       double *out, hipStream_t st);
 hipError_t launch_scale(int64_t n, const double *ss, const double *d, const double *x, double *y, hipStream_t st);
@@ -148,8 +147,9 @@ struct Block {
     std::vector<int32_t> pair_emax;
     DevBuf nbr_ptr, nbr_node, nbr_pair, nbr_off, node_code, node_rowstart, contrib_ptr, contrib, pair_node,
         pair_slot;
-    // device CSR (int32 for the time loop) and values
-    DevBuf rowptr32, colidx32, kval, mdiag;
+    // device values (CSR order) and, for the free-free block, the run records
+    // of the row products of the time loop (fs::RunRows)
+    DevBuf rowrec, slotrec, kval, mdiag;
     bool k_ready = false, m_ready = false;
     void reset() {
         built = k_ready = m_ready = false;
@@ -158,7 +158,7 @@ struct Block {
         colval.clear();
         pair_emax.clear();
         for (DevBuf *b : {&nbr_ptr, &nbr_node, &nbr_pair, &nbr_off, &node_code, &node_rowstart, &contrib_ptr,
-                          &contrib, &pair_node, &pair_slot, &rowptr32, &colidx32, &kval, &mdiag})
+                          &contrib, &pair_node, &pair_slot, &rowrec, &slotrec, &kval, &mdiag})
             b->release();
     }
 };
@@ -490,10 +490,31 @@ int build_block(fs_model *m, int which) {
                 clist[cfill[p]++] = (int32_t)(((e * nblk + bl) << 1) | tr);
             }
     }
+    // run records of the free-free block (fs::RunRows): every row of node I
+    // is the runs of I's neighbours in slot order, each run the neighbour's
+    // free dofs (consecutive columns starting at its first free dof)
+    std::vector<int32_t> rrec, srec;
+    if (which == FS_BLOCK_FF) {
+        rrec.assign(4 * nrows, 0);
+        for (int64_t d = 0; d < nrows; ++d) {
+            const int64_t I = owner[d];
+            rrec[4 * d + 0] = (int32_t)B.rowptr[d];
+            rrec[4 * d + 1] = m->nbr_ptr[I];
+            rrec[4 * d + 2] = m->nbr_ptr[I + 1];
+        }
+        srec.assign(2 * nnb, 0);
+        for (size_t k = 0; k < nnb; ++k) {
+            const int J = m->nbr_list[k];
+            const int nf = nfr[J];
+            if (nf == 0) continue;  // no free dofs: an empty run
+            if (nb_offfree[k] >= (1 << 27))
+                return fail(c, FS_ERR_UNSUPPORTED, "row of more than 2^27 entries");
+            srec[2 * k] = (nb_offfree[k] << 4) | nf;
+            srec[2 * k + 1] = (int32_t)minfr[J];
+        }
+    }
     // upload
     int s;
-    std::vector<int32_t> rp32(nrows + 1);
-    for (int64_t i = 0; i <= nrows; ++i) rp32[i] = (int32_t)B.rowptr[i];
     if ((s = upload(c, B.nbr_ptr, m->nbr_ptr.data(), m->nbr_ptr.size()))) return s;
     if ((s = upload(c, B.nbr_node, m->nbr_list.data(), m->nbr_list.size()))) return s;
     if ((s = upload(c, B.nbr_pair, nb_pair.data(), nb_pair.size()))) return s;
@@ -504,8 +525,10 @@ int build_block(fs_model *m, int which) {
     if ((s = upload(c, B.contrib, clist.data(), clist.size()))) return s;
     if ((s = upload(c, B.pair_node, pnode.data(), pnode.size()))) return s;
     if ((s = upload(c, B.pair_slot, pslot.data(), pslot.size()))) return s;
-    if ((s = upload(c, B.rowptr32, rp32.data(), rp32.size()))) return s;
-    if ((s = upload(c, B.colidx32, B.colval.data(), B.colval.size()))) return s;
+    if (which == FS_BLOCK_FF) {
+        if ((s = upload(c, B.rowrec, rrec.data(), rrec.size()))) return s;
+        if ((s = upload(c, B.slotrec, srec.data(), srec.size()))) return s;
+    }
     if ((s = dalloc(c, B.kval, (size_t)nnz * sizeof(double)))) return s;
     if ((s = dalloc(c, B.mdiag, (size_t)nrows * sizeof(double)))) return s;
     FS_HIP(c, hipStreamSynchronize(c->stream));
@@ -516,6 +539,14 @@ int build_block(fs_model *m, int which) {
     B.built = true;
     B.k_ready = B.m_ready = false;
     return FS_OK;
+}
+
+fs::RunRows runrows_of(const Block &B) {
+    fs::RunRows r;
+    r.n = B.nrows;
+    r.rowrec = B.rowrec.as<int4>();
+    r.slotrec = B.slotrec.as<int2>();
+    return r;
 }
 
 fs::ScatterPlan plan_of(const Block &B) {
@@ -1279,8 +1310,10 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     std::vector<uint8_t> forced(p->nsteps + 1, 0);
     for (int64_t step = 0; step <= p->nsteps; ++step)
         for (int gi = 0; gi < ng && !forced[step]; ++gi) forced[step] = g[step * ng + gi] != 0.0;
-    // rows of the shell stencils hold ~40-60 entries: 8 lanes per row
-    const int tpr = 8;
+    // rows of the shell stencils hold 7 (T3) to 9 (Q4) neighbour runs: 4 lanes
+    // per row, up to three runs per lane loaded together (measured fastest
+    // of 2, 4, 8 and 16 lanes per row on both families)
+    const int tpr = 4;
     const int32_t *rg = batch ? dRG.as<int32_t>() : nullptr;
     const double *gdt = batch ? dGdt.as<double>() : nullptr;
     const fs::MixedOp op = mixed ? mixed_op(m) : fs::MixedOp{};
@@ -1303,8 +1336,7 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
                                              batch ? 0.0 : g[step], p->dt, damped ? 1 : 0, forced[step] ? 1 : 0,
                                              slot, pdst, rg, gdt, gt, st);
             else
-                e = fs::launch_cd_step(tpr, n, B.rowptr32.as<int32_t>(), B.colidx32.as<int32_t>(),
-                                       B.kval.as<double>(), dU[cur].as<double>(), dU[1 - cur].as<double>(),
+                e = fs::launch_cd_step(tpr, runrows_of(B), B.kval.as<double>(), dU[cur].as<double>(), dU[1 - cur].as<double>(),
                                        dV.as<double>(), dA.as<double>(), dInv.as<double>(), dC.as<double>(),
                                        dF.as<double>(), batch ? 0.0 : g[step], p->dt, damped ? 1 : 0, slot, pdst, rg,
                                        gdt, gt, st);
@@ -1369,7 +1401,6 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
     c->err.clear();
     Block &B = m->blk[FS_BLOCK_FF];
     if (!B.k_ready || !B.m_ready) return fail(c, FS_ERR_STATE, "omega_max: FF-block K and M must be computed first");
-    if (!B.rowptr32.p) return fail(c, FS_ERR_UNSUPPORTED, "omega_max: more than 2^31 stored entries");
     FS_HIP(c, hipSetDevice(c->device));
     const int64_t n = B.nrows;
     std::vector<double> M(n), invM(n);
@@ -1382,17 +1413,17 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
         (s = dalloc(c, dpart, 512 * 8)) || (s = upload(c, dss, one, 4)))
         return s;
     double *v = dv.as<double>(), *w = dw.as<double>(), *ss = dss.as<double>();
-    const int32_t *rp = B.rowptr32.as<int32_t>(), *ci = B.colidx32.as<int32_t>();
+    const fs::RunRows rows = runrows_of(B);
     const double *kv = B.kval.as<double>();
     for (int it = 0; it < maxit; ++it) {
-        FS_HIP(c, fs::launch_spmv(n, rp, ci, kv, v, w, c->stream));                        // w = K v
+        FS_HIP(c, fs::launch_spmv(rows, kv, v, w, c->stream));                              // w = K v
         FS_HIP(c, fs::launch_dot(n, 0, w, nullptr, nullptr, dpart.as<double>(), ss, c->stream));
         FS_HIP(c, fs::launch_scale(n, ss, nullptr, w, w, c->stream));                       // w /= |w|
         FS_HIP(c, fs::launch_scale(n, ss + 3, dinv.as<double>(), w, v, c->stream));          // v = invM .* w
         FS_HIP(c, fs::launch_dot(n, 0, v, nullptr, nullptr, dpart.as<double>(), ss + 1, c->stream));
         FS_HIP(c, fs::launch_scale(n, ss + 1, nullptr, v, v, c->stream));                    // v /= |v|
     }
-    FS_HIP(c, fs::launch_spmv(n, rp, ci, kv, v, w, c->stream));
+    FS_HIP(c, fs::launch_spmv(rows, kv, v, w, c->stream));
     FS_HIP(c, fs::launch_dot(n, 1, w, v, nullptr, dpart.as<double>(), ss, c->stream));
     FS_HIP(c, fs::launch_dot(n, 2, v, nullptr, B.mdiag.as<double>(), dpart.as<double>(), ss + 1, c->stream));
     double h[2];

@@ -1531,39 +1531,50 @@ __global__ __launch_bounds__(256) void cd_init_kernel(int64_t n, double dt, doub
 }
 
 // One step for a group of TPR lanes per row (TPR in {4, 8, 16, 32, 64}).
-// Row segment of a CSR product, lane `lane` of TPR: entries b + lane + i*TPR
-// in ascending i, summed in that order.  The column indices and values of a
-// chunk of FS_ROW_UNROLL entries are loaded before any of the gathers they
-// address, so a row costs two dependent memory round trips per chunk instead
-// of two per entry.
-#define FS_ROW_UNROLL 8
+// Row R of K times x, lane `lane` of TPR: the lane takes the neighbour runs
+// rr.y + lane, rr.y + lane + TPR, ... and sums each run's entries in column
+// order; the TPR partial sums are then combined by the caller's butterfly.
+// The run records of up to FS_RUN_UNROLL runs are loaded before the values
+// and displacements they address.
+#ifndef FS_RUN_UNROLL
+#define FS_RUN_UNROLL 3
+#endif
 template <int TPR>
-__device__ __forceinline__ double csr_row_dot(int b, int en, int lane, const int32_t *__restrict__ colidx,
+__device__ __forceinline__ double run_row_dot(int4 rr, int lane, const int2 *__restrict__ slotrec,
                                               const double *__restrict__ kval, const double *__restrict__ x) {
     double e = 0.0;
-    for (int base = b + lane; base < en; base += TPR * FS_ROW_UNROLL) {
-        int c[FS_ROW_UNROLL];
-        double v[FS_ROW_UNROLL];
+    for (int k0 = rr.y + lane; k0 < rr.z; k0 += TPR * FS_RUN_UNROLL) {
+        int2 sr[FS_RUN_UNROLL];
 #pragma unroll
-        for (int t = 0; t < FS_ROW_UNROLL; ++t) {
-            const int k = base + t * TPR;
-            const bool in = k < en;
-            c[t] = in ? colidx[k] : -1;
-            v[t] = in ? kval[k] : 0.0;
+        for (int t = 0; t < FS_RUN_UNROLL; ++t) {
+            const int k = k0 + t * TPR;
+            sr[t] = k < rr.z ? slotrec[k] : make_int2(0, 0);
         }
-        double xv[FS_ROW_UNROLL];
+        double kk[FS_RUN_UNROLL][6], xx[FS_RUN_UNROLL][6];
 #pragma unroll
-        for (int t = 0; t < FS_ROW_UNROLL; ++t) xv[t] = c[t] >= 0 ? x[c[t]] : 0.0;
+        for (int t = 0; t < FS_RUN_UNROLL; ++t) {
+            const int nf = sr[t].x & 15;
+            const double *kv = kval + rr.x + (sr[t].x >> 4);
+            const double *xv = x + sr[t].y;
 #pragma unroll
-        for (int t = 0; t < FS_ROW_UNROLL; ++t)
-            if (c[t] >= 0) e += v[t] * xv[t];
+            for (int c = 0; c < 6; ++c) {
+                kk[t][c] = c < nf ? kv[c] : 0.0;
+                xx[t][c] = c < nf ? xv[c] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < FS_RUN_UNROLL; ++t) {
+            const int nf = sr[t].x & 15;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (c < nf) e += kk[t][c] * xx[t][c];
+        }
     }
     return e;
 }
 
 template <int TPR>
-__global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *__restrict__ rowptr,
-                                                      const int32_t *__restrict__ colidx,
+__global__ __launch_bounds__(256) void cd_step_kernel(RunRows rows,
                                                       const double *__restrict__ kval, const double *__restrict__ Ucur,
                                                       double *__restrict__ Unext, double *__restrict__ V,
                                                       double *__restrict__ A, const double *__restrict__ invMC,
@@ -1577,9 +1588,9 @@ __global__ __launch_bounds__(256) void cd_step_kernel(int64_t n, const int32_t *
     const int64_t gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t row = gt / TPR;
     const int lane = (int)(gt % TPR);
-    const bool active = row < n;
+    const bool active = row < rows.n;
     double e = 0.0;
-    if (active) e = csr_row_dot<TPR>(rowptr[row], rowptr[row + 1], lane, colidx, kval, Ucur);
+    if (active) e = run_row_dot<TPR>(rows.rowrec[row], lane, rows.slotrec, kval, Ucur);
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, TPR);
     if (active && lane == 0) {
@@ -1854,18 +1865,16 @@ __global__ void gather_kernel(int n, const int64_t *__restrict__ idx, const doub
 
 // y = K x (power iteration)
 template <int TPR>
-__global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rowptr,
-                                                   const int32_t *__restrict__ colidx,
-                                                   const double *__restrict__ kval, const double *__restrict__ x,
-                                                   double *__restrict__ y) {
+__global__ __launch_bounds__(256) void spmv_kernel(RunRows rows, const double *__restrict__ kval,
+                                                   const double *__restrict__ x, double *__restrict__ y) {
     const int64_t gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t row = gt / TPR;
     const int lane = (int)(gt % TPR);
     double e = 0.0;
-    if (row < n) e = csr_row_dot<TPR>(rowptr[row], rowptr[row + 1], lane, colidx, kval, x);
+    if (row < rows.n) e = run_row_dot<TPR>(rows.rowrec[row], lane, rows.slotrec, kval, x);
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, TPR);
-    if (row < n && lane == 0) y[row] = e;
+    if (row < rows.n && lane == 0) y[row] = e;
 }
 
 // Deterministic reductions for the power iteration: fixed tree per block,
@@ -2068,17 +2077,17 @@ hipError_t launch_cd_init(int64_t n, double dt, double g0, const double *invMC, 
     return hipGetLastError();
 }
 
-hipError_t launch_cd_step(int tpr, int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval,
+hipError_t launch_cd_step(int tpr, const RunRows &rows, const double *kval,
                           const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
                           const double *Cd, const double *Fmag, double gstep, double dt, int damped,
                           const int32_t *probe_slot, double *probe_dst, const int32_t *row_group,
                           const double *group_dt, const double *gtab, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    const int64_t threads = n * tpr;
+    if (rows.n == 0) return hipSuccess;
+    const int64_t threads = rows.n * tpr;
     const unsigned blocks = (unsigned)((threads + 255) / 256);
 #define FS_CD_CASE(T)                                                                                           \
     case T:                                                                                                     \
-        hipLaunchKernelGGL(cd_step_kernel<T>, dim3(blocks), dim3(256), 0, st, n, rowptr, colidx, kval, Ucur,     \
+        hipLaunchKernelGGL(cd_step_kernel<T>, dim3(blocks), dim3(256), 0, st, rows, kval, Ucur,                  \
                            Unext, V, A, invMC, Cd, Fmag, gstep, dt, damped, probe_slot, probe_dst, row_group,    \
                            group_dt, gtab);                                                                    \
         break;
@@ -2101,12 +2110,10 @@ hipError_t launch_gather(int n, const int64_t *idx, const double *U, double *out
     return hipGetLastError();
 }
 
-hipError_t launch_spmv(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *kval, const double *x,
-                       double *y, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    const int64_t threads = n * 8;
-    hipLaunchKernelGGL(spmv_kernel<8>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, n, rowptr, colidx,
-                       kval, x, y);
+hipError_t launch_spmv(const RunRows &rows, const double *kval, const double *x, double *y, hipStream_t st) {
+    if (rows.n == 0) return hipSuccess;
+    const int64_t threads = rows.n * 4;
+    hipLaunchKernelGGL(spmv_kernel<4>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, rows, kval, x, y);
     return hipGetLastError();
 }
 
