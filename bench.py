@@ -127,22 +127,25 @@ def bench_batch(nmodels, nsteps):
 
 
 def mixed_step_bytes(N, E, nen, kmb_pairs, nsr, nnb, fmag_frac):
-    """Algorithmic bytes of one mixed-precision step: FP32 K_mb slices, the
-    neighbour lists, FP64 shear factor rows, drilling blocks, incidence and
-    connectivity, node coordinates, and the FP64 state over 6 N positions
-    (u gathered once; U V A invMC read, Fmag read on the steps with a nonzero
-    load factor (fraction fmag_frac), U V A written)."""
+    """Algorithmic bytes of one mixed-precision step: FP32 slices, the
+    neighbour lists, FP64 shear factor rows with their incidence records and
+    connectivity (nsr > 0 only), drilling blocks, node coordinates, and the
+    FP64 state over 6 N positions (u gathered once; U V A invMC read, Fmag
+    read on the steps with a nonzero load factor (fraction fmag_frac), U V A
+    written)."""
     n6 = 6 * N
-    return (kmb_pairs * 8 + nnb * 4 + (N + 1) * 4 + E * nsr * 6 * nen * 8 + N * 48 + E * nen * 4 * 2
-            + (N + 1) * 4 + N * 24 + n6 * (8 + 32 + 8 * fmag_frac + 24))
+    shear = (E * nsr * 6 * nen * 8 + E * nen * 4 * 2 + (N + 1) * 4) if nsr > 0 else 0
+    return (kmb_pairs * 8 + nnb * 4 + (N + 1) * 4 + shear + N * 48 + N * 32
+            + n6 * (8 + 32 + 8 * fmag_frac + 24))
 
 
 def bench_mixed(nsteps, nx=1024, ny=512):
     """Flat steel plate 2 x 1 m, t 0.004, T3 nx x ny, clamped, Gaussian
     pressure pulse (sigma 0.1 m, 1e4 Pa peak, centre seeded, 1e-4 s): nsteps
-    central-difference steps at 0.9 of the stable dt in FP64 (CSR) and mixed
-    precision; trajectory error of the mixed run against the FP64 run (which
-    the tests pin to the oracle) over 64 probe dofs every 10 steps."""
+    central-difference steps at 0.9 of the stable dt in FP64 (CSR) and in the
+    two mixed precisions (FP64 shear factors; shear in the FP32 slices);
+    trajectory error of the mixed runs against the FP64 run (which the tests
+    pin to the oracle) over 64 probe dofs every 10 steps."""
     import numpy as np
 
     from flexshell import _lib, explicit, femm as FM, workloads as W
@@ -165,12 +168,13 @@ def bench_mixed(nsteps, nx=1024, ny=512):
     z = np.zeros(nf)
     print(f"[bench] mixed: setup {time.perf_counter() - t0:.1f}s, {nf} free dofs, dt {dt:.3e}", file=sys.stderr,
           flush=True)
-    runs = {}
+    runs, info = {}, {}
     for name, prec in (("fp64", _lib.FS_PREC_FP64), ("mixed", _lib.FS_PREC_MIXED),
                        ("mixed_f32", _lib.FS_PREC_MIXED_F32)):
         explicit.run(f._model, 0.0, z, z, dt, 10, F, g[:11], precision=prec)  # warm (and build the operator)
         r = explicit.run(f._model, 0.0, z, z, dt, nsteps, F, g, probes, 10, precision=prec)
         runs[name] = r
+        info[name] = f._model.mixed_info()
         print(f"[bench] mixed: {name} {r['ms'] / nsteps * 1e3:.1f} us/step", file=sys.stderr, flush=True)
     ref = runs["fp64"]["probes"]
     E, N = c["conn"].shape[0], c["xyz"].shape[0]
@@ -181,13 +185,15 @@ def bench_mixed(nsteps, nx=1024, ny=512):
         out[f"{name}_us_per_step"] = r["ms"] / nsteps * 1e3
         if name != "fp64":
             out[f"{name}_max_rel_traj_err"] = float(np.abs(r["probes"] - ref).max() / np.abs(ref).max())
-    out["mixed_over_fp64"] = out["mixed_dof_steps_per_s"] / out["fp64_dof_steps_per_s"]
-    kmb_pairs, nsr, nnb, _ = f._model.mixed_info()
-    b_mx = mixed_step_bytes(N, E, 3, kmb_pairs, nsr, nnb, float(np.mean(g[1:] != 0)))
-    ach = b_mx / (runs["mixed"]["ms"] / nsteps / 1e3) / 1e9
-    out["roofline_mixed"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": ach / HBM_PEAK_GBS, "kernel": "cd_mixed_step_kernel",
-                             "algorithmic_bytes": b_mx, "bytes_per_dof": b_mx / nf}
+    for name in ("mixed", "mixed_f32"):
+        out[f"{name}_over_fp64"] = out[f"{name}_dof_steps_per_s"] / out["fp64_dof_steps_per_s"]
+        kmb_pairs, nsr, nnb, _ = info[name]
+        b_mx = mixed_step_bytes(N, E, 3, kmb_pairs, nsr, nnb, float(np.mean(g[1:] != 0)))
+        ach = b_mx / (runs[name]["ms"] / nsteps / 1e3) / 1e9
+        out[f"roofline_{name}"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": ach / HBM_PEAK_GBS,
+                                   "kernel": f"cd_mixed_step_kernel<3, {nsr}>",
+                                   "algorithmic_bytes": b_mx, "bytes_per_dof": b_mx / nf}
     f._model.close()
     return out
 

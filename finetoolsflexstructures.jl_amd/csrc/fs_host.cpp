@@ -53,7 +53,7 @@ hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, 
                               const int32_t *conn, const double *dre, double *dr, hipStream_t st);
 hipError_t launch_pack_xyz(int64_t N, const double *x, const double *y, const double *z, double4 *xyz,
                            hipStream_t st);
-hipError_t launch_cd_mixed_step(int nen, int nsr, bool f32prod, const MixedOp &op, const double *Ucur, double *Unext,
+hipError_t launch_cd_mixed_step(int nen, int nsr, const MixedOp &op, const double *Ucur, double *Unext,
                                 double *V, double *A, const double *invMC, const double *Cd, const double *Fmag,
                                 double gstep, double dt, int damped, int forced, const int32_t *probe_slot,
                                 double *probe_dst, const int32_t *row_group, const double *group_dt,
@@ -169,6 +169,7 @@ struct Block {
 struct MixedStore {
     bool built = false;
     uint64_t kgen = 0;   // element-parameter generation it was built from
+    bool shear_fp32 = false;  // transverse shear in the FP32 slices (no factor rows)
     int nsr = 0;
     int64_t nslices = 0;
     int64_t kmb_pairs = 0;  // float2 values
@@ -742,7 +743,7 @@ int element_blocks(fs_model *m, const fs_shell_params *p, float *elem_ms) {
 // stiffness evaluation: split element kernel (membrane + bending blocks,
 // shear factor rows, drilling blocks), then the FP32 slices of K_mb and the
 // per-node drilling blocks.
-int build_mixed(fs_model *m) {
+int build_mixed(fs_model *m, bool shear_fp32) {
     fs_ctx *c = m->ctx;
     Block &B = m->blk[FS_BLOCK_FF];
     MixedStore &X = m->mixed;
@@ -750,7 +751,7 @@ int build_mixed(fs_model *m) {
     const int nen = m->nen;
     const int nblk = nen * (nen + 1) / 2;
     const int K = 6 * nen;
-    const int nsr = nen == 3 ? (m->last_ep.formulation == FS_SHEAR_AVERAGE_K ? 6 : 2) : 8;
+    const int nsr = shear_fp32 ? 0 : nen == 3 ? (m->last_ep.formulation == FS_SHEAR_AVERAGE_K ? 6 : 2) : 8;
     const int64_t N = m->N, E = m->E, nrows = 6 * N;
     // slices of 10 nodes (60 rows): slot count = largest neighbour count of the slice's nodes
     (void)nrows;
@@ -765,23 +766,26 @@ int build_mixed(fs_model *m) {
     int s;
     DevBuf kbm, dre;
     if ((s = dalloc(c, kbm, (size_t)E * nblk * 36 * 8)) || (s = dalloc(c, dre, (size_t)E * nen * 9 * 8)) ||
-        (s = dalloc(c, X.cs, (size_t)E * nsr * K * 8)) || (s = dalloc(c, X.dr, (size_t)N * 6 * 8)) ||
+        (nsr > 0 && (s = dalloc(c, X.cs, (size_t)E * nsr * K * 8))) || (s = dalloc(c, X.dr, (size_t)N * 6 * 8)) ||
         (s = upload(c, X.goff, goff.data(), goff.size())) ||
         (s = dalloc(c, X.kmb, (size_t)goff[ns] * sizeof(float2))))
         return s;
-    // incidence records: element and its nodes, in each node's ascending element order
-    const int ps = nen == 3 ? 4 : 8;
-    std::vector<int32_t> pack((size_t)m->ne_list.size() * ps, 0);
-    for (size_t k = 0; k < m->ne_list.size(); ++k) {
-        const int32_t e = m->ne_list[k];
-        pack[k * ps] = e;
-        for (int a = 0; a < nen; ++a) pack[k * ps + 1 + a] = m->conn[(size_t)e * nen + a];
+    if (nsr > 0) {
+        // incidence records: element and its nodes, in each node's ascending element order
+        const int ps = nen == 3 ? 4 : 8;
+        std::vector<int32_t> pack((size_t)m->ne_list.size() * ps, 0);
+        for (size_t k = 0; k < m->ne_list.size(); ++k) {
+            const int32_t e = m->ne_list[k];
+            pack[k * ps] = e;
+            for (int a = 0; a < nen; ++a) pack[k * ps + 1 + a] = m->conn[(size_t)e * nen + a];
+        }
+        if ((s = upload(c, X.ne_pack, pack.data(), pack.size()))) return s;
     }
-    if ((s = upload(c, X.ne_pack, pack.data(), pack.size())) || (s = dalloc(c, X.xyz, (size_t)N * 32))) return s;
+    if ((s = dalloc(c, X.xyz, (size_t)N * 32))) return s;
     FS_HIP(c, fs::launch_pack_xyz(N, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(), X.xyz.as<double4>(),
                                   c->stream));
     fs::ElemParams ep = m->last_ep;
-    ep.split_cs = X.cs.as<double>();
+    ep.split_cs = nsr > 0 ? X.cs.as<double>() : nullptr;  // nullptr: shear stays in the blocks
     ep.split_dr = dre.as<double>();
     hipEvent_t e0, e1;
     FS_HIP(c, hipEventCreate(&e0));
@@ -802,6 +806,7 @@ int build_mixed(fs_model *m) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     X.nsr = nsr;
+    X.shear_fp32 = shear_fp32;
     X.nslices = ns;
     X.kmb_pairs = goff[ns];
     X.kgen = m->kgen;
@@ -1255,8 +1260,9 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     double build_ms = 0.0;
     if (mixed) {
         if (m->N > (int64_t)INT32_MAX / 6) return fail(c, FS_ERR_UNSUPPORTED, "explicit: model too large for mixed");
-        if (!m->mixed.built || m->mixed.kgen != m->kgen) {
-            if ((s = build_mixed(m))) return s;
+        const bool shear_fp32 = p->precision == FS_PREC_MIXED_F32;
+        if (!m->mixed.built || m->mixed.kgen != m->kgen || m->mixed.shear_fp32 != shear_fp32) {
+            if ((s = build_mixed(m, shear_fp32))) return s;
             build_ms = m->mixed.build_ms;
         }
     }
@@ -1317,7 +1323,6 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     const int32_t *rg = batch ? dRG.as<int32_t>() : nullptr;
     const double *gdt = batch ? dGdt.as<double>() : nullptr;
     const fs::MixedOp op = mixed ? mixed_op(m) : fs::MixedOp{};
-    const bool f32prod = p->precision == FS_PREC_MIXED_F32;
     auto enqueue = [&](hipStream_t st) -> hipError_t {
         hipError_t e = fs::launch_cd_init(nv, p->dt, batch ? 0.0 : g[0], dInv.as<double>(), dF.as<double>(),
                                           dU[0].as<double>(), dV.as<double>(), dA.as<double>(), dU[1].as<double>(),
@@ -1330,7 +1335,7 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
             double *pdst = rec ? dProbe.as<double>() + (step / stride) * nprobes : nullptr;
             const double *gt = batch ? dG.as<double>() + step * ng : nullptr;
             if (mixed)
-                e = fs::launch_cd_mixed_step(m->nen, m->mixed.nsr, f32prod, op, dU[cur].as<double>(),
+                e = fs::launch_cd_mixed_step(m->nen, m->mixed.nsr, op, dU[cur].as<double>(),
                                              dU[1 - cur].as<double>(), dV.as<double>(), dA.as<double>(),
                                              dInv.as<double>(), dC.as<double>(), dF.as<double>(),
                                              batch ? 0.0 : g[step], p->dt, damped ? 1 : 0, forced[step] ? 1 : 0,

@@ -345,7 +345,7 @@ __device__ __forceinline__ void q4_gradn(double xi, double eta, double gN[4][2])
 // GaussRule(2, 2) abscissa as FinEtools tabulates it.
 #define FS_GAUSS2 0.577350269189626
 
-template <int NEN, int NQP, int NR, int NE, int WG, bool SPLIT, int RS>
+template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void elem_kernel(ElemParams p, const double *__restrict__ xs,
                                                   const double *__restrict__ ys,
                                                   const double *__restrict__ zs,
@@ -716,14 +716,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
             }
     }
     const bool cact = cthread && (le3 < ne);
-    constexpr bool split = SPLIT;
+    constexpr bool split_sh = SPLIT == 1;  // shear rows leave as factors
+    constexpr bool split_dr = SPLIT != 0;  // drilling blocks leave separately
     double acc[6][6];
     for (int r = 0; r < 6; ++r)
         for (int c = 0; c < 6; ++c) acc[r][c] = 0.0;
     if (cact) {
         const double *cb = sC + le3 * T::CSTR;
         for (int s = 0; s < T::NS; ++s) {
-            if (split && (s % NR) >= 6) continue;  // shear rows leave as factors
+            if (split_sh && (s % NR) >= 6) continue;
             double va[6], vb[6];
             for (int r = 0; r < 6; ++r) {
                 va[r] = cb[s * T::K + 6 * ba + r];
@@ -742,7 +743,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
             const double kavg = (((((ms[T::M_KP + 0] + ms[T::M_KP + 2]) + ms[T::M_KP + 4]) + ms[T::M_KP + 1]) +
                                    ms[T::M_KP + 3]) + ms[T::M_KP + 5]) / 6 * dscale;
             const double *gv = ms + T::M_G + 3 * ba;
-            if (split) {
+            if (split_dr) {
                 double *dd = p.split_dr + ((size_t)e * NEN + ba) * 9;
                 for (int r = 0; r < 3; ++r)
                     for (int c = 0; c < 3; ++c) dd[r * 3 + c] = valid[node] ? kavg * (gv[r] * gv[c]) : 0.0;
@@ -800,7 +801,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
             }
         }
     }
-    if (split) {
+    if (split_sh) {
         // shear rows of C (global basis) of every element of the block
         constexpr int NSR = NQP * T::NSH;
         for (int i = tid; i < ne * NSR * T::K; i += WG) {
@@ -836,7 +837,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
 // point) stores a compact geometry record: E_G, gradN_e, the material scale
 // factors (and the layup angle for layered shells) and the MITC shear row
 // entries; the material factors are rebuilt per node thread.
-template <int NE, int WG, bool SPLIT, int RS, bool COMP>
+template <int NE, int WG, int SPLIT, int RS, bool COMP>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_q4_seq_kernel(ElemParams p, const double *__restrict__ xs,
                                                          const double *__restrict__ ys,
                                                          const double *__restrict__ zs,
@@ -1029,9 +1030,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 ++cnt;
             }
     }
-    constexpr bool split = SPLIT;
+    constexpr bool split_sh = SPLIT == 1;  // shear rows leave as factors
+    constexpr bool split_dr = SPLIT != 0;  // drilling blocks leave separately
     double acc[6][6];
-    double accs[3][3];  // split: shear part of the rotation block (drilling penalty input)
+    double accs[3][3];  // split_sh: shear part of the rotation block (drilling penalty input)
     for (int r = 0; r < 6; ++r)
         for (int c = 0; c < 6; ++c) acc[r][c] = 0.0;
     for (int r = 0; r < 3; ++r)
@@ -1146,7 +1148,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                     va[r] = cbp[s * K + 6 * ba + r];
                     vb[r] = cbp[s * K + 6 * bb + r];
                 }
-                if (split && s >= 6) {
+                if (split_sh && s >= 6) {
                     if (ba == bb)
                         for (int r = 0; r < 3; ++r)
                             for (int c = 0; c < 3; ++c) accs[r][c] += va[3 + r] * vb[3 + c];
@@ -1156,7 +1158,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                     for (int c = 0; c < 6; ++c) acc[r][c] += va[r] * vb[c];
             }
         }
-        if (split) {
+        if (split_sh) {
             // shear rows of this Gauss point (global basis): factor rows 2q, 2q+1
             for (int i = tid; i < ne * 2 * K; i += WG) {
                 const int le = i / (2 * K), rem = i % (2 * K);
@@ -1202,7 +1204,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     __syncthreads();
     const int grpd = cact ? (p.elem_group ? p.elem_group[e0 + le3] : 0) : 0;
     const double dscale = COMP ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
-    if (split && cact && ba == bb) {
+    if (split_dr && cact && ba == bb) {
         double *dd = p.split_dr + ((size_t)(e0 + le3) * NEN + ba) * 9;
         for (int i = 0; i < 9; ++i) dd[i] = 0.0;
     }
@@ -1223,7 +1225,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
             if (sqrt(n0 * n0 + n1 * n1 + n2 * n2) != 0.0) {
                 const double nv[3] = {n0, n1, n2};
-                if (split) {
+                if (split_dr) {
                     double *dd = p.split_dr + ((size_t)e * NEN + ba) * 9;
                     for (int r = 0; r < 3; ++r)
                         for (int c = 0; c < 3; ++c) dd[r * 3 + c] = kavg * (nv[r] * nv[c]);
@@ -1634,7 +1636,8 @@ __global__ __launch_bounds__(256) void cd_step_kernel(RunRows rows,
 //             + [r >= 3] D_i[r-3, :] . theta_i
 // with the incident elements visited in ascending order, so the result is
 // deterministic.  K_mb values are the FP64 element-order sums of the element
-// blocks, rounded once to FP32.
+// blocks, rounded once to FP32.  In the FP32-shear form (NSR = 0) the slices
+// hold K_mb + K_s and the shear term above is absent.
 
 // K_mb slices from the split element blocks: thread per (node, row, neighbour
 // slot); the sum over the pair's contributions follows scatter_pair_kernel.
@@ -1708,9 +1711,11 @@ __global__ __launch_bounds__(256) void mixed_drill_kernel(int64_t N, int E, cons
 //   K_mb: the row sum runs over the node's neighbours j on the displacement
 //     relative to the rigid motion of node i, du_j = u_j - t_i - th_i x
 //     (x_j - x_i), dth_j = th_j - th_i; K_mb annihilates rigid motions, so
-//     this equals (K_mb u)_R while the FP32 values multiply O(h) differences.
-//     F32PROD: products and their sum in FP32; otherwise widened to FP64.
-//   K_s: for every incident element e (ascending), lane r forms the partial
+//     this equals (K_mb u)_R while the FP32 values multiply O(h) differences;
+//     products widened to FP64.  Transverse shear does no work on a rigid
+//     motion either, so in the FP32-shear form (NSR = 0) the slices carry it
+//     too.
+//   K_s (NSR > 0): for every incident element e (ascending), lane r forms the partial
 //     sums of s_e = C_e u_e over its own component r; the six lanes of the
 //     node add the partials in a fixed order, and each lane adds
 //     C_e[:, (a, r)]' s_e.  FP64 throughout.
@@ -1718,14 +1723,13 @@ __global__ __launch_bounds__(256) void mixed_drill_kernel(int64_t N, int E, cons
 #ifndef MX_CHUNK
 #define MX_CHUNK 1  // neighbour slots whose loads are issued together (1: highest occupancy, fastest)
 #endif
-template <int NEN, int NSR, bool F32PROD>
+template <int NEN, int NSR>
 __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     MixedOp op, const double *__restrict__ Ucur, double *__restrict__ Unext, double *__restrict__ V,
     double *__restrict__ A, const double *__restrict__ invMC, const double *__restrict__ Cd,
     const double *__restrict__ Fmag, double gstep, double dt, int damped, int forced,
     const int32_t *__restrict__ probe_slot, double *__restrict__ probe_dst, const int32_t *__restrict__ row_group,
     const double *__restrict__ group_dt, const double *__restrict__ gtab) {
-    constexpr int K = 6 * NEN;
     const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t g = gid >> 6;  // slice = wavefront
     const int l = (int)(gid & 63);
@@ -1746,7 +1750,6 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     const int nb0 = op.nbr_ptr[i];
     const int cnt = min(op.nbr_ptr[i + 1] - nb0, nsl);
     double e = 0.0;
-    float ef = 0.0f;
     // neighbour slots in chunks of MX_CHUNK: all loads of a chunk are issued
     // before its arithmetic
     for (int s0 = 0; s0 < cnt; s0 += MX_CHUNK) {
@@ -1779,16 +1782,10 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
             const double d2 = uu[t][1].x - (t2 + (h0 * dy - h1 * dx));
             const double d3 = uu[t][1].y - h0, d4 = uu[t][2].x - h1, d5 = uu[t][2].y - h2;
             const float2 k0 = kk[t][0], k1 = kk[t][1], k2 = kk[t][2];
-            if constexpr (F32PROD) {
-                ef += k0.x * (float)d0 + k0.y * (float)d1 + k1.x * (float)d2 + k1.y * (float)d3 +
-                      k2.x * (float)d4 + k2.y * (float)d5;
-            } else {
-                e += (double)k0.x * d0 + (double)k0.y * d1 + (double)k1.x * d2 + (double)k1.y * d3 +
-                     (double)k2.x * d4 + (double)k2.y * d5;
-            }
+            e += (double)k0.x * d0 + (double)k0.y * d1 + (double)k1.x * d2 + (double)k1.y * d3 +
+                 (double)k2.x * d4 + (double)k2.y * d5;
         }
     }
-    if constexpr (F32PROD) e = (double)ef;
     // drilling (FP64)
     if (r >= 3) {
         const double *d = op.dr + i64 * 6;
@@ -1798,36 +1795,39 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
         const double d2 = rr == 0 ? d[2] : (rr == 1 ? d[4] : d[5]);
         e += d0 * h0 + d1 * h1 + d2 * h2;
     }
-    // transverse shear (FP64), incident elements in ascending order; the six
-    // lanes of the node share the element products through cross-lane reads.
-    // Incidence records: element id and its nodes (ne_pack, stride 4 or 8).
-    constexpr int PS = NEN == 3 ? 4 : 8;
-    for (int k = op.ne_ptr[i]; k < op.ne_ptr[i + 1]; ++k) {
-        const int4 pk = reinterpret_cast<const int4 *>(op.ne_pack + (int64_t)k * PS)[0];
-        int nodes[NEN];
-        nodes[0] = pk.y;
-        nodes[1] = pk.z;
-        nodes[2] = pk.w;
-        if constexpr (NEN == 4) nodes[3] = op.ne_pack[(int64_t)k * PS + 4];
-        const double *C = op.cs + (int64_t)pk.x * NSR * K;
-        double ub[NEN];
-        for (int b2 = 0; b2 < NEN; ++b2) ub[b2] = Ucur[6 * (int64_t)nodes[b2] + r];
-        double part[NSR], ca[NSR];
-        for (int q = 0; q < NSR; ++q) {
-            part[q] = 0.0;
-            ca[q] = 0.0;
-        }
-        for (int b2 = 0; b2 < NEN; ++b2) {
+    if constexpr (NSR > 0) {
+        // transverse shear (FP64), incident elements in ascending order; the six
+        // lanes of the node share the element products through cross-lane reads.
+        // Incidence records: element id and its nodes (ne_pack, stride 4 or 8).
+        constexpr int PS = NEN == 3 ? 4 : 8;
+        constexpr int K = 6 * NEN;
+        for (int k = op.ne_ptr[i]; k < op.ne_ptr[i + 1]; ++k) {
+            const int4 pk = reinterpret_cast<const int4 *>(op.ne_pack + (int64_t)k * PS)[0];
+            int nodes[NEN];
+            nodes[0] = pk.y;
+            nodes[1] = pk.z;
+            nodes[2] = pk.w;
+            if constexpr (NEN == 4) nodes[3] = op.ne_pack[(int64_t)k * PS + 4];
+            const double *C = op.cs + (int64_t)pk.x * NSR * K;
+            double ub[NEN];
+            for (int b2 = 0; b2 < NEN; ++b2) ub[b2] = Ucur[6 * (int64_t)nodes[b2] + r];
+            double part[NSR], ca[NSR];
             for (int q = 0; q < NSR; ++q) {
-                const double cqb = C[q * K + 6 * b2 + r];
-                part[q] += cqb * ub[b2];
-                if (nodes[b2] == i) ca[q] = cqb;  // column (a, r) of the node's own block
+                part[q] = 0.0;
+                ca[q] = 0.0;
             }
-        }
-        for (int q = 0; q < NSR; ++q) {
-            double sq = 0.0;
-            for (int c = 0; c < 6; ++c) sq += __shfl(part[q], base + c, 64);
-            e += ca[q] * sq;
+            for (int b2 = 0; b2 < NEN; ++b2) {
+                for (int q = 0; q < NSR; ++q) {
+                    const double cqb = C[q * K + 6 * b2 + r];
+                    part[q] += cqb * ub[b2];
+                    if (nodes[b2] == i) ca[q] = cqb;  // column (a, r) of the node's own block
+                }
+            }
+            for (int q = 0; q < NSR; ++q) {
+                double sq = 0.0;
+                for (int c = 0; c < 6; ++c) sq += __shfl(part[q], base + c, 64);
+                e += ca[q] * sq;
+            }
         }
     }
     double dtr = dt, gr = gstep;
@@ -1931,7 +1931,7 @@ __global__ __launch_bounds__(256) void scale_kernel(int64_t n, const double *__r
 #define FS_T3_RS 2
 #endif
 
-template <int NEN, int NQP, int NR, int NE, int WG, bool SPLIT, int RS = FS_T3_RS>
+template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS = FS_T3_RS>
 static hipError_t launch_elem(const ElemParams &p, const double *x, const double *y, const double *z,
                               const int32_t *conn, const double *nx, const double *ny, const double *nz,
                               const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
@@ -1959,7 +1959,7 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
 #define FS_Q4_RS 2
 #endif
 
-template <int NE, int WG, bool SPLIT, int RS, bool COMP>
+template <int NE, int WG, int SPLIT, int RS, bool COMP>
 static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
@@ -1996,36 +1996,45 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
                                  const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                  const uint8_t *valid, double *Kb, int *status, hipStream_t st, int variant) {
 #define FS_ARGS p, x, y, z, conn, nx, ny, nz, valid, Kb, status, st
-    const bool split = p.split_cs != nullptr;
-    if (split) {
-        if (p.nen == 3) {
-            if (p.formulation == 1) return launch_elem<3, 1, 12, 16, 128, true>(FS_ARGS);
-            return launch_elem<3, 1, 8, 16, 128, true>(FS_ARGS);
+    // split modes of the mixed-precision operator: 1 shear factor rows and
+    // drilling blocks out, 2 drilling blocks out only
+    if (p.split_dr != nullptr) {
+        if (p.split_cs != nullptr) {
+            if (p.nen == 3) {
+                if (p.formulation == 1) return launch_elem<3, 1, 12, 16, 128, 1>(FS_ARGS);
+                return launch_elem<3, 1, 8, 16, 128, 1>(FS_ARGS);
+            }
+            return p.composite ? launch_q4_seq<16, 192, 1, FS_Q4_RS, true>(FS_ARGS)
+                               : launch_q4_seq<16, 192, 1, FS_Q4_RS, false>(FS_ARGS);
         }
-        return p.composite ? launch_q4_seq<16, 192, true, FS_Q4_RS, true>(FS_ARGS)
-                           : launch_q4_seq<16, 192, true, FS_Q4_RS, false>(FS_ARGS);
+        if (p.nen == 3) {
+            if (p.formulation == 1) return launch_elem<3, 1, 12, 16, 128, 2>(FS_ARGS);
+            return launch_elem<3, 1, 8, 16, 128, 2>(FS_ARGS);
+        }
+        return p.composite ? launch_q4_seq<16, 192, 2, FS_Q4_RS, true>(FS_ARGS)
+                           : launch_q4_seq<16, 192, 2, FS_Q4_RS, false>(FS_ARGS);
     }
     if (p.nen == 3) {
         if (p.formulation == 1) {
             switch (variant) {
-                case 1: return launch_elem<3, 1, 12, 8, 64, false>(FS_ARGS);
-                case 2: return launch_elem<3, 1, 12, 32, 256, false>(FS_ARGS);
-                default: return launch_elem<3, 1, 12, 16, 128, false>(FS_ARGS);
+                case 1: return launch_elem<3, 1, 12, 8, 64, 0>(FS_ARGS);
+                case 2: return launch_elem<3, 1, 12, 32, 256, 0>(FS_ARGS);
+                default: return launch_elem<3, 1, 12, 16, 128, 0>(FS_ARGS);
             }
         }
         switch (variant) {
-            case 1: return launch_elem<3, 1, 8, 8, 64, false>(FS_ARGS);
-            case 2: return launch_elem<3, 1, 8, 32, 256, false>(FS_ARGS);
-            default: return launch_elem<3, 1, 8, 16, 128, false>(FS_ARGS);
+            case 1: return launch_elem<3, 1, 8, 8, 64, 0>(FS_ARGS);
+            case 2: return launch_elem<3, 1, 8, 32, 256, 0>(FS_ARGS);
+            default: return launch_elem<3, 1, 8, 16, 128, 0>(FS_ARGS);
         }
     }
     switch (variant) {
-        case 1: return p.composite ? launch_q4_seq<12, 128, false, FS_Q4_RS, true>(FS_ARGS)
-                                   : launch_q4_seq<12, 128, false, FS_Q4_RS, false>(FS_ARGS);
-        case 2: return p.composite ? launch_q4_seq<6, 64, false, FS_Q4_RS, true>(FS_ARGS)
-                                   : launch_q4_seq<6, 64, false, FS_Q4_RS, false>(FS_ARGS);
-        default: return p.composite ? launch_q4_seq<16, 192, false, FS_Q4_RS, true>(FS_ARGS)
-                                    : launch_q4_seq<16, 192, false, FS_Q4_RS, false>(FS_ARGS);
+        case 1: return p.composite ? launch_q4_seq<12, 128, 0, FS_Q4_RS, true>(FS_ARGS)
+                                   : launch_q4_seq<12, 128, 0, FS_Q4_RS, false>(FS_ARGS);
+        case 2: return p.composite ? launch_q4_seq<6, 64, 0, FS_Q4_RS, true>(FS_ARGS)
+                                   : launch_q4_seq<6, 64, 0, FS_Q4_RS, false>(FS_ARGS);
+        default: return p.composite ? launch_q4_seq<16, 192, 0, FS_Q4_RS, true>(FS_ARGS)
+                                    : launch_q4_seq<16, 192, 0, FS_Q4_RS, false>(FS_ARGS);
     }
 #undef FS_ARGS
 }
@@ -2159,7 +2168,7 @@ hipError_t launch_pack_xyz(int64_t N, const double *x, const double *y, const do
     return hipGetLastError();
 }
 
-hipError_t launch_cd_mixed_step(int nen, int nsr, bool f32prod, const MixedOp &op, const double *Ucur, double *Unext,
+hipError_t launch_cd_mixed_step(int nen, int nsr, const MixedOp &op, const double *Ucur, double *Unext,
                                 double *V, double *A, const double *invMC, const double *Cd, const double *Fmag,
                                 double gstep, double dt, int damped, int forced, const int32_t *probe_slot,
                                 double *probe_dst, const int32_t *row_group, const double *group_dt,
@@ -2167,19 +2176,17 @@ hipError_t launch_cd_mixed_step(int nen, int nsr, bool f32prod, const MixedOp &o
     if (op.nnodes == 0) return hipSuccess;
     const int64_t threads = (int64_t)((op.nnodes + 9) / 10) * 64;  // one wavefront per 10-node slice
     const unsigned blocks = (unsigned)((threads + 255) / 256);
-#define FS_MX_CASE(NEN_, NSR_, F32_)                                                                             \
-    if (nen == NEN_ && nsr == NSR_ && f32prod == F32_) {                                                        \
-        hipLaunchKernelGGL((cd_mixed_step_kernel<NEN_, NSR_, F32_>), dim3(blocks), dim3(256), 0, st, op, Ucur,   \
+#define FS_MX_CASE(NEN_, NSR_)                                                                                  \
+    if ((nen == NEN_ || NSR_ == 0) && nsr == NSR_) {                                                            \
+        hipLaunchKernelGGL((cd_mixed_step_kernel<NEN_, NSR_>), dim3(blocks), dim3(256), 0, st, op, Ucur,         \
                            Unext, V, A, invMC, Cd, Fmag, gstep, dt, damped, forced, probe_slot, probe_dst,       \
                            row_group, group_dt, gtab);                                                          \
         return hipGetLastError();                                                                               \
     }
-    FS_MX_CASE(3, 2, false)
-    FS_MX_CASE(3, 2, true)
-    FS_MX_CASE(3, 6, false)
-    FS_MX_CASE(3, 6, true)
-    FS_MX_CASE(4, 8, false)
-    FS_MX_CASE(4, 8, true)
+    FS_MX_CASE(3, 0)  // FP32 shear in the slices (either family)
+    FS_MX_CASE(3, 2)
+    FS_MX_CASE(3, 6)
+    FS_MX_CASE(4, 8)
 #undef FS_MX_CASE
     return hipErrorInvalidValue;
 }

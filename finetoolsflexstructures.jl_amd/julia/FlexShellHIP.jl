@@ -46,6 +46,7 @@ const FS_BLOCK_ALL = Cint(0)
 const FS_BLOCK_FF = Cint(1)
 const FS_PREC_FP64 = Int32(0)
 const FS_PREC_MIXED = Int32(1)
+const FS_PREC_MIXED_F32 = Int32(2)
 
 # ---------------------------------------------------------------- ABI structs
 # Field order and types follow include/flexshell.h exactly.
@@ -308,7 +309,9 @@ end
 left resident by `stiffness(femm, SysmatAssemblerHIPCSR(; block = :ff), ...)`
 and `mass(femm, SysmatAssemblerHIPDiag(; block = :ff), ...)` (same assembler
 context).  The force closure is `F(t) = g(t) * Fmag`; `peek` becomes the probe
-dofs (1-based) recorded every `stride` steps.  Returns (U, V, probe history).
+dofs (1-based) recorded every `stride` steps.  `mixed = true` runs the step in
+FS_PREC_MIXED (FP32 membrane + bending, FP64 shear), `mixed = :f32` in
+FS_PREC_MIXED_F32 (shear in FP32 too).  Returns (U, V, probe history).
 """
 function cd_loop!(femm, assembler, cdamp, U0, V0, tend, dt, Fmag, g; probes = Int[], stride = 1, mixed = false)
     m = assembler.models[femm]
@@ -327,7 +330,8 @@ function cd_loop!(femm, assembler, cdamp, U0, V0, tend, dt, Fmag, g; probes = In
     U = similar(U0); V = similar(V0)
     pr = Int64.(probes .- 1)
     pout = zeros(length(pr), nsteps ÷ stride + 1)
-    p = Ref(CDParams(dt, cdamp, nsteps, mixed ? FS_PREC_MIXED : FS_PREC_FP64, stride, 0, 1, C_NULL, C_NULL, C_NULL))
+    prec = mixed === :f32 ? FS_PREC_MIXED_F32 : (mixed === true ? FS_PREC_MIXED : FS_PREC_FP64)
+    p = Ref(CDParams(dt, cdamp, nsteps, prec, stride, 0, 1, C_NULL, C_NULL, C_NULL))
     check(m.ctx, ccall((:fs_model_explicit, libflexshell), Cint,
                        (Ptr{Cvoid}, Ptr{CDParams}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
                         Ptr{Int64}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),

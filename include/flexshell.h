@@ -55,8 +55,12 @@ extern "C" {
 #define FS_PREC_MIXED 1 /* membrane+bending stiffness held in FP32 (products
                            widened to and summed in FP64); state, mass, shear
                            and drilling stabilisation terms in FP64 */
-#define FS_PREC_MIXED_F32 2 /* as FS_PREC_MIXED with the membrane+bending
-                               products and their row sum in FP32 */
+#define FS_PREC_MIXED_F32 2 /* as FS_PREC_MIXED with the transverse-shear
+                               stiffness also held in FP32 (in the membrane +
+                               bending slices): fewer bytes per step; its
+                               rounding grows with the element size over the
+                               thickness, so it suits meshes with elements of
+                               a few thicknesses or less */
 
 typedef struct fs_ctx fs_ctx;
 typedef struct fs_model fs_model;
@@ -278,10 +282,11 @@ int fs_ctx_set_pipeline(fs_ctx *ctx, int ranges);
 
 /* Storage of the mixed-precision operator built by the last FS_PREC_MIXED /
  * FS_PREC_MIXED_F32 loop (no counterpart in the reference, whose loop runs on
- * the FP64 CSR matrix only): info[0] FP32 value pairs of the membrane +
- * bending slices, info[1] shear factor rows per element, info[2] entries of
- * the node neighbour lists, info[3] 64-row slices.  Zeros before the first
- * mixed loop. */
+ * the FP64 CSR matrix only): info[0] FP32 value pairs of the slices
+ * (membrane + bending, plus shear for FS_PREC_MIXED_F32), info[1] FP64 shear
+ * factor rows per element (0 for FS_PREC_MIXED_F32), info[2] entries of the
+ * node neighbour lists, info[3] 64-row slices.  Zeros before the first mixed
+ * loop. */
 int fs_model_mixed_info(const fs_model *m, int64_t *info);
 
 #ifdef __cplusplus

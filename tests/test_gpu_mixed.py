@@ -1,5 +1,6 @@
-"""Mixed-precision explicit path: FP32 membrane + bending stiffness, FP64
-transverse shear, drilling, mass and state.  The operator agrees with the FP64
+"""Mixed-precision explicit paths: FP32 membrane + bending stiffness with FP64
+transverse shear (FS_PREC_MIXED) or FP32 shear too (FS_PREC_MIXED_F32), and
+FP64 drilling, mass and state.  The operator agrees with the FP64
 stiffness to FP32 rounding, trajectories stay within 1e-6 of the FP64 oracle,
 runs are bitwise reproducible, and a batch gives each model bitwise the same
 trajectory as running it alone."""
@@ -111,6 +112,9 @@ def test_mixed_trajectory_within_1e6_of_oracle(ctx):
         out[prec] = err
         print(f"[gpu] precision {prec}: max rel trajectory error {err:.3e}", flush=True)
     assert out[MIXED] <= 1e-6
+    # FP32 shear on a coarse thin mesh (element size ~16 thicknesses): the
+    # least favourable case for it, still within the same bar
+    assert out[MIXED32] <= 1e-6
     r1 = explicit.run(f._model, 0.0, z, z, dt, 200, F, g[:201], precision=MIXED)
     r2 = explicit.run(f._model, 0.0, z, z, dt, 200, F, g[:201], precision=MIXED)
     assert np.array_equal(r1["U"], r2["U"]) and np.array_equal(r1["V"], r2["V"])
