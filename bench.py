@@ -31,10 +31,11 @@ def assembly_bytes(E, N, nen, nnz):
     return E * (4 * nen + 4 * k * k) + N * (24 + 24 + 1) + nnz * 8
 
 
-def cd_step_bytes(n, nnz):
-    """Algorithmic bytes of one fused FP64 CD step (int32 column indices):
-    K values + indices, row pointers, u gather, U V A invM Fmag reads, U V A writes."""
-    return nnz * 12 + (n + 1) * 4 + n * 72
+def cd_step_bytes(n, nnz, nnb):
+    """Algorithmic bytes of one fused FP64 CD step on the run records: K values,
+    one 16-byte record per row and one 8-byte record per neighbour slot (nnb),
+    u gathered once, U V A invM Fmag reads, U V A writes."""
+    return nnz * 8 + n * 16 + nnb * 8 + n * 72
 
 
 def cpu_baseline(case, nsample, K_host, Mdiag, dt, cd_steps_cpu):
@@ -104,26 +105,29 @@ def bench_batch(nmodels, nsteps):
     E = int(sum(m["conn"].shape[0] for m in models))
     r = bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g, nsteps)
     same = all(np.array_equal(r["U"][bt.model_slice(k)], U_plain[k]) for k in keep)
-    bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g[:11], 10,
-                precision=_lib.FS_PREC_MIXED)  # builds the mixed operator
-    rm = bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g, nsteps,
-                     precision=_lib.FS_PREC_MIXED)
-    errs = [float(np.abs(rm["U"][bt.model_slice(k)] - r["U"][bt.model_slice(k)]).max()
-                  / np.abs(r["U"][bt.model_slice(k)]).max()) for k in range(nm)]
-    bt.close()
     plain = nfree * nsteps / (plain_cd_ms / 1e3)
     batched = nfree * nsteps / (r["ms"] / 1e3)
-    batched_mixed = nfree * nsteps / (rm["ms"] / 1e3)
-    return {"models": nm, "elements": E, "free_dofs": nfree, "steps": nsteps,
-            "plain_dof_steps_per_s": plain, "batched_dof_steps_per_s": batched,
-            "batched_over_plain": batched / plain,
-            "batched_mixed_dof_steps_per_s": batched_mixed, "batched_mixed_over_plain": batched_mixed / plain,
-            "batched_mixed_max_rel_err_final_U": max(errs),
+    out = {"models": nm, "elements": E, "free_dofs": nfree, "steps": nsteps,
+           "plain_dof_steps_per_s": plain, "batched_dof_steps_per_s": batched,
+           "batched_over_plain": batched / plain}
+    for name, prec in (("batched_mixed", _lib.FS_PREC_MIXED), ("batched_mixed_f32", _lib.FS_PREC_MIXED_F32)):
+        bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g[:11], 10,
+                    precision=prec)  # builds the mixed operator
+        rm = bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g, nsteps,
+                         precision=prec)
+        errs = [float(np.abs(rm["U"][bt.model_slice(k)] - r["U"][bt.model_slice(k)]).max()
+                      / np.abs(r["U"][bt.model_slice(k)]).max()) for k in range(nm)]
+        rate = nfree * nsteps / (rm["ms"] / 1e3)
+        out.update({f"{name}_dof_steps_per_s": rate, f"{name}_over_plain": rate / plain,
+                    f"{name}_over_batched": rate / batched, f"{name}_max_rel_err_final_U": max(errs)})
+    bt.close()
+    return {**out,
             "plain_elements_per_s": E / (plain_asm_ms / 1e3), "batched_elements_per_s": E / (batch_asm_ms / 1e3),
             "bitwise_equal_to_plain": bool(same),
             "note": "plain = one model per launch sequence (FP64, no graph); batched = all models per launch, "
                     "hipGraph step loop, FP64; batched_mixed = the same with FP32 membrane+bending stiffness "
-                    "(FP64 shear, drilling, mass, state); error = max over models of max|dU| / max|U_fp64|"}
+                    "(FP64 shear, drilling, mass, state); batched_mixed_f32 = shear in FP32 too; "
+                    "error = max over models of max|dU| / max|U_fp64| (white-noise initial velocities)"}
 
 
 def mixed_step_bytes(N, E, nen, kmb_pairs, nsr, nnb, fmag_frac):
@@ -374,7 +378,7 @@ def main():
     assert np.all(np.isfinite(r["U"])), "non-finite state in the explicit loop"
 
     b_asm = assembly_bytes(E, N, 4, nnz)
-    b_cd = cd_step_bytes(nf, nnz)
+    b_cd = cd_step_bytes(nf, nnz, model.mixed_info()[2])
     ach = b_asm / (dev_ms / 1e3) / 1e9
     ach_cd = b_cd / (cd_ms / ncd / 1e3) / 1e9
     # HBM bytes per assembly from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
