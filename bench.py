@@ -226,7 +226,7 @@ def bench_serve(nreq, rate, slots, plain=True, steps=500, plain_requests=300):
         return [S.Request(s["rid"], s["kind"], m, steps=s["steps"], arrival=s["arrival"], seed=s["rid"])
                 for s, m in zip(specs, models)]
 
-    out = {"requests": nreq, "rate_req_s": rate, "explicit_steps": steps, "explicit_precision": "mixed",
+    out = {"requests": nreq, "rate_req_s": rate, "explicit_steps": steps, "explicit_precision": "fp64",
            "elements_total": int(sum(m["conn"].shape[0] for m in models)),
            "kinds": {k: sum(s["kind"] == k for s in specs) for k in ("assembly", "modal", "explicit")}}
     runs = [("served", slots, nreq)] + ([("plain", 1, min(plain_requests, nreq))] if plain else [])

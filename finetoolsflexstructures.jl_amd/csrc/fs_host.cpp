@@ -16,6 +16,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/flexshell.h"
@@ -30,7 +31,8 @@ void element_variant_shape(int nen, int variant, int *ne, int *wg);
 hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals, hipStream_t st);
 hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const double *y, const double *z,
                           const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
-                          double *nx, double *ny, double *nz, uint8_t *valid, hipStream_t st);
+                          const double *gnrm, const int32_t *egroup, double *nx, double *ny, double *nz,
+                          uint8_t *valid, hipStream_t st);
 hipError_t launch_mass(int nen, int64_t N, int E, int composite, const double *g_rho_t, const double *g_inertia,
                        const int32_t *elem_group, const double *x, const double *y, const double *z,
                        const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, const int64_t *node_dof,
@@ -205,6 +207,21 @@ struct fs_model {
 };
 
 namespace {
+
+// Runs fn(begin, end) over [0, n) in up to hardware_concurrency() chunks on
+// threads of their own; for host loops whose iterations write disjoint data.
+template <class F>
+void parallel_chunks(int64_t n, F fn) {
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), n / 65536));
+    if (nt <= 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int64_t t = 0; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+    for (auto &x : th) x.join();
+}
 
 int fail(fs_ctx *c, int code, const char *fmt, ...) {
     char buf[512];
@@ -392,9 +409,10 @@ int build_block(fs_model *m, int which) {
     } catch (const std::bad_alloc &) {
         return fail(c, FS_ERR_NOMEM, "host pattern of %lld entries", (long long)nnz);
     }
-    {
+    // column lists: every node writes only its own rows
+    parallel_chunks(N, [&](int64_t I0, int64_t I1) {
         std::vector<int32_t> cols;
-        for (int64_t I = 0; I < N; ++I) {
+        for (int64_t I = I0; I < I1; ++I) {
             cols.assign(rowlen[I], 0);
             for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
                 const int J = m->nbr_list[k];
@@ -411,7 +429,7 @@ int build_block(fs_model *m, int which) {
                 std::copy(cols.begin(), cols.end(), B.colval.begin() + B.rowptr[d]);
             }
         }
-    }
+    });
     if (nnz >= (int64_t(1) << 31))
         return fail(c, FS_ERR_UNSUPPORTED, "block with %lld stored entries exceeds 32-bit positions", (long long)nnz);
     // upper node pairs: the neighbours J >= I of each node, in order
@@ -1011,17 +1029,16 @@ int fs_model_set_dofs(fs_model *m, const int64_t *dofnums, int64_t nalldofs, int
     return FS_OK;
 }
 
-int fs_model_normals_isoparametric(fs_model *m, double threshold_angle_deg, double *normals_out, uint8_t *valid_out) {
-    if (!m) return FS_ERR_ARG;
+// Normals kernel in either form (gnrm null: isoparametric) and the copies out.
+static int normals_common(fs_model *m, double threshold_angle_deg, const double *gnrm, const int32_t *egroup,
+                          double *normals_out, uint8_t *valid_out) {
     fs_ctx *c = m->ctx;
-    c->err.clear();
-    FS_HIP(c, hipSetDevice(c->device));
     const double s = std::sin(threshold_angle_deg / 180 * M_PI);
     const double ntol = 1 - std::sqrt(1 - s * s);  // FEMMShellT3FFModule.jl:600
     FS_HIP(c, fs::launch_normals(m->nen, m->N, (int)m->E, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
                                  m->dconn.as<int32_t>(), m->dne_ptr.as<int32_t>(), m->dne_list.as<int32_t>(), ntol,
-                                 m->nx.as<double>(), m->ny.as<double>(), m->nz.as<double>(), m->valid.as<uint8_t>(),
-                                 c->stream));
+                                 gnrm, egroup, m->nx.as<double>(), m->ny.as<double>(), m->nz.as<double>(),
+                                 m->valid.as<uint8_t>(), c->stream));
     if (normals_out) {
         FS_HIP(c, hipMemcpyAsync(normals_out, m->nx.p, m->N * 8, hipMemcpyDeviceToHost, c->stream));
         FS_HIP(c, hipMemcpyAsync(normals_out + m->N, m->ny.p, m->N * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1031,6 +1048,33 @@ int fs_model_normals_isoparametric(fs_model *m, double threshold_angle_deg, doub
     FS_HIP(c, hipStreamSynchronize(c->stream));
     m->normals_ready = true;
     return FS_OK;
+}
+
+int fs_model_normals_isoparametric(fs_model *m, double threshold_angle_deg, double *normals_out, uint8_t *valid_out) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    FS_HIP(c, hipSetDevice(c->device));
+    return normals_common(m, threshold_angle_deg, nullptr, nullptr, normals_out, valid_out);
+}
+
+int fs_model_normals_layered(fs_model *m, double threshold_angle_deg, int32_t ngroups, const double *group_normal,
+                             const int32_t *elem_group, double *normals_out, uint8_t *valid_out) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (ngroups < 1 || !group_normal) return fail(c, FS_ERR_ARG, "normals: layered form needs ngroups >= 1 normals");
+    if (elem_group)
+        for (int64_t e = 0; e < m->E; ++e)
+            if (elem_group[e] < 0 || elem_group[e] >= ngroups)
+                return fail(c, FS_ERR_ARG, "normals: element %lld has group %d", (long long)e, elem_group[e]);
+    FS_HIP(c, hipSetDevice(c->device));
+    DevBuf dg, de;
+    int s;
+    if ((s = upload(c, dg, group_normal, (size_t)ngroups * 3))) return s;
+    if (elem_group && (s = upload(c, de, elem_group, (size_t)m->E))) return s;
+    return normals_common(m, threshold_angle_deg, dg.as<double>(), elem_group ? de.as<int32_t>() : nullptr,
+                          normals_out, valid_out);
 }
 
 int fs_model_normals_upload(fs_model *m, const double *normals, const uint8_t *valid) {
@@ -1069,7 +1113,9 @@ int fs_model_pattern_get(fs_model *m, int block, int64_t *rowptr, int64_t *colva
     if (rowptr)
         for (int64_t i = 0; i <= B.nrows; ++i) rowptr[i] = B.rowptr[i] + index_base;
     if (colval)
-        for (int64_t i = 0; i < B.nnz; ++i) colval[i] = (int64_t)B.colval[i] + index_base;
+        parallel_chunks(B.nnz, [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i) colval[i] = (int64_t)B.colval[i] + index_base;
+        });
     return FS_OK;
 }
 

@@ -1314,6 +1314,11 @@ __global__ __launch_bounds__(256) void scatter_pair_kernel(ScatterPlan sp, const
 // (node_elem lists), reproducing the element-order accumulation of
 // FEMMShellT3FFModule.jl:579-587 / FEMMShellQ4RSModule.jl:487-495, then the
 // normalisation and the validity pass (:597-611 / :508-520).
+// Layered form (gnrm set: the constant layup csys normal of each element
+// group, FEMMShellT3FFCompModule.jl:500-521 / FEMMShellQ4RSCompModule.jl:
+// 462-503): the sum runs over the groups' csys normals (Q4: weighted by the
+// corner Jacobian), and validity is checked against the element normal (T3)
+// or the csys normal (Q4).
 template <int NEN>
 __global__ __launch_bounds__(256) void normals_kernel(int64_t N, int E, const double *__restrict__ xs,
                                                       const double *__restrict__ ys,
@@ -1321,6 +1326,8 @@ __global__ __launch_bounds__(256) void normals_kernel(int64_t N, int E, const do
                                                       const int32_t *__restrict__ conn,
                                                       const int32_t *__restrict__ ne_ptr,
                                                       const int32_t *__restrict__ ne_list, double ntol,
+                                                      const double *__restrict__ gnrm,
+                                                      const int32_t *__restrict__ egroup,
                                                       double *__restrict__ nx, double *__restrict__ ny,
                                                       double *__restrict__ nz, uint8_t *__restrict__ valid) {
     const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1364,18 +1371,28 @@ __global__ __launch_bounds__(256) void normals_kernel(int64_t N, int E, const do
                 w = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
             }
             e_g(EG, J);
+            // direction summed and checked: the element normal, or the
+            // group's csys normal in the layered form
+            double dir[3] = {EG[0][2], EG[1][2], EG[2][2]};
+            if (gnrm) {
+                const double *gn = gnrm + 3 * (egroup ? egroup[e] : 0);
+                for (int j = 0; j < 3; ++j) dir[j] = gn[j];
+            }
             if (pass == 0) {
                 if (NEN == 3) {
-                    acc[0] += EG[0][2];
-                    acc[1] += EG[1][2];
-                    acc[2] += EG[2][2];
+                    acc[0] += dir[0];
+                    acc[1] += dir[1];
+                    acc[2] += dir[2];
                 } else {
-                    acc[0] += w * EG[0][2];
-                    acc[1] += w * EG[1][2];
-                    acc[2] += w * EG[2][2];
+                    acc[0] += w * dir[0];
+                    acc[1] += w * dir[1];
+                    acc[2] += w * dir[2];
                 }
             } else {
-                const double nd = nrm[0] * EG[0][2] + nrm[1] * EG[1][2] + nrm[2] * EG[2][2];
+                // checked against the element normal, except layered Q4: the csys normal
+                const bool vs_elem = !gnrm || NEN == 3;
+                const double nd = vs_elem ? nrm[0] * EG[0][2] + nrm[1] * EG[1][2] + nrm[2] * EG[2][2]
+                                          : nrm[0] * dir[0] + nrm[1] * dir[1] + nrm[2] * dir[2];
                 if (nd < 1 - ntol) ok = 0;
             }
         }
@@ -2049,15 +2066,16 @@ hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals,
 
 hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const double *y, const double *z,
                           const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
-                          double *nx, double *ny, double *nz, uint8_t *valid, hipStream_t st) {
+                          const double *gnrm, const int32_t *egroup, double *nx, double *ny, double *nz,
+                          uint8_t *valid, hipStream_t st) {
     if (N == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((N + 255) / 256);
     if (nen == 3)
         hipLaunchKernelGGL(normals_kernel<3>, dim3(blocks), dim3(256), 0, st, N, E, x, y, z, conn, ne_ptr, ne_list,
-                           ntol, nx, ny, nz, valid);
+                           ntol, gnrm, egroup, nx, ny, nz, valid);
     else
         hipLaunchKernelGGL(normals_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, x, y, z, conn, ne_ptr, ne_list,
-                           ntol, nx, ny, nz, valid);
+                           ntol, gnrm, egroup, nx, ny, nz, valid);
     return hipGetLastError();
 }
 

@@ -95,6 +95,7 @@ SIGNATURES = {
     "fs_model_destroy": (None, [C.c_void_p]),
     "fs_model_set_dofs": (C.c_int, [C.c_void_p, _i64p, C.c_int64, C.c_int64, C.c_int]),
     "fs_model_normals_isoparametric": (C.c_int, [C.c_void_p, C.c_double, _dp, _u8p]),
+    "fs_model_normals_layered": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _dp, _i32p, _dp, _u8p]),
     "fs_model_normals_upload": (C.c_int, [C.c_void_p, _dp, _u8p]),
     "fs_model_pattern": (C.c_int, [C.c_void_p, C.c_int, _i64p, _i64p]),
     "fs_model_pattern_get": (C.c_int, [C.c_void_p, C.c_int, _i64p, _i64p, C.c_int]),
@@ -234,6 +235,18 @@ class Model:
         self.ctx.check(self.ctx.lib.fs_model_normals_isoparametric(self.h, float(threshold_deg), ptr(out),
                                                                     ptr(valid, C.c_uint8)),
                        "fs_model_normals_isoparametric")
+        return out.T.copy(), valid.astype(bool)
+
+    def normals_layered(self, threshold_deg, group_normals, elem_group=None):
+        """Normals of a layered shell from the per-group layup csys normals
+        (group_normals: ngroups x 3; elem_group: nelem ints or None)."""
+        gn = np.ascontiguousarray(np.asarray(group_normals, dtype=np.float64).reshape(-1, 3))
+        eg = None if elem_group is None else np.ascontiguousarray(np.asarray(elem_group, dtype=np.int32))
+        out = np.zeros((3, self.N))
+        valid = np.zeros(self.N, dtype=np.uint8)
+        self.ctx.check(self.ctx.lib.fs_model_normals_layered(self.h, float(threshold_deg), int(gn.shape[0]), ptr(gn),
+                                                              ptr(eg, C.c_int32), ptr(out), ptr(valid, C.c_uint8)),
+                       "fs_model_normals_layered")
         return out.T.copy(), valid.astype(bool)
 
     def normals_upload(self, normals, valid):

@@ -324,56 +324,20 @@ class _CompositeMixin:
         if (self._group < 0).any():
             raise ValueError("every element must belong to a layup group")
 
+    def associategeometry(self, geom0):
+        """Nodal normals of the layered FEMMs (FEMMShellT3FFCompModule.jl:500-521,
+        FEMMShellQ4RSCompModule.jl:462-503) on the device: the layup csys is
+        constant per group, so its normal crosses the ABI per group."""
+        m = self._ensure_model(geom0)
+        gn = np.array([lg[0].csys[:, 2] for lg in self.layup_groups], dtype=float)
+        self._normals, self._normal_valid = m.normals_layered(self.threshold_angle, gn, self._group)
+        self._associatedgeometry = True
+        return self
+
     def _csys_normal_function(self):
         groups = self._group
         lay = [lg[0] for lg in self.layup_groups]
         return lambda X, J, e, q: lay[groups[e]].csys[:, 2]
-
-    def _host_csys_normals(self, xyz, normal_of):
-        """Constant layup csys per group: the nodal normal is the (normalized)
-        sum of the csys normals of the incident elements' groups
-        (FEMMShellT3FFCompModule.jl:500-521, FEMMShellQ4RSCompModule.jl:462-503),
-        validity against the element normal (T3) or the csys normal itself (Q4)."""
-        conn = self.conn
-        N = xyz.shape[0]
-        n3 = np.array([lg[0].csys[:, 2] for lg in self.layup_groups], dtype=float)[self._group]  # E x 3
-        # per (element, node) weights in element-major order, summed per node in
-        # that order (the element loop of the reference)
-        if self.nen == 3:
-            w = np.ones(conn.shape)
-        else:
-            X = xyz[conn]
-            npc = [(-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)]
-            w = np.empty(conn.shape)
-            for a in range(4):
-                xi, eta = npc[a]
-                gN = 0.25 * np.array([[-(1 - eta), -(1 - xi)], [(1 - eta), -(1 + xi)], [(1 + eta), (1 + xi)],
-                                      [-(1 + eta), (1 - xi)]])
-                J0 = gN[0, 0] * X[:, 0] + gN[1, 0] * X[:, 1] + gN[2, 0] * X[:, 2] + gN[3, 0] * X[:, 3]
-                J1 = gN[0, 1] * X[:, 0] + gN[1, 1] * X[:, 1] + gN[2, 1] * X[:, 2] + gN[3, 1] * X[:, 3]
-                w[:, a] = np.linalg.norm(np.cross(J0, J1), axis=1)
-        idx = conn.ravel()
-        nrm = np.stack([np.bincount(idx, weights=(w * n3[:, k:k + 1]).ravel(), minlength=N) for k in range(3)],
-                       axis=1)
-        nn = np.linalg.norm(nrm, axis=1)
-        nrm[nn > 0] /= nn[nn > 0, None]
-        valid = np.ones(N, dtype=bool)
-        s = np.sin(self.threshold_angle / 180 * np.pi)
-        ntol = 1 - np.sqrt(1 - s * s)
-        if self.nen == 3:
-            X = xyz[conn]
-            e1 = X[:, 1] - X[:, 0]
-            e1 /= np.linalg.norm(e1, axis=1)[:, None]
-            e3 = np.cross(e1, X[:, 2] - X[:, 0])
-            e3 /= np.linalg.norm(e3, axis=1)[:, None]
-            ref = e3
-        else:
-            ref = n3
-        for a in range(self.nen):
-            na = nrm[conn[:, a]]
-            bad = (na[:, 0] * ref[:, 0] + na[:, 1] * ref[:, 1] + na[:, 2] * ref[:, 2]) < 1 - ntol
-            valid[conn[bad, a]] = False
-        return nrm, valid
 
     def _shell_params(self):
         p = _lib.ShellParams()

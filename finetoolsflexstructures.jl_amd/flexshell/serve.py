@@ -51,7 +51,7 @@ class Request:
     explicit jobs also take V0 (free-dof velocities) and `steps`."""
 
     def __init__(self, rid, kind, model, steps=0, arrival=0.0, omega_iters=30, seed=0,
-                 precision=_lib.FS_PREC_MIXED):
+                 precision=_lib.FS_PREC_FP64):
         if kind not in (ASSEMBLY, MODAL, EXPLICIT):
             raise ValueError(f"unknown request kind {kind!r}")
         self.rid = rid

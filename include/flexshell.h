@@ -184,6 +184,18 @@ int fs_model_set_dofs(fs_model *m, const int64_t *dofnums, int64_t nalldofs, int
 int fs_model_normals_isoparametric(fs_model *m, double threshold_angle_deg, double *normals_out,
                                    uint8_t *valid_out);
 
+/* associategeometry! of the layered FEMMs (FEMMShellT3FFCompModule.jl:500-521,
+ * FEMMShellQ4RSCompModule.jl:462-503), whose layup coordinate systems are
+ * constant per layup group: the nodal normal is the normalised sum, over the
+ * incident elements in element order, of the element group's csys normal
+ * (weighted by the corner Jacobian for Q4); a node is invalid where that sum
+ * departs from the element normal (T3) or from the csys normal (Q4) by more
+ * than the threshold.  group_normal: ngroups x 3 row-major (third column of
+ * each group's csys); elem_group: nelem group indices, or NULL (all group 0).
+ * Outputs as for fs_model_normals_isoparametric. */
+int fs_model_normals_layered(fs_model *m, double threshold_angle_deg, int32_t ngroups, const double *group_normal,
+                             const int32_t *elem_group, double *normals_out, uint8_t *valid_out);
+
 /*
  * Upload normals computed on the host: the mcsys / layup.csys paths of
  * associategeometry! (FEMMShellT3FFModule.jl:584 with a user CSys,

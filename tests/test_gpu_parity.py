@@ -132,6 +132,36 @@ def test_layered_barbero(ctx, nen):
     assert abs(wmax - ref) / ref < 1e-10
 
 
+@pytest.mark.parametrize("nen", [3, 4])
+def test_layered_normals_device(ctx, nen):
+    """Layered normals on the device (fs_model_normals_layered): one layup
+    against the oracle's constant-csys normals, and two layups with a tilted
+    csys (so that the sum and the validity both vary) against the per-element
+    loop of associategeometry! on the host (generic csys closure path)."""
+    print(f"[gpu] layered normals nen={nen}", flush=True)
+    from flexshell import workloads as W
+    xyz, conn = SC.distorted_mesh(nen, 6, 5, seed=11, warp=0.6)
+    cls = FM.FEMMShellT3FFComp if nen == 3 else FM.FEMMShellQ4RSComp
+    lay_a = W.c5_layup(3, [0.0, 45.0, 90.0])
+    f = cls(conn, lay_a, ctx=ctx)
+    f.associategeometry(xyz)
+    nrm, val = O.csys_normals(nen, xyz, conn, lay_a.csys)
+    assert np.allclose(f._normals, nrm, rtol=0, atol=1e-14)
+    assert np.array_equal(f._normal_valid, val)
+    lay_b = W.c5_layup(4, [0.0, -45.0, 45.0, 90.0])
+    c, s = np.cos(0.9), np.sin(0.9)
+    lay_b.csys = np.array([[1, 0, 0], [0, c, -s], [0, s, c]], dtype=float)
+    E = conn.shape[0]
+    f2 = cls(conn, None, layup_groups=[(lay_a, np.arange(0, E, 2)), (lay_b, np.arange(1, E, 2))], ctx=ctx)
+    f2.associategeometry(xyz)
+    n_loop, v_loop = FM._ShellFEMM._host_csys_normals(f2, xyz, f2._csys_normal_function())
+    assert np.allclose(f2._normals, n_loop, rtol=0, atol=1e-14)
+    assert np.array_equal(f2._normal_valid, v_loop)
+    assert (~v_loop).any() and v_loop.any()
+    f._model.close()
+    f2._model.close()
+
+
 def test_fv12_frequencies_gpu(ctx):
     print("[gpu] fv12", flush=True)
     c = SC.fv12()
