@@ -319,10 +319,13 @@ struct ElemTraits {
 template <int NEN, int NQP, int NR, int NE>
 struct ElemSmem {
     using T = ElemTraits<NEN, NQP, NR>;
-    static constexpr int OFF_C = 0;
-    static constexpr int SZ_C = NE * (T::CSTR > T::KSTAGE ? T::CSTR : T::KSTAGE);
-    static constexpr int OFF_G = SZ_C;
+    // the output stage (NE x KSTAGE, after the last read of C and of the
+    // geometry) spans the C and G regions, so C is sized for C alone where
+    // the two together hold the stage
     static constexpr int SZ_G = NE * NQP * T::GEO;
+    static constexpr int OFF_C = 0;
+    static constexpr int SZ_C = NE * T::CSTR + SZ_G >= NE * T::KSTAGE ? NE * T::CSTR : NE * T::KSTAGE - SZ_G;
+    static constexpr int OFF_G = SZ_C;
     static constexpr int OFF_N = OFF_G + SZ_G;
     static constexpr int SZ_N = NE * NQP * NEN * T::NODE;
     static constexpr int OFF_M = OFF_N + SZ_N;
@@ -811,7 +814,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void el
             p.split_cs[(size_t)(e0 + le) * NSR * T::K + rem] = sC[le * T::CSTR + srow * T::K + col];
         }
     }
-    __syncthreads();  // every read of sC is done: reuse it as the output stage
+    __syncthreads();  // every read of sC and sG is done: reuse them as the output stage
     if (cact) {
         double *ks = sC + (le3 * T::NBLK + blk) * 36;
         for (int r = 0; r < 6; ++r)
