@@ -851,15 +851,21 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                                                          const uint8_t *__restrict__ valid,
                                                          double *__restrict__ Kb, int *__restrict__ status) {
     constexpr int NEN = 4, NQP = 4, NR = 8, K = 24, NBLK = 10;
-    constexpr int GEO = 47;             // EG 9 | gN 8 | s_m s_b s_s m n | bs 24 (+1 pad)
-    constexpr int G_EG = 0, G_GN = 9, G_SC = 17, G_BS = 22;
+    // geometry record per (element, Gauss point): EG 9 | gN 8 | s_m s_b s_s
+    // (layered: s_mb 0 s_s m n) | bs 24 (layered: +1 pad)
+    constexpr int NSC = COMP ? 5 : 3;
+    constexpr int GEO = 17 + NSC + 24 + (COMP ? 1 : 0);
+    constexpr int G_EG = 0, G_GN = 9, G_SC = 17, G_BS = 17 + NSC;
     constexpr int CSTR = NR * K + 2;    // one Gauss point of C, padded
     constexpr int NODE = 6 + NR;
     constexpr int OFF_G = 0;
     constexpr int OFF_C = OFF_G + NE * NQP * GEO;
     constexpr int OFF_N = OFF_C + NE * CSTR;
-    constexpr int OFF_M = OFF_N + NE * NEN * NODE;
-    constexpr int MISC = 4 * 9 + 4 + 1;  // rotation blocks K_rr of the 4 nodes | trace terms
+    // after the Gauss-point loop the node records are dead: the drilling
+    // scratch (rotation blocks K_rr of the 4 nodes | trace terms) takes their place
+    constexpr int OFF_M = OFF_N;
+    constexpr int MISC = 4 * 9 + 4 + 1;
+    static_assert(MISC <= NEN * NODE, "drilling scratch must fit in the node records");
     static_assert(NE * NQP * GEO + NE * CSTR >= NE * NBLK * 36, "output stage must fit in the G+C region");
     extern __shared__ double smem[];
     double *sG = smem + OFF_G;
@@ -944,8 +950,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 g[G_SC + 0] = sqrt(tt * Jac);
                 g[G_SC + 1] = sqrt((tt * tt * tt / 12.0) * Jac);
                 g[G_SC + 2] = sqrt(tt * stab * Jac);
-                g[G_SC + 3] = 0.0;
-                g[G_SC + 4] = 0.0;
             } else {
                 double m, n;
                 layup_angle(EG, p.g_csys + 9 * grp, m, n);
@@ -1061,8 +1065,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 nk_e[2] = 1.0;
             }
             nodal_triad(nk_e, A);
-            const MatRows mr = mat_rows(p, COMP, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], g[G_SC + 3],
-                                        g[G_SC + 4]);
+            const MatRows mr = mat_rows(p, COMP, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
+                                        COMP ? g[G_SC + NSC - 2] : 0.0, COMP ? g[G_SC + NSC - 1] : 0.0);
             const double gNj0 = g[G_GN + 2 * j2], gNj1 = g[G_GN + 2 * j2 + 1];
             const double *bs = g + G_BS + j2 * 6;
             const double m1 = (1 / A[2][2]) * A[0][2];
@@ -1106,8 +1110,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 r2a[l] = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
                 r2b[l] = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
             }
-            const MatRows mr = mat_rows(p, COMP, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2], g[G_SC + 3],
-                                        g[G_SC + 4]);
+            const MatRows mr = mat_rows(p, COMP, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
+                                        COMP ? g[G_SC + NSC - 2] : 0.0, COMP ? g[G_SC + NSC - 1] : 0.0);
             const double *bs = g + G_BS + j2 * 6;
             double *cdst = sC + le2 * CSTR + 6 * j2;
             // global-basis columns of node j for row s of C
@@ -1983,7 +1987,9 @@ template <int NE, int WG, int SPLIT, int RS, bool COMP>
 static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
-    constexpr int total = NE * 4 * 47 + NE * (8 * 24 + 2) + NE * 4 * 14 + NE * 41;
+    // geometry records (homogeneous 44, layered 47 doubles per Gauss point),
+    // one Gauss point of C, node records (also the drilling scratch)
+    constexpr int total = NE * 4 * (COMP ? 47 : 44) + NE * (8 * 24 + 2) + NE * 4 * 14;
     const size_t smem = total * sizeof(double);
     static_assert(NE * 10 <= WG && NE * 4 * RS <= WG, "workgroup too small");
     static_assert(total * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
