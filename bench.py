@@ -38,6 +38,13 @@ def cd_step_bytes(n, nnz, nnb):
     return nnz * 8 + n * 16 + nnb * 8 + n * 72
 
 
+def resultant_bytes(E, N, nen, npts):
+    """Algorithmic HBM bytes of one stress-resultant launch over all elements:
+    connectivity 4 nen E, per node coordinates 24 + normal and flag 25 + the
+    six displacements 48, and 8 resultants of 8 B per integration point."""
+    return 4 * nen * E + (24 + 25 + 48) * N + 64 * npts * E
+
+
 def cpu_baseline(case, nsample, K_host, Mdiag, dt, cd_steps_cpu):
     """Time the oracle (C restatement, OpenMP) on a bounded sample of the same
     workload: element matrices + COO assembly for nsample elements, and
@@ -377,6 +384,24 @@ def main():
           f"omega_max {om:.4e}, dt {dt:.3e}", file=sys.stderr, flush=True)
     assert np.all(np.isfinite(r["U"])), "non-finite state in the explicit loop"
 
+    # stress resultants (inspectintegpoints, all three quantities) of the
+    # final CD state at the 4 Gauss points of every element
+    Ufull = d.scattersysvec(r["U"])
+    for _ in range(2):
+        f.resultants(case["xyz"], Ufull)
+    res_ms = []
+    t0 = time.perf_counter()
+    nres = 5
+    for _ in range(nres):
+        f.resultants(case["xyz"], Ufull)
+        res_ms.append(model.timing(5))
+    res_wall = (time.perf_counter() - t0) / nres * 1e3
+    res_dev = float(np.mean(res_ms))
+    b_res = resultant_bytes(E, N, 4, 4)
+    ach_res = b_res / (res_dev / 1e3) / 1e9
+    print(f"[bench] resultants: kernel {res_dev * 1e3:.1f} us, call {res_wall:.1f} ms (host copies, point "
+          f"locations)", file=sys.stderr, flush=True)
+
     b_asm = assembly_bytes(E, N, 4, nnz)
     b_cd = cd_step_bytes(nf, nnz, model.mixed_info()[2])
     ach = b_asm / (dev_ms / 1e3) / 1e9
@@ -422,6 +447,11 @@ def main():
                               "element_span_ms": float(np.mean(span_ms)),
                               "unpipelined_ms": float(np.mean(elem_ms) + np.mean(scat_ms))},
         "elem_launch_shape": dict(zip(("variant", "elements_per_workgroup", "workgroup"), ctx.tuning(4, 0))),
+        "resultants": {"kernel_us": res_dev * 1e3, "points_per_s": 4 * E / (res_dev / 1e3),
+                       "call_ms": res_wall,
+                       "roofline": {"bound": "hbm", "achieved": ach_res, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": ach_res / HBM_PEAK_GBS, "kernel": "resultants_kernel<4, false, false>",
+                                    "algorithmic_bytes": b_res}},
     }
     if not args.no_cpu:
         import scipy.sparse as sp

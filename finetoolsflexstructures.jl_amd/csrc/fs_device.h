@@ -110,6 +110,30 @@ struct ScatterPlan {
     const int32_t *pair_slot;     // 2 * npairs: kI, kJ
 };
 
+// Parameters of one stress-resultant launch (inspectintegpoints): one thread
+// per (listed element, integration point).
+//   g_mat  homogeneous: per group 16 doubles -- Dps 9 (row-major), 5/6 Dt 4,
+//          thickness, stab_c, unused; layered: per group 41 -- A 9, B 9, D 9,
+//          H 4 (row-major), layup csys 9 (row-major [i][j]), thickness
+//   ocs    output csys (row-major [i][j]): one matrix (ocs_kind 1) or
+//          nlist x npts matrices (ocs_kind 2); ocs_kind 0 takes E_G
+//   out    nlist x npts x 8: N11 N22 N12 M11 M22 M12 Q1 Q2
+//   status [0] singular metric, [1] output csys normal off the element normal
+struct ResParams {
+    int32_t nlist;
+    int32_t nelem;
+    int32_t ocs_kind;
+    int32_t twice;          // layered Q4: displacements transformed twice (reference form)
+    double stab_c;          // layered shells
+    const int32_t *felist;  // nlist, or nullptr for elements 0..nlist-1
+    const double *u;        // N x 6 nodal displacements and rotations (global basis)
+    const double *ocs;
+    const double *g_mat;
+    const int32_t *elem_group;  // nelem or nullptr
+    double *out;
+    int *status;
+};
+
 // Rows of the free-free CSR matrix for the row products of the time loop and
 // the power iteration.  Each row of a shell matrix is a sequence of runs of
 // consecutive columns, one run per neighbour node (its free dofs), so the

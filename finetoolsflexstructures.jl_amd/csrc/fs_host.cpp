@@ -33,6 +33,9 @@ hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const doub
                           const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
                           const double *gnrm, const int32_t *egroup, double *nx, double *ny, double *nz,
                           uint8_t *valid, hipStream_t st);
+hipError_t launch_resultants(int nen, int composite, const ResParams &rp, const double *x, const double *y,
+                             const double *z, const int32_t *conn, const double *nx, const double *ny,
+                             const double *nz, const uint8_t *valid, hipStream_t st);
 hipError_t launch_mass(int nen, int64_t N, int E, int composite, const double *g_rho_t, const double *g_inertia,
                        const int32_t *elem_group, const double *x, const double *y, const double *z,
                        const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, const int64_t *node_dof,
@@ -198,7 +201,7 @@ struct fs_model {
     Block blk[2];
     // group data for the last stiffness/mass call
     DevBuf g_Labd, g_Lh, g_csys, g_t, g_group, g_inertia, m_group, g_homog, g_rho_t;
-    double timing[5] = {0, 0, 0, 0, 0};
+    double timing[6] = {0, 0, 0, 0, 0, 0};
     // parameters of the last element-stiffness launch (device pointers into
     // the group buffers above) and their generation
     fs::ElemParams last_ep{};
@@ -1272,6 +1275,120 @@ int fs_model_mass(fs_model *m, const fs_mass_params *p, int block, double *diag_
     return FS_OK;
 }
 
+int fs_model_resultants(fs_model *m, const fs_shell_params *p, const double *u, int64_t nlist,
+                        const int32_t *felist, int32_t ocs_kind, const double *ocs, int32_t flags, double *out,
+                        int32_t *mismatch) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (!p || !u || !out || nlist < 0) return fail(c, FS_ERR_ARG, "resultants: bad arguments");
+    if (!m->normals_ready) return fail(c, FS_ERR_STATE, "resultants: normals not computed (associategeometry!)");
+    if (ocs_kind < FS_OCSYS_ISOPARAMETRIC || ocs_kind > FS_OCSYS_PER_POINT || (ocs_kind != 0 && !ocs))
+        return fail(c, FS_ERR_ARG, "resultants: output csys kind %d", ocs_kind);
+    if (flags & ~FS_RES_LAYERED_Q4_SINGLE_T) return fail(c, FS_ERR_ARG, "resultants: unknown flags %d", flags);
+    if (nlist > (int64_t)INT32_MAX / 4) return fail(c, FS_ERR_UNSUPPORTED, "resultants: %lld elements", (long long)nlist);
+    const int npts = m->nen == 3 ? 1 : 4;
+    if (felist)
+        for (int64_t l = 0; l < nlist; ++l)
+            if (felist[l] < 0 || felist[l] >= m->E)
+                return fail(c, FS_ERR_ARG, "resultants: element %d out of range", felist[l]);
+    if (!felist && nlist > m->E) return fail(c, FS_ERR_ARG, "resultants: nlist exceeds the element count");
+    FS_HIP(c, hipSetDevice(c->device));
+    fs::ResParams rp{};
+    rp.nlist = (int32_t)nlist;
+    rp.nelem = (int32_t)m->E;
+    rp.ocs_kind = ocs_kind;
+    rp.twice = (p->composite && m->nen == 4 && !(flags & FS_RES_LAYERED_Q4_SINGLE_T)) ? 1 : 0;
+    rp.stab_c = p->stab_c;
+    int s = 0;
+    DevBuf dmat, dgrp, du, dlist, docs, dout, dst;
+    std::vector<double> rec;
+    int ng = 1;
+    const int32_t *egroup = nullptr;
+    if (!p->composite) {
+        // per group: Dps (row-major), 5/6 Dt, thickness, stab_c
+        const bool grouped = p->group_homog != nullptr;
+        ng = grouped ? p->ngroups : 1;
+        if (ng < 1) return fail(c, FS_ERR_ARG, "resultants: group_homog needs ngroups >= 1");
+        rec.assign(16 * (size_t)ng, 0.0);
+        for (int g = 0; g < ng; ++g) {
+            const double *src = grouped ? p->group_homog + 16 * g : nullptr;
+            double *r = &rec[16 * g];
+            cm2rm(3, grouped ? src : p->Dps, r);
+            cm2rm(2, grouped ? src + 9 : p->Dt, r + 9);
+            for (int i = 0; i < 4; ++i) r[9 + i] *= 5.0 / 6.0;  // FEMMShellT3FFModule.jl:875-876, Q4RS :1091-1092
+            r[13] = grouped ? src[13] : p->thickness;
+            r[14] = grouped ? src[14] : p->stab_c;
+            if (!(r[13] > 0.0)) return fail(c, FS_ERR_ARG, "resultants: thickness of group %d must be positive", g);
+        }
+        if (grouped) egroup = p->elem_group;
+    } else {
+        ng = p->ngroups;
+        if (ng < 1 || !p->group_ABDH || !p->group_csys || !p->group_thickness)
+            return fail(c, FS_ERR_ARG, "resultants: layered shell needs ngroups >= 1 and group arrays");
+        rec.assign(41 * (size_t)ng, 0.0);
+        for (int g = 0; g < ng; ++g) {
+            const double *src = p->group_ABDH + 31 * g;
+            double *r = &rec[41 * g];
+            cm2rm(3, src, r);
+            cm2rm(3, src + 9, r + 9);
+            cm2rm(3, src + 18, r + 18);
+            cm2rm(2, src + 27, r + 27);
+            cm2rm(3, p->group_csys + 9 * g, r + 31);
+            r[40] = p->group_thickness[g];
+        }
+        egroup = p->elem_group;
+    }
+    if (egroup) {
+        for (int64_t e = 0; e < m->E; ++e)
+            if (egroup[e] < 0 || egroup[e] >= ng)
+                return fail(c, FS_ERR_ARG, "resultants: element %lld has group %d", (long long)e, egroup[e]);
+        if ((s = upload(c, dgrp, egroup, (size_t)m->E))) return s;
+        rp.elem_group = dgrp.as<int32_t>();
+    }
+    if ((s = upload(c, dmat, rec.data(), rec.size()))) return s;
+    rp.g_mat = dmat.as<double>();
+    if ((s = upload(c, du, u, (size_t)m->N * 6))) return s;
+    rp.u = du.as<double>();
+    if (felist) {
+        if ((s = upload(c, dlist, felist, (size_t)nlist))) return s;
+        rp.felist = dlist.as<int32_t>();
+    }
+    if (ocs_kind != FS_OCSYS_ISOPARAMETRIC) {
+        const size_t nm = ocs_kind == FS_OCSYS_MATRIX ? 1 : (size_t)nlist * npts;
+        std::vector<double> rm(9 * nm);
+        for (size_t k = 0; k < nm; ++k) cm2rm(3, ocs + 9 * k, &rm[9 * k]);
+        if ((s = upload(c, docs, rm.data(), rm.size()))) return s;
+        rp.ocs = docs.as<double>();
+    }
+    if ((s = dalloc(c, dout, (size_t)nlist * npts * 8 * sizeof(double)))) return s;
+    if ((s = dalloc(c, dst, 2 * sizeof(int)))) return s;
+    rp.out = dout.as<double>();
+    rp.status = dst.as<int>();
+    FS_HIP(c, hipMemsetAsync(dst.p, 0, 2 * sizeof(int), c->stream));
+    hipEvent_t e0, e1;
+    FS_HIP(c, hipEventCreate(&e0));
+    FS_HIP(c, hipEventCreate(&e1));
+    FS_HIP(c, hipEventRecord(e0, c->stream));
+    FS_HIP(c, fs::launch_resultants(m->nen, p->composite ? 1 : 0, rp, m->x.as<double>(), m->y.as<double>(),
+                                    m->z.as<double>(), m->dconn.as<int32_t>(), m->nx.as<double>(),
+                                    m->ny.as<double>(), m->nz.as<double>(), m->valid.as<uint8_t>(), c->stream));
+    FS_HIP(c, hipEventRecord(e1, c->stream));
+    int st[2] = {0, 0};
+    FS_HIP(c, hipMemcpyAsync(st, dst.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipMemcpyAsync(out, dout.p, (size_t)nlist * npts * 8 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    FS_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    m->timing[5] = ms;
+    if (st[0]) return fail(c, FS_ERR_NUMERIC, "resultants: singular metric matrix in _gradN_e!");
+    if (mismatch) *mismatch = st[1] ? 1 : 0;
+    return FS_OK;
+}
+
 int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, const double *V0, const double *Fmag,
                       const double *g, const int64_t *probe_dofs, int32_t nprobes, double *probe_out, double *U_out,
                       double *V_out) {
@@ -1529,7 +1646,7 @@ int fs_model_mixed_info(const fs_model *m, int64_t *info) {
 }
 
 int fs_model_last_timing(const fs_model *m, int which, double *ms) {
-    if (!m || !ms || which < 0 || which > 4) return FS_ERR_ARG;
+    if (!m || !ms || which < 0 || which > 5) return FS_ERR_ARG;
     *ms = m->timing[which];
     return FS_OK;
 }

@@ -348,6 +348,54 @@ __device__ __forceinline__ void q4_gradn(double xi, double eta, double gN[4][2])
 // GaussRule(2, 2) abscissa as FinEtools tabulates it.
 #define FS_GAUSS2 0.577350269189626
 
+// MITC transverse shear strain-displacement entries at (r, s) before the
+// transformation (_Bsmat, FEMMShellQ4RSModule.jl:753-804): bs[node][comp
+// w, tx, ty][row], from the element coordinates ec relative to the centroid.
+__device__ __forceinline__ void q4_mitc_bs(const double ec[4][2], double r, double s, double *bs) {
+    const double X1 = ec[0][0], Y1 = ec[0][1], X2 = ec[1][0], Y2 = ec[1][1];
+    const double X3 = ec[2][0], Y3 = ec[2][1], X4 = ec[3][0], Y4 = ec[3][1];
+    const double J11 = (X1 * (s - 1) / 4 - X2 * (s - 1) / 4 + X3 * (s + 1) / 4 - X4 * (s + 1) / 4);
+    const double J21 = (Y1 * (s - 1) / 4 - Y2 * (s - 1) / 4 + Y3 * (s + 1) / 4 - Y4 * (s + 1) / 4);
+    const double J12 = (X1 * (r - 1) / 4 - X2 * (r + 1) / 4 + X3 * (r + 1) / 4 - X4 * (r - 1) / 4);
+    const double J22 = (Y1 * (r - 1) / 4 - Y2 * (r + 1) / 4 + Y3 * (r + 1) / 4 - Y4 * (r - 1) / 4);
+    const double A = sqrt(J11 * J11 + J21 * J21);
+    const double B = sqrt(J12 * J12 + J22 * J22);
+    const double iA = 1.0 / A, iB = 1.0 / B;
+    const double ca = J11 * iA, sa = J21 * iA, cb = J12 * iB, sb = J22 * iB;
+    const double detJ = J11 * J22 - J12 * J21;
+    const double Ax = X1 - X2 - X3 + X4, Ay = Y1 - Y2 - Y3 + Y4;
+    const double Bx = X1 - X2 + X3 - X4, By = Y1 - Y2 + Y3 - Y4;
+    const double Cx = X1 + X2 - X3 - X4, Cy = Y1 + Y2 - Y3 - Y4;
+    const double P = sqrt((Ax + Bx * s) * (Ax + Bx * s) + (Ay + By * s) * (Ay + By * s));
+    const double Q = sqrt((Bx * r + Cx) * (Bx * r + Cx) + (By * r + Cy) * (By * r + Cy));
+    const double i16 = 1.0 / (16 * detJ), i32 = 0.5 * i16;  // exact: power-of-two scale
+    bs[0 * 6 + 0 * 2 + 0] = (sa * (r + 1) * P - sb * (s + 1) * Q) * i16;
+    bs[0 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
+    bs[0 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
+    bs[0 * 6 + 0 * 2 + 1] = (-ca * (r + 1) * P + cb * (s + 1) * Q) * i16;
+    bs[0 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
+    bs[0 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
+    bs[1 * 6 + 0 * 2 + 0] = (-sa * (r - 1) * P + sb * (s + 1) * Q) * i16;
+    bs[1 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
+    bs[1 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
+    bs[1 * 6 + 0 * 2 + 1] = (ca * (r - 1) * P - cb * (s + 1) * Q) * i16;
+    bs[1 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
+    bs[1 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
+    bs[2 * 6 + 0 * 2 + 0] = (sa * (r - 1) * P - sb * (s - 1) * Q) * i16;
+    bs[2 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
+    bs[2 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
+    bs[2 * 6 + 0 * 2 + 1] = (-ca * (r - 1) * P + cb * (s - 1) * Q) * i16;
+    bs[2 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
+    bs[2 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
+    bs[3 * 6 + 0 * 2 + 0] = (-sa * (r + 1) * P + sb * (s - 1) * Q) * i16;
+    bs[3 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
+    bs[3 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
+    bs[3 * 6 + 0 * 2 + 1] = (ca * (r + 1) * P - cb * (s - 1) * Q) * i16;
+    bs[3 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
+    bs[3 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
+}
+
+
 template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void elem_kernel(ElemParams p, const double *__restrict__ xs,
                                                   const double *__restrict__ ys,
@@ -961,50 +1009,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             }
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j) g[G_EG + i * 3 + j] = EG[i][j];
-            // MITC transverse shear entries (_Bsmat, FEMMShellQ4RSModule.jl:753-804)
-            const double X1 = ec[0][0], Y1 = ec[0][1], X2 = ec[1][0], Y2 = ec[1][1];
-            const double X3 = ec[2][0], Y3 = ec[2][1], X4 = ec[3][0], Y4 = ec[3][1];
-            const double r = xi, s = eta;
-            const double J11 = (X1 * (s - 1) / 4 - X2 * (s - 1) / 4 + X3 * (s + 1) / 4 - X4 * (s + 1) / 4);
-            const double J21 = (Y1 * (s - 1) / 4 - Y2 * (s - 1) / 4 + Y3 * (s + 1) / 4 - Y4 * (s + 1) / 4);
-            const double J12 = (X1 * (r - 1) / 4 - X2 * (r + 1) / 4 + X3 * (r + 1) / 4 - X4 * (r - 1) / 4);
-            const double J22 = (Y1 * (r - 1) / 4 - Y2 * (r + 1) / 4 + Y3 * (r + 1) / 4 - Y4 * (r - 1) / 4);
-            const double A = sqrt(J11 * J11 + J21 * J21);
-            const double B = sqrt(J12 * J12 + J22 * J22);
-            const double iA = 1.0 / A, iB = 1.0 / B;
-            const double ca = J11 * iA, sa = J21 * iA, cb = J12 * iB, sb = J22 * iB;
-            const double detJ = J11 * J22 - J12 * J21;
-            const double Ax = X1 - X2 - X3 + X4, Ay = Y1 - Y2 - Y3 + Y4;
-            const double Bx = X1 - X2 + X3 - X4, By = Y1 - Y2 + Y3 - Y4;
-            const double Cx = X1 + X2 - X3 - X4, Cy = Y1 + Y2 - Y3 - Y4;
-            const double P = sqrt((Ax + Bx * s) * (Ax + Bx * s) + (Ay + By * s) * (Ay + By * s));
-            const double Q = sqrt((Bx * r + Cx) * (Bx * r + Cx) + (By * r + Cy) * (By * r + Cy));
-            const double i16 = 1.0 / (16 * detJ), i32 = 0.5 * i16;  // exact: power-of-two scale
-            double *bs = g + G_BS;  // [node][comp w,tx,ty][row]
-            bs[0 * 6 + 0 * 2 + 0] = (sa * (r + 1) * P - sb * (s + 1) * Q) * i16;
-            bs[0 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
-            bs[0 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
-            bs[0 * 6 + 0 * 2 + 1] = (-ca * (r + 1) * P + cb * (s + 1) * Q) * i16;
-            bs[0 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
-            bs[0 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
-            bs[1 * 6 + 0 * 2 + 0] = (-sa * (r - 1) * P + sb * (s + 1) * Q) * i16;
-            bs[1 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
-            bs[1 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
-            bs[1 * 6 + 0 * 2 + 1] = (ca * (r - 1) * P - cb * (s + 1) * Q) * i16;
-            bs[1 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
-            bs[1 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
-            bs[2 * 6 + 0 * 2 + 0] = (sa * (r - 1) * P - sb * (s - 1) * Q) * i16;
-            bs[2 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
-            bs[2 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
-            bs[2 * 6 + 0 * 2 + 1] = (-ca * (r - 1) * P + cb * (s - 1) * Q) * i16;
-            bs[2 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
-            bs[2 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
-            bs[3 * 6 + 0 * 2 + 0] = (-sa * (r + 1) * P + sb * (s - 1) * Q) * i16;
-            bs[3 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
-            bs[3 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
-            bs[3 * 6 + 0 * 2 + 1] = (ca * (r + 1) * P - cb * (s - 1) * Q) * i16;
-            bs[3 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
-            bs[3 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
+            q4_mitc_bs(ec, xi, eta, g + G_BS);
         }
     }
 
@@ -1521,6 +1526,320 @@ __global__ __launch_bounds__(256) void mass_kernel(int64_t N, int E, MassParams 
         const int64_t d = node_dof[I * 6 + c];
         if (d >= 0 && d < nrows) diag[d] = c < 3 ? tm : rm;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Stress resultants at the integration points (inspectintegpoints)
+// ---------------------------------------------------------------------------
+//
+// One thread per (listed element, integration point) computes the membrane
+// forces, bending moments and transverse shear forces of the point at once
+// (FEMMShellT3FFModule.jl:850-962, FEMMShellQ4RSModule.jl:1061-1170,
+// FEMMShellT3FFCompModule.jl:809-943, FEMMShellQ4RSCompModule.jl:1077-1209).
+// The element displacements are taken into the element frame node by node
+// (the block structure of T = Tae Tga: each node's 3 x 3 triad blocks, plus
+// the coupling of the in-plane rotations to every node's translations) instead
+// of through the dense 18 x 18 / 24 x 24 product, then the strain rows of
+// Bm, Bb and Bs are applied, then the material law and the output frame.
+
+// Pieces of t_e = Tae Tga t_g at one point (_TransfmatGToA,
+// FEMMShellT3FFModule.jl:398-419; _transfmat_a_to_e!, :421-463), node by
+// node: un_j = A_j' E_G' t_j (translations), rn_j = the first two components
+// of A_j' E_G' r_j (the drilling component does not enter the rows of T), and
+// W0 = sum_j gN_j0 un_j, W1 = sum_j gN_j1 un_j, so that the coupling of node
+// i's in-plane rotations to all translations is
+// sum_j sum_k a3(i, j, k) un_j[k] = (A_i[1] . W0 - A_i[0] . W1) / 2.
+template <int NEN>
+__device__ __forceinline__ void nodal_frame(const double EG[3][3], const double A[NEN][3][3],
+                                            const double gN[NEN][2], const double tg[NEN][6], double un[NEN][3],
+                                            double rn[NEN][2], double W0[3], double W1[3]) {
+    for (int k = 0; k < 3; ++k) W0[k] = W1[k] = 0.0;
+    for (int j = 0; j < NEN; ++j) {
+        double gt[3], gr[3];
+        for (int k = 0; k < 3; ++k) {
+            gt[k] = EG[0][k] * tg[j][0] + EG[1][k] * tg[j][1] + EG[2][k] * tg[j][2];
+            gr[k] = EG[0][k] * tg[j][3] + EG[1][k] * tg[j][4] + EG[2][k] * tg[j][5];
+        }
+        for (int r = 0; r < 3; ++r) un[j][r] = A[j][0][r] * gt[0] + A[j][1][r] * gt[1] + A[j][2][r] * gt[2];
+        for (int r = 0; r < 2; ++r) rn[j][r] = A[j][0][r] * gr[0] + A[j][1][r] * gr[1] + A[j][2][r] * gr[2];
+        for (int k = 0; k < 3; ++k) {
+            W0[k] += gN[j][0] * un[j][k];
+            W1[k] += gN[j][1] * un[j][k];
+        }
+    }
+}
+
+// Element-frame translations te[0..2] and in-plane rotations te[3..4] of node i.
+__device__ __forceinline__ void node_element_frame(const double Ai[3][3], const double un[3], const double rn[2],
+                                                   const double W0[3], const double W1[3], double te[5]) {
+    for (int rw = 0; rw < 3; ++rw) te[rw] = Ai[rw][0] * un[0] + Ai[rw][1] * un[1] + Ai[rw][2] * un[2];
+    const double ia33 = 1 / Ai[2][2];
+    const double cpl = 0.5 * ((Ai[1][0] * W0[0] + Ai[1][1] * W0[1] + Ai[1][2] * W0[2]) -
+                              (Ai[0][0] * W1[0] + Ai[0][1] * W1[1] + Ai[0][2] * W1[2]));
+    for (int rw = 0; rw < 2; ++rw) {
+        double acc = 0.0;
+        for (int cl = 0; cl < 2; ++cl) acc += (Ai[rw][cl] - ia33 * Ai[rw][2] * Ai[cl][2]) * rn[cl];
+        te[3 + rw] = acc + (ia33 * Ai[rw][2]) * cpl;
+    }
+}
+
+// (o2_e' X o2_e) for the symmetric 2 x 2 tensor X = [x0 x2; x2 x1], o2_e = [m n; -n m].
+__device__ __forceinline__ void out_tensor(double m, double n, const double x[3], double *o) {
+    const double t00 = m * x[0] - n * x[2], t01 = m * x[2] - n * x[1];
+    const double t10 = n * x[0] + m * x[2], t11 = n * x[2] + m * x[1];
+    o[0] = t00 * m - t01 * n;
+    o[1] = t10 * n + t11 * m;
+    o[2] = t00 * n + t01 * m;
+}
+
+// T3 transverse shear entries bs[node][w, tx, ty][row]: _Bsmat!, the mean of
+// the three orderings of _add_Bsmat_o! (FEMMShellT3FFModule.jl:486-537),
+// accumulated in the reference's order.
+__device__ __forceinline__ void t3_bs(const double ec[3][2], double Ae, double bs[18]) {
+    for (int i = 0; i < 18; ++i) bs[i] = 0.0;
+    const double mlt = (1 / 2.0 / Ae);
+    for (int o = 0; o < 3; ++o) {
+        const int s = o, p = (o + 1) % 3, q = (o + 2) % 3;
+        const double a = ec[p][0] - ec[s][0], b = ec[p][1] - ec[s][1];
+        const double c = ec[q][0] - ec[s][0], d = ec[q][1] - ec[s][1];
+        bs[s * 6 + 0 * 2 + 0] += mlt * (b - d);
+        bs[s * 6 + 2 * 2 + 0] += mlt * (Ae);
+        bs[s * 6 + 0 * 2 + 1] += mlt * (c - a);
+        bs[s * 6 + 1 * 2 + 1] += mlt * (-Ae);
+        bs[p * 6 + 0 * 2 + 0] += mlt * (d);
+        bs[p * 6 + 1 * 2 + 0] += mlt * (-b * d / 2);
+        bs[p * 6 + 2 * 2 + 0] += mlt * (a * d / 2);
+        bs[p * 6 + 0 * 2 + 1] += mlt * (-c);
+        bs[p * 6 + 1 * 2 + 1] += mlt * (b * c / 2);
+        bs[p * 6 + 2 * 2 + 1] += mlt * (-a * c / 2);
+        bs[q * 6 + 0 * 2 + 0] += mlt * (-b);
+        bs[q * 6 + 1 * 2 + 0] += mlt * (b * d / 2);
+        bs[q * 6 + 2 * 2 + 0] += mlt * (-b * c / 2);
+        bs[q * 6 + 0 * 2 + 1] += mlt * (a);
+        bs[q * 6 + 1 * 2 + 1] += mlt * (-a * d / 2);
+        bs[q * 6 + 2 * 2 + 1] += mlt * (a * c / 2);
+    }
+    for (int i = 0; i < 18; ++i) bs[i] *= (1.0 / 3);
+}
+
+// TWICE (layered Q4 as the reference computes it): the element-frame vectors
+// are transformed by T a second time before the strain rows are applied.
+template <int NEN, bool COMP, bool TWICE>
+__global__ __launch_bounds__(256) void resultants_kernel(ResParams rp, const double *__restrict__ xs,
+                                                         const double *__restrict__ ys,
+                                                         const double *__restrict__ zs,
+                                                         const int32_t *__restrict__ conn,
+                                                         const double *__restrict__ nxs,
+                                                         const double *__restrict__ nys,
+                                                         const double *__restrict__ nzs,
+                                                         const uint8_t *__restrict__ valid) {
+    constexpr int NQP = NEN == 3 ? 1 : 4;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (int64_t)rp.nlist * NQP) return;
+    const int64_t l = gid / NQP;
+    const int q = (int)(gid % NQP);
+    const int E = rp.nelem;
+    const int e = rp.felist ? rp.felist[l] : (int)l;
+    int nd[NEN];
+    for (int a = 0; a < NEN; ++a) nd[a] = conn[a * E + e];
+    // ---- geometry of the point: E_G, element-frame gradients, shear entries, h
+    double EG[3][3], gN[NEN][2], h;
+    double ec[NEN][2], Ae = 0.0;  // element coordinates for the shear rows
+    {
+        double X[NEN][3];
+        for (int a = 0; a < NEN; ++a) {
+            X[a][0] = xs[nd[a]];
+            X[a][1] = ys[nd[a]];
+            X[a][2] = zs[nd[a]];
+        }
+        if (NEN == 3) {
+            double J0[3][2];
+            for (int j = 0; j < 3; ++j) {
+                J0[j][0] = X[1][j] - X[0][j];
+                J0[j][1] = X[2][j] - X[0][j];
+            }
+            e_g(EG, J0);
+            ec[0][0] = ec[0][1] = 0.0;
+            for (int k = 0; k < 2; ++k) {
+                ec[1][k] = J0[0][0] * EG[0][k] + J0[1][0] * EG[1][k] + J0[2][0] * EG[2][k];
+                ec[2][k] = J0[0][1] * EG[0][k] + J0[1][1] * EG[1][k] + J0[2][1] * EG[2][k];
+            }
+            const double a = ec[1][0], b = ec[1][1], c = ec[2][0], d = ec[2][1];
+            const double Jd = a * d - b * c;  // _gradN_e_Ae!, FEMMShellT3FFModule.jl:465-484
+            gN[0][0] = (b - d) / Jd;
+            gN[1][0] = d / Jd;
+            gN[2][0] = -b / Jd;
+            gN[0][1] = (c - a) / Jd;
+            gN[1][1] = -c / Jd;
+            gN[2][1] = a / Jd;
+            Ae = Jd / 2;
+            h = sqrt(2 * Ae);
+        } else {
+            const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+            const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+            double gNp[4][2];
+            q4_gradn(xi, eta, gNp);
+            double J[3][2];
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 2; ++k) {
+                    double acc = 0.0;
+                    for (int a = 0; a < 4; ++a) acc += X[a][i] * gNp[a][k];
+                    J[i][k] = acc;
+                }
+            e_g(EG, J);
+            double cen[3] = {0.0, 0.0, 0.0};
+            for (int a = 0; a < 4; ++a)
+                for (int k = 0; k < 3; ++k) cen[k] += X[a][k];
+            for (int k = 0; k < 3; ++k) cen[k] /= 4;
+            for (int a = 0; a < 4; ++a)
+                for (int k = 0; k < 2; ++k) {
+                    double dd = 0.0;
+                    for (int m = 0; m < 3; ++m) dd += (X[a][m] - cen[m]) * EG[m][k];
+                    ec[a][k] = dd;
+                }
+            const double G00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
+            const double G01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
+            const double G11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
+            const double detG = G00 * G11 - G01 * G01;
+            if (detG == 0.0) rp.status[0] = 1;
+            const double idetG = 1.0 / detG;
+            const double i00 = G11 * idetG, i01 = -G01 * idetG, i11 = G00 * idetG;
+            for (int a = 0; a < 4; ++a) {
+                const double t0 = i00 * gNp[a][0] + i01 * gNp[a][1];
+                const double t1 = i01 * gNp[a][0] + i11 * gNp[a][1];
+                double gg[3];
+                for (int i = 0; i < 3; ++i) gg[i] = J[i][0] * t0 + J[i][1] * t1;
+                gN[a][0] = EG[0][0] * gg[0] + EG[1][0] * gg[1] + EG[2][0] * gg[2];
+                gN[a][1] = EG[0][1] * gg[0] + EG[1][1] * gg[1] + EG[2][1] * gg[2];
+            }
+            double maxd = 0.0;  // _quad_diameter, FEMMShellQ4RSModule.jl:861-870
+            for (int i = 1; i < 4; ++i) {
+                const double dx = X[i][0] - X[0][0], dy = X[i][1] - X[0][1], dz = X[i][2] - X[0][2];
+                const double dist = (dx * dx + dy * dy + dz * dz);
+                maxd = dist > maxd ? dist : maxd;
+            }
+            h = sqrt(maxd);
+        }
+    }
+    // ---- nodal triads, the nodal-frame vectors and the strain rows
+    double A[NEN][3][3];
+    for (int a = 0; a < NEN; ++a) {
+        double nk_e[3];
+        if (valid[nd[a]]) {
+            const double n0 = nxs[nd[a]], n1 = nys[nd[a]], n2 = nzs[nd[a]];
+            for (int i = 0; i < 3; ++i) nk_e[i] = EG[0][i] * n0 + EG[1][i] * n1 + EG[2][i] * n2;
+        } else {
+            nk_e[0] = 0.0;
+            nk_e[1] = 0.0;
+            nk_e[2] = 1.0;
+        }
+        nodal_triad(nk_e, A[a]);
+    }
+    double un[NEN][3], rn[NEN][2], W0[3], W1[3];
+    {
+        double tg[NEN][6];
+        for (int a = 0; a < NEN; ++a)
+            for (int c = 0; c < 6; ++c) tg[a][c] = rp.u[(int64_t)nd[a] * 6 + c];
+        nodal_frame<NEN>(EG, A, gN, tg, un, rn, W0, W1);
+        if (TWICE) {
+            for (int i = 0; i < NEN; ++i) {
+                node_element_frame(A[i], un[i], rn[i], W0, W1, tg[i]);
+                tg[i][5] = 0.0;
+            }
+            nodal_frame<NEN>(EG, A, gN, tg, un, rn, W0, W1);
+        }
+    }
+    // strains: Bm, Bb (FEMMShellT3FFModule.jl:539-561), node by node; then
+    // Bs from the element coordinates once the triads are dead
+    double eps[3] = {0.0, 0.0, 0.0}, kap[3] = {0.0, 0.0, 0.0}, gam[2] = {0.0, 0.0};
+    double te[NEN][5];
+    for (int i = 0; i < NEN; ++i) {
+        node_element_frame(A[i], un[i], rn[i], W0, W1, te[i]);
+        eps[0] += gN[i][0] * te[i][0];
+        eps[1] += gN[i][1] * te[i][1];
+        eps[2] += gN[i][1] * te[i][0] + gN[i][0] * te[i][1];
+        kap[0] += gN[i][0] * te[i][4];
+        kap[1] += -gN[i][1] * te[i][3];
+        kap[2] += -gN[i][0] * te[i][3] + gN[i][1] * te[i][4];
+    }
+    {
+        double bs[6 * NEN];  // [node][w, tx, ty][row]
+        if (NEN == 3)
+            t3_bs((const double(*)[2])ec, Ae, bs);
+        else
+            q4_mitc_bs((const double(*)[2])ec, (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2,
+                       (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2, bs);
+        for (int i = 0; i < NEN; ++i)
+            for (int row = 0; row < 2; ++row)
+                gam[row] += bs[i * 6 + 0 * 2 + row] * te[i][2] + bs[i * 6 + 1 * 2 + row] * te[i][3] +
+                            bs[i * 6 + 2 * 2 + row] * te[i][4];
+    }
+    // ---- material law
+    const int grp = rp.elem_group ? rp.elem_group[e] : 0;
+    double f[3], mo[3], qs[2];
+    if (!COMP) {
+        const double *g = rp.g_mat + 16 * grp;
+        const double t = g[13], stab_c = g[14];
+        const double stab = (t * t) / (t * t + stab_c * (h * h));
+        const double cb = (t * t * t) / 12, cs = t * stab;
+        for (int i = 0; i < 3; ++i) {
+            f[i] = t * (g[i * 3 + 0] * eps[0] + g[i * 3 + 1] * eps[1] + g[i * 3 + 2] * eps[2]);
+            mo[i] = cb * (g[i * 3 + 0] * kap[0] + g[i * 3 + 1] * kap[1] + g[i * 3 + 2] * kap[2]);
+        }
+        for (int i = 0; i < 2; ++i) qs[i] = cs * (g[9 + i * 2 + 0] * gam[0] + g[9 + i * 2 + 1] * gam[1]);
+    } else {
+        // layup frame: strains through Tb, resultants back through Tb' (the
+        // sA = Tb' A Tb, ... of FEMMShellT3FFCompModule.jl:914-921)
+        const double *g = rp.g_mat + 41 * grp;
+        const double t = g[40];
+        const double stab = (t * t) / (t * t + rp.stab_c * (h * h));
+        double m, n;
+        layup_angle(EG, g + 31, m, n);
+        const double Tb[3][3] = {{m * m, n * n, -(m * n)}, {n * n, m * m, (m * n)},
+                                 {2 * (m * n), -2 * (m * n), m * m - n * n}};
+        double el[3], kl[3], Nl[3], Ml[3];
+        for (int i = 0; i < 3; ++i) {
+            el[i] = Tb[i][0] * eps[0] + Tb[i][1] * eps[1] + Tb[i][2] * eps[2];
+            kl[i] = Tb[i][0] * kap[0] + Tb[i][1] * kap[1] + Tb[i][2] * kap[2];
+        }
+        for (int i = 0; i < 3; ++i) {
+            double a = 0.0, b = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                a += g[i * 3 + j] * el[j] + g[9 + i * 3 + j] * kl[j];
+                b += g[9 + i * 3 + j] * el[j] + g[18 + i * 3 + j] * kl[j];
+            }
+            Nl[i] = a;
+            Ml[i] = b;
+        }
+        for (int i = 0; i < 3; ++i) {
+            f[i] = Tb[0][i] * Nl[0] + Tb[1][i] * Nl[1] + Tb[2][i] * Nl[2];
+            mo[i] = Tb[0][i] * Ml[0] + Tb[1][i] * Ml[1] + Tb[2][i] * Ml[2];
+        }
+        // sH = Tt' H Tt with Tt = [m -n; n m]
+        const double gl0 = m * gam[0] - n * gam[1], gl1 = n * gam[0] + m * gam[1];
+        const double Ql0 = g[27] * gl0 + g[28] * gl1, Ql1 = g[29] * gl0 + g[30] * gl1;
+        qs[0] = stab * (m * Ql0 + n * Ql1);
+        qs[1] = stab * (-n * Ql0 + m * Ql1);
+    }
+    // ---- output frame
+    double L[9];
+    if (rp.ocs_kind == 0) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) L[i * 3 + j] = EG[i][j];
+    } else {
+        const double *src = rp.ocs + (rp.ocs_kind == 1 ? 0 : (l * NQP + q) * 9);
+        for (int i = 0; i < 9; ++i) L[i] = src[i];
+        if (L[2] * EG[0][2] + L[5] * EG[1][2] + L[8] * EG[2][2] < 0.95) rp.status[1] = 1;
+    }
+    double om, on;
+    layup_angle(EG, L, om, on);
+    double *o = rp.out + (l * NQP + q) * 8;
+    double ov[8];
+    out_tensor(om, on, f, ov);
+    out_tensor(om, on, mo, ov + 3);
+    ov[6] = om * qs[0] - on * qs[1];
+    ov[7] = on * qs[0] + om * qs[1];
+    for (int k = 0; k < 8; ++k) o[k] = ov[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -2086,6 +2405,27 @@ hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const doub
         hipLaunchKernelGGL(normals_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, x, y, z, conn, ne_ptr, ne_list,
                            ntol, gnrm, egroup, nx, ny, nz, valid);
     return hipGetLastError();
+}
+
+hipError_t launch_resultants(int nen, int composite, const ResParams &rp, const double *x, const double *y,
+                             const double *z, const int32_t *conn, const double *nx, const double *ny,
+                             const double *nz, const uint8_t *valid, hipStream_t st) {
+    const int64_t nthreads = (int64_t)rp.nlist * (nen == 3 ? 1 : 4);
+    if (nthreads == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((nthreads + 255) / 256);
+#define FS_RES_CASE(NN, CC, TW)                                                                               \
+    if (nen == NN && (composite != 0) == CC && (rp.twice != 0) == TW) {                                         \
+        hipLaunchKernelGGL((resultants_kernel<NN, CC, TW>), dim3(blocks), dim3(256), 0, st, rp, x, y, z, conn, nx, \
+                           ny, nz, valid);                                                                      \
+        return hipGetLastError();                                                                               \
+    }
+    FS_RES_CASE(3, false, false)
+    FS_RES_CASE(3, true, false)
+    FS_RES_CASE(4, false, false)
+    FS_RES_CASE(4, true, false)
+    FS_RES_CASE(4, true, true)
+#undef FS_RES_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_mass(int nen, int64_t N, int E, int composite, const double *g_rho_t, const double *g_inertia,

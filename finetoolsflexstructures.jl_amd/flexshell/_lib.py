@@ -19,6 +19,10 @@ FS_SHEAR_AVERAGE_K = 1
 FS_PREC_FP64 = 0
 FS_PREC_MIXED = 1
 FS_PREC_MIXED_F32 = 2
+FS_OCSYS_ISOPARAMETRIC = 0
+FS_OCSYS_MATRIX = 1
+FS_OCSYS_PER_POINT = 2
+FS_RES_LAYERED_Q4_SINGLE_T = 1
 
 _STATUS = {
     -1: "invalid argument",
@@ -105,6 +109,8 @@ SIGNATURES = {
     "fs_model_explicit": (C.c_int, [C.c_void_p, C.POINTER(CDParams), _dp, _dp, _dp, _dp, _i64p, C.c_int32,
                                     _dp, _dp, _dp]),
     "fs_model_omega_max": (C.c_int, [C.c_void_p, _dp, C.c_int32, _dp]),
+    "fs_model_resultants": (C.c_int, [C.c_void_p, C.POINTER(ShellParams), _dp, C.c_int64, _i32p, C.c_int32, _dp,
+                                      C.c_int32, _dp, _i32p]),
     "fs_model_last_timing": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "fs_model_mixed_info": (C.c_int, [C.c_void_p, _i64p]),
     "fs_ctx_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i32p]),
@@ -278,6 +284,33 @@ class Model:
         self.ctx.check(self.ctx.lib.fs_model_element_blocks(self.h, C.byref(params), ptr(out)),
                        "fs_model_element_blocks")
         return out
+
+    def resultants(self, params, u, felist=None, ocs=None, flags=0):
+        """Stress resultants N11 N22 N12 M11 M22 M12 Q1 Q2 at the integration
+        points (nlist x npts x 8) and the csys-mismatch flag.  u: N x 6 nodal
+        values; felist: 0-based element numbers or None (all); ocs: None
+        (isoparametric), one 3 x 3 matrix, or nlist x npts x 3 x 3."""
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(self.N, 6))
+        fl = None if felist is None else np.ascontiguousarray(np.asarray(felist, dtype=np.int32).ravel())
+        nlist = self.E if fl is None else fl.shape[0]
+        npts = 1 if self.nen == 3 else 4
+        if ocs is None:
+            kind, oc = FS_OCSYS_ISOPARAMETRIC, None
+        else:
+            m = np.asarray(ocs, dtype=np.float64)
+            if m.size == 9:
+                kind = FS_OCSYS_MATRIX
+            elif m.size == 9 * nlist * npts:
+                kind = FS_OCSYS_PER_POINT
+            else:
+                raise ValueError("ocs must be one 3 x 3 matrix or nlist x npts x 3 x 3")
+            oc = np.ascontiguousarray(np.swapaxes(m.reshape(-1, 3, 3), 1, 2))  # column-major matrices
+        out = np.zeros((nlist, npts, 8))
+        mis = C.c_int32(0)
+        self.ctx.check(self.ctx.lib.fs_model_resultants(self.h, C.byref(params), ptr(u), int(nlist),
+                                                        ptr(fl, C.c_int32), int(kind), ptr(oc), int(flags), ptr(out),
+                                                        C.byref(mis)), "fs_model_resultants")
+        return out, bool(mis.value)
 
     def mass(self, params, block, nrows=None):
         """Lumped-mass diagonal of the block; nrows None keeps it on the device only."""

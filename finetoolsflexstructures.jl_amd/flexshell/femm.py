@@ -11,6 +11,8 @@ Julia's `!` suffix is dropped (associategeometry! -> associategeometry).  The
 stabilisation closure stab_fun(t, h) = t^2 / (t^2 + c h^2) is given by its
 coefficient c (`stab_c`), which covers every form the reference uses.
 """
+import warnings
+
 import numpy as np
 import scipy.sparse as sp
 
@@ -264,6 +266,88 @@ class _ShellFEMM:
         d = m.mass(self._mass_params(), block, nrows)
         return sp.diags(d, format="csr")
 
+    # -- inspectintegpoints --
+    def _points(self, xyz, fl, tangents=False):
+        """Location (nlist x npts x 3) and, if asked, tangent matrix (nlist x
+        npts x 3 x 2) of every integration point of the listed elements: the
+        centroid and J0 for T3 (FEMMShellT3FFModule.jl:908-911), loc and J of
+        locjac! at the Gauss points for Q4."""
+        X = xyz[self.conn[fl]]
+        if self.nen == 3:
+            loc = X.mean(axis=1)[:, None, :]
+            J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]], axis=2)[:, None] if tangents else None
+            return loc, J
+        g = 0.577350269189626
+        pts = [(-g, -g), (-g, g), (g, -g), (g, g)]
+        Nm = np.array([_q4_N(xi, eta) for xi, eta in pts])  # points x nodes
+        loc = np.matmul(Nm, X)
+        J = None
+        if tangents:
+            G = np.array([_q4_gradN(xi, eta) for xi, eta in pts])  # points x nodes x 2
+            J = np.einsum("eak,pal->epkl", X, G)
+        return loc, J
+
+    def resultants(self, geom0, u, felist=None, outputcsys=None, transform_once=False):
+        """All stress resultants of the listed elements on the device, as
+        inspectintegpoints computes them: (nlist x npts x 8 with N11 N22 N12
+        M11 M22 M12 Q1 Q2 in the output csys, point locations nlist x npts x 3).
+        outputcsys: None (the FEMM's default: mcsys, or the first layup csys),
+        a 3 x 3 matrix, or a callable (X, J, e, q) -> 3 x 3 evaluated per point.
+        transform_once: for Q4RSComp, take the displacements into the element
+        frame once instead of the reference's twice."""
+        assert self._associatedgeometry, "associategeometry must be called first"
+        xyz = np.asarray(geom0, dtype=float)
+        m = self._ensure_model(xyz)
+        vals = np.asarray(getattr(u, "values", u), dtype=float).reshape(xyz.shape[0], 6)
+        fl = np.arange(self.conn.shape[0]) if felist is None else np.asarray(felist, dtype=np.int64).ravel()
+        if fl.size and (fl.min() < 0 or fl.max() >= self.conn.shape[0]):
+            raise IndexError("inspectintegpoints: element number out of range")
+        ocs = outputcsys
+        if ocs is None:
+            ocs = self.layup_groups[0][0].csys if self.composite else self.mcsys
+        loc, J = self._points(xyz, fl, tangents=callable(ocs))
+        if callable(ocs):
+            npts = loc.shape[1]
+            ocs = np.array([[np.asarray(ocs(loc[l, j], J[l, j], int(fl[l]), j), dtype=float)
+                             for j in range(npts)] for l in range(fl.shape[0])]).reshape(-1, 3, 3)
+        flags = _lib.FS_RES_LAYERED_Q4_SINGLE_T if transform_once else 0
+        out, mismatch = m.resultants(self._shell_params(), vals, fl, ocs, flags)
+        if mismatch:
+            warnings.warn("Coordinate systems mismatched?")
+        return out, loc
+
+    _QUANTITY = {"bending": 3, "moment": 3, "bending_moment": 3, "transverse_shear": 6, "transverse": 6,
+                 "shear": 6, "membrane_force": 0, "membrane": 0}
+
+    def inspectintegpoints(self, geom0, u, *args, outputcsys=None, transform_once=False):
+        """inspectintegpoints(femm, geom0, u[, dT], felist, inspector, idat[,
+        quantity]; outputcsys) (FEMMShellT3FFModule.jl:850-962,
+        FEMMShellQ4RSModule.jl:1061-1170, FEMMShellT3FFCompModule.jl:809-943,
+        FEMMShellQ4RSCompModule.jl:1077-1209).  The resultants come from the
+        device in one launch; the inspector is then called per element and
+        point as idat = inspector(idat, e, conn, ecoords, out, loc) with
+        out = the moments (default, :bending/:moment), the membrane forces
+        (:membrane) or the shear forces (:shear, out[2] = 0).  Element numbers
+        are 0-based."""
+        if len(args) >= 3 and callable(args[1]):
+            felist, inspector, idat, rest = args[0], args[1], args[2], args[3:]
+        else:
+            felist, inspector, idat, rest = args[1], args[2], args[3], args[4:]
+        quantity = str(rest[0]).lstrip(":") if rest else "Cauchy"
+        k0 = self._QUANTITY.get(quantity, 3)
+        nk = 2 if k0 == 6 else 3
+        fl = np.asarray(felist, dtype=np.int64).ravel()
+        res, loc = self.resultants(geom0, u, fl, outputcsys, transform_once)
+        xyz = np.asarray(geom0, dtype=float)
+        for l, e in enumerate(fl):
+            conn = self.conn[e]
+            xe = xyz[conn]
+            for j in range(res.shape[1]):
+                out = np.zeros(3)
+                out[:nk] = res[l, j, k0:k0 + nk]
+                idat = inspector(idat, int(e), conn, xe, out, loc[l, j])
+        return idat
+
     def num_normals(self):
         v = self._normal_valid
         return len(v), int(np.count_nonzero(~v))
@@ -287,6 +371,11 @@ def _element_normal(J0):
 
 def _q4_N(xi, eta):
     return 0.25 * np.array([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)])
+
+
+def _q4_gradN(xi, eta):
+    return 0.25 * np.array([[-(1 - eta), -(1 - xi)], [(1 - eta), -(1 + xi)], [(1 + eta), (1 + xi)],
+                            [-(1 + eta), (1 - xi)]])
 
 
 def _q4_jac(X, xi, eta):

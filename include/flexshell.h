@@ -261,6 +261,48 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
  */
 int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *blocks_out);
 
+/* Output coordinate systems of fs_model_resultants. */
+#define FS_OCSYS_ISOPARAMETRIC 0 /* E_G of each point: the default mcsys of the
+                                    homogeneous FEMMs (isoparametric!, and
+                                    _isoparametric! FEMMShellQ4RSModule.jl:201-203) */
+#define FS_OCSYS_MATRIX 1        /* one 3x3 matrix (column-major) for every point:
+                                    a constant CSys, e.g. the default output csys
+                                    of the layered FEMMs, layup_groups[1][1].csys */
+#define FS_OCSYS_PER_POINT 2     /* nlist x npts column-major matrices that the caller
+                                    evaluated with updatecsmat!(outputcsys, ...) */
+/* fs_model_resultants flags */
+#define FS_RES_LAYERED_Q4_SINGLE_T 1 /* Q4RSComp: transform the displacements into the
+                                        element frame once (see below) */
+
+/*
+ * Stress resultants at the integration points: inspectintegpoints for the
+ * three quantities :membrane_force, :bending_moment and :transverse_shear at
+ * once (FEMMShellT3FFModule.jl:850-962, FEMMShellQ4RSModule.jl:1061-1170,
+ * FEMMShellT3FFCompModule.jl:809-943, FEMMShellQ4RSCompModule.jl:1077-1209).
+ * The inspector callback stays with the caller, which hands it, for listed
+ * element l and point j, out[l][j][0..2] (membrane), [3..5] (bending) or
+ * [6..7] (shear).
+ *   p         the material of stiffness(); formulation and drilling scale unused
+ *   u         N x 6 nodal displacements and rotations (u.values, global basis,
+ *             row-major), host memory
+ *   felist    nlist 0-based element numbers (host); NULL = elements 0..nlist-1
+ *   ocs_kind  FS_OCSYS_*, with ocs the matrices it needs (host)
+ *   flags     0 reproduces the reference.  Its layered Q4 form multiplies the
+ *             displacements already taken into the element frame,
+ *             edisp_e = T u (FEMMShellQ4RSCompModule.jl:1163), by B T again
+ *             (:1168-1170, :1196); FS_RES_LAYERED_Q4_SINGLE_T applies T once,
+ *             as the other three FEMMs do
+ *   out       nlist x npts x 8 (npts 1 for T3 at the centroid, 4 for Q4 at the
+ *             Gauss points): N11 N22 N12 M11 M22 M12 Q1 Q2 in the output csys
+ *   mismatch  optional: 1 if some output csys normal is less than 0.95 aligned
+ *             with the element normal (the reference's "Coordinate systems
+ *             mismatched?" warning), else 0
+ * Requires the normals (associategeometry!).
+ */
+int fs_model_resultants(fs_model *m, const fs_shell_params *p, const double *u, int64_t nlist,
+                        const int32_t *felist, int32_t ocs_kind, const double *ocs, int32_t flags, double *out,
+                        int32_t *mismatch);
+
 /* Device time (ms) of the last fs_model_stiffness / fs_model_explicit call,
  * measured with HIP events on the context stream: which = 0 element kernels
  * (first launch to last element range done; with the assembly pipeline this
@@ -268,7 +310,7 @@ int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *block
  * (the whole scatter kernel when the pipeline has one range), 2 whole
  * stiffness (0 + 1), 3 explicit loop, 4 construction of the mixed-precision
  * operator (0 when the last loop ran in FP64 or reused an operator built
- * earlier). */
+ * earlier), 5 the stress-resultant kernel of the last fs_model_resultants. */
 int fs_model_last_timing(const fs_model *m, int which, double *ms);
 
 /* Element-kernel launch shape chosen by the autotuner for a family (FS_T3 /

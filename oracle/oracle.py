@@ -83,6 +83,9 @@ def lib():
         L.or_cd_loop.restype = None
         L.or_pwr.argtypes = [C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_int]
         L.or_pwr.restype = C.c_double
+        L.or_resultants_batch.argtypes = [C.c_int, _dp, _i64p, _dp, _u8p, C.c_int, _dp, _dp, _dp, _i32p,
+                                          C.c_double, C.c_double, _dp, C.c_int64, _i64p, C.c_int, _dp, C.c_int, _dp]
+        L.or_resultants_batch.restype = C.c_int
         L.or_ply_matrices.argtypes = [_dp, _dp, _dp]
         L.or_ply_matrices.restype = None
         L.or_laminate.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp,
@@ -232,6 +235,50 @@ def element_matrices(nen, xyz, conn, normals_, valid, *, Dps=None, Dt=None, t=No
     if st != 0:
         raise ArithmeticError("Singular metric matrix in _gradN_e!")
     return Ke
+
+
+def resultants(nen, xyz, conn, normals_, valid, u, *, Dps=None, Dt=None, t=None, groups=None, elem_group=None,
+               stab_c=None, felist=None, ocs=None, twice=False):
+    """Stress resultants N11 N22 N12 M11 M22 M12 Q1 Q2 at the integration
+    points of the listed elements (nlist x npts x 8; npts 1 for T3, 4 for Q4):
+    inspectintegpoints for all three quantities.  u: N x 6 nodal values.
+    ocs: None (isoparametric csys), one 3 x 3 matrix, or nlist x npts x 3 x 3.
+    twice: the layered Q4 form that applies T to the displacements twice."""
+    xyz = _c(xyz, np.float64)
+    conn = _c(conn, np.int64)
+    nrm = _c(normals_, np.float64)
+    val = _c(valid, np.uint8)
+    u = _c(u, np.float64)
+    E = conn.shape[0]
+    fl = _c(np.arange(E) if felist is None else felist, np.int64)
+    npts = 1 if nen == 3 else 4
+    out = np.zeros((fl.shape[0], npts, 8))
+    if stab_c is None:
+        stab_c = 5 / 12 / 1.5 if nen == 3 else 0.1
+    if groups is None:
+        mat = _c(np.concatenate([np.asarray(Dps, float).ravel(), np.asarray(Dt, float).ravel()]), np.float64)
+        cs, tg, eg, comp = None, None, None, 0
+    else:
+        mat = _c(np.concatenate([np.concatenate([np.asarray(A).ravel(), np.asarray(B).ravel(), np.asarray(D).ravel(),
+                                                 np.asarray(H).ravel()]) for (A, B, D, H, _, _) in groups]),
+                 np.float64)
+        cs = _c(np.concatenate([np.asarray(c_, float).ravel() for (_, _, _, _, c_, _) in groups]), np.float64)
+        tg = _c([g[5] for g in groups], np.float64)
+        eg = None if elem_group is None else _c(elem_group, np.int32)
+        comp = 1
+    if ocs is None:
+        kind, oc = 0, None
+    else:
+        oc = _c(ocs, np.float64)
+        kind = 1 if oc.size == 9 else 2
+        if kind == 2 and oc.size != fl.shape[0] * npts * 9:
+            raise ValueError("ocs must be one 3 x 3 matrix or nlist x npts x 3 x 3")
+    st = lib().or_resultants_batch(nen, _p(xyz), _p(conn, C.c_int64), _p(nrm), _p(val, C.c_uint8), comp, _p(mat),
+                                   _p(cs), _p(tg), _p(eg, C.c_int32), float(t or 0.0), float(stab_c), _p(u),
+                                   fl.shape[0], _p(fl, C.c_int64), kind, _p(oc), int(bool(twice)), _p(out))
+    if st != 0:
+        raise ArithmeticError("Singular metric matrix in _gradN_e!")
+    return out
 
 
 def element_masses(nen, xyz, conn, *, rho=None, t=None, m0=None, m1=None):

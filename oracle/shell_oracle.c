@@ -1123,3 +1123,282 @@ int or_stiffness_batch(int nen, int64_t E, const double *xyz, const int64_t *con
     }
     return status;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Stress resultants (inspectintegpoints)                                     */
+/* ------------------------------------------------------------------------ */
+
+static void matvec(int n, const double *A, const double *x, double *y) {
+    for (int i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (int k = 0; k < n; ++k) acc += A[i * n + k] * x[k];
+        y[i] = acc;
+    }
+}
+
+static void rectvec(int m, int n, const double *A, const double *x, double *y) {
+    for (int i = 0; i < m; ++i) {
+        double acc = 0.0;
+        for (int k = 0; k < n; ++k) acc += A[i * n + k] * x[k];
+        y[i] = acc;
+    }
+}
+
+/* o2_e' * [a1 a3; a3 a2] * o2_e -> (X11, X22, X12), evaluated as (o2' m) o2. */
+static void rotate_tensor(const double o2[2][2], const double *a, double *out) {
+    double m[2][2] = {{a[0], a[2]}, {a[2], a[1]}}, t[2][2], X[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) t[i][j] = o2[0][i] * m[0][j] + o2[1][i] * m[1][j];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) X[i][j] = t[i][0] * o2[0][j] + t[i][1] * o2[1][j];
+    out[0] = X[0][0];
+    out[1] = X[1][1];
+    out[2] = X[0][1];
+}
+
+/* Material law and output frame of one integration point.  Homogeneous
+ * (FEMMShellT3FFModule.jl:929-957, FEMMShellQ4RSModule.jl:1139-1163):
+ * N = (t Dps) e, M = (t^3/12 Dps) k, Q = (t stab (5/6 Dt)) g.  Layered
+ * (FEMMShellT3FFCompModule.jl:891-937, FEMMShellQ4RSCompModule.jl:1157-1201):
+ * A, B, D, H rotated into the element frame by the layup angle, N = A e + B k,
+ * M = B e + D k, Q = (stab H) g.  Then the tensors are rotated into the
+ * output csys with o2_e = [m n; -n m] from Layup2ElementAngle(E_G, ocs); a
+ * NULL ocs is the isoparametric csys, which is E_G itself. */
+static void point_resultants(int composite, const double *mat, const double *lcs, const double EG[3][3], double t,
+                             double stab, const double *ocs, const double *eps, const double *kap,
+                             const double *gam, double *out) {
+    double f[3], mo[3], q[2];
+    if (!composite) {
+        const double *Dps = mat;
+        double Dt[4];
+        for (int i = 0; i < 4; ++i) Dt[i] = mat[9 + i] * (5.0 / 6.0);
+        double cb = (t * t * t) / 12;
+        double cs = t * stab;
+        for (int i = 0; i < 3; ++i) {
+            double a = 0.0, b = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                a += (t * Dps[i * 3 + j]) * eps[j];
+                b += (cb * Dps[i * 3 + j]) * kap[j];
+            }
+            f[i] = a;
+            mo[i] = b;
+        }
+        for (int i = 0; i < 2; ++i) q[i] = (cs * Dt[i * 2 + 0]) * gam[0] + (cs * Dt[i * 2 + 1]) * gam[1];
+    } else {
+        double m, n, sA[9], sB[9], sD[9], sH[4];
+        layup2element_angle(EG, (const double(*)[3])lcs, &m, &n);
+        rotate_laminate(m, n, mat, mat + 9, mat + 18, mat + 27, sA, sB, sD, sH);
+        for (int i = 0; i < 3; ++i) {
+            double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                a += sA[i * 3 + j] * eps[j];
+                b += sB[i * 3 + j] * kap[j];
+                c += sB[i * 3 + j] * eps[j];
+                d += sD[i * 3 + j] * kap[j];
+            }
+            f[i] = a + b;
+            mo[i] = c + d;
+        }
+        for (int i = 0; i < 2; ++i) q[i] = (stab * sH[i * 2 + 0]) * gam[0] + (stab * sH[i * 2 + 1]) * gam[1];
+    }
+    double om, on;
+    if (ocs)
+        layup2element_angle(EG, (const double(*)[3])ocs, &om, &on);
+    else
+        layup2element_angle(EG, EG, &om, &on);
+    double o2[2][2] = {{om, on}, {-on, om}};
+    rotate_tensor(o2, mo, out + 3);
+    rotate_tensor(o2, f, out + 0);
+    out[6] = o2[0][0] * q[0] + o2[1][0] * q[1];
+    out[7] = o2[0][1] * q[0] + o2[1][1] * q[1];
+}
+
+/*
+ * T3FF / T3FFComp inspectintegpoints (FEMMShellT3FFModule.jl:850-962,
+ * FEMMShellT3FFCompModule.jl:809-943) at the one point (the centroid),
+ * all three quantities: out[8] = N11 N22 N12 M11 M22 M12 Q1 Q2.
+ *   ue: 18 element displacements and rotations, global basis.
+ *   edisp_n = Tga ue, edisp_e = Tae edisp_n; e = Bm edisp_e, k = Bb edisp_e,
+ *   g = Bs edisp_e (_Bsmat!, the average of the three orderings), h = sqrt(2 Ae).
+ */
+void or_t3ff_resultants(const double *X, const double *nrm, const uint8_t *valid, int composite,
+                        const double *mat, const double *lcs, double t, double stab_c, const double *ue,
+                        const double *ocs, double *out) {
+    double J0[3][2], EG[3][3], ec[3][2], gradN[3][2], Ae, centroid[3];
+    t3_geometry(X, J0, EG, ec, gradN, &Ae, centroid);
+    double AEs[3][3][3];
+    int nvalid[3];
+    double T[18 * 18], en[18], ee[18];
+    nodal_triads_e(3, AEs, nvalid, EG, nrm, valid);
+    transfmat_g_to_a(3, T, AEs, EG);
+    matvec(18, T, ue, en);
+    transfmat_a_to_e(3, T, AEs, &gradN[0][0]);
+    matvec(18, T, en, ee);
+    double Bm[3 * 18], Bb[3 * 18], Bs[2 * 18];
+    memset(Bm, 0, sizeof(Bm));
+    memset(Bb, 0, sizeof(Bb));
+    memset(Bs, 0, sizeof(Bs));
+    for (int i = 0; i < 3; ++i) {
+        Bm[0 * 18 + 6 * i + 0] = gradN[i][0];
+        Bm[1 * 18 + 6 * i + 1] = gradN[i][1];
+        Bm[2 * 18 + 6 * i + 0] = gradN[i][1];
+        Bm[2 * 18 + 6 * i + 1] = gradN[i][0];
+        Bb[0 * 18 + 6 * i + 4] = gradN[i][0];
+        Bb[1 * 18 + 6 * i + 3] = -gradN[i][1];
+        Bb[2 * 18 + 6 * i + 3] = -gradN[i][0];
+        Bb[2 * 18 + 6 * i + 4] = gradN[i][1];
+    }
+    add_bsmat_o(Bs, ec, Ae, 0, 1, 2);
+    add_bsmat_o(Bs, ec, Ae, 1, 2, 0);
+    add_bsmat_o(Bs, ec, Ae, 2, 0, 1);
+    for (int i = 0; i < 36; ++i) Bs[i] *= (1.0 / 3);
+    double eps[3], kap[3], gam[2];
+    rectvec(3, 18, Bm, ee, eps);
+    rectvec(3, 18, Bb, ee, kap);
+    rectvec(2, 18, Bs, ee, gam);
+    double h = sqrt(2 * Ae);
+    double stab = (t * t) / (t * t + stab_c * (h * h));
+    point_resultants(composite, mat, lcs, EG, t, stab, ocs, eps, kap, gam, out);
+}
+
+/*
+ * Q4RS / Q4RSComp inspectintegpoints (FEMMShellQ4RSModule.jl:1061-1170,
+ * FEMMShellQ4RSCompModule.jl:1077-1209) at the four Gauss points:
+ * out[4][8] as above, ocs: NULL or 4 x 9 (one csys matrix per point).
+ *   T = Tae Tga; e = (Bm_loc T) ue, k = (Bb_loc T) ue, g = (Bs_loc T) ue.
+ *   twice != 0 restates the layered form, which multiplies the already
+ *   transformed edisp_e = T ue by B_loc T again (:1163, :1168-1170, :1196).
+ */
+int or_q4rs_resultants(const double *X, const double *nrm, const uint8_t *valid, int composite,
+                       const double *mat, const double *lcs, double t, double stab_c, const double *ue,
+                       const double *ocs, int twice, double *out) {
+    double maxd = 0.0;
+    for (int i = 1; i < 4; ++i) {
+        double dx = X[i * 3 + 0] - X[0], dy = X[i * 3 + 1] - X[1], dz = X[i * 3 + 2] - X[2];
+        double dist = (dx * dx + dy * dy + dz * dz);
+        maxd = (dist > maxd) ? dist : maxd;
+    }
+    double h = sqrt(maxd);
+    double stab = (t * t) / (t * t + stab_c * (h * h));
+    for (int j = 0; j < 4; ++j) {
+        double xi, eta, N[4], gN[4][2], J[3][2], EG[3][3];
+        q4_gauss_point(j, &xi, &eta);
+        q4_bfun(xi, eta, N, gN);
+        q4_jac(X, gN, J);
+        e_g(EG, J);
+        double cen[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 4; ++a)
+            for (int k = 0; k < 3; ++k) cen[k] += X[a * 3 + k];
+        for (int k = 0; k < 3; ++k) cen[k] /= 4;
+        double ec[4][2];
+        for (int a = 0; a < 4; ++a)
+            for (int k = 0; k < 2; ++k) {
+                double d = 0.0;
+                for (int m = 0; m < 3; ++m) d += (X[a * 3 + m] - cen[m]) * EG[m][k];
+                ec[a][k] = d;
+            }
+        double G[2][2];
+        for (int p = 0; p < 2; ++p)
+            for (int q = 0; q < 2; ++q) G[p][q] = J[0][p] * J[0][q] + J[1][p] * J[1][q] + J[2][p] * J[2][q];
+        double detG = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+        if (detG == 0.0) return -1;
+        double iG[2][2] = {{G[1][1] / detG, -G[0][1] / detG}, {-G[1][0] / detG, G[0][0] / detG}};
+        double gradN[4][2];
+        for (int a = 0; a < 4; ++a) {
+            double t2[2] = {iG[0][0] * gN[a][0] + iG[0][1] * gN[a][1], iG[1][0] * gN[a][0] + iG[1][1] * gN[a][1]};
+            double g[3];
+            for (int i = 0; i < 3; ++i) g[i] = J[i][0] * t2[0] + J[i][1] * t2[1];
+            gradN[a][0] = EG[0][0] * g[0] + EG[1][0] * g[1] + EG[2][0] * g[2];
+            gradN[a][1] = EG[0][1] * g[0] + EG[1][1] * g[1] + EG[2][1] * g[2];
+        }
+        double AEs[4][3][3];
+        int nvalid[4];
+        double Tga[24 * 24], Tae[24 * 24], T[24 * 24];
+        nodal_triads_e(4, AEs, nvalid, EG, nrm, valid);
+        transfmat_g_to_a(4, Tga, AEs, EG);
+        transfmat_a_to_e(4, Tae, AEs, &gradN[0][0]);
+        for (int a = 0; a < 24; ++a)
+            for (int b = 0; b < 24; ++b) {
+                double acc = 0.0;
+                for (int k = 0; k < 24; ++k) acc += Tae[a * 24 + k] * Tga[k * 24 + b];
+                T[a * 24 + b] = acc;
+            }
+        double Bl[3 * 24], Bm[3 * 24], Bb[3 * 24], Bsl[2 * 24], Bs[2 * 24];
+        memset(Bl, 0, sizeof(Bl));
+        for (int i = 0; i < 4; ++i) {
+            int off = i * NDOF;
+            Bl[0 * 24 + off + 0] = gradN[i][0];
+            Bl[1 * 24 + off + 1] = gradN[i][1];
+            Bl[2 * 24 + off + 0] = gradN[i][1];
+            Bl[2 * 24 + off + 1] = gradN[i][0];
+        }
+        bt(3, Bl, T, Bm);
+        memset(Bl, 0, sizeof(Bl));
+        for (int i = 0; i < 4; ++i) {
+            int off = i * NDOF;
+            Bl[0 * 24 + off + 4] = gradN[i][0];
+            Bl[1 * 24 + off + 3] = -gradN[i][1];
+            Bl[2 * 24 + off + 3] = -gradN[i][0];
+            Bl[2 * 24 + off + 4] = gradN[i][1];
+        }
+        bt(3, Bl, T, Bb);
+        q4_bsmat_loc(Bsl, ec, xi, eta);
+        bt(2, Bsl, T, Bs);
+        double ud[24];
+        if (twice)
+            matvec(24, T, ue, ud);
+        else
+            memcpy(ud, ue, sizeof(ud));
+        double eps[3], kap[3], gam[2];
+        rectvec(3, 24, Bm, ud, eps);
+        rectvec(3, 24, Bb, ud, kap);
+        rectvec(2, 24, Bs, ud, gam);
+        point_resultants(composite, mat, lcs, EG, t, stab, ocs ? ocs + 9 * j : NULL, eps, kap, gam, out + 8 * j);
+    }
+    return 0;
+}
+
+/* Resultants of the listed elements.  u: N x 6 nodal values (global basis);
+ * felist: nlist 0-based element numbers; ocs_kind 0: isoparametric csys,
+ * 1: one 3 x 3 matrix (row-major) for every point, 2: nlist x npts x 9;
+ * out: nlist x npts x 8.  Material arguments as for or_stiffness_batch. */
+int or_resultants_batch(int nen, const double *xyz, const int64_t *conn, const double *normals,
+                        const uint8_t *valid, int composite, const double *mat, const double *csys,
+                        const double *tgroup, const int32_t *elem_group, double t, double stab_c, const double *u,
+                        int64_t nlist, const int64_t *felist, int ocs_kind, const double *ocs, int twice,
+                        double *out) {
+    int npts = nen == 3 ? 1 : 4;
+    int status = 0;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t l = 0; l < nlist; ++l) {
+        int64_t el = felist[l];
+        double X[12], nrm[12], ue[24];
+        uint8_t vl[4];
+        for (int a = 0; a < nen; ++a) {
+            int64_t n = conn[el * nen + a];
+            for (int c = 0; c < 3; ++c) {
+                X[a * 3 + c] = xyz[n * 3 + c];
+                nrm[a * 3 + c] = normals[n * 3 + c];
+            }
+            for (int c = 0; c < NDOF; ++c) ue[a * NDOF + c] = u[n * NDOF + c];
+            vl[a] = valid[n];
+        }
+        int g = elem_group ? elem_group[el] : 0;
+        const double *m = mat + (composite ? 31 : 13) * g;
+        const double *cs = composite ? csys + 9 * g : NULL;
+        double tt = tgroup ? tgroup[g] : t;
+        const double *oc = ocs_kind == 0 ? NULL : ocs_kind == 1 ? ocs : ocs + l * npts * 9;
+        double *o = out + l * npts * 8;
+        if (nen == 3) {
+            or_t3ff_resultants(X, nrm, vl, composite, m, cs, tt, stab_c, ue, oc, o);
+        } else {
+            double oc4[36];
+            if (ocs_kind == 1)
+                for (int j = 0; j < 4; ++j) memcpy(oc4 + 9 * j, ocs, 9 * sizeof(double));
+            if (or_q4rs_resultants(X, nrm, vl, composite, m, cs, tt, stab_c, ue,
+                                   ocs_kind == 0 ? NULL : ocs_kind == 1 ? oc4 : oc, twice, o))
+                status = -1;
+        }
+    }
+    return status;
+}
