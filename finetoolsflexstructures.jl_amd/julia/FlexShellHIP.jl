@@ -340,6 +340,112 @@ function cd_loop!(femm, assembler, cdamp, U0, V0, tend, dt, Fmag, g; probes = In
     return U, V, permutedims(pout)
 end
 
-export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context
+# ---------------------------------------------------------------- stress resultants
+const FS_OCSYS_ISOPARAMETRIC = Int32(0)
+const FS_OCSYS_MATRIX = Int32(1)
+const FS_OCSYS_PER_POINT = Int32(2)
+const FS_RES_LAYERED_Q4_SINGLE_T = Int32(1)
+
+struct _InspectModels                  # device models of inspectintegpoints, per FEMM
+    ctx::Context
+    models::IdDict{Any,DeviceModel}
+end
+const _inspect = Ref{Union{Nothing,_InspectModels}}(nothing)
+
+_quantity_offset(q) = q in (:membrane_force, :membrane) ? 0 :
+                      q in (:transverse_shear, :transverse, :shear) ? 6 : 3
+
+"""
+    inspectintegpoints(femm, geom0, u, dT, felist, inspector, idat, quantity = :Cauchy;
+                       outputcsys = nothing, transform_once = false)
+
+`inspectintegpoints` of the four shell FEMMs (FEMMShellT3FFModule.jl:850-962,
+FEMMShellQ4RSModule.jl:1061-1170, FEMMShellT3FFCompModule.jl:809-943,
+FEMMShellQ4RSCompModule.jl:1077-1209) with all resultants of the listed
+elements computed on the GPU in one launch (fs_model_resultants); the
+inspector is then called per element and point as in the reference.
+`outputcsys = nothing` is the FEMM's default (the isoparametric mcsys of the
+homogeneous FEMMs, the first layup csys of the layered ones); a constant
+`CSys` crosses the ABI as one matrix; any other `CSys` is evaluated per point
+with `updatecsmat!` at the point's location and tangents, as the reference
+does.  `transform_once = true` sets FS_RES_LAYERED_Q4_SINGLE_T.
+"""
+function inspectintegpoints(self, geom0::NodalField{Float64}, u::NodalField{T}, dT, felist::AbstractVector,
+                            inspector::F, idat, quantity = :Cauchy; outputcsys = nothing,
+                            transform_once::Bool = false) where {T<:Number,F<:Function}
+    @assert self._associatedgeometry == true
+    _inspect[] === nothing && (_inspect[] = _InspectModels(default_context(), IdDict{Any,DeviceModel}()))
+    m = _device_model!(_inspect[], self, geom0, u)
+    ctx = m.ctx
+    nen = _nen(self)
+    npts = nen == 3 ? 1 : 4
+    fes = self.integdomain.fes
+    fl = Int32.(collect(felist) .- 1)
+    vals = permutedims(u.values)                       # N x 6 row-major
+    layered = self isa Union{FEMMShellT3FFComp,FEMMShellQ4RSComp}
+    ocs = outputcsys === nothing ? (layered ? self.layup_groups[1][1].csys : nothing) : outputcsys
+    kind, ocm = FS_OCSYS_ISOPARAMETRIC, Float64[]
+    if ocs !== nothing && ocs.isconstant
+        kind, ocm = FS_OCSYS_MATRIX, vec(copy(csmat(ocs)))
+    elseif ocs !== nothing
+        kind = FS_OCSYS_PER_POINT
+        ocm = zeros(9, npts, length(fl))
+        npr, Ns, gradNparams, _, _ = integrationdata(self.integdomain, self.integdomain.integration_rule)
+        loc, J = zeros(1, 3), zeros(3, 2)
+        for (l, e) in enumerate(felist), j in 1:npts
+            X = geom0.values[collect(fes.conn[e]), :]
+            if nen == 3                                # centroid and J0 (FEMMShellT3FFModule.jl:908-911)
+                loc .= sum(X, dims = 1) ./ 3
+                J .= hcat(X[2, :] - X[1, :], X[3, :] - X[1, :])
+            else
+                locjac!(loc, J, X, Ns[j], gradNparams[j])
+            end
+            updatecsmat!(ocs, loc, J, e, nen == 3 ? 0 : j)
+            ocm[:, j, l] = vec(csmat(ocs))
+        end
+    end
+    out = zeros(8, npts, length(fl))
+    mis = Ref{Int32}(0)
+    flags = transform_once ? FS_RES_LAYERED_Q4_SINGLE_T : Int32(0)
+    if layered
+        abdh, csys, tg, _, grp = _layup_arrays(self)
+        GC.@preserve abdh csys tg grp vals fl ocm out begin
+            p = Ref(ShellParams(1, self.transv_shear_formulation, 0.0, _stab_coefficient(self.stab_fun),
+                                self.drilling_stiffness_scale, ntuple(_ -> 0.0, 9), ntuple(_ -> 0.0, 4),
+                                length(tg), 0, pointer(abdh), pointer(csys), pointer(tg), pointer(grp), C_NULL))
+            check(ctx, ccall((:fs_model_resultants, libflexshell), Cint,
+                             (Ptr{Cvoid}, Ptr{ShellParams}, Ptr{Float64}, Int64, Ptr{Int32}, Int32, Ptr{Float64},
+                              Int32, Ptr{Float64}, Ptr{Int32}),
+                             m.h, p, vals, length(fl), fl, kind, isempty(ocm) ? C_NULL : pointer(ocm), flags, out,
+                             mis), "fs_model_resultants")
+        end
+    else
+        p = Ref(_homog_params(self))
+        GC.@preserve vals fl ocm out begin
+            check(ctx, ccall((:fs_model_resultants, libflexshell), Cint,
+                             (Ptr{Cvoid}, Ptr{ShellParams}, Ptr{Float64}, Int64, Ptr{Int32}, Int32, Ptr{Float64},
+                              Int32, Ptr{Float64}, Ptr{Int32}),
+                             m.h, p, vals, length(fl), fl, kind, isempty(ocm) ? C_NULL : pointer(ocm), flags, out,
+                             mis), "fs_model_resultants")
+        end
+    end
+    mis[] != 0 && @warn "Coordinate systems mismatched?"
+    k0 = _quantity_offset(quantity)
+    nk = k0 == 6 ? 2 : 3
+    o = zeros(3)
+    npr, Ns, _, _, _ = integrationdata(self.integdomain, self.integdomain.integration_rule)
+    for (l, e) in enumerate(felist)
+        X = geom0.values[collect(fes.conn[e]), :]
+        for j in 1:npts
+            fill!(o, 0.0)
+            o[1:nk] .= out[(k0 + 1):(k0 + nk), j, l]
+            loc = nen == 3 ? sum(X, dims = 1) ./ 3 : Ns[j]' * X
+            idat = inspector(idat, e, fes.conn[e], X, o, loc)
+        end
+    end
+    return idat
+end
+
+export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context, inspectintegpoints
 
 end # module

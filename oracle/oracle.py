@@ -39,6 +39,12 @@ What pins each function (tests/test_oracle_pins.py unless noted):
   or_cd_loop                  undamped single-dof closed form (harmonic
                               oscillator under the CD recurrence); energy.
   or_pwr                      largest eigenvalue by scipy on small models.
+  or_resultants_batch         membrane, bending and shear patch closed forms;
+  (inspectintegpoints)        rigid motions on a tilted flat mesh; energy
+                              identity against or_*_elmat (homogeneous and
+                              layered); isotropic ply = homogeneous; the
+                              layered Q4 double transform on a rotated grid
+                              (tests/test_resultants_pins.py).
 """
 import ctypes as C
 import os
