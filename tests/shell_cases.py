@@ -1,6 +1,8 @@
 """Reference test scenarios restated as plain arrays (meshes, BCs, loads),
 shared by the oracle pin tests and the GPU parity tests.  Values quoted from
 the reference's test files are cited next to them."""
+import os
+
 import numpy as np
 
 from flexshell import fe
@@ -127,3 +129,28 @@ def distorted_mesh(nen, nx, ny, seed, warp=0.0, jitter=0.2):
     z = warp * (xy[:, 0] ** 2 - 0.5 * xy[:, 1] ** 2 + 0.3 * xy[:, 0] * xy[:, 1])
     xyz = np.stack([xy[:, 0], xy[:, 1], z], axis=1)
     return xyz, conn
+
+# test/test_shell_statics.jl:226-234: Raasch hook, T3FF on the Abaqus S4
+# meshes split by Q4toT3, percentage of the analytical 5.02 (rtol 1e-4)
+RAASCH_T3 = {"1x9": 91.7059961843231, "3x18": 95.9355786538892, "5x36": 97.19276899988246,
+             "10x72": 98.38896641657374}
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def raasch(mesh):
+    """Raasch hook (test/test_shell_statics.jl:148-224): E 3300, nu 0.35,
+    t 2, clamped at x = 0, line load 0.05 along z on the tip edge."""
+    from flexshell import meshio
+    m = meshio.import_abaqus(os.path.join(GOLDEN, "inp", f"raasch_s4_{mesh}.inp"))
+    xyz, conn, _ = meshio.compact_nodes(m["xyz"], m["fesets"][0][1])
+    conn = meshio.q4_to_t3(conn)
+    t = 2.0
+    tol = t / 2
+    dch = fe.DofField(xyz.shape[0])
+    dch.setebc(fe.selectnode(xyz, [0, 0, -np.inf, np.inf, -np.inf, np.inf], tol), [1, 2, 3, 4, 5, 6])
+    dch.numberdofs()
+    tip = meshio.select_edges(xyz, fe.boundary_edges(conn), [97.9615, 97.9615, -16, -16, 0, 20], tol)
+    F = fe.edge_load(xyz, tip, [0, 0, 0.05, 0, 0, 0], dch.nalldofs, dch.dofnums)
+    probe = fe.selectnode(xyz, [97.9615, 97.9615, -16, -16, 0, 0], tol)[0]
+    return dict(nen=3, xyz=xyz, conn=conn, dchi=dch, E=3300.0, nu=0.35, rho=0.0, t=t, stab_c=0.2, F=F,
+                probe=probe, ntip=len(tip))

@@ -132,6 +132,27 @@ def test_layered_barbero(ctx, nen):
     assert abs(wmax - ref) / ref < 1e-10
 
 
+@pytest.mark.parametrize("mesh", ["1x9", "10x72"])
+def test_raasch_hook_gpu(ctx, mesh):
+    """GPU stiffness on the reference's Abaqus meshes (test_shell_statics.jl:
+    148-234): CSR equal to the oracle's and the tip deflection percentage of
+    the reference."""
+    print(f"[gpu] raasch hook {mesh}", flush=True)
+    c = SC.raasch(mesh)
+    d = c["dchi"]
+    f = _homog_femm(ctx, 3, c["conn"], c["E"], c["nu"], 0.0, c["t"], stab_c=c["stab_c"])
+    f.associategeometry(c["xyz"])
+    Kg = f.stiffness(FM.SysmatAssemblerSparseCSRSymm(), c["xyz"], None, None, d)
+    Ko = O.assemble_csr(3, c["conn"], d.dofnums, d.nalldofs,
+                        _oracle_homog(3, c["xyz"], c["conn"], c["E"], c["nu"], c["t"], stab_c=c["stab_c"]))
+    assert np.array_equal(Kg.indices, Ko.indices) and np.array_equal(Kg.indptr, Ko.indptr)
+    assert _rel(Kg, Ko) < KTOL
+    nf = d.nfreedofs
+    U = sla.spsolve(Kg[:nf, :nf].tocsc(), c["F"][:nf])
+    pct = d.scattersysvec(U)[c["probe"], 2] / 5.02 * 100
+    assert abs(pct - SC.RAASCH_T3[mesh]) / SC.RAASCH_T3[mesh] < 1e-9
+
+
 @pytest.mark.parametrize("nen", [3, 4])
 def test_layered_normals_device(ctx, nen):
     """Layered normals on the device (fs_model_normals_layered): one layup

@@ -247,3 +247,17 @@ def test_pwr_largest_eigenvalue():
     v0 = np.random.default_rng(0).random(n)
     om = O.pwr(sp.diags(kd).tocsr(), md, v0, maxit=60)
     assert abs(om - np.sqrt(10.0 / 2.0)) < 1e-12
+
+
+@pytest.mark.parametrize("mesh", ["1x9", "3x18", "5x36", "10x72"])
+def test_raasch_hook_from_abaqus_meshes(mesh):
+    """T3FF on the reference's Abaqus meshes (raasch_s4_*.inp read by
+    flexshell.meshio, compacted, split by Q4toT3): the tip deflection in
+    percent of 5.02 of test/test_shell_statics.jl:226-234 (rtol 1e-4 there)."""
+    c = SC.raasch(mesh)
+    d = c["dchi"]
+    K = _homog_K(c)
+    nf = d.nfreedofs
+    U = sla.spsolve(K[:nf, :nf].tocsc(), c["F"][:nf])
+    pct = d.scattersysvec(U)[c["probe"], 2] / 5.02 * 100
+    assert abs(pct - SC.RAASCH_T3[mesh]) / SC.RAASCH_T3[mesh] < 1e-9
