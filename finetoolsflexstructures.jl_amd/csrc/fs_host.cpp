@@ -33,6 +33,15 @@ hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const doub
                           const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
                           const double *gnrm, const int32_t *egroup, double *nx, double *ny, double *nz,
                           uint8_t *valid, hipStream_t st);
+hipError_t launch_number_dofs(int64_t n, const uint8_t *fixed, int64_t *tile_free, int64_t *total, int64_t *dof,
+                              hipStream_t st);
+int64_t number_dofs_tiles(int64_t n);
+hipError_t launch_surface_load(int nen, int npts, int64_t N, int E, const double *x, const double *y,
+                               const double *z, const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list,
+                               int per_point, const double *force, const int64_t *node_dof, int64_t nrows, double *F,
+                               hipStream_t st);
+hipError_t launch_load_points(int nen, int npts, int E, const double *x, const double *y, const double *z,
+                              const int32_t *conn, double *out, hipStream_t st);
 hipError_t launch_resultants(int nen, int composite, const ResParams &rp, const double *x, const double *y,
                              const double *z, const int32_t *conn, const double *nx, const double *ny,
                              const double *nz, const uint8_t *valid, hipStream_t st);
@@ -1030,6 +1039,106 @@ int fs_model_set_dofs(fs_model *m, const int64_t *dofnums, int64_t nalldofs, int
     FS_HIP(c, hipStreamSynchronize(c->stream));
     m->dofs_ready = true;
     return FS_OK;
+}
+
+int fs_model_number_dofs(fs_model *m, const uint8_t *is_fixed, const int64_t *perm, int64_t *dofnums_out,
+                         int64_t *nfree_out) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (!is_fixed) return fail(c, FS_ERR_ARG, "number_dofs: NULL flags");
+    const int64_t N = m->N, n = N * 6;
+    std::vector<uint8_t> pf;
+    if (perm) {
+        // flags in numbering order: node perm[k] is the k-th node numbered
+        std::vector<uint8_t> seen((size_t)N, 0);
+        pf.resize((size_t)n);
+        for (int64_t k = 0; k < N; ++k) {
+            const int64_t I = perm[k];
+            if (I < 0 || I >= N || seen[I]) return fail(c, FS_ERR_ARG, "number_dofs: perm is not a permutation");
+            seen[I] = 1;
+            for (int cc = 0; cc < 6; ++cc) pf[k * 6 + cc] = is_fixed[I * 6 + cc];
+        }
+    }
+    FS_HIP(c, hipSetDevice(c->device));
+    DevBuf dfix, dtile, dtot;
+    int s;
+    if ((s = upload(c, dfix, perm ? pf.data() : is_fixed, (size_t)n)) ||
+        (s = dalloc(c, dtile, fs::number_dofs_tiles(n) * 8 + 8)) || (s = dalloc(c, dtot, 8)) ||
+        (s = dalloc(c, m->node_dof, (size_t)n * 8)))
+        return s;
+    FS_HIP(c, hipMemsetAsync(dtot.p, 0, 8, c->stream));
+    FS_HIP(c, fs::launch_number_dofs(n, dfix.as<uint8_t>(), dtile.as<int64_t>(), dtot.as<int64_t>(),
+                                     m->node_dof.as<int64_t>(), c->stream));
+    std::vector<int64_t> dof((size_t)n);
+    int64_t nfree = 0;
+    FS_HIP(c, hipMemcpyAsync(&nfree, dtot.p, 8, hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipMemcpyAsync(dof.data(), m->node_dof.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    if (perm) {
+        // back to node order on the host, and the device copy replaced
+        std::vector<int64_t> byn((size_t)n);
+        for (int64_t k = 0; k < N; ++k)
+            for (int cc = 0; cc < 6; ++cc) byn[perm[k] * 6 + cc] = dof[k * 6 + cc];
+        dof.swap(byn);
+        if ((s = upload(c, m->node_dof, dof.data(), dof.size()))) return s;
+        FS_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    // the host copy feeds the pattern planner
+    m->dof.swap(dof);
+    m->nall = n;
+    m->nfree = nfree;
+    m->blk[0].reset();
+    m->blk[1].reset();
+    m->mixed.reset();
+    m->dofs_ready = true;
+    if (dofnums_out)
+        for (int64_t I = 0; I < m->N; ++I)
+            for (int cc = 0; cc < 6; ++cc) dofnums_out[cc * m->N + I] = m->dof[I * 6 + cc];
+    if (nfree_out) *nfree_out = nfree;
+    return FS_OK;
+}
+
+static bool load_rule_ok(int nen, int npts) { return nen == 3 ? (npts == 1 || npts == 3 || npts == 6) : npts == 4; }
+
+int fs_model_surface_load(fs_model *m, int32_t npts, int32_t kind, const double *force, int block, double *F_out) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (!force || !F_out) return fail(c, FS_ERR_ARG, "surface_load: NULL arguments");
+    if (kind != FS_LOAD_UNIFORM && kind != FS_LOAD_PER_POINT) return fail(c, FS_ERR_ARG, "surface_load: kind %d", kind);
+    if (block != FS_BLOCK_ALL && block != FS_BLOCK_FF) return fail(c, FS_ERR_ARG, "unknown block %d", block);
+    if (!m->dofs_ready) return fail(c, FS_ERR_STATE, "surface_load: equation numbers not set");
+    if (!load_rule_ok(m->nen, npts)) return fail(c, FS_ERR_ARG, "surface_load: %d-point rule", npts);
+    FS_HIP(c, hipSetDevice(c->device));
+    const int64_t nrows = block == FS_BLOCK_FF ? m->nfree : m->nall;
+    DevBuf dforce, dF;
+    int s;
+    if ((s = upload(c, dforce, force, kind == FS_LOAD_UNIFORM ? 6 : (size_t)m->E * npts * 6)) ||
+        (s = dalloc(c, dF, (size_t)nrows * 8)))
+        return s;
+    FS_HIP(c, fs::launch_surface_load(m->nen, npts, m->N, (int)m->E, m->x.as<double>(), m->y.as<double>(),
+                                      m->z.as<double>(), m->dconn.as<int32_t>(), m->dne_ptr.as<int32_t>(),
+                                      m->dne_list.as<int32_t>(), kind == FS_LOAD_PER_POINT ? 1 : 0,
+                                      dforce.as<double>(), m->node_dof.as<int64_t>(), nrows, dF.as<double>(),
+                                      c->stream));
+    return d2h(c, F_out, dF.p, (size_t)nrows * 8);
+}
+
+int fs_model_load_points(fs_model *m, int32_t npts, double *xyz_out) {
+    if (!m) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    if (!xyz_out) return fail(c, FS_ERR_ARG, "load_points: NULL output");
+    if (!load_rule_ok(m->nen, npts)) return fail(c, FS_ERR_ARG, "load_points: %d-point rule", npts);
+    FS_HIP(c, hipSetDevice(c->device));
+    const size_t n = (size_t)m->E * npts * 3;
+    DevBuf d;
+    int s = dalloc(c, d, n * 8);
+    if (s) return s;
+    FS_HIP(c, fs::launch_load_points(m->nen, npts, (int)m->E, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
+                                     m->dconn.as<int32_t>(), d.as<double>(), c->stream));
+    return d2h(c, xyz_out, d.p, n * 8);
 }
 
 // Normals kernel in either form (gnrm null: isoparametric) and the copies out.

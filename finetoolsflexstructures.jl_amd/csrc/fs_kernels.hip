@@ -1529,6 +1529,213 @@ __global__ __launch_bounds__(256) void mass_kernel(int64_t N, int E, MassParams 
 }
 
 // ---------------------------------------------------------------------------
+// Load / boundary-condition / dof front end
+// ---------------------------------------------------------------------------
+//
+// numberdofs! (free dofs first in node-major order, then the fixed ones) as a
+// three-pass scan over the N x 6 fixed flags: per-tile free counts, one
+// workgroup scanning the tile counts, then each tile renumbers its entries.
+// A free entry i gets the number of free entries before it; a fixed entry gets
+// nfree + the number of fixed entries before it.
+#define FS_DOF_TILE 4096  // entries per workgroup (256 threads x 16)
+
+__global__ __launch_bounds__(256) void dof_count_kernel(int64_t n, const uint8_t *__restrict__ fixed,
+                                                        int64_t *__restrict__ tile_free) {
+    __shared__ int64_t red[256];
+    const int64_t base = (int64_t)blockIdx.x * FS_DOF_TILE + (int64_t)threadIdx.x * 16;
+    int64_t cnt = 0;
+    for (int k = 0; k < 16; ++k) {
+        const int64_t i = base + k;
+        if (i < n && !fixed[i]) ++cnt;
+    }
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_free[blockIdx.x] = red[0];
+}
+
+// Exclusive scan of the tile counts in place by one workgroup; total[0] = nfree.
+__global__ __launch_bounds__(256) void dof_scan_kernel(int64_t ntiles, int64_t *__restrict__ tile_free,
+                                                       int64_t *__restrict__ total) {
+    __shared__ int64_t sh[256];
+    int64_t carry = 0;
+    for (int64_t b0 = 0; b0 < ntiles; b0 += 256) {
+        const int64_t i = b0 + threadIdx.x;
+        const int64_t v = i < ntiles ? tile_free[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int s = 1; s < 256; s <<= 1) {  // inclusive Hillis-Steele scan
+            const int64_t add = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < ntiles) tile_free[i] = carry + sh[threadIdx.x] - v;
+        const int64_t tile_sum = sh[255];
+        __syncthreads();
+        carry += tile_sum;
+    }
+    if (threadIdx.x == 0) total[0] = carry;
+}
+
+__global__ __launch_bounds__(256) void dof_number_kernel(int64_t n, const uint8_t *__restrict__ fixed,
+                                                         const int64_t *__restrict__ tile_free,
+                                                         const int64_t *__restrict__ total,
+                                                         int64_t *__restrict__ dof) {
+    __shared__ int64_t sh[256];
+    const int64_t tile0 = (int64_t)blockIdx.x * FS_DOF_TILE;
+    const int64_t base = tile0 + (int64_t)threadIdx.x * 16;
+    int64_t cnt = 0;
+    for (int k = 0; k < 16; ++k) {
+        const int64_t i = base + k;
+        if (i < n && !fixed[i]) ++cnt;
+    }
+    sh[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = 1; s < 256; s <<= 1) {
+        const int64_t add = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int64_t freeb = tile_free[blockIdx.x] + sh[threadIdx.x] - cnt;  // free entries before `base`
+    const int64_t nfree = total[0];
+    for (int k = 0; k < 16; ++k) {
+        const int64_t i = base + k;
+        if (i >= n) break;
+        if (!fixed[i]) {
+            dof[i] = freeb;
+            ++freeb;
+        } else {
+            dof[i] = nfree + (i - freeb);
+        }
+    }
+}
+
+// Integration rules of the surface loads, as FinEtools tabulates them:
+// TriRule(1), TriRule(3), TriRule(6) (param_coords, weights) for T3 and
+// GaussRule(2, 2) for Q4.
+__constant__ double c_tri_rule[10][3] = {
+    {1.0 / 3, 1.0 / 3, 1.0 / 2},  // TriRule(1)
+    {1.0 / 6, 1.0 / 6, 1.0 / 6},  // TriRule(3)
+    {2.0 / 3, 1.0 / 6, 1.0 / 6},
+    {1.0 / 6, 2.0 / 3, 1.0 / 6},
+    {0.816847572980459, 0.091576213509771, 0.109951743655322 / 2},  // TriRule(6)
+    {0.091576213509771, 0.816847572980459, 0.109951743655322 / 2},
+    {0.091576213509771, 0.091576213509771, 0.109951743655322 / 2},
+    {0.108103018168070, 0.445948490915965, 0.223381589678011 / 2},
+    {0.445948490915965, 0.108103018168070, 0.223381589678011 / 2},
+    {0.445948490915965, 0.445948490915965, 0.223381589678011 / 2}};
+
+template <int NEN>
+__device__ __forceinline__ void load_point(int npts, int q, double N[NEN], double gNp[NEN][2], double &w) {
+    if constexpr (NEN == 3) {
+        const int r = (npts == 1 ? 0 : npts == 3 ? 1 : 4) + q;
+        const double xi = c_tri_rule[r][0], eta = c_tri_rule[r][1];
+        N[0] = 1.0 - xi - eta;
+        N[1] = xi;
+        N[2] = eta;
+        gNp[0][0] = -1.0;
+        gNp[0][1] = -1.0;
+        gNp[1][0] = 1.0;
+        gNp[1][1] = 0.0;
+        gNp[2][0] = 0.0;
+        gNp[2][1] = 1.0;
+        w = c_tri_rule[r][2];
+    } else {
+        const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+        const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
+        q4_gradn(xi, eta, gNp);
+        N[0] = 0.25 * ((1 - xi) * (1 - eta));
+        N[1] = 0.25 * ((1 + xi) * (1 - eta));
+        N[2] = 0.25 * ((1 + xi) * (1 + eta));
+        N[3] = 0.25 * ((1 - xi) * (1 + eta));
+        w = 1.0;
+    }
+}
+
+// Consistent nodal loads of a surface force intensity (distribloads with
+// ForceIntensity, FinEtools; e.g. plate_expl_examples.jl:184-203): thread per
+// node, summing over its incident elements in ascending order each element's
+// entry Fe[a][c] = sum_q (N_a f_c) (Jac w), as the element-order assembly of
+// the element vectors does.  force: one 6-vector (per_point 0) or E x npts x 6.
+template <int NEN>
+__global__ __launch_bounds__(256) void surface_load_kernel(int64_t N, int E, int npts,
+                                                           const double *__restrict__ xs,
+                                                           const double *__restrict__ ys,
+                                                           const double *__restrict__ zs,
+                                                           const int32_t *__restrict__ conn,
+                                                           const int32_t *__restrict__ ne_ptr,
+                                                           const int32_t *__restrict__ ne_list, int per_point,
+                                                           const double *__restrict__ force,
+                                                           const int64_t *__restrict__ node_dof, int64_t nrows,
+                                                           double *__restrict__ F) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = ne_ptr[I]; k < ne_ptr[I + 1]; ++k) {
+        const int e = ne_list[k];
+        double X[NEN][3];
+        int la = 0;
+        for (int a = 0; a < NEN; ++a) {
+            const int n = conn[a * E + e];
+            if (n == I) la = a;
+            X[a][0] = xs[n];
+            X[a][1] = ys[n];
+            X[a][2] = zs[n];
+        }
+        double fe[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < npts; ++q) {
+            double Nq[NEN], gNp[NEN][2], w;
+            load_point<NEN>(npts, q, Nq, gNp, w);
+            double J[3][2];
+            for (int i = 0; i < 3; ++i)
+                for (int kk = 0; kk < 2; ++kk) {
+                    double s = 0.0;
+                    for (int a = 0; a < NEN; ++a) s += X[a][i] * gNp[a][kk];
+                    J[i][kk] = s;
+                }
+            const double c0 = -J[2][0] * J[1][1] + J[1][0] * J[2][1];
+            const double c1 = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+            const double c2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
+            const double Jw = sqrt(c0 * c0 + c1 * c1 + c2 * c2) * w;
+            const double *f = per_point ? force + ((int64_t)e * npts + q) * 6 : force;
+            for (int c = 0; c < 6; ++c) fe[c] += (Nq[la] * f[c]) * Jw;
+        }
+        for (int c = 0; c < 6; ++c) acc[c] += fe[c];
+    }
+    for (int c = 0; c < 6; ++c) {
+        const int64_t d = node_dof[I * 6 + c];
+        if (d < nrows) F[d] = acc[c];
+    }
+}
+
+// Locations of the surface-load integration points, E x npts x 3.
+template <int NEN>
+__global__ __launch_bounds__(256) void load_points_kernel(int E, int npts, const double *__restrict__ xs,
+                                                          const double *__restrict__ ys,
+                                                          const double *__restrict__ zs,
+                                                          const int32_t *__restrict__ conn,
+                                                          double *__restrict__ out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (int64_t)E * npts) return;
+    const int e = (int)(gid / npts), q = (int)(gid % npts);
+    double Nq[NEN], gNp[NEN][2], w;
+    load_point<NEN>(npts, q, Nq, gNp, w);
+    double p[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < NEN; ++a) {
+        const int n = conn[a * E + e];
+        p[0] += Nq[a] * xs[n];
+        p[1] += Nq[a] * ys[n];
+        p[2] += Nq[a] * zs[n];
+    }
+    for (int k = 0; k < 3; ++k) out[gid * 3 + k] = p[k];
+}
+
+// ---------------------------------------------------------------------------
 // Stress resultants at the integration points (inspectintegpoints)
 // ---------------------------------------------------------------------------
 //
@@ -2404,6 +2611,45 @@ hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const doub
     else
         hipLaunchKernelGGL(normals_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, x, y, z, conn, ne_ptr, ne_list,
                            ntol, gnrm, egroup, nx, ny, nz, valid);
+    return hipGetLastError();
+}
+
+hipError_t launch_number_dofs(int64_t n, const uint8_t *fixed, int64_t *tile_free, int64_t *total, int64_t *dof,
+                              hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int64_t ntiles = (n + FS_DOF_TILE - 1) / FS_DOF_TILE;
+    hipLaunchKernelGGL(dof_count_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n, fixed, tile_free);
+    hipLaunchKernelGGL(dof_scan_kernel, dim3(1), dim3(256), 0, st, ntiles, tile_free, total);
+    hipLaunchKernelGGL(dof_number_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n, fixed, tile_free, total, dof);
+    return hipGetLastError();
+}
+
+int64_t number_dofs_tiles(int64_t n) { return (n + FS_DOF_TILE - 1) / FS_DOF_TILE; }
+
+hipError_t launch_surface_load(int nen, int npts, int64_t N, int E, const double *x, const double *y,
+                               const double *z, const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list,
+                               int per_point, const double *force, const int64_t *node_dof, int64_t nrows, double *F,
+                               hipStream_t st) {
+    if (N == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((N + 255) / 256);
+    if (nen == 3)
+        hipLaunchKernelGGL(surface_load_kernel<3>, dim3(blocks), dim3(256), 0, st, N, E, npts, x, y, z, conn, ne_ptr,
+                           ne_list, per_point, force, node_dof, nrows, F);
+    else
+        hipLaunchKernelGGL(surface_load_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, npts, x, y, z, conn, ne_ptr,
+                           ne_list, per_point, force, node_dof, nrows, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_load_points(int nen, int npts, int E, const double *x, const double *y, const double *z,
+                              const int32_t *conn, double *out, hipStream_t st) {
+    const int64_t nt = (int64_t)E * npts;
+    if (nt == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((nt + 255) / 256);
+    if (nen == 3)
+        hipLaunchKernelGGL(load_points_kernel<3>, dim3(blocks), dim3(256), 0, st, E, npts, x, y, z, conn, out);
+    else
+        hipLaunchKernelGGL(load_points_kernel<4>, dim3(blocks), dim3(256), 0, st, E, npts, x, y, z, conn, out);
     return hipGetLastError();
 }
 

@@ -23,6 +23,8 @@ FS_OCSYS_ISOPARAMETRIC = 0
 FS_OCSYS_MATRIX = 1
 FS_OCSYS_PER_POINT = 2
 FS_RES_LAYERED_Q4_SINGLE_T = 1
+FS_LOAD_UNIFORM = 0
+FS_LOAD_PER_POINT = 1
 
 _STATUS = {
     -1: "invalid argument",
@@ -109,6 +111,9 @@ SIGNATURES = {
     "fs_model_explicit": (C.c_int, [C.c_void_p, C.POINTER(CDParams), _dp, _dp, _dp, _dp, _i64p, C.c_int32,
                                     _dp, _dp, _dp]),
     "fs_model_omega_max": (C.c_int, [C.c_void_p, _dp, C.c_int32, _dp]),
+    "fs_model_number_dofs": (C.c_int, [C.c_void_p, _u8p, _i64p, _i64p, _i64p]),
+    "fs_model_surface_load": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int, _dp]),
+    "fs_model_load_points": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "fs_model_resultants": (C.c_int, [C.c_void_p, C.POINTER(ShellParams), _dp, C.c_int64, _i32p, C.c_int32, _dp,
                                       C.c_int32, _dp, _i32p]),
     "fs_model_last_timing": (C.c_int, [C.c_void_p, C.c_int, _dp]),
@@ -283,6 +288,35 @@ class Model:
         out = np.zeros((self.E, nblk, 6, 6))
         self.ctx.check(self.ctx.lib.fs_model_element_blocks(self.h, C.byref(params), ptr(out)),
                        "fs_model_element_blocks")
+        return out
+
+    def number_dofs(self, is_fixed, perm=None):
+        """numberdofs! on the device: (dofnums N x 6, nfree); the numbering is
+        installed in the model as set_dofs would."""
+        fx = np.ascontiguousarray(np.asarray(is_fixed, dtype=np.uint8).reshape(self.N, 6))
+        pm = None if perm is None else np.ascontiguousarray(np.asarray(perm, dtype=np.int64))
+        cm = np.zeros(6 * self.N, dtype=np.int64)
+        nf = C.c_int64(0)
+        self.ctx.check(self.ctx.lib.fs_model_number_dofs(self.h, ptr(fx, C.c_uint8), ptr(pm, C.c_int64),
+                                                         ptr(cm, C.c_int64), C.byref(nf)), "fs_model_number_dofs")
+        self.nall, self.nfree = 6 * self.N, int(nf.value)
+        return cm.reshape(6, self.N).T.copy(), int(nf.value)
+
+    def load_points(self, npts):
+        out = np.zeros((self.E, npts, 3))
+        self.ctx.check(self.ctx.lib.fs_model_load_points(self.h, int(npts), ptr(out)), "fs_model_load_points")
+        return out
+
+    def surface_load(self, npts, force, block, nrows):
+        """Consistent nodal loads (distribloads) over the block's rows; force:
+        a 6-vector or E x npts x 6 values at the rule's points."""
+        f = np.ascontiguousarray(np.asarray(force, dtype=np.float64))
+        kind = FS_LOAD_UNIFORM if f.size == 6 else FS_LOAD_PER_POINT
+        if kind == FS_LOAD_PER_POINT and f.size != self.E * npts * 6:
+            raise ValueError("force must be a 6-vector or nelem x npts x 6")
+        out = np.zeros(nrows)
+        self.ctx.check(self.ctx.lib.fs_model_surface_load(self.h, int(npts), int(kind), ptr(f), int(block), ptr(out)),
+                       "fs_model_surface_load")
         return out
 
     def resultants(self, params, u, felist=None, ocs=None, flags=0):

@@ -153,50 +153,6 @@ class DofField:
         return out
 
 
-def _tri_bfun(xi, eta):
-    return np.array([1.0 - xi - eta, xi, eta]), np.array([[-1.0, -1.0], [1.0, 0.0], [0.0, 1.0]])
-
-
-def _quad_bfun(xi, eta):
-    N = 0.25 * np.array([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)])
-    gN = 0.25 * np.array([[-(1 - eta), -(1 - xi)], [(1 - eta), -(1 + xi)], [(1 + eta), (1 + xi)],
-                          [-(1 + eta), (1 - xi)]])
-    return N, gN
-
-
-def surface_load(xyz, conn, force, nalldofs, dofnums, rule=None):
-    """distribloads(FEMMBase(IntegDomain(fes, rule)), geom, dchi, ForceIntensity(f), 2)
-    for a uniform or position-dependent surface force on T3/Q4 elements.
-    force: 6-vector, or callable(x (3,)) -> 6-vector.  Returns the assembled
-    vector over all dofs (length nalldofs)."""
-    xyz = np.asarray(xyz)
-    conn = np.asarray(conn)
-    nen = conn.shape[1]
-    if nen == 3:
-        pts = np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]) if rule is None else rule[0]
-        wts = np.array([1 / 6, 1 / 6, 1 / 6]) if rule is None else rule[1]
-        bf = _tri_bfun
-    else:
-        g = GAUSS2
-        pts = np.array([[-g, -g], [-g, g], [g, -g], [g, g]]) if rule is None else rule[0]
-        wts = np.ones(4) if rule is None else rule[1]
-        bf = _quad_bfun
-    F = np.zeros(nalldofs)
-    fconst = None if callable(force) else np.asarray(force, dtype=float)
-    for e in range(conn.shape[0]):
-        X = xyz[conn[e]]
-        fe = np.zeros((nen, 6))
-        for (xi, eta), w in zip(pts, wts):
-            N, gN = bf(xi, eta)
-            J = X.T @ gN
-            Jac = np.linalg.norm(np.cross(J[:, 0], J[:, 1]))
-            f = fconst if fconst is not None else np.asarray(force(N @ X), dtype=float)
-            fe += np.outer(N, f) * (Jac * w)
-        for a in range(nen):
-            F[dofnums[conn[e, a]]] += fe[a]
-    return F
-
-
 def edge_load(xyz, edges, force, nalldofs, dofnums):
     """distribloads on L2 boundary elements with GaussRule(1, 2) (uniform force 6-vector)."""
     F = np.zeros(nalldofs)

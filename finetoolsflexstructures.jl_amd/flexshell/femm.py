@@ -266,6 +266,36 @@ class _ShellFEMM:
         d = m.mass(self._mass_params(), block, nrows)
         return sp.diags(d, format="csr")
 
+    # -- load / dof front end on the device --
+    def number_dofs(self, geom0, dchi, perm=None):
+        """numberdofs!(dchi[, perm]) on the device: fills dchi.dofnums and
+        dchi.nfreedofs from the fixed flags set by setebc, and installs the
+        numbering in the device model."""
+        m = self._ensure_model(geom0)
+        dn, nf = m.number_dofs(dchi.is_fixed, perm)
+        dchi.dofnums, dchi.nfreedofs = dn, nf
+        self._dofs_key = (dchi.nalldofs, dchi.nfreedofs, hash(dchi.dofnums.tobytes()))
+        self._patterns = {}
+        if self._normals is not None:
+            m.normals_upload(self._normals, self._normal_valid)
+        return dchi
+
+    def distribloads(self, geom0, dchi, force, npts=None, block=_lib.FS_BLOCK_FF):
+        """distribloads(FEMMBase(IntegDomain(fes, rule)), geom0, dchi,
+        ForceIntensity(f), 2) on the device: the free rows (block FF, as
+        SysvecAssemblerFBlock) or all rows.  npts: 1, 3 (default) or 6 for
+        TriRule on T3, 4 for GaussRule(2, 2) on Q4.  force: a 6-vector, or a
+        callable taking the rule's point locations (n x 3) and returning n x 6
+        force intensities (the ForceIntensity function)."""
+        m = self._ensure_model(geom0)
+        self._ensure_dofs(dchi)
+        npts = (3 if self.nen == 3 else 4) if npts is None else int(npts)
+        if callable(force):
+            X = m.load_points(npts).reshape(-1, 3)
+            force = np.asarray(force(X), dtype=float).reshape(self.conn.shape[0], npts, 6)
+        nrows = dchi.nfreedofs if block == _lib.FS_BLOCK_FF else dchi.nalldofs
+        return m.surface_load(npts, force, block, nrows)
+
     # -- inspectintegpoints --
     def _points(self, xyz, fl, tangents=False):
         """Location (nlist x npts x 3) and, if asked, tangent matrix (nlist x

@@ -261,6 +261,43 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
  */
 int fs_model_element_blocks(fs_model *m, const fs_shell_params *p, double *blocks_out);
 
+/*
+ * numberdofs! on the device: free dofs first, then the fixed ones, nodes in
+ * their order or in the order of a permutation (FinEtools NodalField
+ * numberdofs!(dchi) and numberdofs!(dchi, perm) after setebc!/applyebc!:
+ * test_shell_statics.jl:205, plate_expl_examples.jl:147), and the equation
+ * numbers installed as fs_model_set_dofs would.
+ *   is_fixed     N x 6 flags (row-major: node, dof), host
+ *   perm         N 0-based node numbers in numbering order, or NULL
+ *   dofnums_out  N x 6 0-based numbers, column-major like fs_model_set_dofs'
+ *                input (may be NULL)
+ *   nfree_out    number of free dofs (may be NULL)
+ */
+int fs_model_number_dofs(fs_model *m, const uint8_t *is_fixed, const int64_t *perm, int64_t *dofnums_out,
+                         int64_t *nfree_out);
+
+/* fs_model_surface_load kinds */
+#define FS_LOAD_UNIFORM 0   /* one 6-vector force intensity for the whole mesh */
+#define FS_LOAD_PER_POINT 1 /* E x npts x 6 values at the rule's points (fs_model_load_points) */
+
+/*
+ * Consistent nodal loads of a distributed surface force: distribloads(
+ * FEMMBase(IntegDomain(fes, rule)), geom0, dchi, ForceIntensity(f), 2)
+ * (test_shell_statics.jl:84-86; plate_expl_examples.jl:197-199).  npts picks
+ * the rule: 1, 3 or 6 for TriRule(npts) on T3, 4 for GaussRule(2, 2) on Q4.
+ * Each element's vector sum_q N f Jac w is summed into the nodes in element
+ * order, as the element-by-element assembly does.  block FS_BLOCK_FF gives
+ * the free rows (SysvecAssemblerFBlock, F[1:nfreedofs]), FS_BLOCK_ALL all
+ * rows.  F_out: host.
+ */
+int fs_model_surface_load(fs_model *m, int32_t npts, int32_t kind, const double *force, int block, double *F_out);
+
+/* Locations of the integration points of fs_model_surface_load (E x npts x 3,
+ * host), where a position-dependent ForceIntensity is evaluated
+ * (updateforce!(fi, loc, J, label, qp), e.g. computetrac! of
+ * plate_expl_examples.jl:184-194); distribloads computes them internally. */
+int fs_model_load_points(fs_model *m, int32_t npts, double *xyz_out);
+
 /* Output coordinate systems of fs_model_resultants. */
 #define FS_OCSYS_ISOPARAMETRIC 0 /* E_G of each point: the default mcsys of the
                                     homogeneous FEMMs (isoparametric!, and

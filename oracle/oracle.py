@@ -39,6 +39,11 @@ What pins each function (tests/test_oracle_pins.py unless noted):
   or_cd_loop                  undamped single-dof closed form (harmonic
                               oscillator under the CD recurrence); energy.
   or_pwr                      largest eigenvalue by scipy on small models.
+  surface_load, number_dofs   Scordelis-Lo deflections (the load of
+                              test_shell_statics.jl:84-86); exactness of each
+                              rule on polynomial loads of its degree; the
+                              numbering by brute force on small flag sets
+                              (tests/test_frontend_pins.py).
   or_resultants_batch         membrane, bending and shear patch closed forms;
   (inspectintegpoints)        rigid motions on a tilted flat mesh; energy
                               identity against or_*_elmat (homogeneous and
@@ -285,6 +290,89 @@ def resultants(nen, xyz, conn, normals_, valid, u, *, Dps=None, Dt=None, t=None,
     if st != 0:
         raise ArithmeticError("Singular metric matrix in _gradN_e!")
     return out
+
+
+# ---------------------------------------------------------------- loads and numbering
+# FinEtools integration rules (IntegRuleModule): TriRule(1), TriRule(3),
+# TriRule(6) and GaussRule(2, 2) as (points, weights).
+G2 = 0.577350269189626
+RULES = {
+    (3, 1): (np.array([[1 / 3, 1 / 3]]), np.array([1 / 2])),
+    (3, 3): (np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]), np.array([1 / 6, 1 / 6, 1 / 6])),
+    (3, 6): (np.array([[0.816847572980459, 0.091576213509771], [0.091576213509771, 0.816847572980459],
+                       [0.091576213509771, 0.091576213509771], [0.108103018168070, 0.445948490915965],
+                       [0.445948490915965, 0.108103018168070], [0.445948490915965, 0.445948490915965]]),
+             np.array([0.109951743655322] * 3 + [0.223381589678011] * 3) / 2),
+    (4, 4): (np.array([[-G2, -G2], [-G2, G2], [G2, -G2], [G2, G2]]), np.ones(4)),
+}
+
+
+def _tri_bfun(xi, eta):
+    return np.array([1.0 - xi - eta, xi, eta]), np.array([[-1.0, -1.0], [1.0, 0.0], [0.0, 1.0]])
+
+
+def _quad_bfun(xi, eta):
+    N = 0.25 * np.array([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)])
+    gN = 0.25 * np.array([[-(1 - eta), -(1 - xi)], [(1 - eta), -(1 + xi)], [(1 + eta), (1 + xi)],
+                          [-(1 + eta), (1 - xi)]])
+    return N, gN
+
+
+def load_points(nen, xyz, conn, npts):
+    """Locations of the integration points of the rule (E x npts x 3)."""
+    pts, _ = RULES[(nen, npts)]
+    bf = _tri_bfun if nen == 3 else _quad_bfun
+    X = np.asarray(xyz)[np.asarray(conn)]
+    return np.stack([np.einsum("a,eak->ek", bf(*p)[0], X) for p in pts], axis=1)
+
+
+def surface_load(nen, xyz, conn, force, nalldofs, dofnums, npts=None):
+    """distribloads(FEMMBase(IntegDomain(fes, rule)), geom, dchi,
+    ForceIntensity(f), 2) (FinEtools; test_shell_statics.jl:84-86) for T3/Q4
+    elements: per element Fe = sum_q outer(N, f) Jac w, assembled in element
+    order.  force: a 6-vector, or E x npts x 6 values at the rule's points.
+    Returns the vector over all dofs (length nalldofs)."""
+    xyz = np.asarray(xyz)
+    conn = np.asarray(conn)
+    npts = (3 if nen == 3 else 4) if npts is None else npts
+    pts, wts = RULES[(nen, npts)]
+    bf = _tri_bfun if nen == 3 else _quad_bfun
+    F = np.zeros(nalldofs)
+    f_all = np.asarray(force, dtype=float)
+    uniform = f_all.size == 6
+    for e in range(conn.shape[0]):
+        X = xyz[conn[e]]
+        fe = np.zeros((nen, 6))
+        for q, ((xi, eta), w) in enumerate(zip(pts, wts)):
+            N, gN = bf(xi, eta)
+            J = X.T @ gN
+            Jac = np.linalg.norm(np.cross(J[:, 0], J[:, 1]))
+            f = f_all.reshape(6) if uniform else f_all[e, q]
+            fe += np.outer(N, f) * (Jac * w)
+        for a in range(nen):
+            F[dofnums[conn[e, a]]] += fe[a]
+    return F
+
+
+def number_dofs(is_fixed, perm=None):
+    """numberdofs!(dchi[, perm]) (FinEtools NodalField): free dofs first,
+    node by node in node order (or in the order of perm), then the fixed ones
+    in the same order.  Returns (dofnums N x 6, nfree)."""
+    fx = np.asarray(is_fixed, dtype=bool)
+    N = fx.shape[0]
+    order = np.arange(N) if perm is None else np.asarray(perm)
+    dn = np.zeros((N, 6), dtype=np.int64)
+    nfree = int((~fx).sum())
+    nf = nx = 0
+    for I in order:
+        for c in range(6):
+            if fx[I, c]:
+                dn[I, c] = nfree + nx
+                nx += 1
+            else:
+                dn[I, c] = nf
+                nf += 1
+    return dn, nfree
 
 
 def element_masses(nen, xyz, conn, *, rho=None, t=None, m0=None, m1=None):

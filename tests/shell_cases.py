@@ -58,7 +58,7 @@ def scordelis(nen, n):
     dch.numberdofs()
     s40 = np.sin(40 / 360 * 2 * np.pi) * 25
     nl = fe.selectnode(xyz, [s40, s40, L / 2, L / 2, -np.inf, np.inf], tol)[0]
-    F = fe.surface_load(xyz, conn, [0, 0, -90, 0, 0, 0], dch.nalldofs, dch.dofnums)
+    F = O.surface_load(nen, xyz, conn, [0, 0, -90, 0, 0, 0], dch.nalldofs, dch.dofnums)
     return dict(nen=nen, xyz=xyz, conn=conn, dchi=dch, E=4.32e8, nu=0.0, rho=0.0, t=0.25, stab_c=0.2, F=F,
                 probe=nl)
 
