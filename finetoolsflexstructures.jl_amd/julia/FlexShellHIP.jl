@@ -340,6 +340,72 @@ function cd_loop!(femm, assembler, cdamp, U0, V0, tend, dt, Fmag, g; probes = In
     return U, V, permutedims(pout)
 end
 
+# ---------------------------------------------------------------- dofs and loads
+const FS_LOAD_UNIFORM = Int32(0)
+const FS_LOAD_PER_POINT = Int32(1)
+
+"""
+    numberdofs!(femm, geom0, dchi[, perm]; ctx = default_context())
+
+`numberdofs!` of FinEtools on the device (fs_model_number_dofs): free dofs
+first, then the fixed ones, in node order or in the order of `perm`; the
+numbering is written into `dchi` and installed in the device model.
+"""
+function numberdofs!(femm, geom0::NodalField{Float64}, dchi::NodalField, perm = nothing)
+    _inspect[] === nothing && (_inspect[] = _InspectModels(default_context(), IdDict{Any,DeviceModel}()))
+    m = _device_model!(_inspect[], femm, geom0, dchi)
+    N = size(dchi.values, 1)
+    fixed = permutedims(UInt8.(dchi.is_fixed))          # N x 6 row-major
+    pm = perm === nothing ? C_NULL : Int64.(collect(perm) .- 1)
+    dn = zeros(Int64, N, 6)                             # column-major, 0-based
+    nf = Ref{Int64}(0)
+    GC.@preserve fixed pm dn begin
+        check(m.ctx, ccall((:fs_model_number_dofs, libflexshell), Cint,
+                           (Ptr{Cvoid}, Ptr{UInt8}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}),
+                           m.h, fixed, pm, dn, nf), "fs_model_number_dofs")
+    end
+    dchi.dofnums .= dn .+ 1
+    dchi.nfreedofs = nf[]
+    m.dofs_key = hash(dchi.dofnums, hash(nfreedofs(dchi)))
+    return dchi
+end
+
+"""
+    distribloads(femm, geom0, dchi, fi::ForceIntensity, npts; block = :ff)
+
+`distribloads(FEMMBase(IntegDomain(fes, rule)), geom0, dchi, fi, 2)` on the
+device (fs_model_surface_load) with rule = TriRule(npts) (npts 1, 3, 6) for
+T3 and GaussRule(2, 2) (npts 4) for Q4.  A constant `fi` crosses as one
+6-vector; otherwise `fi` is evaluated with `updateforce!` at the rule's points
+(fs_model_load_points).  `block = :ff` returns the free rows.
+"""
+function distribloads(femm, geom0::NodalField{Float64}, dchi::NodalField, fi, npts::Integer; block::Symbol = :ff)
+    _inspect[] === nothing && (_inspect[] = _InspectModels(default_context(), IdDict{Any,DeviceModel}()))
+    m = _device_model!(_inspect[], femm, geom0, dchi)
+    E = count(femm.integdomain.fes)
+    if fi.isconstant
+        kind, f = FS_LOAD_UNIFORM, copy(fi._cache)
+    else
+        X = zeros(3, npts, E)
+        check(m.ctx, ccall((:fs_model_load_points, libflexshell), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}),
+                           m.h, npts, X), "fs_model_load_points")
+        f = zeros(6, npts, E)
+        J = zeros(3, 2)
+        for e in 1:E, q in 1:npts
+            f[:, q, e] .= updateforce!(fi, reshape(X[:, q, e], 1, 3), J, e, q)
+        end
+        kind = FS_LOAD_PER_POINT
+    end
+    blk = block === :ff ? FS_BLOCK_FF : FS_BLOCK_ALL
+    F = zeros(block === :ff ? nfreedofs(dchi) : nalldofs(dchi))
+    GC.@preserve f F begin
+        check(m.ctx, ccall((:fs_model_surface_load, libflexshell), Cint,
+                           (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Cint, Ptr{Float64}),
+                           m.h, npts, kind, f, blk, F), "fs_model_surface_load")
+    end
+    return F
+end
+
 # ---------------------------------------------------------------- stress resultants
 const FS_OCSYS_ISOPARAMETRIC = Int32(0)
 const FS_OCSYS_MATRIX = Int32(1)
@@ -446,6 +512,7 @@ function inspectintegpoints(self, geom0::NodalField{Float64}, u::NodalField{T}, 
     return idat
 end
 
-export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context, inspectintegpoints
+export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context, inspectintegpoints,
+    numberdofs!, distribloads
 
 end # module
