@@ -345,13 +345,13 @@ const FS_LOAD_UNIFORM = Int32(0)
 const FS_LOAD_PER_POINT = Int32(1)
 
 """
-    numberdofs!(femm, geom0, dchi[, perm]; ctx = default_context())
+    numberdofs_hip!(femm, geom0, dchi[, perm])
 
 `numberdofs!` of FinEtools on the device (fs_model_number_dofs): free dofs
 first, then the fixed ones, in node order or in the order of `perm`; the
 numbering is written into `dchi` and installed in the device model.
 """
-function numberdofs!(femm, geom0::NodalField{Float64}, dchi::NodalField, perm = nothing)
+function numberdofs_hip!(femm, geom0::NodalField{Float64}, dchi::NodalField, perm = nothing)
     _inspect[] === nothing && (_inspect[] = _InspectModels(default_context(), IdDict{Any,DeviceModel}()))
     m = _device_model!(_inspect[], femm, geom0, dchi)
     N = size(dchi.values, 1)
@@ -371,7 +371,7 @@ function numberdofs!(femm, geom0::NodalField{Float64}, dchi::NodalField, perm = 
 end
 
 """
-    distribloads(femm, geom0, dchi, fi::ForceIntensity, npts; block = :ff)
+    distribloads_hip(femm, geom0, dchi, fi::ForceIntensity, npts; block = :ff)
 
 `distribloads(FEMMBase(IntegDomain(fes, rule)), geom0, dchi, fi, 2)` on the
 device (fs_model_surface_load) with rule = TriRule(npts) (npts 1, 3, 6) for
@@ -379,7 +379,7 @@ T3 and GaussRule(2, 2) (npts 4) for Q4.  A constant `fi` crosses as one
 6-vector; otherwise `fi` is evaluated with `updateforce!` at the rule's points
 (fs_model_load_points).  `block = :ff` returns the free rows.
 """
-function distribloads(femm, geom0::NodalField{Float64}, dchi::NodalField, fi, npts::Integer; block::Symbol = :ff)
+function distribloads_hip(femm, geom0::NodalField{Float64}, dchi::NodalField, fi, npts::Integer; block::Symbol = :ff)
     _inspect[] === nothing && (_inspect[] = _InspectModels(default_context(), IdDict{Any,DeviceModel}()))
     m = _device_model!(_inspect[], femm, geom0, dchi)
     E = count(femm.integdomain.fes)
@@ -422,8 +422,8 @@ _quantity_offset(q) = q in (:membrane_force, :membrane) ? 0 :
                       q in (:transverse_shear, :transverse, :shear) ? 6 : 3
 
 """
-    inspectintegpoints(femm, geom0, u, dT, felist, inspector, idat, quantity = :Cauchy;
-                       outputcsys = nothing, transform_once = false)
+    inspectintegpoints_hip(femm, geom0, u, dT, felist, inspector, idat, quantity = :Cauchy;
+                           outputcsys = nothing, transform_once = false)
 
 `inspectintegpoints` of the four shell FEMMs (FEMMShellT3FFModule.jl:850-962,
 FEMMShellQ4RSModule.jl:1061-1170, FEMMShellT3FFCompModule.jl:809-943,
@@ -436,7 +436,7 @@ homogeneous FEMMs, the first layup csys of the layered ones); a constant
 with `updatecsmat!` at the point's location and tangents, as the reference
 does.  `transform_once = true` sets FS_RES_LAYERED_Q4_SINGLE_T.
 """
-function inspectintegpoints(self, geom0::NodalField{Float64}, u::NodalField{T}, dT, felist::AbstractVector,
+function inspectintegpoints_hip(self, geom0::NodalField{Float64}, u::NodalField{T}, dT, felist::AbstractVector,
                             inspector::F, idat, quantity = :Cauchy; outputcsys = nothing,
                             transform_once::Bool = false) where {T<:Number,F<:Function}
     @assert self._associatedgeometry == true
@@ -512,7 +512,7 @@ function inspectintegpoints(self, geom0::NodalField{Float64}, u::NodalField{T}, 
     return idat
 end
 
-export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context, inspectintegpoints,
-    numberdofs!, distribloads
+export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context,
+    inspectintegpoints_hip, numberdofs_hip!, distribloads_hip
 
 end # module
