@@ -154,3 +154,148 @@ def raasch(mesh):
     probe = fe.selectnode(xyz, [97.9615, 97.9615, -16, -16, 0, 0], tol)[0]
     return dict(nen=3, xyz=xyz, conn=conn, dchi=dch, E=3300.0, nu=0.35, rho=0.0, t=t, stab_c=0.2, F=F,
                 probe=probe, ntip=len(tip))
+
+
+def _csr_solve_max(K, F, d, comp, sign):
+    """Solve the free-free system and return max (sign > 0) or min of the
+    given nodal component (FinEtools solve_blocked! with zero prescribed
+    values)."""
+    import scipy.sparse.linalg as sla
+    nf = d.nfreedofs
+    U = sla.spsolve(K[:nf, :nf].tocsc(), F[:nf])
+    v = d.scattersysvec(U)[:, comp]
+    return v.max() if sign > 0 else v.min()
+
+
+# test/test_composite_shell_statics.jl: T3FFComp plates of Barbero's examples
+# and variants, reference values of the @test lines (default rtol of isapprox)
+COMP_T3 = {
+    "c1": 0.010276258430770539,     # :200, 10 pairs [0/90], max w (mm)
+    "c2": 3.3161098653904544,       # :323, [+45/-45/0/0/-45/+45], max u (mm)
+    "c6": -1502.0324040447633,      # :753, [0/90] 50 x 50, pressure, min w (mm)
+    "c7": -1176.8346659735575,      # :844, [-45/45] 50 x 50, pressure, min w (mm)
+    "c8+": 0.22004349767718365,     # :954, [90/0], edge load +, max w (mm)
+    "c8-": -0.22004349767718365,    # :952, edge load -, min w (mm)
+    "c9+": 0.22004349767718365,     # :1070, rotated plate and csys, max u (mm)
+    "c9-": -0.22004349767718365,    # :1068, min u (mm)
+    "c10w": 1.9529295928169146,     # :1181, Sacco [0/90], normalized max w
+    "c10u": 0.12375916912652352,    # :1183, normalized max u
+}
+
+
+# test/test_composite_shell_statics_q4rs.jl: the same setups with Q4RSComp on
+# Q4block meshes (pressure integrated with GaussRule(2, 2)); these values are
+# printed with fewer digits, so they hold to the @test default rtol (1.5e-8)
+COMP_Q4 = {
+    "c1": 0.010650045488315574,     # :200
+    "c2": 3.3161098653904544,       # :323
+    "c6": -1503.4206439821,         # :752
+    "c7": -1177.349547131,          # :843
+    "c8+": 0.22785850060074,        # :953
+    "c8-": -0.22785850060074,       # :951
+    "c9+": 0.227858500600,          # :1069
+    "c9-": -0.22785850060,          # :1067
+    "c10w": 2.0134037916683,        # :1180-1181
+    "c10u": 0.128716225,            # :1182-1183
+}
+
+
+def comp_t3(case, nen=3):
+    """The setups of mcompshell1/2/6/7/8/9/10 (test_composite_shell_statics.jl:
+    117-200, 234-323, 683-753, 774-844, 870-954, 978-1070, 1099-1183) as plain
+    arrays: mesh, layup plies (D6, thickness, angle, rho), csys, fixed dofs,
+    loads ("edges": list of (box, 6-vector per length); "pressure": 6-vector
+    per area with TriRule(1)), and the quantity to compare (component, sign,
+    scale)."""
+    from flexshell import meshio
+    mm, MPa = 1e-3, 1e6
+    key = case.rstrip("+-")
+    if key in ("c10w", "c10u"):
+        key = "c10"
+    sign = -1.0 if case.endswith("-") else 1.0
+    lam = (133860 * MPa, 7706 * MPa, 0.301, 4306 * MPa, 4306 * MPa, 2760 * MPa)
+    ax = ay = 2000 * mm
+    nx = ny = 8
+    csys = np.eye(3)
+    edges, pressure, quantity = [], None, None
+    fixes = []
+    if key == "c1":
+        th = 10 * mm
+        angles = [0, 90] * 10
+        pt = [th / 20] * 20
+        fixes = [([ax, ax, 0, 0, -np.inf, np.inf], [1, 2, 3]), ([0, 0, 0, 0, -np.inf, np.inf], [2]),
+                 ("boundary", [3])]
+        edges = [([-np.inf, np.inf, 0, 0, 0, 0], [0, 0.1 * MPa * th, 0, 0, 0, 0]),
+                 ([-np.inf, np.inf, ay, ay, 0, 0], [0, -0.1 * MPa * th, 0, 0, 0, 0])]
+        quantity = (2, +1, 1 / mm)
+    elif key == "c2":
+        lam = (37880 * MPa, 9407 * MPa, 0.299, 3405 * MPa, 3405 * MPa, 3308 * MPa)
+        angles = [45, -45, 0, 0, -45, 45]
+        pt = [1 * mm] * 6
+        fixes = [([0, 0, 0, 0, -np.inf, np.inf], [1, 2, 3]), ([ax, ax, 0, 0, -np.inf, np.inf], [2]),
+                 ("boundary", [3])]
+        edges = [([0, 0, -np.inf, np.inf, 0, 0], [-200 * 1e3, 0, 0, 0, 0, 0]),
+                 ([ax, ax, -np.inf, np.inf, 0, 0], [200 * 1e3, 0, 0, 0, 0, 0])]
+        quantity = (0, +1, 1 / mm)
+    elif key in ("c6", "c7"):
+        ax = ay = 1000 * mm
+        nx = ny = 50
+        th = 10 * mm
+        angles = [0, 90] if key == "c6" else [-45, 45]
+        pt = [th / 2] * 2
+        fixes = [("boundary", [1, 2, 3])]
+        pressure = [0, 0, -1.0 * MPa, 0, 0, 0]
+        quantity = (2, -1, 1 / mm)
+    elif key == "c8":
+        th = 10 * mm
+        angles = [90, 0]
+        pt = [th / 2] * 2
+        fixes = [([ax, ax, 0, 0, -np.inf, np.inf], [1, 2, 3]), ([0, 0, 0, 0, -np.inf, np.inf], [2]),
+                 ("boundary", [3])]
+        edges = [([0, 0, -np.inf, np.inf, 0, 0], [0.1 * MPa * sign * th, 0, 0, 0, 0, 0]),
+                 ([ax, ax, -np.inf, np.inf, 0, 0], [-0.1 * MPa * sign * th, 0, 0, 0, 0, 0])]
+        quantity = (2, sign, 1 / mm)
+    elif key == "c9":
+        th = 10 * mm
+        angles = [90, 0]
+        pt = [th / 2] * 2
+        csys = np.array([[0.0, 0, 1], [1, 0, 0], [0, 1, 0]])
+        fixes = [([0, 0, 0, 0, 0, 0], [1, 2, 3]), ([0, 0, 0, 0, ax, ax], [2]), ("boundary", [1])]
+        edges = [([0, 0, -np.inf, np.inf, ax, ax], [0, 0, 0.1 * MPa * sign * th, 0, 0, 0]),
+                 ([0, 0, -np.inf, np.inf, 0, 0], [0, 0, -0.1 * MPa * sign * th, 0, 0, 0])]
+        quantity = (0, sign, 1 / mm)
+    elif key == "c10":
+        a = 250 * mm
+        ax = ay = a
+        E1 = 100e9
+        E2 = E1 / 25
+        lam = (E1, E2, 0.25, E2 * 0.5, E2 * 0.5, E2 * 0.2)
+        th = a / 10
+        angles = [0, 90]
+        pt = [th / 2] * 2
+        q = 10.0e3
+        fixes = [([0, 0, -np.inf, np.inf, 0, 0], [2, 4]), ([a, a, -np.inf, np.inf, 0, 0], [2, 4]),
+                 ([-np.inf, np.inf, 0, 0, 0, 0], [1, 5]), ([-np.inf, np.inf, a, a, 0, 0], [1, 5]),
+                 ("boundary", [3])]
+        pressure = [0, 0, q, 0, 0, 0]
+        norm = (100 * E2) / (q * th * (a / th) ** 4)
+        quantity = (2, +1, norm) if case == "c10w" else (0, +1, norm)
+    xy, conn = (fe.T3block if nen == 3 else fe.Q4block)(ax, ay, nx, ny)
+    xyz = fe.xyz3(xy)
+    if key == "c9":
+        xyz = xyz[:, [2, 0, 1]]  # (x, y, z) -> (z, x, y)
+    tol = ax / nx / 100
+    d = fe.DofField(xyz.shape[0])
+    for box, comps in fixes:
+        nodes = fe.boundary_nodes(conn) if isinstance(box, str) else fe.selectnode(xyz, box, tol)
+        d.setebc(nodes, comps)
+    d.numberdofs()
+    F = np.zeros(d.nalldofs)
+    bed = fe.boundary_edges(conn)
+    for box, f in edges:
+        F += fe.edge_load(xyz, meshio.select_edges(xyz, bed, box, tol), f, d.nalldofs, d.dofnums)
+    D6 = O.lamina_D6(*lam)
+    plies = [(D6, t, float(ang), 0.0) for t, ang in zip(pt, angles)]
+    return dict(nen=nen, xyz=xyz, conn=conn, dchi=d, plies=plies, csys=csys, th=float(sum(pt)), F=F,
+                pressure=pressure, load_npts=1 if nen == 3 else 4, quantity=quantity, lamina=lam,
+                ref=(COMP_T3 if nen == 3 else COMP_Q4)[case], rtol=1e-9 if nen == 3 else 1.5e-8)

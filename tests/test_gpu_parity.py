@@ -11,7 +11,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as sla
 
 import shell_cases as SC
-from flexshell import explicit, fe, femm as FM
+from flexshell import _lib, explicit, fe, femm as FM
 from flexshell.material import CompositeLayup, MatDeforElastIso, Ply, lamina_material
 from oracle import oracle as O
 
@@ -130,6 +130,28 @@ def test_layered_barbero(ctx, nen):
     wmax = d.scattersysvec(U)[:, 2].max() / SC.mm
     ref = SC.BARBERO_T3 if nen == 3 else SC.BARBERO_Q4
     assert abs(wmax - ref) / ref < 1e-10
+
+
+@pytest.mark.parametrize("nen", [3, 4])
+@pytest.mark.parametrize("case", ["c1", "c2", "c6", "c7", "c8+", "c9-", "c10w", "c10u"])
+def test_composite_reference_values_gpu(ctx, case, nen):
+    """The T3FFComp / Q4RSComp plates of test_composite_shell_statics.jl and
+    test_composite_shell_statics_q4rs.jl with K and the pressure load from the
+    device: the values of their @test lines."""
+    print(f"[gpu] composite reference {case} nen={nen}", flush=True)
+    c = SC.comp_t3(case, nen)
+    d = c["dchi"]
+    lay = _flexshell_layup(c)
+    cls = FM.FEMMShellT3FFComp if nen == 3 else FM.FEMMShellQ4RSComp
+    f = cls(c["conn"], lay, ctx=ctx)
+    f.associategeometry(c["xyz"])
+    Kg = f.stiffness(FM.SysmatAssemblerSparseCSRSymm(), c["xyz"], None, None, d)
+    F = c["F"].copy()
+    if c["pressure"] is not None:
+        F += f.distribloads(c["xyz"], d, c["pressure"], c["load_npts"], _lib.FS_BLOCK_ALL)
+    comp, sign, scale = c["quantity"]
+    v = SC._csr_solve_max(Kg, F, d, comp, sign) * scale
+    assert abs(v - c["ref"]) / abs(c["ref"]) < c["rtol"], (v, c["ref"])
 
 
 @pytest.mark.parametrize("mesh", ["1x9", "10x72"])

@@ -261,3 +261,21 @@ def test_raasch_hook_from_abaqus_meshes(mesh):
     U = sla.spsolve(K[:nf, :nf].tocsc(), c["F"][:nf])
     pct = d.scattersysvec(U)[c["probe"], 2] / 5.02 * 100
     assert abs(pct - SC.RAASCH_T3[mesh]) / SC.RAASCH_T3[mesh] < 1e-9
+
+
+@pytest.mark.parametrize("nen", [3, 4])
+@pytest.mark.parametrize("case", ["c1", "c2", "c6", "c7", "c8+", "c8-", "c9+", "c9-", "c10w", "c10u"])
+def test_composite_reference_values(case, nen):
+    """T3FFComp / Q4RSComp plates of test/test_composite_shell_statics.jl and
+    test_composite_shell_statics_q4rs.jl (layups of 2 to 20 plies at 0, 90,
+    +-45 degrees; edge and pressure loads; a plate and layup csys rotated in
+    3-D): the values of their @test lines."""
+    c = SC.comp_t3(case, nen)
+    d = c["dchi"]
+    K, _ = _layered_K(c)
+    F = c["F"].copy()
+    if c["pressure"] is not None:
+        F += O.surface_load(nen, c["xyz"], c["conn"], c["pressure"], d.nalldofs, d.dofnums, c["load_npts"])
+    comp, sign, scale = c["quantity"]
+    v = SC._csr_solve_max(K, F, d, comp, sign) * scale
+    assert abs(v - c["ref"]) / abs(c["ref"]) < c["rtol"], (v, c["ref"])
