@@ -378,6 +378,39 @@ class _ShellFEMM:
                 idat = inspector(idat, int(e), conn, xe, out, loc[l, j])
         return idat
 
+    def fieldfromintegpoints(self, geom0, u, quantity, component, outputcsys=None, transform_once=False):
+        """Nodal field of one resultant component (FinEtools
+        fieldfromintegpoints with inverse-distance interpolation, as
+        test_shell_statics.jl:690-728 uses it): every integration point
+        contributes its value to the element's nodes with weights 1/d^2 (d
+        the point-node distance, floored at 1e-9 of the smallest nonzero one
+        of the element), and each node takes the weighted mean.  quantity:
+        membrane, moment or shear; component: 0-based.  The resultants come
+        from the device; the interpolation runs on the host."""
+        k0 = self._QUANTITY.get(str(quantity).lstrip(":"), 3)
+        res, loc = self.resultants(geom0, u, None, outputcsys, transform_once)
+        v = res[:, :, k0 + int(component)]
+        X = np.asarray(geom0, dtype=float)[self.conn]               # E x nen x 3
+        d2 = np.sum((X[:, None, :, :] - loc[:, :, None, :]) ** 2, axis=3)  # E x npts x nen
+        pos = np.where(d2 > 0, d2, np.inf).min(axis=2, keepdims=True)
+        d2 = np.maximum(d2, pos / 1e9)
+        w = 1.0 / d2
+        N = np.asarray(geom0).shape[0]
+        num = np.zeros(N)
+        den = np.zeros(N)
+        np.add.at(num, np.repeat(self.conn[:, None, :], v.shape[1], axis=1).ravel(), (w * v[:, :, None]).ravel())
+        np.add.at(den, np.repeat(self.conn[:, None, :], v.shape[1], axis=1).ravel(), w.ravel())
+        out = np.zeros(N)
+        np.divide(num, den, out=out, where=den > 0)
+        return out
+
+    def elemfieldfromintegpoints(self, geom0, u, quantity, component, outputcsys=None, transform_once=False):
+        """Element field of one resultant component: the mean over the
+        element's integration points (FinEtools elemfieldfromintegpoints)."""
+        k0 = self._QUANTITY.get(str(quantity).lstrip(":"), 3)
+        res, _ = self.resultants(geom0, u, None, outputcsys, transform_once)
+        return res[:, :, k0 + int(component)].mean(axis=1)
+
     def num_normals(self):
         v = self._normal_valid
         return len(v), int(np.count_nonzero(~v))

@@ -126,32 +126,27 @@ def q4_to_t3(conn, orientation="default"):
 
 
 def merge_nodes(xyz, conn, tolerance):
-    """Nodes closer than `tolerance` (Euclidean) become one; the kept node is
-    the first of a group in node order and the others are removed.  Returns
-    (xyz, conn)."""
+    """FinEtools mergenodes: in node order, every node not yet merged takes
+    all later nodes whose L1 distance (sum of the absolute coordinate
+    differences) to it is below `tolerance`; the kept nodes stay in their
+    order.  Returns (xyz, conn)."""
     xyz = np.asarray(xyz, dtype=float)
     N = xyz.shape[0]
-    rep = np.arange(N)
-    order = np.lexsort((xyz[:, 2], xyz[:, 1], xyz[:, 0]))
+    rep = np.full(N, -1, dtype=np.int64)
+    order = np.argsort(xyz[:, 0], kind="stable")
     xs = xyz[order, 0]
-    for ii, i in enumerate(order):
-        if rep[i] != i:
-            continue
-        j = ii + 1
-        while j < N and xs[j] - xyz[i, 0] <= tolerance:
-            k = order[j]
-            if rep[k] == k and k != i and np.linalg.norm(xyz[k] - xyz[i]) < tolerance:
-                if k < i:
-                    rep[i] = k
-                else:
-                    rep[k] = i
-            j += 1
-    # resolve chains to the smallest node of each group
     for i in range(N):
-        r = i
-        while rep[r] != r:
-            r = rep[r]
-        rep[i] = r
+        if rep[i] >= 0:
+            continue
+        rep[i] = i
+        # candidates: |x_j - x_i| < tolerance (a necessary condition of the L1 test)
+        lo = np.searchsorted(xs, xyz[i, 0] - tolerance, side="left")
+        hi = np.searchsorted(xs, xyz[i, 0] + tolerance, side="right")
+        cand = order[lo:hi]
+        cand = cand[(cand > i) & (rep[cand] < 0)]
+        if cand.size:
+            d = np.abs(xyz[cand] - xyz[i]).sum(axis=1)
+            rep[cand[d < tolerance]] = i
     conn = rep[np.asarray(conn)]
     return compact_nodes(xyz, conn)[:2]
 

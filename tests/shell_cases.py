@@ -299,3 +299,62 @@ def comp_t3(case, nen=3):
     return dict(nen=nen, xyz=xyz, conn=conn, dchi=d, plies=plies, csys=csys, th=float(sum(pt)), F=F,
                 pressure=pressure, load_npts=1 if nen == 3 else 4, quantity=quantity, lamina=lam,
                 ref=(COMP_T3 if nen == 3 else COMP_Q4)[case], rtol=1e-9 if nen == 3 else 1.5e-8)
+
+
+# test/test_shell_statics.jl:690-728: minimum and maximum of the nodal
+# resultant fields of the barrel vault (fieldfromintegpoints, cylindrical
+# output csys), rtol 0.01
+BARREL_RESULTANTS = {
+    "moment": [(-1520.6449167366522, 14.067403309095397), (-73.8262145426215, 425.93651541819503),
+               (-0.005341121492547284, 953.0929383629322)],
+    "membrane": [(-306.83173146926623, 309.5860993742647), (-1011.4832977998705, 2167.0403478574167),
+                 (-687.3500043290137, 69.38703021678862)],
+    "shear": [(-13.784764125688811, 21.165312065421237), (-7.4216963152916255, 25.679801383392967)],
+}
+
+
+def barrel_vault():
+    """Barrel vault on the irregular Abaqus STRI3 mesh (test_shell_statics.jl:
+    578-670): compacted, merged (thickness / 10), T3FF with stab 0.2 h^2 and
+    drilling scale 0.1, load -0.625 along x per area (TriRule(3)), and the
+    cylindrical output csys of the resultants (:676-688)."""
+    from flexshell import meshio
+    m = meshio.import_abaqus(os.path.join(GOLDEN, "inp", "barrelvault_stri3_irreg.inp"))
+    xyz, conn, _ = meshio.compact_nodes(m["xyz"], m["fesets"][0][1])
+    t = 3.0
+    xyz, conn = meshio.merge_nodes(xyz, conn, t / 10)
+    L = 50.0 * 12
+    tol = t / 20
+    d = fe.DofField(xyz.shape[0])
+    d.setebc(fe.selectnode(xyz, [-np.inf, np.inf, -np.inf, np.inf, 0, 0], tol), [1, 2])
+    d.setebc(fe.selectnode(xyz, [-np.inf, np.inf, -np.inf, np.inf, L / 2, L / 2], tol), [3, 4, 5])
+    d.setebc(fe.selectnode(xyz, [-np.inf, np.inf, 0, 0, -np.inf, np.inf], tol), [2, 4, 6])
+    d.numberdofs()
+
+    def cylindrical(X, J=None, e=None, q=None):
+        r = -np.asarray(X, dtype=float).copy()
+        r[2] = 0.0
+        c3 = r / np.linalg.norm(r)
+        c2 = np.array([0.0, 0.0, 1.0])
+        return np.stack([np.cross(c2, c3), c2, c3], axis=1)
+
+    return dict(nen=3, xyz=xyz, conn=conn, dchi=d, E=3.0e6, nu=0.0, t=t, stab_c=0.2, drill=0.1,
+                force=[-0.625, 0, 0, 0, 0, 0], csys=cylindrical)
+
+
+def inverse_distance_field(xyz, conn, loc, vals):
+    """Nodal field by inverse squared point-node distance weights (FinEtools
+    fieldfromintegpoints, interpolation :invdistance) from point values
+    (E x npts) at point locations (E x npts x 3); test-side restatement."""
+    X = np.asarray(xyz)[conn]
+    N = np.asarray(xyz).shape[0]
+    num = np.zeros(N)
+    den = np.zeros(N)
+    for e in range(conn.shape[0]):
+        for q in range(vals.shape[1]):
+            d2 = np.sum((X[e] - loc[e, q]) ** 2, axis=1)
+            d2 = np.maximum(d2, d2[d2 > 0].min() / 1e9)
+            for a in range(conn.shape[1]):
+                num[conn[e, a]] += vals[e, q] / d2[a]
+                den[conn[e, a]] += 1.0 / d2[a]
+    return np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)

@@ -184,3 +184,33 @@ def test_resultants_errors(ctx):
     st = ctx.lib.fs_model_resultants(f._model.h, _lib.C.byref(p), _lib.ptr(u), 2, None, 0, None, 0, _lib.ptr(o),
                                      None)
     assert st == 0
+
+
+def test_barrel_vault_fields_gpu(ctx):
+    """The barrel vault of test_shell_statics.jl:578-728 through the device
+    (stiffness, TriRule(3) load, resultants) and the mirror's
+    fieldfromintegpoints: the reference's minimum and maximum of every nodal
+    resultant field within its rtol 0.01, and the same fields as the
+    test-side interpolation of the oracle's resultants to 1e-9."""
+    import scipy.sparse.linalg as sla
+    print("[gpu] barrel vault resultant fields", flush=True)
+    c = SC.barrel_vault()
+    xyz, conn, d = c["xyz"], c["conn"], c["dchi"]
+    f = FM.make(conn, MatDeforElastIso(c["E"], c["nu"], 0.0), thickness=c["t"], stab_c=c["stab_c"], ctx=ctx)
+    f.drilling_stiffness_scale = c["drill"]
+    f.associategeometry(xyz)
+    K = f.stiffness(FM.SysmatAssemblerSparseCSRSymm(), xyz, None, None, d)
+    F = f.distribloads(xyz, d, c["force"], 3, _lib.FS_BLOCK_ALL)
+    nf = d.nfreedofs
+    u = d.scattersysvec(sla.spsolve(K[:nf, :nf].tocsc(), F[:nf]))
+    for q, refs in SC.BARREL_RESULTANTS.items():
+        for nc, (lo, hi) in enumerate(refs):
+            fl = f.fieldfromintegpoints(xyz, u, q, nc, outputcsys=c["csys"])
+            assert abs(fl.min() - lo) <= 0.01 * abs(lo), (q, nc, fl.min(), lo)
+            assert abs(fl.max() - hi) <= 0.01 * abs(hi), (q, nc, fl.max(), hi)
+    res, loc = f.resultants(xyz, u, outputcsys=c["csys"])
+    fl = f.fieldfromintegpoints(xyz, u, "moment", 0, outputcsys=c["csys"])
+    ref = SC.inverse_distance_field(xyz, conn, loc, res[:, :, 3])
+    assert np.abs(fl - ref).max() <= 1e-9 * np.abs(ref).max()
+    ef = f.elemfieldfromintegpoints(xyz, u, "membrane", 1, outputcsys=c["csys"])
+    assert np.array_equal(ef, res[:, 0, 1])

@@ -248,3 +248,30 @@ def test_layered_equals_homogeneous_for_isotropic_ply():
         c = O.resultants(nen, xyz, conn, nrm, val, u, groups=[(A, B, D, H, np.eye(3), T_)], ocs=np.eye(3))
         for s in (slice(0, 3), slice(3, 6), slice(6, 8)):
             assert np.abs(h[..., s] - c[..., s]).max() < 1e-11 * np.abs(h[..., s]).max()
+
+
+def test_barrel_vault_resultant_fields():
+    """The barrel vault of test_shell_statics.jl:578-728: minimum and maximum
+    of the nodal moment, membrane and shear fields in the cylindrical csys,
+    within the reference's rtol 0.01 (the only shell resultant values the
+    reference's tests hold; its nodal interpolation is FinEtools'
+    inverse-distance fieldfromintegpoints, restated by the test)."""
+    import scipy.sparse.linalg as sla
+    c = SC.barrel_vault()
+    xyz, conn, d = c["xyz"], c["conn"], c["dchi"]
+    Dps, Dt = O.shell_D(O.iso_D6(c["E"], c["nu"]))
+    nrm, val = O.normals(3, xyz, conn)
+    Ke = O.element_matrices(3, xyz, conn, nrm, val, Dps=Dps, Dt=Dt, t=c["t"], stab_c=c["stab_c"], drill=c["drill"])
+    K = O.assemble_csr(3, conn, d.dofnums, d.nalldofs, Ke)
+    F = O.surface_load(3, xyz, conn, c["force"], d.nalldofs, d.dofnums, 3)
+    nf = d.nfreedofs
+    u = d.scattersysvec(sla.spsolve(K[:nf, :nf].tocsc(), F[:nf]))
+    loc = xyz[conn].mean(axis=1)[:, None, :]
+    ocs = np.array([c["csys"](p) for p in loc[:, 0]]).reshape(-1, 1, 3, 3)
+    res = O.resultants(3, xyz, conn, nrm, val, u, Dps=Dps, Dt=Dt, t=c["t"], stab_c=c["stab_c"], ocs=ocs)
+    off = {"membrane": 0, "moment": 3, "shear": 6}
+    for q, refs in SC.BARREL_RESULTANTS.items():
+        for nc, (lo, hi) in enumerate(refs):
+            fl = SC.inverse_distance_field(xyz, conn, loc, res[:, :, off[q] + nc])
+            assert abs(fl.min() - lo) <= 0.01 * abs(lo), (q, nc, fl.min(), lo)
+            assert abs(fl.max() - hi) <= 0.01 * abs(hi), (q, nc, fl.max(), hi)
