@@ -358,3 +358,72 @@ def inverse_distance_field(xyz, conn, loc, vals):
                 num[conn[e, a]] += vals[e, q] / d2[a]
                 den[conn[e, a]] += 1.0 / d2[a]
     return np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
+
+
+# test/test_shell_statics.jl:364-381 (T3FF) and test_q4rs_shell_statics.jl:
+# 360-377 (Q4RS): twisted beam tip deflection in percent of the reference
+# solution, for (thick, dir 2), (thick, dir 3), (thin, dir 2), (thin, dir 3)
+# and meshes 2n x n, n = 2, 4, 8, 16 (rtol 1e-3 there)
+TWISTED_T3 = [39.709921740907355, 68.87306876326497, 86.01944734315117, 95.04101960524827,
+              53.10262177376127, 83.8593790803426, 94.91359387874728, 98.21549248655576,
+              48.16757753755567, 79.43420077873479, 92.54464819755955, 96.85008269135751,
+              50.577029703967334, 80.34160167730624, 92.48675665271801, 96.7096641005938]
+TWISTED_Q4 = [57.52004303100694, 82.57318302555431, 94.20131995162048, 98.37454367262973,
+              76.28227868667001, 93.90901921611645, 98.28145099724978, 99.43562910255783,
+              68.51008267652144, 90.66189189880987, 97.4223162100464, 99.27913153086936,
+              76.09971224380796, 93.00031411228612, 97.58071557550575, 99.2319932521005]
+_TWISTED_PARAMS = [dict(t=0.32, force=1.0, dir=2, uex=0.001753248285256),
+                   dict(t=0.32, force=1.0, dir=3, uex=0.005424534868469),
+                   dict(t=0.0032, force=1.0e-6, dir=2, uex=0.001294),
+                   dict(t=0.0032, force=1.0e-6, dir=3, uex=0.005256)]
+
+
+def twisted_beam(nen, which, n):
+    """Twisted cantilever (test_shell_statics.jl:289-348,
+    test_q4rs_shell_statics.jl:283-342): 12 x 1.1, twisted by 90 degrees
+    along its length, clamped at x = 0, a point force at the tip centre.
+    which: 0..3 as in _TWISTED_PARAMS; mesh 2n x n."""
+    p = _TWISTED_PARAMS[which]
+    E, nu, W, L = 0.29e8, 0.22, 1.1, 12.0
+    nL, nW = 2 * n, n
+    tol = W / nW / 100
+    xy, conn = (fe.T3block if nen == 3 else fe.Q4block)(L, W, nL, nW)
+    xyz = fe.xyz3(xy)
+    a = xyz[:, 0] / L * (np.pi / 2)
+    y = xyz[:, 1] - W / 2
+    z = xyz[:, 2]
+    xyz = np.stack([xyz[:, 0], y * np.cos(a) - z * np.sin(a), y * np.sin(a) + z * np.cos(a)], axis=1)
+    d = fe.DofField(xyz.shape[0])
+    d.setebc(fe.selectnode(xyz, [0, 0, -np.inf, np.inf, -np.inf, np.inf], tol), [1, 2, 3, 4, 5, 6])
+    d.numberdofs()
+    tip = fe.selectnode(xyz, [L, L, 0, 0, 0, 0], tol)
+    F = np.zeros(d.nalldofs)
+    F[d.dofnums[tip[0], p["dir"] - 1]] = p["force"]
+    ref = (TWISTED_T3 if nen == 3 else TWISTED_Q4)[4 * which + [2, 4, 8, 16].index(n)]
+    return dict(nen=nen, xyz=xyz, conn=conn, dchi=d, E=E, nu=nu, rho=0.0, t=p["t"],
+                stab_c=(None if nen == 3 else 0.05), F=F, probe=tip[0], dir=p["dir"], uex=p["uex"], ref=ref)
+
+
+# test/test_shell_statics.jl:521-531: LE5 Z-section cantilever (NAFEMS), min
+# and max of the deflection w (default rtol of isapprox)
+LE5_W = (-0.01568028415401719, 0.015401663810490641)
+
+
+def le5():
+    """LE5 Z-section cantilever under torsion (test_shell_statics.jl:438-519)
+    on the reference's Abaqus mesh nle5xf3c.inp: T3FF, E 210e9, nu 0.3,
+    t 0.1, stab 0.2 h^2, translations fixed at x = 0, point forces +-0.6e6
+    along z at (10, +-1, 0)."""
+    from flexshell import meshio
+    m = meshio.import_abaqus(os.path.join(GOLDEN, "inp", "nle5xf3c.inp"))
+    xyz, conn, _ = meshio.compact_nodes(m["xyz"], m["fesets"][0][1])
+    t = 0.1
+    tol = t / 1000
+    d = fe.DofField(xyz.shape[0])
+    d.setebc(fe.selectnode(xyz, [0, 0, -np.inf, np.inf, -np.inf, np.inf], tol), [1, 2, 3])
+    d.numberdofs()
+    F = np.zeros(d.nalldofs)
+    for y, fz in ((1.0, 0.6e6), (-1.0, -0.6e6)):
+        nl = fe.selectnode(xyz, [10.0, 10.0, y, y, 0, 0], tol)
+        F[d.dofnums[nl[0], 2]] += fz
+    return dict(nen=3, xyz=xyz, conn=conn, dchi=d, E=210e9, nu=0.3, rho=0.0, t=t, stab_c=0.2, F=F)

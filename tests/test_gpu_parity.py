@@ -154,6 +154,45 @@ def test_composite_reference_values_gpu(ctx, case, nen):
     assert abs(v - c["ref"]) / abs(c["ref"]) < c["rtol"], (v, c["ref"])
 
 
+@pytest.mark.parametrize("nen", [3, 4])
+@pytest.mark.parametrize("which", [0, 1, 2, 3])
+def test_twisted_beam_gpu(ctx, nen, which):
+    """Twisted cantilever (test_shell_statics.jl:350-386,
+    test_q4rs_shell_statics.jl:346-382) on the 16 x 8 mesh with K from the
+    device: CSR within 1e-12 of the oracle's and the reference percentage
+    (bars as in the oracle pin: 1e-8 thick, 1e-4 thin)."""
+    print(f"[gpu] twisted beam nen={nen} case={which}", flush=True)
+    c = SC.twisted_beam(nen, which, 8)
+    d = c["dchi"]
+    f = _homog_femm(ctx, nen, c["conn"], c["E"], c["nu"], 0.0, c["t"], stab_c=c["stab_c"])
+    f.associategeometry(c["xyz"])
+    Kg = f.stiffness(FM.SysmatAssemblerSparseCSRSymm(), c["xyz"], None, None, d)
+    Ko = O.assemble_csr(nen, c["conn"], d.dofnums, d.nalldofs,
+                        _oracle_homog(nen, c["xyz"], c["conn"], c["E"], c["nu"], c["t"], stab_c=c["stab_c"]))
+    assert np.array_equal(Kg.indices, Ko.indices) and _rel(Kg, Ko) < KTOL
+    nf = d.nfreedofs
+    U = sla.spsolve(Kg[:nf, :nf].tocsc(), c["F"][:nf])
+    v = d.scattersysvec(U)[c["probe"], c["dir"] - 1] / c["uex"] * 100
+    assert abs(v - c["ref"]) / c["ref"] < (1e-8 if which < 2 else 1e-4), (v, c["ref"])
+
+
+def test_le5_z_cantilever_gpu(ctx):
+    """LE5 Z-section cantilever (test_shell_statics.jl:438-531) with K from
+    the device, normals from the device (folds: invalid normals): the
+    reference's minimum and maximum deflection."""
+    print("[gpu] LE5 Z cantilever", flush=True)
+    c = SC.le5()
+    d = c["dchi"]
+    f = _homog_femm(ctx, 3, c["conn"], c["E"], c["nu"], 0.0, c["t"], stab_c=c["stab_c"])
+    f.associategeometry(c["xyz"])
+    assert (~f._normal_valid).any()
+    Kg = f.stiffness(FM.SysmatAssemblerSparseCSRSymm(), c["xyz"], None, None, d)
+    nf = d.nfreedofs
+    w = d.scattersysvec(sla.spsolve(Kg[:nf, :nf].tocsc(), c["F"][:nf]))[:, 2]
+    assert abs(w.min() - SC.LE5_W[0]) / abs(SC.LE5_W[0]) < 1e-9, w.min()
+    assert abs(w.max() - SC.LE5_W[1]) / abs(SC.LE5_W[1]) < 1e-9, w.max()
+
+
 @pytest.mark.parametrize("mesh", ["1x9", "10x72"])
 def test_raasch_hook_gpu(ctx, mesh):
     """GPU stiffness on the reference's Abaqus meshes (test_shell_statics.jl:

@@ -279,3 +279,39 @@ def test_composite_reference_values(case, nen):
     comp, sign, scale = c["quantity"]
     v = SC._csr_solve_max(K, F, d, comp, sign) * scale
     assert abs(v - c["ref"]) / abs(c["ref"]) < c["rtol"], (v, c["ref"])
+
+
+@pytest.mark.parametrize("nen", [3, 4])
+@pytest.mark.parametrize("which", [0, 1, 2, 3])
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+def test_twisted_beam(nen, which, n):
+    """Twisted cantilever of test_shell_statics.jl:350-386 (T3FF) and
+    test_q4rs_shell_statics.jl:346-382 (Q4RS), thick and thin, both load
+    directions, four meshes: the reference percentages (rtol 1e-3 there).
+    Thick section (t = 0.32): within 1e-8.  Thin section (t = 0.0032 on a
+    12-long beam): the stiffness condition number grows like (L/t)^2 ~ 1e7, so
+    rounding-level differences of K and of the sparse solver (not the
+    reference's) show at up to 2e-5 on the finest mesh; held to 1e-4."""
+    c = SC.twisted_beam(nen, which, n)
+    d = c["dchi"]
+    Dps, Dt = O.shell_D(O.iso_D6(c["E"], c["nu"]))
+    nrm, val = O.normals(nen, c["xyz"], c["conn"])
+    Ke = O.element_matrices(nen, c["xyz"], c["conn"], nrm, val, Dps=Dps, Dt=Dt, t=c["t"], stab_c=c["stab_c"])
+    K = O.assemble_csr(nen, c["conn"], d.dofnums, d.nalldofs, Ke)
+    nf = d.nfreedofs
+    U = sla.spsolve(K[:nf, :nf].tocsc(), c["F"][:nf])
+    v = d.scattersysvec(U)[c["probe"], c["dir"] - 1] / c["uex"] * 100
+    assert abs(v - c["ref"]) / c["ref"] < (1e-8 if which < 2 else 1e-4), (v, c["ref"])
+
+
+def test_le5_z_cantilever():
+    """LE5 Z-section cantilever of test_shell_statics.jl:438-531 on the
+    reference's folded Abaqus mesh (nodal normals invalid along the folds):
+    the minimum and maximum deflection of its @test lines."""
+    c = SC.le5()
+    d = c["dchi"]
+    K = _homog_K(c)
+    nf = d.nfreedofs
+    w = d.scattersysvec(sla.spsolve(K[:nf, :nf].tocsc(), c["F"][:nf]))[:, 2]
+    assert abs(w.min() - SC.LE5_W[0]) / abs(SC.LE5_W[0]) < 1e-9, w.min()
+    assert abs(w.max() - SC.LE5_W[1]) / abs(SC.LE5_W[1]) < 1e-9, w.max()
