@@ -427,3 +427,32 @@ def le5():
         nl = fe.selectnode(xyz, [10.0, 10.0, y, y, 0, 0], tol)
         F[d.dofnums[nl[0], 2]] += fz
     return dict(nen=3, xyz=xyz, conn=conn, dchi=d, E=210e9, nu=0.3, rho=0.0, t=t, stab_c=0.2, F=F)
+
+
+# test/test_composite_shell_dynamics.jl:93-104 (and the same list for the
+# layered model): 8 lowest frequencies (Hz) of a simply supported square
+# plate, isotropic, homogeneous T3FF and [45/-45/45/-45] T3FFComp
+PETYT_FS = [21.822774909379287, 54.52203720717488, 54.60475772077202, 86.5480437899711, 108.65349048567792,
+            108.78050954575491, 138.29506871044424, 140.89370172063866]
+
+
+def petyt(composite):
+    """mcompshelldyn0 (test_composite_shell_dynamics.jl:14-92 and 106-190):
+    1 m square plate, t = 1 mm, E 1000 MPa, nu 0.396, rho 2, 8 x 8 T3 mesh;
+    boundary translations fixed (the layered model also pins a corner)."""
+    ax = ay = 1.0
+    nx = ny = 8
+    tol = ax / nx / 100
+    xy, conn = fe.T3block(ax, ay, nx, ny)
+    xyz = fe.xyz3(xy)
+    d = fe.DofField(xyz.shape[0])
+    if composite:
+        d.setebc(fe.selectnode(xyz, [ax, ax, 0, 0, -np.inf, np.inf], tol), [1, 2, 3])
+        d.setebc(fe.selectnode(xyz, [0, 0, 0, 0, -np.inf, np.inf], tol), [2])
+    d.setebc(fe.boundary_nodes(conn), [1, 2, 3])
+    d.numberdofs()
+    t = ax / 1000
+    D6 = O.iso_D6(1000e6, 0.396)
+    plies = [(D6, t / 4, a, 2.0) for a in (45.0, -45.0, 45.0, -45.0)]
+    return dict(nen=3, xyz=xyz, conn=conn, dchi=d, E=1000e6, nu=0.396, rho=2.0, t=t, stab_c=None, plies=plies,
+                csys=np.eye(3), th=t)

@@ -176,6 +176,29 @@ def test_twisted_beam_gpu(ctx, nen, which):
     assert abs(v - c["ref"]) / c["ref"] < (1e-8 if which < 2 else 1e-4), (v, c["ref"])
 
 
+@pytest.mark.parametrize("composite", [False, True])
+def test_petyt_plate_frequencies_gpu(ctx, composite):
+    """mcompshelldyn0 of test_composite_shell_dynamics.jl with the FF-block K
+    and lumped M from the device: the 8 reference frequencies within 1e-9."""
+    print(f"[gpu] petyt plate composite={composite}", flush=True)
+    c = SC.petyt(composite)
+    d = c["dchi"]
+    nf = d.nfreedofs
+    if composite:
+        mat = lamina_material(c["rho"], c["E"], c["nu"])  # isotropic lamina, as the reference's
+        lay = CompositeLayup("petyt", [Ply(f"p{i}", mat, t, ang) for i, (_, t, ang, _) in enumerate(c["plies"])],
+                             c["csys"])
+        f = FM.FEMMShellT3FFComp(c["conn"], lay, ctx=ctx)
+    else:
+        f = _homog_femm(ctx, 3, c["conn"], c["E"], c["nu"], c["rho"], c["t"])
+    f.associategeometry(c["xyz"])
+    K = f.stiffness(FM.SysmatAssemblerFFBlock(nf), c["xyz"], None, None, d)
+    M = f.mass(FM.SysmatAssemblerFFBlock(FM.SysmatAssemblerSparseDiag(), nf, nf), c["xyz"], d).diagonal()
+    w = sl.eigh(K.toarray(), np.diag(M), eigvals_only=True, subset_by_index=[0, 7])
+    fs = np.sqrt(w) / (2 * np.pi)
+    assert np.linalg.norm(fs - SC.PETYT_FS) < 1e-9 * np.linalg.norm(fs), fs
+
+
 def test_le5_z_cantilever_gpu(ctx):
     """LE5 Z-section cantilever (test_shell_statics.jl:438-531) with K from
     the device, normals from the device (folds: invalid normals): the

@@ -315,3 +315,23 @@ def test_le5_z_cantilever():
     w = d.scattersysvec(sla.spsolve(K[:nf, :nf].tocsc(), c["F"][:nf]))[:, 2]
     assert abs(w.min() - SC.LE5_W[0]) / abs(SC.LE5_W[0]) < 1e-9, w.min()
     assert abs(w.max() - SC.LE5_W[1]) / abs(SC.LE5_W[1]) < 1e-9, w.max()
+
+
+@pytest.mark.parametrize("composite", [False, True])
+def test_petyt_plate_frequencies(composite):
+    """mcompshelldyn0 of test_composite_shell_dynamics.jl: the 8 lowest
+    frequencies of the homogeneous T3FF plate and of the [45/-45/45/-45]
+    T3FFComp plate of isotropic plies (reference bar: 1e-3 of the norm)."""
+    c = SC.petyt(composite)
+    d = c["dchi"]
+    nf = d.nfreedofs
+    if composite:
+        K, (m0, m1) = _layered_K(c)
+        tm, rm = O.element_masses(3, c["xyz"], c["conn"], m0=m0, m1=m1)
+    else:
+        K = _homog_K(c)
+        tm, rm = O.element_masses(3, c["xyz"], c["conn"], rho=c["rho"], t=c["t"])
+    M = O.lumped_diag(3, c["conn"], d.dofnums, d.nalldofs, tm, rm)
+    w = sl.eigh(K[:nf, :nf].toarray(), np.diag(M[:nf]), eigvals_only=True, subset_by_index=[0, 7])
+    fs = np.sqrt(w) / (2 * np.pi)
+    assert np.linalg.norm(fs - SC.PETYT_FS) < 1e-9 * np.linalg.norm(fs), fs
