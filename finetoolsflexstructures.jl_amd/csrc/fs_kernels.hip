@@ -397,7 +397,7 @@ __device__ __forceinline__ void q4_mitc_bs(const double ec[4][2], double r, doub
 
 
 template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void elem_kernel(ElemParams p, const double *__restrict__ xs,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_kernel(ElemParams p, const double *__restrict__ xs,
                                                   const double *__restrict__ ys,
                                                   const double *__restrict__ zs,
                                                   const int32_t *__restrict__ conn,
