@@ -22,6 +22,7 @@ sys.path.insert(0, os.path.join(ROOT, "finetoolsflexstructures.jl_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+FP64_PEAK_TFS = 78.6  # MI355X FP64 vector (and FP64 MFMA), dense
 
 
 def assembly_bytes(E, N, nen, nnz):
@@ -409,9 +410,14 @@ def main():
     # HBM bytes per assembly from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     # summary of this workload (512 x 512 only)
     traffic = None
+    elem_flops = None
     tf = os.path.join(ROOT, "profiles", "traffic_q4_cylinder_512x512.json")
     if args.n == 512 and os.path.exists(tf):
-        traffic = json.load(open(tf)).get("assembly_bytes_per_launch")
+        tj = json.load(open(tf))
+        traffic = tj.get("assembly_bytes_per_launch")
+        # FP64 operations the element kernel issues per launch (rocprofv3
+        # SQ_INSTS_VALU_*_F64 of this workload)
+        elem_flops = tj.get("elem_kernel", {}).get("fp64_ops_issued")
 
     out = {
         "metric": "FP64 shell elements assembled/s + DOF-steps/s; p50/p99 latency vs plain GPU",
@@ -441,6 +447,12 @@ def main():
         "roofline_cd": {"bound": "hbm", "achieved": ach_cd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach_cd / HBM_PEAK_GBS, "kernel": "cd_step_kernel",
                         "algorithmic_bytes": b_cd},
+        "roofline_elem": (None if elem_flops is None else
+                          {"bound": "fp64-valu", "achieved": elem_flops / (np.mean(elem_ms) / 1e3) / 1e12,
+                           "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                           "frac": elem_flops / (np.mean(elem_ms) / 1e3) / 1e12 / FP64_PEAK_TFS,
+                           "kernel": "elem_q4_seq_kernel (FP64 operations issued, from the committed counters)",
+                           "fp64_ops_per_launch": elem_flops}),
         "kernels_ms": {"elem_kernel": float(np.mean(elem_ms)), "scatter_kernel": float(np.mean(scat_ms)),
                        "note": "each kernel alone (pipeline of 1 range)"},
         "assembly_pipeline": {"ranges": args.pipeline, "device_ms": dev_ms,
