@@ -7,8 +7,12 @@
 // normals and the time loop runs in the kernels of fs_kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <sched.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <cstdarg>
@@ -107,6 +111,10 @@ struct fs_ctx {
     hipStream_t stream2 = nullptr;
     int asm_chunks = 1;
     std::vector<hipEvent_t> ev;  // ordering events of the pipeline (no timing)
+    // pinned staging buffers of the large device-to-host copies (allocated
+    // on first use, FS_STAGE_BYTES each) and their completion events
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -152,7 +160,12 @@ struct Block {
     int64_t nnz = 0;
     int64_t nnodes = 0;
     std::vector<int64_t> rowptr;   // nrows + 1
-    std::vector<int32_t> colval;   // nnz
+    // column lists, kept as per-node structure and expanded on request
+    // (fs_model_pattern_get): code[J*6+c] offset of comp c within J's free
+    // (0..5) or fixed (8..13) run, -1 not in the block; off_free[k] /
+    // off_fix[k] offset of neighbour slot k's free / fixed run in the row
+    std::vector<int8_t> code;
+    std::vector<int32_t> off_free, off_fix;
     // scatter plan (device)
     int64_t npairs = 0;
     // running maximum over the pairs, in pair order, of the last element
@@ -169,7 +182,9 @@ struct Block {
         built = k_ready = m_ready = false;
         nrows = nnz = npairs = 0;
         rowptr.clear();
-        colval.clear();
+        code.clear();
+        off_free.clear();
+        off_fix.clear();
         pair_emax.clear();
         for (DevBuf *b : {&nbr_ptr, &nbr_node, &nbr_pair, &nbr_off, &node_code, &node_rowstart, &contrib_ptr,
                           &contrib, &pair_node, &pair_slot, &rowrec, &slotrec, &kval, &mdiag})
@@ -220,11 +235,28 @@ struct fs_model {
 
 namespace {
 
-// Runs fn(begin, end) over [0, n) in up to hardware_concurrency() chunks on
-// threads of their own; for host loops whose iterations write disjoint data.
+// Host threads for the parallel loops: the CPUs this process may run on,
+// capped by OMP_NUM_THREADS when that is set.
+int64_t host_threads() {
+    static const int64_t nt = [] {
+        int64_t n = std::max<int64_t>(1, std::thread::hardware_concurrency());
+        cpu_set_t set;
+        if (sched_getaffinity(0, sizeof set, &set) == 0) n = std::max(1, CPU_COUNT(&set));
+        if (const char *e = std::getenv("OMP_NUM_THREADS")) {
+            const long v = std::strtol(e, nullptr, 10);
+            if (v > 0) n = std::min<int64_t>(n, v);
+        }
+        return n;
+    }();
+    return nt;
+}
+
+// Runs fn(begin, end) over [0, n) in up to host_threads() chunks of at
+// least `grain` iterations on threads of their own; for host loops whose
+// iterations write disjoint data.
 template <class F>
-void parallel_chunks(int64_t n, F fn) {
-    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), n / 65536));
+void parallel_chunks(int64_t n, F fn, int64_t grain = 65536) {
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(host_threads(), n / grain));
     if (nt <= 1) {
         fn((int64_t)0, n);
         return;
@@ -299,12 +331,42 @@ int upload(fs_ctx *c, DevBuf &b, const T *src, size_t n) {
     return FS_OK;
 }
 
+constexpr size_t FS_STAGE_BYTES = size_t(32) << 20;
+
 // Device-to-host copy ordered on the context's own stream, so that contexts
-// sharing a device never serialise on the null stream.
+// sharing a device never serialise on the null stream.  Copies of more than
+// two staging buffers go through the context's pinned buffers in chunks: the
+// DMA of chunk i+1 runs while host threads copy chunk i into `dst` (which
+// also spreads the first-touch page faults of a fresh destination over the
+// threads).
 int d2h(fs_ctx *c, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return FS_OK;
-    FS_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
-    FS_HIP(c, hipStreamSynchronize(c->stream));
+    if (bytes <= 2 * FS_STAGE_BYTES) {
+        FS_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        FS_HIP(c, hipStreamSynchronize(c->stream));
+        return FS_OK;
+    }
+    for (int i = 0; i < 2; ++i) {
+        if (!c->stage[i]) FS_HIP(c, hipHostMalloc(&c->stage[i], FS_STAGE_BYTES, hipHostMallocDefault));
+        if (!c->stage_ev[i]) FS_HIP(c, hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
+    }
+    const size_t nch = (bytes + FS_STAGE_BYTES - 1) / FS_STAGE_BYTES;
+    auto chunk = [&](size_t i) { return std::min(FS_STAGE_BYTES, bytes - i * FS_STAGE_BYTES); };
+    auto enqueue = [&](size_t i) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(c->stage[i % 2], (const char *)src + i * FS_STAGE_BYTES, chunk(i),
+                                      hipMemcpyDeviceToHost, c->stream);
+        return e != hipSuccess ? e : hipEventRecord(c->stage_ev[i % 2], c->stream);
+    };
+    FS_HIP(c, enqueue(0));
+    for (size_t i = 0; i < nch; ++i) {
+        // buffer (i+1) % 2 was last read by the host copy of chunk i-1, which has returned
+        if (i + 1 < nch) FS_HIP(c, enqueue(i + 1));
+        FS_HIP(c, hipEventSynchronize(c->stage_ev[i % 2]));
+        const char *from = (const char *)c->stage[i % 2];
+        char *to = (char *)dst + i * FS_STAGE_BYTES;
+        parallel_chunks((int64_t)chunk(i), [&](int64_t b0, int64_t b1) { std::memcpy(to + b0, from + b0, b1 - b0); },
+                        int64_t(1) << 20);
+    }
     return FS_OK;
 }
 
@@ -341,46 +403,66 @@ int build_block(fs_model *m, int which) {
     const int64_t nrows = which == FS_BLOCK_FF ? m->nfree : m->nall;
     const int64_t nfree = m->nfree;
     // per-node codes: offset of each comp within the node's free / fixed run
-    std::vector<int8_t> code(N * 6, -1);
+    std::vector<int8_t> &code = B.code;
+    code.assign(N * 6, -1);
     std::vector<int8_t> nfr(N, 0), nfx(N, 0);
     std::vector<int64_t> minfr(N, INT64_MAX), minfx(N, INT64_MAX);
-    for (int64_t I = 0; I < N; ++I) {
-        int64_t fr[6], fx[6];
-        int a = 0, b = 0;
-        for (int cc = 0; cc < 6; ++cc) {
-            const int64_t d = m->dof[I * 6 + cc];
-            if (d >= nrows) continue;
-            if (d < nfree)
-                fr[a++] = d;
-            else
-                fx[b++] = d;
+    // lowest node (2 I + 0: free, 2 I + 1: fixed) whose dofs are not consecutive
+    std::atomic<int64_t> bad{INT64_MAX};
+    auto flag = [&](int64_t v) {
+        int64_t cur = bad.load();
+        while (v < cur && !bad.compare_exchange_weak(cur, v)) {
         }
-        std::sort(fr, fr + a);
-        std::sort(fx, fx + b);
-        if (a && fr[a - 1] - fr[0] + 1 != a)
-            return fail(c, FS_ERR_UNSUPPORTED, "node %lld: free dofs are not consecutive numbers", (long long)I);
-        if (b && fx[b - 1] - fx[0] + 1 != b)
-            return fail(c, FS_ERR_UNSUPPORTED, "node %lld: fixed dofs are not consecutive numbers", (long long)I);
-        for (int cc = 0; cc < 6; ++cc) {
-            const int64_t d = m->dof[I * 6 + cc];
-            if (d >= nrows) continue;
-            if (d < nfree)
-                code[I * 6 + cc] = (int8_t)(d - fr[0]);
-            else
-                code[I * 6 + cc] = (int8_t)(8 + (d - fx[0]));
+    };
+    parallel_chunks(N, [&](int64_t I0, int64_t I1) {
+        for (int64_t I = I0; I < I1; ++I) {
+            int64_t fr[6], fx[6];
+            int a = 0, b = 0;
+            for (int cc = 0; cc < 6; ++cc) {
+                const int64_t d = m->dof[I * 6 + cc];
+                if (d >= nrows) continue;
+                if (d < nfree)
+                    fr[a++] = d;
+                else
+                    fx[b++] = d;
+            }
+            std::sort(fr, fr + a);
+            std::sort(fx, fx + b);
+            if (a && fr[a - 1] - fr[0] + 1 != a) {
+                flag(2 * I);
+                continue;
+            }
+            if (b && fx[b - 1] - fx[0] + 1 != b) {
+                flag(2 * I + 1);
+                continue;
+            }
+            for (int cc = 0; cc < 6; ++cc) {
+                const int64_t d = m->dof[I * 6 + cc];
+                if (d >= nrows) continue;
+                if (d < nfree)
+                    code[I * 6 + cc] = (int8_t)(d - fr[0]);
+                else
+                    code[I * 6 + cc] = (int8_t)(8 + (d - fx[0]));
+            }
+            nfr[I] = (int8_t)a;
+            nfx[I] = (int8_t)b;
+            if (a) minfr[I] = fr[0];
+            if (b) minfx[I] = fx[0];
         }
-        nfr[I] = (int8_t)a;
-        nfx[I] = (int8_t)b;
-        if (a) minfr[I] = fr[0];
-        if (b) minfx[I] = fx[0];
-    }
+    });
+    if (bad.load() != INT64_MAX)
+        return fail(c, FS_ERR_UNSUPPORTED, "node %lld: %s dofs are not consecutive numbers",
+                    (long long)(bad.load() / 2), bad.load() % 2 ? "fixed" : "free");
     // per node: sorted column list = runs of the neighbours ordered by first dof
     // (runs are contiguous integer ranges, so sorting runs sorts the columns)
+    const size_t nnb = m->nbr_list.size();
     std::vector<int32_t> rowlen(N, 0);
-    std::vector<int32_t> nb_offfree(m->nbr_list.size(), -1), nb_offfix(m->nbr_list.size(), -1);
-    {
+    std::vector<int32_t> &nb_offfree = B.off_free, &nb_offfix = B.off_fix;
+    nb_offfree.assign(nnb, -1);
+    nb_offfix.assign(nnb, -1);
+    parallel_chunks(N, [&](int64_t I0, int64_t I1) {
         std::vector<std::pair<int64_t, int64_t>> runs;  // (first dof, nbr slot*2 + kind)
-        for (int64_t I = 0; I < N; ++I) {
+        for (int64_t I = I0; I < I1; ++I) {
             runs.clear();
             for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
                 const int J = m->nbr_list[k];
@@ -402,7 +484,7 @@ int build_block(fs_model *m, int which) {
             }
             rowlen[I] = pos;
         }
-    }
+    });
     // row owners
     std::vector<int64_t> owner(nrows, -1);
     for (int64_t I = 0; I < N; ++I)
@@ -416,42 +498,19 @@ int build_block(fs_model *m, int which) {
         B.rowptr[d + 1] = B.rowptr[d] + rowlen[owner[d]];
     }
     const int64_t nnz = B.rowptr[nrows];
-    try {
-        B.colval.assign(nnz, 0);
-    } catch (const std::bad_alloc &) {
-        return fail(c, FS_ERR_NOMEM, "host pattern of %lld entries", (long long)nnz);
-    }
-    // column lists: every node writes only its own rows
-    parallel_chunks(N, [&](int64_t I0, int64_t I1) {
-        std::vector<int32_t> cols;
-        for (int64_t I = I0; I < I1; ++I) {
-            cols.assign(rowlen[I], 0);
-            for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
-                const int J = m->nbr_list[k];
-                for (int cc = 0; cc < 6; ++cc) {
-                    const int cd = code[(int64_t)J * 6 + cc];
-                    if (cd < 0) continue;
-                    const int pos = cd < 8 ? nb_offfree[k] + cd : nb_offfix[k] + cd - 8;
-                    cols[pos] = (int32_t)m->dof[(int64_t)J * 6 + cc];
-                }
-            }
-            for (int cc = 0; cc < 6; ++cc) {
-                const int64_t d = m->dof[I * 6 + cc];
-                if (d >= nrows) continue;
-                std::copy(cols.begin(), cols.end(), B.colval.begin() + B.rowptr[d]);
-            }
-        }
-    });
     if (nnz >= (int64_t(1) << 31))
         return fail(c, FS_ERR_UNSUPPORTED, "block with %lld stored entries exceeds 32-bit positions", (long long)nnz);
     // upper node pairs: the neighbours J >= I of each node, in order
     std::vector<int64_t> pair_base(N + 1, 0);
-    for (int64_t I = 0; I < N; ++I) {
-        int cnt = 0;
-        for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k)
-            if (m->nbr_list[k] >= I) ++cnt;
-        pair_base[I + 1] = pair_base[I] + cnt;
-    }
+    parallel_chunks(N, [&](int64_t I0, int64_t I1) {
+        for (int64_t I = I0; I < I1; ++I) {
+            int cnt = 0;
+            for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k)
+                if (m->nbr_list[k] >= I) ++cnt;
+            pair_base[I + 1] = cnt;
+        }
+    });
+    for (int64_t I = 0; I < N; ++I) pair_base[I + 1] += pair_base[I];
     const int64_t npairs = pair_base[N];
     if (npairs >= (int64_t(1) << 27)) return fail(c, FS_ERR_UNSUPPORTED, "too many node pairs (%lld)", (long long)npairs);
     auto slot_of = [&](int64_t I, int64_t J) -> int {
@@ -459,90 +518,117 @@ int build_block(fs_model *m, int which) {
         const int32_t *e0 = m->nbr_list.data() + m->nbr_ptr[I + 1];
         return (int)(std::lower_bound(b0, e0, (int32_t)J) - m->nbr_list.data());
     };
-    auto pair_index = [&](int64_t lo, int64_t hi) -> int64_t {
-        return pair_base[lo] + (slot_of(lo, hi) - slot_of(lo, lo));
-    };
-    // neighbour tables: pair, side, run offsets; node row starts
-    const size_t nnb = m->nbr_list.size();
+    // slot of each node in its own list: the upper pairs of I are its slots
+    // from diag[I] on, numbered from pair_base[I]
+    std::vector<int32_t> diag(N);
+    parallel_chunks(N, [&](int64_t I0, int64_t I1) {
+        for (int64_t I = I0; I < I1; ++I) diag[I] = slot_of(I, I);
+    });
+    // neighbour tables: pair, side, run offsets; node row starts (each node
+    // writes its own slots, its own upper pairs and its own rows)
     std::vector<int32_t> nb_pair(nnb), nb_off(2 * nnb), rowstart(N * 6, -1);
     std::vector<int32_t> pnode(2 * npairs), pslot(2 * npairs);
-    for (int64_t I = 0; I < N; ++I) {
-        for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
-            const int J = m->nbr_list[k];
-            const int64_t p = I <= J ? pair_index(I, J) : pair_index(J, I);
-            nb_pair[k] = (int32_t)((p << 1) | (I > J ? 1 : 0));
-            nb_off[2 * k] = nb_offfree[k];
-            nb_off[2 * k + 1] = nb_offfix[k];
-            if (I <= J) {
-                pnode[2 * p] = (int32_t)I;
-                pnode[2 * p + 1] = J;
-                pslot[2 * p] = k;
-                pslot[2 * p + 1] = slot_of(J, I);
+    parallel_chunks(N, [&](int64_t I0, int64_t I1) {
+        for (int64_t I = I0; I < I1; ++I) {
+            for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
+                const int J = m->nbr_list[k];
+                const int64_t p = I <= J ? pair_base[I] + (k - diag[I]) : pair_base[J] + (slot_of(J, I) - diag[J]);
+                nb_pair[k] = (int32_t)((p << 1) | (I > J ? 1 : 0));
+                nb_off[2 * k] = nb_offfree[k];
+                nb_off[2 * k + 1] = nb_offfix[k];
+                if (I <= J) {
+                    pnode[2 * p] = (int32_t)I;
+                    pnode[2 * p + 1] = J;
+                    pslot[2 * p] = k;
+                    pslot[2 * p + 1] = slot_of(J, I);
+                }
+            }
+            for (int cc = 0; cc < 6; ++cc) {
+                const int64_t d = m->dof[I * 6 + cc];
+                if (d < nrows) rowstart[I * 6 + cc] = (int32_t)B.rowptr[d];
             }
         }
-        for (int cc = 0; cc < 6; ++cc) {
-            const int64_t d = m->dof[I * 6 + cc];
-            if (d < nrows) rowstart[I * 6 + cc] = (int32_t)B.rowptr[d];
-        }
-    }
+    });
     // contributions, element order within each pair
     const int nen = m->nen;
     const int nblk = nen * (nen + 1) / 2;
     if ((int64_t)m->E * nblk >= (int64_t(1) << 30))
         return fail(c, FS_ERR_UNSUPPORTED, "too many element blocks (%lld)", (long long)(m->E * nblk));
-    std::vector<int32_t> cptr(npairs + 1, 0);
-    std::vector<int64_t> cpair((size_t)m->E * nblk);
-    for (int64_t e = 0; e < m->E; ++e) {
-        int bl = 0;
-        for (int a = 0; a < nen; ++a)
-            for (int b = a; b < nen; ++b, ++bl) {
-                const int64_t I = m->conn[e * nen + a], J = m->conn[e * nen + b];
-                const int64_t p = pair_index(std::min(I, J), std::max(I, J));
-                cpair[e * nblk + bl] = p;
-                cptr[p + 1]++;
+    // The contributions of the upper pair (I, J) are the elements incident to
+    // both nodes (an intersection of the two ascending incidence lists, so in
+    // element order), each with the local block (a <= b) that holds the pair
+    // and whether the block's row node is J (transposed).  Two passes over
+    // the pairs: counts, then the lists.
+    auto pair_contribs = [&](int64_t p, int32_t *outl) -> int {
+        const int I = pnode[2 * p], J = pnode[2 * p + 1];
+        int ki = m->ne_ptr[I], kj = m->ne_ptr[J], cnt = 0;
+        const int ei = m->ne_ptr[I + 1], ej = m->ne_ptr[J + 1];
+        while (ki < ei && kj < ej) {
+            const int32_t e1 = m->ne_list[ki], e2 = m->ne_list[kj];
+            if (e1 < e2) {
+                ++ki;
+            } else if (e2 < e1) {
+                ++kj;
+            } else {
+                if (outl) {
+                    const int32_t *ce = m->conn.data() + (int64_t)e1 * nen;
+                    int a = 0, b = 0;
+                    for (int q = 0; q < nen; ++q) {
+                        if (ce[q] == I) a = q;
+                        if (ce[q] == J) b = q;
+                    }
+                    const int lo = std::min(a, b), hi = std::max(a, b);
+                    const int bl = lo * nen - lo * (lo - 1) / 2 + (hi - lo);  // upper pairs in row order
+                    const int tr = ce[lo] > ce[hi] ? 1 : 0;
+                    outl[cnt] = (int32_t)((((int64_t)e1 * nblk + bl) << 1) | tr);
+                }
+                ++cnt;
+                ++ki;
+                ++kj;
             }
-    }
-    for (int64_t p = 0; p < npairs; ++p) cptr[p + 1] += cptr[p];
-    std::vector<int32_t> cfill(cptr.begin(), cptr.end() - 1), clist(cptr[npairs]);
-    B.pair_emax.assign(npairs, -1);
-    for (int64_t e = 0; e < m->E; ++e)
-        for (int bl = 0; bl < nblk; ++bl) {
-            int32_t &x = B.pair_emax[cpair[e * nblk + bl]];
-            x = std::max(x, (int32_t)e);
         }
+        return cnt;
+    };
+    std::vector<int32_t> cptr(npairs + 1, 0);
+    parallel_chunks(npairs, [&](int64_t p0, int64_t p1) {
+        for (int64_t p = p0; p < p1; ++p) cptr[p + 1] = pair_contribs(p, nullptr);
+    });
+    for (int64_t p = 0; p < npairs; ++p) cptr[p + 1] += cptr[p];
+    std::vector<int32_t> clist(cptr[npairs]);
+    B.pair_emax.assign(npairs, -1);
+    parallel_chunks(npairs, [&](int64_t p0, int64_t p1) {
+        for (int64_t p = p0; p < p1; ++p) {
+            const int n = pair_contribs(p, clist.data() + cptr[p]);
+            if (n) B.pair_emax[p] = (clist[cptr[p] + n - 1] >> 1) / nblk;  // last (largest) element
+        }
+    });
     for (int64_t p = 1; p < npairs; ++p) B.pair_emax[p] = std::max(B.pair_emax[p], B.pair_emax[p - 1]);
-    for (int64_t e = 0; e < m->E; ++e) {
-        int bl = 0;
-        for (int a = 0; a < nen; ++a)
-            for (int b = a; b < nen; ++b, ++bl) {
-                const int64_t I = m->conn[e * nen + a], J = m->conn[e * nen + b];
-                const int tr = (I > J) ? 1 : 0;
-                const int64_t p = cpair[e * nblk + bl];
-                clist[cfill[p]++] = (int32_t)(((e * nblk + bl) << 1) | tr);
-            }
-    }
     // run records of the free-free block (fs::RunRows): every row of node I
     // is the runs of I's neighbours in slot order, each run the neighbour's
     // free dofs (consecutive columns starting at its first free dof)
     std::vector<int32_t> rrec, srec;
     if (which == FS_BLOCK_FF) {
         rrec.assign(4 * nrows, 0);
-        for (int64_t d = 0; d < nrows; ++d) {
-            const int64_t I = owner[d];
-            rrec[4 * d + 0] = (int32_t)B.rowptr[d];
-            rrec[4 * d + 1] = m->nbr_ptr[I];
-            rrec[4 * d + 2] = m->nbr_ptr[I + 1];
-        }
+        parallel_chunks(nrows, [&](int64_t d0, int64_t d1) {
+            for (int64_t d = d0; d < d1; ++d) {
+                const int64_t I = owner[d];
+                rrec[4 * d + 0] = (int32_t)B.rowptr[d];
+                rrec[4 * d + 1] = m->nbr_ptr[I];
+                rrec[4 * d + 2] = m->nbr_ptr[I + 1];
+            }
+        });
+        for (int64_t I = 0; I < N; ++I)
+            if (rowlen[I] >= (1 << 27)) return fail(c, FS_ERR_UNSUPPORTED, "row of more than 2^27 entries");
         srec.assign(2 * nnb, 0);
-        for (size_t k = 0; k < nnb; ++k) {
-            const int J = m->nbr_list[k];
-            const int nf = nfr[J];
-            if (nf == 0) continue;  // no free dofs: an empty run
-            if (nb_offfree[k] >= (1 << 27))
-                return fail(c, FS_ERR_UNSUPPORTED, "row of more than 2^27 entries");
-            srec[2 * k] = (nb_offfree[k] << 4) | nf;
-            srec[2 * k + 1] = (int32_t)minfr[J];
-        }
+        parallel_chunks((int64_t)nnb, [&](int64_t k0, int64_t k1) {
+            for (int64_t k = k0; k < k1; ++k) {
+                const int J = m->nbr_list[k];
+                const int nf = nfr[J];
+                if (nf == 0) continue;  // no free dofs: an empty run
+                srec[2 * k] = (nb_offfree[k] << 4) | nf;
+                srec[2 * k + 1] = (int32_t)minfr[J];
+            }
+        });
     }
     // upload
     int s;
@@ -910,6 +996,10 @@ void fs_ctx_destroy(fs_ctx *ctx) {
         (void)hipStreamDestroy(ctx->stream2);
     }
     for (hipEvent_t ev : ctx->ev) (void)hipEventDestroy(ev);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->stage[i]) (void)hipHostFree(ctx->stage[i]);
+        if (ctx->stage_ev[i]) (void)hipEventDestroy(ctx->stage_ev[i]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -958,31 +1048,42 @@ int fs_model_create(fs_ctx *ctx, int nen, int64_t nnodes, const double *xyz, int
         for (int64_t e = 0; e < nelem; ++e)
             for (int a = 0; a < nen; ++a) m->ne_list[fill[m->conn[e * nen + a]]++] = (int32_t)e;
     }
-    // node -> neighbour lists (sorted, including the node itself)
+    // node -> neighbour lists (sorted, including the node itself); a node in
+    // no element has no neighbours: its rows stay empty, as in the COO
+    // assembly.  Two passes over the nodes (sizes, then lists).
     m->nbr_ptr.assign(nnodes + 1, 0);
-    {
-        std::vector<int32_t> tmp;
-        std::vector<std::vector<int32_t>> lists(nnodes);
-        for (int64_t n = 0; n < nnodes; ++n) {
-            tmp.clear();
-            for (int k = m->ne_ptr[n]; k < m->ne_ptr[n + 1]; ++k) {
-                const int e = m->ne_list[k];
-                for (int a = 0; a < nen; ++a) tmp.push_back(m->conn[(int64_t)e * nen + a]);
-            }
-            std::sort(tmp.begin(), tmp.end());
-            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-            // a node in no element has no neighbours: its rows stay empty, as in the COO assembly
-            lists[n] = tmp;
-            m->nbr_ptr[n + 1] = m->nbr_ptr[n] + (int32_t)tmp.size();
+    auto neighbours = [&](int64_t n, std::vector<int32_t> &tmp) {
+        tmp.clear();
+        for (int k = m->ne_ptr[n]; k < m->ne_ptr[n + 1]; ++k) {
+            const int e = m->ne_list[k];
+            for (int a = 0; a < nen; ++a) tmp.push_back(m->conn[(int64_t)e * nen + a]);
         }
-        m->nbr_list.resize(m->nbr_ptr[nnodes]);
-        for (int64_t n = 0; n < nnodes; ++n) std::copy(lists[n].begin(), lists[n].end(), m->nbr_list.begin() + m->nbr_ptr[n]);
-    }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    };
+    parallel_chunks(nnodes, [&](int64_t n0, int64_t n1) {
+        std::vector<int32_t> tmp;
+        for (int64_t n = n0; n < n1; ++n) {
+            neighbours(n, tmp);
+            m->nbr_ptr[n + 1] = (int32_t)tmp.size();
+        }
+    });
+    for (int64_t n = 0; n < nnodes; ++n) m->nbr_ptr[n + 1] += m->nbr_ptr[n];
+    m->nbr_list.resize(m->nbr_ptr[nnodes]);
+    parallel_chunks(nnodes, [&](int64_t n0, int64_t n1) {
+        std::vector<int32_t> tmp;
+        for (int64_t n = n0; n < n1; ++n) {
+            neighbours(n, tmp);
+            std::copy(tmp.begin(), tmp.end(), m->nbr_list.begin() + m->nbr_ptr[n]);
+        }
+    });
     // device geometry (SoA) and connectivity planes
     int s;
     std::vector<int32_t> planes((size_t)nelem * nen);
-    for (int64_t e = 0; e < nelem; ++e)
-        for (int a = 0; a < nen; ++a) planes[(size_t)a * nelem + e] = m->conn[e * nen + a];
+    parallel_chunks(nelem, [&](int64_t e0, int64_t e1) {
+        for (int64_t e = e0; e < e1; ++e)
+            for (int a = 0; a < nen; ++a) planes[(size_t)a * nelem + e] = m->conn[e * nen + a];
+    });
     if ((s = upload(ctx, m->x, xyz, nnodes)) || (s = upload(ctx, m->y, xyz + nnodes, nnodes)) ||
         (s = upload(ctx, m->z, xyz + 2 * nnodes, nnodes)) || (s = upload(ctx, m->dconn, planes.data(), planes.size())) ||
         (s = upload(ctx, m->dne_ptr, m->ne_ptr.data(), m->ne_ptr.size())) ||
@@ -1223,11 +1324,37 @@ int fs_model_pattern_get(fs_model *m, int block, int64_t *rowptr, int64_t *colva
     if (s) return s;
     const Block &B = m->blk[block];
     if (rowptr)
-        for (int64_t i = 0; i <= B.nrows; ++i) rowptr[i] = B.rowptr[i] + index_base;
-    if (colval)
-        parallel_chunks(B.nnz, [&](int64_t i0, int64_t i1) {
-            for (int64_t i = i0; i < i1; ++i) colval[i] = (int64_t)B.colval[i] + index_base;
+        parallel_chunks(B.nrows + 1, [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i) rowptr[i] = B.rowptr[i] + index_base;
         });
+    if (colval) {
+        // every row of node I holds the same columns: the runs of I's
+        // neighbours, each at its offset within the row
+        const int64_t nrows = B.nrows;
+        parallel_chunks(m->N, [&](int64_t I0, int64_t I1) {
+            std::vector<int64_t> cols;
+            for (int64_t I = I0; I < I1; ++I) {
+                int64_t r0 = -1;
+                for (int cc = 0; cc < 6 && r0 < 0; ++cc)
+                    if (m->dof[I * 6 + cc] < nrows) r0 = m->dof[I * 6 + cc];
+                if (r0 < 0) continue;  // no row of this node in the block
+                cols.assign(B.rowptr[r0 + 1] - B.rowptr[r0], 0);
+                for (int k = m->nbr_ptr[I]; k < m->nbr_ptr[I + 1]; ++k) {
+                    const int J = m->nbr_list[k];
+                    for (int cc = 0; cc < 6; ++cc) {
+                        const int cd = B.code[(int64_t)J * 6 + cc];
+                        if (cd < 0) continue;
+                        const int pos = cd < 8 ? B.off_free[k] + cd : B.off_fix[k] + cd - 8;
+                        cols[pos] = m->dof[(int64_t)J * 6 + cc] + index_base;
+                    }
+                }
+                for (int cc = 0; cc < 6; ++cc) {
+                    const int64_t d = m->dof[I * 6 + cc];
+                    if (d < nrows) std::memcpy(colval + B.rowptr[d], cols.data(), cols.size() * sizeof(int64_t));
+                }
+            }
+        });
+    }
     return FS_OK;
 }
 
