@@ -445,7 +445,7 @@ def main():
         "dof_steps_per_s": dof_steps,
         "cd_us_per_step": cd_ms / ncd * 1e3,
         "roofline_cd": {"bound": "hbm", "achieved": ach_cd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": ach_cd / HBM_PEAK_GBS, "kernel": "cd_step_kernel",
+                        "frac": ach_cd / HBM_PEAK_GBS, "kernel": "cd_node_step_kernel (8 lanes per node)",
                         "algorithmic_bytes": b_cd},
         "roofline_elem": (None if elem_flops is None else
                           {"bound": "fp64-valu", "achieved": elem_flops / (np.mean(elem_ms) / 1e3) / 1e12,

@@ -148,4 +148,15 @@ struct RunRows {
     const int2 *slotrec;
 };
 
+// Nodes of the free-free block for the node-grouped step (cd_node_step_kernel):
+// one record per node with at least one free dof, in node order.
+//   rec[i]  = {CSR position of the node's first row, that row (first free
+//              dof d0), first and end neighbour slot}
+//   info[i] = row length << 3 | number of rows (free dofs, rows d0 .. d0+nr-1)
+struct NodeRows {
+    int64_t n;
+    const int4 *rec;
+    const int32_t *info;
+};
+
 }  // namespace fs

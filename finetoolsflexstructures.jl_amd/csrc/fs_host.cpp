@@ -62,6 +62,11 @@ hipError_t launch_cd_step(int tpr, const RunRows &rows, const double *kval,
                           const int32_t *probe_slot, double *probe_dst, const int32_t *row_group,
                           const double *group_dt, const double *gtab, hipStream_t st);
 hipError_t launch_spmv(const RunRows &rows, const double *kval, const double *x, double *y, hipStream_t st);
+hipError_t launch_cd_node_step(int nl, int pj, const NodeRows &nrw, const int2 *slotrec, const double *kval,
+                               const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
+                               const double *Cd, const double *Fmag, double gstep, double dt, int damped,
+                               const int32_t *probe_slot, double *probe_dst, const int32_t *row_group,
+                               const double *group_dt, const double *gtab, hipStream_t st);
 hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, const double *d, double *part,
                       double *out, hipStream_t st);
 hipError_t launch_scale(int64_t n, const double *ss, const double *d, const double *x, double *y, hipStream_t st);
@@ -115,6 +120,11 @@ struct fs_ctx {
     // on first use, FS_STAGE_BYTES each) and their completion events
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // row products of the FP64 explicit step: 0 = lanes per row
+    // (cd_step_kernel), 8/16 = lanes per node (cd_node_step_kernel) with
+    // cd_pj positions per lane (0: from the block's longest row)
+    int cd_lanes = 8;
+    int cd_pj = 0;
 };
 
 namespace {
@@ -177,17 +187,22 @@ struct Block {
     // device values (CSR order) and, for the free-free block, the run records
     // of the row products of the time loop (fs::RunRows)
     DevBuf rowrec, slotrec, kval, mdiag;
+    // free-free block: node records of the node-grouped step (fs::NodeRows)
+    DevBuf noderec, nodeinfo;
+    int64_t nrownodes = 0;
+    int32_t max_rowlen = 0;
     bool k_ready = false, m_ready = false;
     void reset() {
         built = k_ready = m_ready = false;
-        nrows = nnz = npairs = 0;
+        nrows = nnz = npairs = nrownodes = 0;
+        max_rowlen = 0;
         rowptr.clear();
         code.clear();
         off_free.clear();
         off_fix.clear();
         pair_emax.clear();
         for (DevBuf *b : {&nbr_ptr, &nbr_node, &nbr_pair, &nbr_off, &node_code, &node_rowstart, &contrib_ptr,
-                          &contrib, &pair_node, &pair_slot, &rowrec, &slotrec, &kval, &mdiag})
+                          &contrib, &pair_node, &pair_slot, &rowrec, &slotrec, &kval, &mdiag, &noderec, &nodeinfo})
             b->release();
     }
 };
@@ -606,8 +621,18 @@ int build_block(fs_model *m, int which) {
     // run records of the free-free block (fs::RunRows): every row of node I
     // is the runs of I's neighbours in slot order, each run the neighbour's
     // free dofs (consecutive columns starting at its first free dof)
-    std::vector<int32_t> rrec, srec;
+    std::vector<int32_t> rrec, srec, nrec, ninfo;
     if (which == FS_BLOCK_FF) {
+        // node records: the nodes with free dofs, in node order
+        for (int64_t I = 0; I < N; ++I)
+            if (nfr[I]) {
+                nrec.push_back((int32_t)B.rowptr[minfr[I]]);
+                nrec.push_back((int32_t)minfr[I]);
+                nrec.push_back(m->nbr_ptr[I]);
+                nrec.push_back(m->nbr_ptr[I + 1]);
+                ninfo.push_back((rowlen[I] << 3) | nfr[I]);
+                B.max_rowlen = std::max(B.max_rowlen, rowlen[I]);
+            }
         rrec.assign(4 * nrows, 0);
         parallel_chunks(nrows, [&](int64_t d0, int64_t d1) {
             for (int64_t d = d0; d < d1; ++d) {
@@ -645,6 +670,9 @@ int build_block(fs_model *m, int which) {
     if (which == FS_BLOCK_FF) {
         if ((s = upload(c, B.rowrec, rrec.data(), rrec.size()))) return s;
         if ((s = upload(c, B.slotrec, srec.data(), srec.size()))) return s;
+        if ((s = upload(c, B.noderec, nrec.data(), nrec.size()))) return s;
+        if ((s = upload(c, B.nodeinfo, ninfo.data(), ninfo.size()))) return s;
+        B.nrownodes = (int64_t)ninfo.size();
     }
     if ((s = dalloc(c, B.kval, (size_t)nnz * sizeof(double)))) return s;
     if ((s = dalloc(c, B.mdiag, (size_t)nrows * sizeof(double)))) return s;
@@ -663,6 +691,14 @@ fs::RunRows runrows_of(const Block &B) {
     r.n = B.nrows;
     r.rowrec = B.rowrec.as<int4>();
     r.slotrec = B.slotrec.as<int2>();
+    return r;
+}
+
+fs::NodeRows noderows_of(const Block &B) {
+    fs::NodeRows r;
+    r.n = B.nrownodes;
+    r.rec = B.noderec.as<int4>();
+    r.info = B.nodeinfo.as<int32_t>();
     return r;
 }
 
@@ -1719,6 +1755,13 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     // per row, up to three runs per lane loaded together (measured fastest
     // of 2, 4, 8 and 16 lanes per row on both families)
     const int tpr = 4;
+    // node-grouped step: positions per lane so that one pass covers the
+    // longest row where the register budget allows (8 lanes: 4..8, 16 lanes: 2..4)
+    int cd_pj = c->cd_pj;
+    if (cd_pj == 0 && c->cd_lanes > 0) {
+        const int lo = c->cd_lanes == 8 ? 4 : 2, hi = c->cd_lanes == 8 ? 8 : 4;
+        cd_pj = std::min(hi, std::max(lo, (B.max_rowlen + c->cd_lanes - 1) / c->cd_lanes));
+    }
     const int32_t *rg = batch ? dRG.as<int32_t>() : nullptr;
     const double *gdt = batch ? dGdt.as<double>() : nullptr;
     const fs::MixedOp op = mixed ? mixed_op(m) : fs::MixedOp{};
@@ -1739,6 +1782,11 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
                                              dInv.as<double>(), dC.as<double>(), dF.as<double>(),
                                              batch ? 0.0 : g[step], p->dt, damped ? 1 : 0, forced[step] ? 1 : 0,
                                              slot, pdst, rg, gdt, gt, st);
+            else if (c->cd_lanes > 0)
+                e = fs::launch_cd_node_step(c->cd_lanes, cd_pj, noderows_of(B), B.slotrec.as<int2>(), B.kval.as<double>(),
+                                            dU[cur].as<double>(), dU[1 - cur].as<double>(), dV.as<double>(),
+                                            dA.as<double>(), dInv.as<double>(), dC.as<double>(), dF.as<double>(),
+                                            batch ? 0.0 : g[step], p->dt, damped ? 1 : 0, slot, pdst, rg, gdt, gt, st);
             else
                 e = fs::launch_cd_step(tpr, runrows_of(B), B.kval.as<double>(), dU[cur].as<double>(), dU[1 - cur].as<double>(),
                                        dV.as<double>(), dA.as<double>(), dInv.as<double>(), dC.as<double>(),
@@ -1868,6 +1916,17 @@ int fs_ctx_set_tuning(fs_ctx *ctx, int nen, int formulation, int32_t variant) {
 int fs_ctx_set_pipeline(fs_ctx *ctx, int ranges) {
     if (!ctx || ranges < 1 || ranges > 64) return FS_ERR_ARG;
     ctx->asm_chunks = ranges;
+    return FS_OK;
+}
+
+int fs_ctx_set_cd_lanes(fs_ctx *ctx, int lanes_per_node, int positions_per_lane) {
+    if (!ctx) return FS_ERR_ARG;
+    const int pj = positions_per_lane;
+    const bool ok = lanes_per_node == 0 || (lanes_per_node == 8 && (pj == 0 || (pj >= 4 && pj <= 8))) ||
+                    (lanes_per_node == 16 && (pj == 0 || (pj >= 2 && pj <= 4)));
+    if (!ok) return fail(ctx, FS_ERR_ARG, "cd lanes: %d lanes per node with %d positions per lane", lanes_per_node, pj);
+    ctx->cd_lanes = lanes_per_node;
+    ctx->cd_pj = lanes_per_node ? pj : 0;
     return FS_OK;
 }
 

@@ -2173,6 +2173,118 @@ __global__ __launch_bounds__(256) void cd_step_kernel(RunRows rows,
     }
 }
 
+// Node-grouped form of the same step: NL lanes per node, for the rows of the
+// node's free dofs.  Those rows are consecutive in the free-free block and
+// hold the same columns (the runs of the node's neighbours), so they are one
+// contiguous stretch of nr x rowlen values.  Lane `lane` takes the positions
+// lane, lane + NL, ... of the row: it finds each position's column in the
+// run records once, loads that displacement once, and multiplies it with the
+// value at that position in each of the nr rows; consecutive lanes read
+// consecutive values of a row, so every load instruction reads contiguous
+// spans of NL values.  The NL partial sums of each row are combined by a
+// butterfly and lane r updates row d0 + r exactly as cd_step_kernel does.
+// The sum of a row depends only on positions within the row, so a model
+// gives bitwise the same result alone and inside a batch.
+// PJ positions per lane are loaded together (the host picks PJ so that NL x
+// PJ covers the longest row of the block where it can).
+template <int NL, int PJ>
+__global__ __launch_bounds__(256) void cd_node_step_kernel(NodeRows nrw, const int2 *__restrict__ slotrec,
+                                                           const double *__restrict__ kval, const double *__restrict__ Ucur,
+                                                           double *__restrict__ Unext, double *__restrict__ V,
+                                                           double *__restrict__ A, const double *__restrict__ invMC,
+                                                           const double *__restrict__ Cd, const double *__restrict__ Fmag,
+                                                           double gstep, double dt, int damped,
+                                                           const int32_t *__restrict__ probe_slot,
+                                                           double *__restrict__ probe_dst,
+                                                           const int32_t *__restrict__ row_group,
+                                                           const double *__restrict__ group_dt,
+                                                           const double *__restrict__ gtab) {
+    const int64_t gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int64_t node = gt / NL;
+    const int lane = (int)(gt % NL);
+    const bool active = node < nrw.n;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int nr = 0, d0 = 0;
+    if (active) {
+        const int4 rec = nrw.rec[node];
+        const int info = nrw.info[node];
+        const int64_t kstart = rec.x;
+        d0 = rec.y;
+        nr = info & 7;
+        const int rowlen = info >> 3;
+        for (int base = lane; base < rowlen; base += NL * PJ) {
+            // values first: their addresses need only the node record
+            double kv[6][PJ];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                const double *kr = kval + kstart + (int64_t)r * rowlen + base;
+#pragma unroll
+                for (int j = 0; j < PJ; ++j)
+                    kv[r][j] = (r < nr && base + j * NL < rowlen) ? kr[j * NL] : 0.0;
+            }
+            int col[PJ];
+#pragma unroll
+            for (int j = 0; j < PJ; ++j) col[j] = 0;
+            for (int k0 = rec.z; k0 < rec.w; k0 += 4) {
+                int2 sr[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) sr[t] = k0 + t < rec.w ? slotrec[k0 + t] : make_int2(0, 0);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int off = sr[t].x >> 4, len = sr[t].x & 15;
+#pragma unroll
+                    for (int j = 0; j < PJ; ++j) {
+                        const unsigned d = (unsigned)(base + j * NL - off);
+                        if (d < (unsigned)len) col[j] = sr[t].y + (int)d;
+                    }
+                }
+            }
+            // every position below rowlen lies in exactly one run
+#pragma unroll
+            for (int j = 0; j < PJ; ++j) {
+                const double xv = base + j * NL < rowlen ? Ucur[col[j]] : 0.0;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) acc[r] += kv[r][j] * xv;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = NL / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) acc[r] += __shfl_xor(acc[r], off, NL);
+    if (active && lane < nr) {
+        double e = acc[0];
+#pragma unroll
+        for (int r = 1; r < 6; ++r)
+            if (lane == r) e = acc[r];
+        const int64_t row = (int64_t)d0 + lane;
+        double dtr = dt, gr = gstep;
+        if (row_group) {
+            const int gi = row_group[row];
+            dtr = group_dt[gi];
+            gr = gtab[gi];
+        }
+        const double dt_2 = (dtr / 2), dt2_2 = ((dtr * dtr) / 2);
+        const double v = V[row], a = A[row];
+        double f = gr * Fmag[row];
+        if (damped)
+            f -= e + Cd[row] * (v + dt_2 * a);
+        else
+            f -= e;
+        double vn = v + dt_2 * a;
+        const double an = invMC[row] * f;
+        vn += dt_2 * an;
+        const double u = Ucur[row];
+        V[row] = vn;
+        A[row] = an;
+        Unext[row] = u + (dtr * vn + dt2_2 * an);
+        if (probe_slot) {
+            const int ps = probe_slot[row];
+            if (ps >= 0) probe_dst[ps] = u;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Mixed-precision explicit step (FP32 membrane + bending, FP64 elsewhere)
 // ---------------------------------------------------------------------------
@@ -2724,6 +2836,33 @@ hipError_t launch_cd_step(int tpr, const RunRows &rows, const double *kval,
     }
 #undef FS_CD_CASE
     return hipGetLastError();
+}
+
+hipError_t launch_cd_node_step(int nl, int pj, const NodeRows &nrw, const int2 *slotrec, const double *kval,
+                               const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
+                               const double *Cd, const double *Fmag, double gstep, double dt, int damped,
+                               const int32_t *probe_slot, double *probe_dst, const int32_t *row_group,
+                               const double *group_dt, const double *gtab, hipStream_t st) {
+    if (nrw.n == 0) return hipSuccess;
+    const int64_t threads = nrw.n * nl;
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+#define FS_CDN_CASE(T, P)                                                                                       \
+    if (nl == T && pj == P) {                                                                                   \
+        hipLaunchKernelGGL((cd_node_step_kernel<T, P>), dim3(blocks), dim3(256), 0, st, nrw, slotrec, kval, Ucur, \
+                           Unext, V, A, invMC, Cd, Fmag, gstep, dt, damped, probe_slot, probe_dst, row_group,    \
+                           group_dt, gtab);                                                                    \
+        return hipGetLastError();                                                                               \
+    }
+    FS_CDN_CASE(8, 4)
+    FS_CDN_CASE(8, 5)
+    FS_CDN_CASE(8, 6)
+    FS_CDN_CASE(8, 7)
+    FS_CDN_CASE(8, 8)
+    FS_CDN_CASE(16, 2)
+    FS_CDN_CASE(16, 3)
+    FS_CDN_CASE(16, 4)
+#undef FS_CDN_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_gather(int n, const int64_t *idx, const double *U, double *out, hipStream_t st) {

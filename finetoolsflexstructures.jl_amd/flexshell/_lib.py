@@ -121,6 +121,7 @@ SIGNATURES = {
     "fs_ctx_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i32p]),
     "fs_ctx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int32]),
     "fs_ctx_set_pipeline": (C.c_int, [C.c_void_p, C.c_int]),
+    "fs_ctx_set_cd_lanes": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
 }
 
 _lib = None
@@ -188,6 +189,14 @@ class Context:
         """Element ranges of the assembly pipeline (1: no overlap of the
         element kernel and the scatter)."""
         self.check(self.lib.fs_ctx_set_pipeline(self.h, int(ranges)), "fs_ctx_set_pipeline")
+
+    def set_cd_lanes(self, lanes_per_node, positions_per_lane=0):
+        """Row products of the FP64 explicit step: 0 = four lanes per row,
+        8 / 16 = lanes per node over the node's rows together, each lane
+        loading `positions_per_lane` values of a row at once (0: chosen from
+        the longest row)."""
+        self.check(self.lib.fs_ctx_set_cd_lanes(self.h, int(lanes_per_node), int(positions_per_lane)),
+                   "fs_ctx_set_cd_lanes")
 
     def close(self):
         if self.h:

@@ -371,6 +371,17 @@ int fs_ctx_set_tuning(fs_ctx *ctx, int nen, int formulation, int32_t variant);
  * Every setting gives bitwise the same matrix.  Default 1; 1 <= ranges <= 64. */
 int fs_ctx_set_pipeline(fs_ctx *ctx, int ranges);
 
+/* Form of the row products of the FP64 explicit step (no counterpart in the
+ * reference, whose step is a CSR product on the host): 0 = four lanes per
+ * row over the neighbour runs; 8 or 16 = that many lanes per node over the
+ * node's rows together (consecutive lanes read consecutive values), each
+ * lane loading positions_per_lane values of a row at once (8 lanes: 4..8,
+ * 16 lanes: 2..4; 0 = chosen from the block's longest row).  All forms give
+ * the same sums up to their order of addition; every setting is bitwise
+ * reproducible and gives a model bitwise the same result alone and in a
+ * batch.  Default 8 lanes per node, positions chosen. */
+int fs_ctx_set_cd_lanes(fs_ctx *ctx, int lanes_per_node, int positions_per_lane);
+
 /* Storage of the mixed-precision operator built by the last FS_PREC_MIXED /
  * FS_PREC_MIXED_F32 loop (no counterpart in the reference, whose loop runs on
  * the FP64 CSR matrix only): info[0] FP32 value pairs of the slices
