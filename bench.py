@@ -32,11 +32,12 @@ def assembly_bytes(E, N, nen, nnz):
     return E * (4 * nen + 4 * k * k) + N * (24 + 24 + 1) + nnz * 8
 
 
-def cd_step_bytes(n, nnz, nnb):
-    """Algorithmic bytes of one fused FP64 CD step on the run records: K values,
-    one 16-byte record per row and one 8-byte record per neighbour slot (nnb),
-    u gathered once, U V A invM Fmag reads, U V A writes."""
-    return nnz * 8 + n * 16 + nnb * 8 + n * 72
+def cd_step_bytes(n, nnz, nnb, nnodes):
+    """Algorithmic bytes of one fused FP64 CD step in the node-grouped form: K
+    values, one 20-byte record per node with free dofs (at most nnodes), one
+    8-byte run record per neighbour slot (nnb), u gathered once, U V A invM
+    Fmag reads, U V A writes."""
+    return nnz * 8 + nnodes * 20 + nnb * 8 + n * 72
 
 
 def resultant_bytes(E, N, nen, npts):
@@ -404,7 +405,7 @@ def main():
           f"locations)", file=sys.stderr, flush=True)
 
     b_asm = assembly_bytes(E, N, 4, nnz)
-    b_cd = cd_step_bytes(nf, nnz, model.mixed_info()[2])
+    b_cd = cd_step_bytes(nf, nnz, model.mixed_info()[2], N)
     ach = b_asm / (dev_ms / 1e3) / 1e9
     ach_cd = b_cd / (cd_ms / ncd / 1e3) / 1e9
     # HBM bytes per assembly from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
