@@ -62,6 +62,8 @@ hipError_t launch_cd_step(int tpr, const RunRows &rows, const double *kval,
                           const int32_t *probe_slot, double *probe_dst, const int32_t *row_group,
                           const double *group_dt, const double *gtab, hipStream_t st);
 hipError_t launch_spmv(const RunRows &rows, const double *kval, const double *x, double *y, hipStream_t st);
+hipError_t launch_spmv_node(int nl, int pj, const NodeRows &nrw, const int2 *slotrec, const double *kval,
+                            const double *x, double *y, hipStream_t st);
 hipError_t launch_cd_node_step(int nl, int pj, const NodeRows &nrw, const int2 *slotrec, const double *kval,
                                const double *Ucur, double *Unext, double *V, double *A, const double *invMC,
                                const double *Cd, const double *Fmag, double gstep, double dt, int damped,
@@ -700,6 +702,15 @@ fs::NodeRows noderows_of(const Block &B) {
     r.rec = B.noderec.as<int4>();
     r.info = B.nodeinfo.as<int32_t>();
     return r;
+}
+
+// Positions per lane of the node-grouped row products: the context's setting,
+// or enough for one pass over the longest row of the block where the
+// register budget allows (8 lanes: 4..8, 16 lanes: 2..4).
+int node_pj(const fs_ctx *c, const Block &B) {
+    if (c->cd_lanes == 0 || c->cd_pj != 0) return c->cd_pj;
+    const int lo = c->cd_lanes == 8 ? 4 : 2, hi = c->cd_lanes == 8 ? 8 : 4;
+    return std::min(hi, std::max(lo, (B.max_rowlen + c->cd_lanes - 1) / c->cd_lanes));
 }
 
 fs::ScatterPlan plan_of(const Block &B) {
@@ -1755,13 +1766,7 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     // per row, up to three runs per lane loaded together (measured fastest
     // of 2, 4, 8 and 16 lanes per row on both families)
     const int tpr = 4;
-    // node-grouped step: positions per lane so that one pass covers the
-    // longest row where the register budget allows (8 lanes: 4..8, 16 lanes: 2..4)
-    int cd_pj = c->cd_pj;
-    if (cd_pj == 0 && c->cd_lanes > 0) {
-        const int lo = c->cd_lanes == 8 ? 4 : 2, hi = c->cd_lanes == 8 ? 8 : 4;
-        cd_pj = std::min(hi, std::max(lo, (B.max_rowlen + c->cd_lanes - 1) / c->cd_lanes));
-    }
+    const int cd_pj = node_pj(c, B);
     const int32_t *rg = batch ? dRG.as<int32_t>() : nullptr;
     const double *gdt = batch ? dGdt.as<double>() : nullptr;
     const fs::MixedOp op = mixed ? mixed_op(m) : fs::MixedOp{};
@@ -1866,16 +1871,23 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
         return s;
     double *v = dv.as<double>(), *w = dw.as<double>(), *ss = dss.as<double>();
     const fs::RunRows rows = runrows_of(B);
+    const fs::NodeRows nrw = noderows_of(B);
     const double *kv = B.kval.as<double>();
+    const int pj = node_pj(c, B);
+    // w = K v in the step's form (node-grouped unless the row form is set)
+    auto spmv = [&]() {
+        return c->cd_lanes > 0 ? fs::launch_spmv_node(c->cd_lanes, pj, nrw, B.slotrec.as<int2>(), kv, v, w, c->stream)
+                               : fs::launch_spmv(rows, kv, v, w, c->stream);
+    };
     for (int it = 0; it < maxit; ++it) {
-        FS_HIP(c, fs::launch_spmv(rows, kv, v, w, c->stream));                              // w = K v
+        FS_HIP(c, spmv());                                                                  // w = K v
         FS_HIP(c, fs::launch_dot(n, 0, w, nullptr, nullptr, dpart.as<double>(), ss, c->stream));
         FS_HIP(c, fs::launch_scale(n, ss, nullptr, w, w, c->stream));                       // w /= |w|
         FS_HIP(c, fs::launch_scale(n, ss + 3, dinv.as<double>(), w, v, c->stream));          // v = invM .* w
         FS_HIP(c, fs::launch_dot(n, 0, v, nullptr, nullptr, dpart.as<double>(), ss + 1, c->stream));
         FS_HIP(c, fs::launch_scale(n, ss + 1, nullptr, v, v, c->stream));                    // v /= |v|
     }
-    FS_HIP(c, fs::launch_spmv(rows, kv, v, w, c->stream));
+    FS_HIP(c, spmv());
     FS_HIP(c, fs::launch_dot(n, 1, w, v, nullptr, dpart.as<double>(), ss, c->stream));
     FS_HIP(c, fs::launch_dot(n, 2, v, nullptr, B.mdiag.as<double>(), dpart.as<double>(), ss + 1, c->stream));
     double h[2];

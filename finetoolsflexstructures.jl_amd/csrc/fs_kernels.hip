@@ -2187,6 +2187,53 @@ __global__ __launch_bounds__(256) void cd_step_kernel(RunRows rows,
 // gives bitwise the same result alone and inside a batch.
 // PJ positions per lane are loaded together (the host picks PJ so that NL x
 // PJ covers the longest row of the block where it can).
+// Row sums of node `rec` (its nr rows, rowlen values each) times x for lane
+// `lane` of NL: the lane's positions lane, lane + NL, ..., PJ at a time,
+// values of all rows first, then the columns from the run records.
+template <int NL, int PJ>
+__device__ __forceinline__ void node_rows_dot(int4 rec, int info, int lane, const int2 *__restrict__ slotrec,
+                                              const double *__restrict__ kval, const double *__restrict__ Ucur,
+                                              double acc[6]) {
+    const int64_t kstart = rec.x;
+    const int nr = info & 7;
+    const int rowlen = info >> 3;
+    for (int base = lane; base < rowlen; base += NL * PJ) {
+        // values first: their addresses need only the node record
+        double kv[6][PJ];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const double *kr = kval + kstart + (int64_t)r * rowlen + base;
+#pragma unroll
+            for (int j = 0; j < PJ; ++j)
+                kv[r][j] = (r < nr && base + j * NL < rowlen) ? kr[j * NL] : 0.0;
+        }
+        int col[PJ];
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) col[j] = 0;
+        for (int k0 = rec.z; k0 < rec.w; k0 += 4) {
+            int2 sr[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sr[t] = k0 + t < rec.w ? slotrec[k0 + t] : make_int2(0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int off = sr[t].x >> 4, len = sr[t].x & 15;
+#pragma unroll
+                for (int j = 0; j < PJ; ++j) {
+                    const unsigned d = (unsigned)(base + j * NL - off);
+                    if (d < (unsigned)len) col[j] = sr[t].y + (int)d;
+                }
+            }
+        }
+        // every position below rowlen lies in exactly one run
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const double xv = base + j * NL < rowlen ? Ucur[col[j]] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) acc[r] += kv[r][j] * xv;
+        }
+    }
+}
+
 template <int NL, int PJ>
 __global__ __launch_bounds__(256) void cd_node_step_kernel(NodeRows nrw, const int2 *__restrict__ slotrec,
                                                            const double *__restrict__ kval, const double *__restrict__ Ucur,
@@ -2208,45 +2255,9 @@ __global__ __launch_bounds__(256) void cd_node_step_kernel(NodeRows nrw, const i
     if (active) {
         const int4 rec = nrw.rec[node];
         const int info = nrw.info[node];
-        const int64_t kstart = rec.x;
         d0 = rec.y;
         nr = info & 7;
-        const int rowlen = info >> 3;
-        for (int base = lane; base < rowlen; base += NL * PJ) {
-            // values first: their addresses need only the node record
-            double kv[6][PJ];
-#pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                const double *kr = kval + kstart + (int64_t)r * rowlen + base;
-#pragma unroll
-                for (int j = 0; j < PJ; ++j)
-                    kv[r][j] = (r < nr && base + j * NL < rowlen) ? kr[j * NL] : 0.0;
-            }
-            int col[PJ];
-#pragma unroll
-            for (int j = 0; j < PJ; ++j) col[j] = 0;
-            for (int k0 = rec.z; k0 < rec.w; k0 += 4) {
-                int2 sr[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) sr[t] = k0 + t < rec.w ? slotrec[k0 + t] : make_int2(0, 0);
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int off = sr[t].x >> 4, len = sr[t].x & 15;
-#pragma unroll
-                    for (int j = 0; j < PJ; ++j) {
-                        const unsigned d = (unsigned)(base + j * NL - off);
-                        if (d < (unsigned)len) col[j] = sr[t].y + (int)d;
-                    }
-                }
-            }
-            // every position below rowlen lies in exactly one run
-#pragma unroll
-            for (int j = 0; j < PJ; ++j) {
-                const double xv = base + j * NL < rowlen ? Ucur[col[j]] : 0.0;
-#pragma unroll
-                for (int r = 0; r < 6; ++r) acc[r] += kv[r][j] * xv;
-            }
-        }
+        node_rows_dot<NL, PJ>(rec, info, lane, slotrec, kval, Ucur, acc);
     }
 #pragma unroll
     for (int off = NL / 2; off > 0; off >>= 1)
@@ -2537,6 +2548,37 @@ __global__ __launch_bounds__(256) void spmv_kernel(RunRows rows, const double *_
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, TPR);
     if (row < rows.n && lane == 0) y[row] = e;
+}
+
+// y = K x in the node-grouped form of cd_node_step_kernel (same row sums).
+template <int NL, int PJ>
+__global__ __launch_bounds__(256) void spmv_node_kernel(NodeRows nrw, const int2 *__restrict__ slotrec,
+                                                        const double *__restrict__ kval, const double *__restrict__ x,
+                                                        double *__restrict__ y) {
+    const int64_t gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int64_t node = gt / NL;
+    const int lane = (int)(gt % NL);
+    const bool active = node < nrw.n;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int nr = 0, d0 = 0;
+    if (active) {
+        const int4 rec = nrw.rec[node];
+        const int info = nrw.info[node];
+        d0 = rec.y;
+        nr = info & 7;
+        node_rows_dot<NL, PJ>(rec, info, lane, slotrec, kval, x, acc);
+    }
+#pragma unroll
+    for (int off = NL / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) acc[r] += __shfl_xor(acc[r], off, NL);
+    if (active && lane < nr) {
+        double e = acc[0];
+#pragma unroll
+        for (int r = 1; r < 6; ++r)
+            if (lane == r) e = acc[r];
+        y[(int64_t)d0 + lane] = e;
+    }
 }
 
 // Deterministic reductions for the power iteration: fixed tree per block,
@@ -2876,6 +2918,27 @@ hipError_t launch_spmv(const RunRows &rows, const double *kval, const double *x,
     const int64_t threads = rows.n * 4;
     hipLaunchKernelGGL(spmv_kernel<4>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, rows, kval, x, y);
     return hipGetLastError();
+}
+
+hipError_t launch_spmv_node(int nl, int pj, const NodeRows &nrw, const int2 *slotrec, const double *kval,
+                            const double *x, double *y, hipStream_t st) {
+    if (nrw.n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((nrw.n * nl + 255) / 256);
+#define FS_SPN_CASE(T, P)                                                                                         \
+    if (nl == T && pj == P) {                                                                                     \
+        hipLaunchKernelGGL((spmv_node_kernel<T, P>), dim3(blocks), dim3(256), 0, st, nrw, slotrec, kval, x, y);   \
+        return hipGetLastError();                                                                                 \
+    }
+    FS_SPN_CASE(8, 4)
+    FS_SPN_CASE(8, 5)
+    FS_SPN_CASE(8, 6)
+    FS_SPN_CASE(8, 7)
+    FS_SPN_CASE(8, 8)
+    FS_SPN_CASE(16, 2)
+    FS_SPN_CASE(16, 3)
+    FS_SPN_CASE(16, 4)
+#undef FS_SPN_CASE
+    return hipErrorInvalidValue;
 }
 
 static constexpr int kDotBlocks = 512;

@@ -371,8 +371,9 @@ int fs_ctx_set_tuning(fs_ctx *ctx, int nen, int formulation, int32_t variant);
  * Every setting gives bitwise the same matrix.  Default 1; 1 <= ranges <= 64. */
 int fs_ctx_set_pipeline(fs_ctx *ctx, int ranges);
 
-/* Form of the row products of the FP64 explicit step (no counterpart in the
- * reference, whose step is a CSR product on the host): 0 = four lanes per
+/* Form of the row products of the FP64 explicit step and of the power
+ * iteration of fs_model_omega_max (no counterpart in the reference, whose
+ * products are CSR products on the host): 0 = four lanes per
  * row over the neighbour runs; 8 or 16 = that many lanes per node over the
  * node's rows together (consecutive lanes read consecutive values), each
  * lane loading positions_per_lane values of a row at once (8 lanes: 4..8,

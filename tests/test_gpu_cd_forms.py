@@ -1,5 +1,6 @@
-"""Row products of the FP64 explicit step: the node-grouped form (lanes per
-node, positions per lane) against the row form and the oracle.
+"""Row products of the FP64 explicit step and the power iteration: the
+node-grouped form (lanes per node, positions per lane) against the row form
+and the oracle.
 
 The forms add the same products in different orders, so the trajectories
 agree to rounding (bound 1e-12 relative) and each form is bitwise
@@ -77,10 +78,13 @@ def test_cd_forms_agree(ctx, mesh):
     cdamp = 50.0
     probes = np.array([0, nf // 3, nf - 1])
     Uo, Vo, _ = O.cd_loop(K, M.diagonal(), cdamp, dt, nsteps, Fmag, g, U0, V0, probes)
+    v0 = rng.random(nf)
+    oo = O.pwr(K, M.diagonal(), v0, maxit=30)
     ref = None
     try:
         for lanes, pj in FORMS:
             ctx.set_cd_lanes(lanes, pj)
+            assert abs(explicit.omega_max(f, v0, maxit=30) - oo) < 1e-10 * oo, (lanes, pj)
             r = explicit.run(f._model, cdamp, U0, V0, dt, nsteps, Fmag, g, probes, 1)
             r2 = explicit.run(f._model, cdamp, U0, V0, dt, nsteps, Fmag, g, probes, 1)
             assert np.array_equal(r["U"], r2["U"]) and np.array_equal(r["V"], r2["V"]), (lanes, pj)
