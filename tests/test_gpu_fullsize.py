@@ -1,0 +1,90 @@
+"""The two single-GPU benchmark workloads at their full sizes, checked through
+properties that do not depend on size:
+
+- the 512 x 512 Q4RS cylinder panel (262,144 elements, 1,575,936 free dofs):
+  the free-free pattern equals the structural union of the element dof
+  pairs built independently here (scipy, from the connectivity and the dof
+  numbers), bit for bit; K is bitwise symmetric; a seeded sample of element
+  matrices matches the oracle's within 1e-12; 20 central-difference steps
+  match the oracle's loop on the same K and M within 1e-10; the loop is
+  linear, so doubling the initial velocity doubles the state exactly (a
+  power-of-two scaling commutes with every rounding);
+- the 1024 x 512 T3 plate (1,048,576 elements, 3,136,518 free dofs): the
+  same step checks."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from flexshell import _lib, explicit, femm as FM, workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _pattern_reference(conn, dofnums, nf):
+    """Free-free CSR pattern of sum_e (dof pairs of e), built from scratch."""
+    N = dofnums.shape[0]
+    E, nen = conn.shape
+    inc = sp.csr_matrix((np.ones(E * nen, dtype=np.int8), (np.repeat(np.arange(E), nen), conn.ravel())),
+                        shape=(E, N))
+    adj = (inc.T @ inc).tocoo()  # node pairs sharing an element (including I = J)
+    I, J = adj.row.astype(np.int64), adj.col.astype(np.int64)
+    r = np.repeat(dofnums[I], 6, axis=1)  # (pairs, 36): dof of I for each (a, b)
+    c = np.tile(dofnums[J], (1, 6))
+    keep = (r < nf) & (c < nf)
+    pat = sp.csr_matrix((np.ones(int(keep.sum()), dtype=np.int8), (r[keep], c[keep])), shape=(nf, nf))
+    pat.sort_indices()
+    return pat
+
+
+def _step_checks(f, xyz, d, seed, nsteps=20):
+    nf = d.nfreedofs
+    K = f.stiffness(FM.SysmatAssemblerFFBlock(nf), xyz, None, None, d)
+    M = f.mass(FM.SysmatAssemblerFFBlock(FM.SysmatAssemblerSparseDiag(), nf, nf), xyz, d).diagonal()
+    f.stiffness(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), xyz, None, None, d)
+    f.mass(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), xyz, d)
+    rng = np.random.default_rng(seed)
+    V0 = W.gaussian_velocity(d, 1e-3, seed=seed)
+    dt = 0.9 * explicit.stable_dt(explicit.omega_max(f, rng.random(nf), maxit=30))
+    z = np.zeros(nf)
+    g = np.zeros(nsteps + 1)
+    r = explicit.run(f._model, 0.0, z, V0, dt, nsteps, z, g)
+    Uo, Vo, _ = O.cd_loop(K, M, 0.0, dt, nsteps, z, g, z, V0)
+    assert np.linalg.norm(r["U"] - Uo) <= 1e-10 * np.linalg.norm(Uo)
+    assert np.linalg.norm(r["V"] - Vo) <= 1e-10 * np.linalg.norm(Vo)
+    r2 = explicit.run(f._model, 0.0, z, 2.0 * V0, dt, nsteps, z, g)
+    assert np.array_equal(r2["U"], 2.0 * r["U"]) and np.array_equal(r2["V"], 2.0 * r["V"])
+    return K
+
+
+def test_cylinder_q4_512_full_size(ctx):
+    print("[gpu] full size: Q4RS cylinder 512x512", flush=True)
+    c = W.cylinder_panel_c2(n=512)
+    d, xyz, conn = c["dchi"], c["xyz"], c["conn"]
+    nf = d.nfreedofs
+    f = FM.FEMMShellQ4RS(conn, c["material"], thickness=c["t"], ctx=ctx)
+    f.associategeometry(xyz)
+    K = _step_checks(f, xyz, d, seed=31)
+    # pattern, bit for bit, and bitwise symmetry
+    P = _pattern_reference(conn, d.dofnums, nf)
+    assert np.array_equal(K.indptr, P.indptr) and np.array_equal(K.indices, P.indices)
+    KT = K.T.tocsr()
+    KT.sort_indices()
+    assert np.array_equal(KT.indptr, K.indptr) and np.array_equal(KT.indices, K.indices)
+    assert np.array_equal(KT.data, K.data)
+    # element matrices of a seeded sample against the oracle (normals of the whole mesh)
+    sample = np.sort(np.random.default_rng(7).choice(conn.shape[0], 256, replace=False))
+    nrm, val = O.normals(4, xyz, conn)
+    Dps, Dt = O.shell_D(O.iso_D6(70e9, 0.33))
+    Ko = O.element_matrices(4, xyz, conn[sample], nrm, val, Dps=Dps, Dt=Dt, t=c["t"])
+    Kg = f.element_matrices(xyz)[sample]
+    err = np.linalg.norm(Kg - Ko, axis=(1, 2)) / np.linalg.norm(Ko, axis=(1, 2))
+    assert err.max() < 1e-12, err.max()
+
+
+def test_plate_t3_1024x512_full_size(ctx):
+    print("[gpu] full size: T3 plate 1024x512", flush=True)
+    c = W.plate_c4()
+    f = FM.FEMMShellT3FF(c["conn"], c["material"], thickness=c["t"], ctx=ctx)
+    f.associategeometry(c["xyz"])
+    _step_checks(f, c["xyz"], c["dchi"], seed=32)
