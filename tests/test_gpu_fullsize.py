@@ -10,7 +10,7 @@ properties that do not depend on size:
   linear, so doubling the initial velocity doubles the state exactly (a
   power-of-two scaling commutes with every rounding);
 - the 1024 x 512 T3 plate (1,048,576 elements, 3,136,518 free dofs): the
-  same step checks."""
+  same checks."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -57,34 +57,40 @@ def _step_checks(f, xyz, d, seed, nsteps=20):
     return K
 
 
-def test_cylinder_q4_512_full_size(ctx):
-    print("[gpu] full size: Q4RS cylinder 512x512", flush=True)
-    c = W.cylinder_panel_c2(n=512)
-    d, xyz, conn = c["dchi"], c["xyz"], c["conn"]
-    nf = d.nfreedofs
-    f = FM.FEMMShellQ4RS(conn, c["material"], thickness=c["t"], ctx=ctx)
-    f.associategeometry(xyz)
-    K = _step_checks(f, xyz, d, seed=31)
-    # pattern, bit for bit, and bitwise symmetry
-    P = _pattern_reference(conn, d.dofnums, nf)
+def _matrix_checks(f, nen, xyz, conn, d, K, E, nu, t):
+    """Bit-exact pattern, bitwise symmetry, and a seeded element sample."""
+    P = _pattern_reference(conn, d.dofnums, d.nfreedofs)
     assert np.array_equal(K.indptr, P.indptr) and np.array_equal(K.indices, P.indices)
+    del P
     KT = K.T.tocsr()
     KT.sort_indices()
     assert np.array_equal(KT.indptr, K.indptr) and np.array_equal(KT.indices, K.indices)
     assert np.array_equal(KT.data, K.data)
-    # element matrices of a seeded sample against the oracle (normals of the whole mesh)
+    del KT
     sample = np.sort(np.random.default_rng(7).choice(conn.shape[0], 256, replace=False))
-    nrm, val = O.normals(4, xyz, conn)
-    Dps, Dt = O.shell_D(O.iso_D6(70e9, 0.33))
-    Ko = O.element_matrices(4, xyz, conn[sample], nrm, val, Dps=Dps, Dt=Dt, t=c["t"])
+    nrm, val = O.normals(nen, xyz, conn)
+    Dps, Dt = O.shell_D(O.iso_D6(E, nu))
+    Ko = O.element_matrices(nen, xyz, conn[sample], nrm, val, Dps=Dps, Dt=Dt, t=t)
     Kg = f.element_matrices(xyz)[sample]
     err = np.linalg.norm(Kg - Ko, axis=(1, 2)) / np.linalg.norm(Ko, axis=(1, 2))
     assert err.max() < 1e-12, err.max()
 
 
+def test_cylinder_q4_512_full_size(ctx):
+    print("[gpu] full size: Q4RS cylinder 512x512", flush=True)
+    c = W.cylinder_panel_c2(n=512)
+    d, xyz, conn = c["dchi"], c["xyz"], c["conn"]
+    f = FM.FEMMShellQ4RS(conn, c["material"], thickness=c["t"], ctx=ctx)
+    f.associategeometry(xyz)
+    K = _step_checks(f, xyz, d, seed=31)
+    _matrix_checks(f, 4, xyz, conn, d, K, 70e9, 0.33, c["t"])
+
+
 def test_plate_t3_1024x512_full_size(ctx):
     print("[gpu] full size: T3 plate 1024x512", flush=True)
     c = W.plate_c4()
-    f = FM.FEMMShellT3FF(c["conn"], c["material"], thickness=c["t"], ctx=ctx)
-    f.associategeometry(c["xyz"])
-    _step_checks(f, c["xyz"], c["dchi"], seed=32)
+    d, xyz, conn = c["dchi"], c["xyz"], c["conn"]
+    f = FM.FEMMShellT3FF(conn, c["material"], thickness=c["t"], ctx=ctx)
+    f.associategeometry(xyz)
+    K = _step_checks(f, xyz, d, seed=32)
+    _matrix_checks(f, 3, xyz, conn, d, K, W.STEEL["E"], W.STEEL["nu"], c["t"])
