@@ -109,6 +109,18 @@ function check(ctx::Context, st::Cint, what::AbstractString)
     error("$what: status $st: $msg")   # the reference's error() convention
 end
 
+# Launch settings of a context (no counterpart in the reference): element
+# ranges of the assembly pipeline, and the form of the explicit step's row
+# products (0 = lanes per row; 8 or 16 lanes per node, positions per lane 0 =
+# chosen from the longest row).  Every setting gives the same results up to
+# the order of addition (bitwise the same for the pipeline).
+set_pipeline!(ctx::Context, ranges::Integer) =
+    check(ctx, ccall((:fs_ctx_set_pipeline, libflexshell), Cint, (Ptr{Cvoid}, Cint), ctx.h, ranges),
+          "fs_ctx_set_pipeline")
+set_cd_lanes!(ctx::Context, lanes_per_node::Integer, positions_per_lane::Integer = 0) =
+    check(ctx, ccall((:fs_ctx_set_cd_lanes, libflexshell), Cint, (Ptr{Cvoid}, Cint, Cint), ctx.h,
+                     lanes_per_node, positions_per_lane), "fs_ctx_set_cd_lanes")
+
 const _default_ctx = Ref{Union{Nothing,Context}}(nothing)
 default_context() = something(_default_ctx[], (_default_ctx[] = Context(parse(Int, get(ENV, "FLEXSHELL_DEVICE", "0")))))
 
@@ -513,6 +525,6 @@ function inspectintegpoints_hip(self, geom0::NodalField{Float64}, u::NodalField{
 end
 
 export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context,
-    inspectintegpoints_hip, numberdofs_hip!, distribloads_hip
+    inspectintegpoints_hip, numberdofs_hip!, distribloads_hip, set_pipeline!, set_cd_lanes!
 
 end # module
