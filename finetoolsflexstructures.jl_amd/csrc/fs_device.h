@@ -49,6 +49,47 @@ struct ElemParams {
     // point, Gauss point major).
     double *split_cs;
     double *split_dr;
+    // Fused assembly (TilePlan): workgroup b computes the elements
+    // tile_elem[b NE .. b NE + NE - 1] (element numbers; a tile's unused
+    // slots repeat one of its elements) instead of ebegin + b NE + le.
+    const int32_t *tile_elem;
+};
+
+// Fused assembly: the element kernel of each tile (a compact patch of NE
+// elements of one connected component) sums, in its LDS output stage, the
+// blocks of every upper node pair its elements touch, in ascending element
+// order, and writes the pairs whose contributing elements all lie in the
+// tile straight into the CSR values; the other (boundary) pairs leave a 6 x 6
+// partial sum that pair_finalize_kernel adds up in tile order.
+//   tp_ptr[t] .. tp_ptr[t+1]   the pair entries of tile t, ordered by (I, J)
+//   trec[3q .. 3q+2]  the CSR placement of the entry's pair (I, J), so that
+//                 its writes need no other table: per node {CSR position of
+//                 the first free row, of the first fixed row (-1: none), row
+//                 length, codes} for I then J, where codes holds 4 bits per
+//                 comp c (0..5 offset in the node's free run, 8..13 in its
+//                 fixed run, 15 not in the block) and bit 30 of I's word
+//                 marks I = J; then {offset of J's free run in I's rows, of
+//                 J's fixed run, of I's free run in J's rows, of I's fixed
+//                 run} (-1: none)
+//   tcontrib[q] = contribution count (bits 0..2) and up to six contributions
+//                 of 10 bits from bit 3 on: tile slot le (5 bits) | local block
+//                 (4 bits) << 5 | transposed << 9, in element order
+//   tslot[q]    = partial slot of a boundary entry, -1 for a pair completed
+//                 in the tile
+//   brec[3b .. 3b+2], bp_ptr[b] .. bp_ptr[b+1]: the boundary pairs
+//                 (placement records as trec) and their partial slots, which
+//                 are consecutive per pair and in tile order
+//   partial     nslots x 36 FP64 partial sums (row-major 6 x 6, rows of I)
+struct TilePlan {
+    int64_t ntiles;
+    int64_t nbnd;                 // boundary pairs
+    const int32_t *tp_ptr;
+    const int4 *trec;
+    const uint64_t *tcontrib;
+    const int32_t *tslot;
+    const int4 *brec;
+    const int32_t *bp_ptr;
+    double *partial;
 };
 
 // Mixed-precision operator of one model for the explicit step, on node-major

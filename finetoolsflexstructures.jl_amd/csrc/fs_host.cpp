@@ -10,6 +10,7 @@
 #include <sched.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -33,6 +34,12 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
 int element_variants(int nen);
 void element_variant_shape(int nen, int variant, int *ne, int *wg);
 hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals, hipStream_t st);
+int fused_tile_elems(int nen);
+hipError_t launch_element_fused(const ElemParams &p, const double *x, const double *y, const double *z,
+                                const int32_t *conn, const double *nx, const double *ny, const double *nz,
+                                const uint8_t *valid, int *status, const ScatterPlan &sp, const TilePlan &tp,
+                                double *vals, hipStream_t st);
+hipError_t launch_pair_finalize(const ScatterPlan &sp, const TilePlan &tp, double *vals, hipStream_t st);
 hipError_t launch_normals(int nen, int64_t N, int E, const double *x, const double *y, const double *z,
                           const int32_t *conn, const int32_t *ne_ptr, const int32_t *ne_list, double ntol,
                           const double *gnrm, const int32_t *egroup, double *nx, double *ny, double *nz,
@@ -78,6 +85,8 @@ hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, 
                               const int32_t *conn, const double *dre, double *dr, hipStream_t st);
 hipError_t launch_pack_xyz(int64_t N, const double *x, const double *y, const double *z, double4 *xyz,
                            hipStream_t st);
+hipError_t launch_coo_reduce(int64_t nnz, const int64_t *cptr, const int64_t *clist, const double *mats,
+                             double *out, hipStream_t st);
 hipError_t launch_cd_mixed_step(int nen, int nsr, const MixedOp &op, const double *Ucur, double *Unext,
                                 double *V, double *A, const double *invMC, const double *Cd, const double *Fmag,
                                 double gstep, double dt, int damped, int forced, const int32_t *probe_slot,
@@ -105,6 +114,10 @@ struct BufCache {
 struct fs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the caller's handle plus one per live model: fs_ctx_destroy drops the
+    // caller's reference, the last release tears the context down, so a
+    // model may outlive fs_ctx_destroy (it may then only be destroyed)
+    std::atomic<int> refs{1};
     std::string err;
     BufCache *cache = nullptr;
     // element-kernel launch shape per (family T3/Q4, shear formulation):
@@ -127,6 +140,10 @@ struct fs_ctx {
     // cd_pj positions per lane (0: from the block's longest row)
     int cd_lanes = 8;
     int cd_pj = 0;
+    // stiffness assembly: 1 = fused element tiles (element kernel writes the
+    // CSR values of the pairs it completes, boundary pairs finalised after);
+    // 0 = element blocks to HBM, then the pair scatter
+    int assembly = 1;
 };
 
 namespace {
@@ -166,6 +183,20 @@ struct DevBuf {
     T *as() const { return static_cast<T *>(p); }
 };
 
+// Fused-assembly plan of a block (fs::TilePlan): the element tiles, their
+// pair entries and the boundary pairs; built on the first fused stiffness.
+struct TileStore {
+    int ne = 0;  // elements per tile it was built for (0: not built)
+    int64_t ntiles = 0, nent = 0, nbnd = 0, nslots = 0;
+    DevBuf tile_elem, tp_ptr, trec, tcontrib, tslot, brec, bp_ptr, partial;
+    void reset() {
+        ne = 0;
+        ntiles = nent = nbnd = nslots = 0;
+        for (DevBuf *b : {&tile_elem, &tp_ptr, &trec, &tcontrib, &tslot, &brec, &bp_ptr, &partial})
+            b->release();
+    }
+};
+
 struct Block {
     bool built = false;
     int64_t nrows = 0;
@@ -194,8 +225,22 @@ struct Block {
     int64_t nrownodes = 0;
     int32_t max_rowlen = 0;
     bool k_ready = false, m_ready = false;
+    // host copies for the fused-assembly planner: neighbour slot -> (pair <<
+    // 1 | side), pair nodes and slots, contributions per pair (counts)
+    std::vector<int32_t> h_nb_pair, h_pnode, h_pslot, h_cptr;
+    // per node {first free row's CSR position, first fixed row's (-1), row
+    // length, 4-bit comp codes}; per neighbour slot the free / fixed run offsets
+    std::vector<int32_t> h_node_rec, h_nb_off;
+    TileStore tiles;
     void reset() {
         built = k_ready = m_ready = false;
+        h_nb_pair.clear();
+        h_pnode.clear();
+        h_pslot.clear();
+        h_cptr.clear();
+        h_node_rec.clear();
+        h_nb_off.clear();
+        tiles.reset();
         nrows = nnz = npairs = nrownodes = 0;
         max_rowlen = 0;
         rowptr.clear();
@@ -250,6 +295,18 @@ struct fs_model {
     MixedStore mixed;
 };
 
+// Buffered element matrices of a FEMM with no element kernel here, reduced
+// into CSR on the device (fs_coo_*): the pattern and, per stored entry, the
+// positions of its contributions in the element-matrix buffer.
+struct fs_coo {
+    fs_ctx *ctx = nullptr;
+    int64_t nrows = 0, ncols = 0, nmat = 0;
+    int32_t kr = 0, kc = 0;
+    int64_t nnz = 0;
+    std::vector<int64_t> rowptr, colval;  // 0-based
+    DevBuf cptr, clist, mats, vals;
+};
+
 namespace {
 
 // Host threads for the parallel loops: the CPUs this process may run on,
@@ -284,13 +341,25 @@ void parallel_chunks(int64_t n, F fn, int64_t grain = 65536) {
     for (auto &x : th) x.join();
 }
 
+// Records the message of a failing call and waits for the work the call
+// had already queued on the context's streams, so that the device buffers
+// the call releases on its way out (returned to the context's spare list)
+// have no pending kernels or copies.  Every ABI call therefore ends with its
+// streams synchronised, on the error paths too.
 int fail(fs_ctx *c, int code, const char *fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    if (c) c->err = buf;
+    if (c) {
+        c->err = buf;
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    }
+    // the error is reported: clear the runtime's last-error state, which the
+    // launch checks (hipGetLastError) of later calls would otherwise read
+    (void)hipGetLastError();
     return code;
 }
 
@@ -679,6 +748,20 @@ int build_block(fs_model *m, int which) {
     if ((s = dalloc(c, B.kval, (size_t)nnz * sizeof(double)))) return s;
     if ((s = dalloc(c, B.mdiag, (size_t)nrows * sizeof(double)))) return s;
     FS_HIP(c, hipStreamSynchronize(c->stream));
+    B.h_node_rec.assign(4 * N, -1);
+    for (int64_t I = 0; I < N; ++I) {
+        int32_t codes = 0;
+        for (int cc = 0; cc < 6; ++cc) codes |= (code[I * 6 + cc] < 0 ? 15 : (int)code[I * 6 + cc]) << (4 * cc);
+        B.h_node_rec[4 * I] = nfr[I] ? (int32_t)B.rowptr[minfr[I]] : -1;
+        B.h_node_rec[4 * I + 1] = nfx[I] ? (int32_t)B.rowptr[minfx[I]] : -1;
+        B.h_node_rec[4 * I + 2] = rowlen[I];
+        B.h_node_rec[4 * I + 3] = codes;
+    }
+    B.h_nb_off = std::move(nb_off);
+    B.h_nb_pair = std::move(nb_pair);
+    B.h_pnode = std::move(pnode);
+    B.h_pslot = std::move(pslot);
+    B.h_cptr = std::move(cptr);
     B.npairs = npairs;
     B.nnodes = N;
     B.nrows = nrows;
@@ -732,6 +815,254 @@ fs::ScatterPlan plan_of(const Block &B) {
     return sp;
 }
 
+fs::TilePlan tileplan_of(const Block &B) {
+    fs::TilePlan tp;
+    tp.ntiles = B.tiles.ntiles;
+    tp.nbnd = B.tiles.nbnd;
+    tp.tp_ptr = B.tiles.tp_ptr.as<int32_t>();
+    tp.trec = B.tiles.trec.as<int4>();
+    tp.tcontrib = B.tiles.tcontrib.as<uint64_t>();
+    tp.tslot = B.tiles.tslot.as<int32_t>();
+    tp.brec = B.tiles.brec.as<int4>();
+    tp.bp_ptr = B.tiles.bp_ptr.as<int32_t>();
+    tp.partial = B.tiles.partial.as<double>();
+    return tp;
+}
+
+// 3 x 21-bit Morton code of a point scaled into [0, 1)^3.
+uint64_t morton3(double x, double y, double z) {
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    auto q = [](double t) {
+        const double u = t * 2097152.0;
+        return (uint64_t)(u < 0.0 ? 0.0 : (u > 2097151.0 ? 2097151.0 : u));
+    };
+    return spread(q(x)) | spread(q(y)) << 1 | spread(q(z)) << 2;
+}
+
+// Fused-assembly plan of a block.  Elements are grouped into tiles of NE:
+// per connected component (components in the order of their first
+// element), elements sorted along a Morton curve of their centroids scaled
+// by the component's bounding box, then cut into consecutive runs of NE, so
+// a tile is a compact patch and never spans two components (a model gives
+// the same tiles alone and inside a batch).  A tile's elements are kept in
+// ascending order.  For every upper node pair a tile touches, its
+// contributions within the tile are listed in element order; the pair is
+// completed in the tile when they are all its contributions, otherwise (or
+// when more than six fall in one tile, in runs of six) it leaves partial
+// sums, added up afterwards in tile order.
+int build_tiles(fs_model *m, int which, int NE) {
+    fs_ctx *c = m->ctx;
+    Block &B = m->blk[which];
+    TileStore &T = B.tiles;
+    T.reset();
+    const int nen = m->nen;
+    const int nblk = nen * (nen + 1) / 2;
+    const int64_t E = m->E, N = m->N;
+    if (NE > 32 || nblk > 16) return fail(c, FS_ERR_UNSUPPORTED, "fused assembly: tile of %d elements", NE);
+    // components: union-find over the nodes of each element
+    std::vector<int32_t> par(N);
+    for (int64_t i = 0; i < N; ++i) par[i] = (int32_t)i;
+    auto find = [&](int32_t x) {
+        while (par[x] != x) {
+            par[x] = par[par[x]];
+            x = par[x];
+        }
+        return x;
+    };
+    for (int64_t e = 0; e < E; ++e)
+        for (int a = 1; a < nen; ++a) {
+            const int32_t r0 = find(m->conn[e * nen]), r1 = find(m->conn[e * nen + a]);
+            if (r0 != r1) par[std::max(r0, r1)] = std::min(r0, r1);
+        }
+    std::vector<int32_t> comp(E), cid(N, -1);
+    int32_t ncomp = 0;
+    for (int64_t e = 0; e < E; ++e) {
+        const int32_t r = find(m->conn[e * nen]);
+        if (cid[r] < 0) cid[r] = ncomp++;
+        comp[e] = cid[r];
+    }
+    // centroids, component boxes, Morton keys
+    std::vector<double> cen(3 * E), lo(3 * ncomp, 1e300), hi(3 * ncomp, -1e300);
+    std::vector<double> X(3 * N);
+    {
+        std::vector<double> x(N), y(N), z(N);
+        FS_HIP(c, hipMemcpyAsync(x.data(), m->x.p, N * 8, hipMemcpyDeviceToHost, c->stream));
+        FS_HIP(c, hipMemcpyAsync(y.data(), m->y.p, N * 8, hipMemcpyDeviceToHost, c->stream));
+        FS_HIP(c, hipMemcpyAsync(z.data(), m->z.p, N * 8, hipMemcpyDeviceToHost, c->stream));
+        FS_HIP(c, hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < N; ++i) {
+            X[3 * i] = x[i];
+            X[3 * i + 1] = y[i];
+            X[3 * i + 2] = z[i];
+        }
+    }
+    for (int64_t e = 0; e < E; ++e) {
+        for (int k = 0; k < 3; ++k) {
+            double v = 0.0;
+            for (int a = 0; a < nen; ++a) v += X[3 * (int64_t)m->conn[e * nen + a] + k];
+            v /= nen;
+            cen[3 * e + k] = v;
+            lo[3 * comp[e] + k] = std::min(lo[3 * comp[e] + k], v);
+            hi[3 * comp[e] + k] = std::max(hi[3 * comp[e] + k], v);
+        }
+    }
+    std::vector<uint64_t> key(E);
+    parallel_chunks(E, [&](int64_t e0, int64_t e1) {
+        for (int64_t e = e0; e < e1; ++e) {
+            const int g = comp[e];
+            double ext = 0.0;
+            for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[3 * g + k] - lo[3 * g + k]);
+            const double sc = ext > 0.0 ? 1.0 / (ext * (1.0 + 1e-9)) : 0.0;
+            key[e] = morton3((cen[3 * e] - lo[3 * g]) * sc, (cen[3 * e + 1] - lo[3 * g + 1]) * sc,
+                             (cen[3 * e + 2] - lo[3 * g + 2]) * sc);
+        }
+    });
+    std::vector<int32_t> order(E);
+    for (int64_t e = 0; e < E; ++e) order[e] = (int32_t)e;
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        if (comp[a] != comp[b]) return comp[a] < comp[b];
+        if (key[a] != key[b]) return key[a] < key[b];
+        return a < b;
+    });
+    // tiles
+    std::vector<int64_t> tstart;
+    for (int64_t i = 0; i < E; ++i)
+        if (i == 0 || comp[order[i]] != comp[order[i - 1]] || i - tstart.back() == NE) tstart.push_back(i);
+    const int64_t ntiles = (int64_t)tstart.size();
+    tstart.push_back(E);
+    if (ntiles >= (int64_t(1) << 31) / 64) return fail(c, FS_ERR_UNSUPPORTED, "fused assembly: too many tiles");
+    std::vector<int32_t> telem((size_t)ntiles * NE);
+    for (int64_t t = 0; t < ntiles; ++t) {
+        std::sort(order.begin() + tstart[t], order.begin() + tstart[t + 1]);
+        const int64_t n = tstart[t + 1] - tstart[t];
+        for (int le = 0; le < NE; ++le) telem[t * NE + le] = order[tstart[t] + (le < n ? le : 0)];
+    }
+    // pair entries per tile
+    auto slot_of = [&](int64_t I, int64_t J) -> int32_t {
+        const int32_t *b0 = m->nbr_list.data() + m->nbr_ptr[I];
+        const int32_t *e0 = m->nbr_list.data() + m->nbr_ptr[I + 1];
+        return (int32_t)(std::lower_bound(b0, e0, (int32_t)J) - m->nbr_list.data());
+    };
+    struct Ent {
+        int32_t p;
+        uint64_t cw;
+        int32_t boundary;  // 1: leaves a partial sum
+    };
+    std::vector<std::vector<Ent>> tent(ntiles);
+    std::atomic<int> bad{0};
+    parallel_chunks(ntiles, [&](int64_t t0, int64_t t1) {
+        std::vector<std::array<int32_t, 4>> tup;  // (pair, element, code, 0)
+        for (int64_t t = t0; t < t1; ++t) {
+            tup.clear();
+            const int64_t n = tstart[t + 1] - tstart[t];
+            for (int le = 0; le < n; ++le) {
+                const int32_t e = telem[t * NE + le];
+                const int32_t *ce = m->conn.data() + (int64_t)e * nen;
+                int bl = 0;
+                for (int a = 0; a < nen; ++a)
+                    for (int b = a; b < nen; ++b, ++bl) {
+                        const int32_t na = ce[a], nb = ce[b];
+                        const int32_t I = std::min(na, nb), J = std::max(na, nb);
+                        const int32_t k = slot_of(I, J);
+                        const int32_t pp = B.h_nb_pair[k] >> 1;
+                        const int32_t code = le | (bl << 5) | ((na > nb ? 1 : 0) << 9);
+                        tup.push_back({pp, e, code, 0});
+                    }
+            }
+            std::sort(tup.begin(), tup.end(), [](const std::array<int32_t, 4> &x, const std::array<int32_t, 4> &y) {
+                return x[0] != y[0] ? x[0] < y[0] : x[1] < y[1];
+            });
+            std::vector<Ent> &out = tent[t];
+            for (size_t i = 0; i < tup.size();) {
+                size_t j = i;
+                while (j < tup.size() && tup[j][0] == tup[i][0]) ++j;
+                const int32_t pp = tup[i][0];
+                const int cnt = (int)(j - i);
+                const int gcnt = B.h_cptr[pp + 1] - B.h_cptr[pp];
+                if (cnt > gcnt) bad = 1;
+                const bool complete = cnt == gcnt && cnt <= 6;
+                for (size_t q = i; q < j; q += 6) {
+                    const int nq = (int)std::min<size_t>(6, j - q);
+                    uint64_t cw = (uint64_t)nq;
+                    for (int r = 0; r < nq; ++r) cw |= (uint64_t)tup[q + r][2] << (3 + 10 * r);
+                    out.push_back({pp, cw, complete ? 0 : 1});
+                }
+                i = j;
+            }
+        }
+    }, 256);
+    if (bad) return fail(c, FS_ERR_STATE, "fused assembly: tile contributions exceed the pair's");
+    // flatten; partial slots in tile order; boundary pairs and their slots
+    std::vector<int32_t> tp_ptr(ntiles + 1, 0);
+    for (int64_t t = 0; t < ntiles; ++t) tp_ptr[t + 1] = tp_ptr[t] + (int32_t)tent[t].size();
+    const int64_t nent = tp_ptr[ntiles];
+    // placement record of upper pair pp (fs::TilePlan trec)
+    auto place = [&](int64_t pp, int32_t *rec) {
+        const int64_t I = B.h_pnode[2 * pp], J = B.h_pnode[2 * pp + 1];
+        const int64_t kI = B.h_pslot[2 * pp], kJ = B.h_pslot[2 * pp + 1];
+        for (int k = 0; k < 4; ++k) {
+            rec[k] = B.h_node_rec[4 * I + k];
+            rec[4 + k] = B.h_node_rec[4 * J + k];
+        }
+        if (I == J) rec[3] |= 1 << 30;
+        rec[8] = B.h_nb_off[2 * kI];
+        rec[9] = B.h_nb_off[2 * kI + 1];
+        rec[10] = B.h_nb_off[2 * kJ];
+        rec[11] = B.h_nb_off[2 * kJ + 1];
+    };
+    std::vector<int32_t> trec(12 * nent), tslot(nent);
+    std::vector<uint64_t> tcw(nent);
+    // partial slots: the slots of one boundary pair are consecutive (its
+    // tiles in tile order), so the finalize reads each pair's partials as one
+    // stretch
+    std::vector<int32_t> pcount(B.npairs + 1, 0);
+    for (int64_t t = 0; t < ntiles; ++t)
+        for (const Ent &en : tent[t])
+            if (en.boundary) pcount[en.p + 1]++;
+    std::vector<int32_t> bidx(B.npairs, -1), brec, bp_ptr(1, 0);
+    for (int64_t pp = 0; pp < B.npairs; ++pp)
+        if (pcount[pp + 1]) {
+            bidx[pp] = (int32_t)(bp_ptr.size() - 1);
+            brec.resize(brec.size() + 12);
+            place(pp, &brec[brec.size() - 12]);
+            bp_ptr.push_back(bp_ptr.back() + pcount[pp + 1]);
+        }
+    const int64_t nbnd = (int64_t)bp_ptr.size() - 1;
+    const int64_t nslots = bp_ptr.back();
+    std::vector<int32_t> bfill(bp_ptr.begin(), bp_ptr.end() - 1);
+    for (int64_t t = 0; t < ntiles; ++t)
+        for (size_t i = 0; i < tent[t].size(); ++i) {
+            const Ent &en = tent[t][i];
+            const int64_t q = tp_ptr[t] + (int64_t)i;
+            place(en.p, &trec[12 * q]);
+            tcw[q] = en.cw;
+            tslot[q] = en.boundary ? bfill[bidx[en.p]]++ : -1;
+        }
+    int st;
+    if ((st = upload(c, T.tile_elem, telem.data(), telem.size())) || (st = upload(c, T.tp_ptr, tp_ptr.data(), tp_ptr.size())) ||
+        (st = upload(c, T.trec, trec.data(), trec.size())) || (st = upload(c, T.tcontrib, tcw.data(), tcw.size())) ||
+        (st = upload(c, T.tslot, tslot.data(), tslot.size())) ||
+        (st = upload(c, T.brec, brec.data(), brec.size())) ||
+        (st = upload(c, T.bp_ptr, bp_ptr.data(), bp_ptr.size())) ||
+        (st = dalloc(c, T.partial, (size_t)std::max<int64_t>(nslots, 1) * 36 * 8)))
+        return st;
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    T.ne = NE;
+    T.ntiles = ntiles;
+    T.nent = nent;
+    T.nbnd = nbnd;
+    T.nslots = nslots;
+    return FS_OK;
+}
+
 // Element-kernel launch on the model's device arrays.
 hipError_t launch_elements(fs_model *m, const fs::ElemParams &ep, int variant, hipStream_t st) {
     return fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
@@ -743,7 +1074,7 @@ hipError_t launch_elements(fs_model *m, const fs::ElemParams &ep, int variant, h
 // Validate the stiffness parameters, upload the per-group material records,
 // size m->Kb, clear the status flag and pick the element-kernel launch shape
 // (the autotuner times every shape on the first evaluation of a kind).
-int prepare_elements(fs_model *m, const fs_shell_params *p, fs::ElemParams &ep, int *variant_out) {
+int prepare_elements(fs_model *m, const fs_shell_params *p, fs::ElemParams &ep, int *variant_out, bool blocks = true) {
     fs_ctx *c = m->ctx;
     if (!p) return fail(c, FS_ERR_ARG, "stiffness: NULL parameters");
     if (!m->normals_ready) return fail(c, FS_ERR_STATE, "stiffness: normals not computed (associategeometry!)");
@@ -834,10 +1165,14 @@ int prepare_elements(fs_model *m, const fs_shell_params *p, fs::ElemParams &ep, 
             ep.elem_group = m->g_group.as<int32_t>();
         }
     }
+    FS_HIP(c, hipMemsetAsync(m->status.p, 0, sizeof(int), c->stream));
+    if (!blocks) {
+        *variant_out = 0;  // the fused path has one launch shape per family
+        return FS_OK;
+    }
     const int nblk = m->nen * (m->nen + 1) / 2;
     const size_t kbytes = (size_t)m->E * nblk * 36 * sizeof(double);
     if (m->Kb.bytes < kbytes && (s = dalloc(c, m->Kb, kbytes))) return s;
-    FS_HIP(c, hipMemsetAsync(m->status.p, 0, sizeof(int), c->stream));
     int &variant = c->elem_variant[m->nen == 4 ? 1 : 0][ep.formulation == FS_SHEAR_AVERAGE_K ? 1 : 0];
     if (variant < 0 && !c->autotune) variant = 0;
     if (variant < 0) {
@@ -1024,20 +1359,15 @@ int fs_ctx_create(int device, fs_ctx **out) {
     return FS_OK;
 }
 
-void fs_ctx_destroy(fs_ctx *ctx) {
-    if (!ctx) return;
+}  // extern "C"
+
+namespace {
+// Drops one reference; the last one (the caller's handle or the last live
+// model) destroys the streams, events, staging buffers and the context.
+void ctx_release(fs_ctx *ctx) {
+    if (--ctx->refs > 0) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    int left;
-    {
-        std::lock_guard<std::mutex> lk(ctx->cache->mu);
-        for (auto &kv : ctx->cache->spare) (void)hipFree(kv.second);
-        ctx->cache->spare.clear();
-        ctx->cache->spare_bytes = 0;
-        ctx->cache->alive = false;  // buffers of models still alive are freed when released
-        left = --ctx->cache->refs;
-    }
-    if (left == 0) delete ctx->cache;
     if (ctx->stream2) {
         (void)hipStreamSynchronize(ctx->stream2);
         (void)hipStreamDestroy(ctx->stream2);
@@ -1049,6 +1379,28 @@ void fs_ctx_destroy(fs_ctx *ctx) {
     }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+}  // namespace
+
+extern "C" {
+
+void fs_ctx_destroy(fs_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+    int left;
+    {
+        std::lock_guard<std::mutex> lk(ctx->cache->mu);
+        for (auto &kv : ctx->cache->spare) (void)hipFree(kv.second);
+        ctx->cache->spare.clear();
+        ctx->cache->spare_bytes = 0;
+        ctx->cache->alive = false;  // buffers of models still alive are freed when released
+        left = --ctx->cache->refs;
+    }
+    if (left == 0) delete ctx->cache;
+    ctx->cache = nullptr;
+    ctx_release(ctx);
 }
 
 const char *fs_last_error(const fs_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
@@ -1145,15 +1497,19 @@ int fs_model_create(fs_ctx *ctx, int nen, int64_t nnodes, const double *xyz, int
         delete m;
         return fail(ctx, FS_ERR_DEVICE, "upload of the mesh failed");
     }
+    ++ctx->refs;
     *out = m;
     return FS_OK;
 }
 
 void fs_model_destroy(fs_model *m) {
     if (!m) return;
-    (void)hipSetDevice(m->ctx->device);
-    (void)hipStreamSynchronize(m->ctx->stream);
-    delete m;
+    fs_ctx *c = m->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    delete m;  // its buffers go back to the context's spare list, or to hipFree once the context is destroyed
+    ctx_release(c);
 }
 
 int fs_model_set_dofs(fs_model *m, const int64_t *dofnums, int64_t nalldofs, int64_t nfreedofs, int index_base) {
@@ -1432,6 +1788,49 @@ int fs_model_stiffness(fs_model *m, const fs_shell_params *p, int block, double 
     Block &B = m->blk[block];
     fs::ElemParams ep;
     int variant = 0;
+    if (c->assembly == 1) {
+        // fused element tiles: one element kernel writes the CSR values of the
+        // pairs its tile completes and the partial sums of the others; the
+        // finalize kernel adds the partials of the boundary pairs
+        const int NE = fs::fused_tile_elems(m->nen);
+        if (B.tiles.ne != NE && (s = build_tiles(m, block, NE))) return s;
+        if ((s = prepare_elements(m, p, ep, &variant, false))) return s;
+        ep.ebegin = 0;
+        ep.eend = (int32_t)(B.tiles.ntiles * NE);
+        ep.tile_elem = B.tiles.tile_elem.as<int32_t>();
+        const fs::ScatterPlan sp = plan_of(B);
+        const fs::TilePlan tp = tileplan_of(B);
+        hipEvent_t t0, t1, t2;
+        FS_HIP(c, hipEventCreate(&t0));
+        FS_HIP(c, hipEventCreate(&t1));
+        FS_HIP(c, hipEventCreate(&t2));
+        FS_HIP(c, hipEventRecord(t0, c->stream));
+        FS_HIP(c, fs::launch_element_fused(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
+                                           m->dconn.as<int32_t>(), m->nx.as<double>(), m->ny.as<double>(),
+                                           m->nz.as<double>(), m->valid.as<uint8_t>(), m->status.as<int>(), sp, tp,
+                                           B.kval.as<double>(), c->stream));
+        FS_HIP(c, hipEventRecord(t1, c->stream));
+        FS_HIP(c, fs::launch_pair_finalize(sp, tp, B.kval.as<double>(), c->stream));
+        FS_HIP(c, hipEventRecord(t2, c->stream));
+        int st = 0;
+        FS_HIP(c, hipMemcpyAsync(&st, m->status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        FS_HIP(c, hipStreamSynchronize(c->stream));
+        float ms_elem = 0.f, ms_all = 0.f;
+        FS_HIP(c, hipEventElapsedTime(&ms_elem, t0, t1));
+        FS_HIP(c, hipEventElapsedTime(&ms_all, t0, t2));
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        (void)hipEventDestroy(t2);
+        ep.tile_elem = nullptr;
+        ep.eend = (int32_t)m->E;
+        if ((s = finish_elements(m, ep, st))) return s;
+        m->timing[0] = ms_elem;
+        m->timing[1] = ms_all - ms_elem;
+        m->timing[2] = ms_all;
+        B.k_ready = true;
+        if (nzval_out) return d2h(c, nzval_out, B.kval.p, (size_t)B.nnz * 8);
+        return FS_OK;
+    }
     if ((s = prepare_elements(m, p, ep, &variant))) return s;
     // Element ranges of whole workgroups.  After range k the pairs whose
     // contributing elements all lie in ranges 0..k are complete (pair_emax),
@@ -1729,11 +2128,11 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     std::vector<int32_t> rgv(batch ? nv : 0, 0);
     if ((s = d2h(c, M.data(), B.mdiag.p, n * 8))) return s;
     for (int64_t i = 0; i < n; ++i) {
-        const double dt = batch ? p->group_dt[p->row_group[i]] : p->dt;
+        const double dt = batch ? p->group_dt[p->row_group[i]] : (p->dt_mass > 0.0 ? p->dt_mass : p->dt);
         const double cd = batch ? p->group_cdamp[p->row_group[i]] : p->cdamp;
         const int64_t q = pos[i];
         Cd[q] = cd * M[i];
-        invMC[q] = 1.0 / (M[i] + (dt / 2) * Cd[q]);  // plate_expl_examples.jl:68-69
+        invMC[q] = 1.0 / (M[i] + (dt / 2) * Cd[q]);  // plate_expl_examples.jl:68-71, before the dt adjustment
         U0v[q] = U0[i];
         V0v[q] = V0[i];
         Fv[q] = Fmag[i];
@@ -1931,6 +2330,13 @@ int fs_ctx_set_pipeline(fs_ctx *ctx, int ranges) {
     return FS_OK;
 }
 
+int fs_ctx_set_assembly(fs_ctx *ctx, int mode) {
+    if (!ctx) return FS_ERR_ARG;
+    if (mode != 0 && mode != 1) return fail(ctx, FS_ERR_ARG, "assembly mode %d", mode);
+    ctx->assembly = mode;
+    return FS_OK;
+}
+
 int fs_ctx_set_cd_lanes(fs_ctx *ctx, int lanes_per_node, int positions_per_lane) {
     if (!ctx) return FS_ERR_ARG;
     const int pj = positions_per_lane;
@@ -1940,6 +2346,155 @@ int fs_ctx_set_cd_lanes(fs_ctx *ctx, int lanes_per_node, int positions_per_lane)
     ctx->cd_lanes = lanes_per_node;
     ctx->cd_pj = lanes_per_node ? pj : 0;
     return FS_OK;
+}
+
+int fs_coo_create(fs_ctx *ctx, int64_t nrows, int64_t ncols, int32_t kr, int32_t kc, int64_t nmat,
+                  const int64_t *dofnums_row, const int64_t *dofnums_col, int index_base, fs_coo **out) {
+    if (!ctx || !out) return FS_ERR_ARG;
+    *out = nullptr;
+    ctx->err.clear();
+    if (nrows < 0 || ncols < 0 || kr < 1 || kc < 1 || kr > 4096 || kc > 4096 || nmat < 0 ||
+        (nmat > 0 && (!dofnums_row || !dofnums_col)))
+        return fail(ctx, FS_ERR_ARG, "coo: bad sizes (nrows %lld, ncols %lld, k %d x %d, %lld matrices)",
+                    (long long)nrows, (long long)ncols, kr, kc, (long long)nmat);
+    const int64_t ke = (int64_t)kr * kc;
+    for (int64_t i = 0; i < nmat * kr; ++i)
+        if (dofnums_row[i] - index_base < 0) return fail(ctx, FS_ERR_ARG, "coo: row dof number below the index base");
+    for (int64_t i = 0; i < nmat * kc; ++i)
+        if (dofnums_col[i] - index_base < 0) return fail(ctx, FS_ERR_ARG, "coo: column dof number below the index base");
+    FS_HIP(ctx, hipSetDevice(ctx->device));
+    fs_coo *h = new (std::nothrow) fs_coo;
+    if (!h) return fail(ctx, FS_ERR_NOMEM, "coo handle");
+    h->ctx = ctx;
+    h->nrows = nrows;
+    h->ncols = ncols;
+    h->nmat = nmat;
+    h->kr = kr;
+    h->kc = kc;
+    // The COO stream of assemble! (element order; within an element
+    // column-major: for j in columns, for i in rows) is bucketed by row by
+    // threads that each own a row range and scan the stream in order, so a
+    // row receives its entries in stream order.
+    std::vector<int64_t> cnt(nrows + 1, 0);
+    auto rowpass = [&](std::vector<int64_t> *fill, int64_t *bcol, int64_t *bpos) {
+        const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(host_threads(), nrows / 4096 + 1));
+        std::vector<std::thread> th;
+        for (int64_t t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                const int64_t r0 = nrows * t / nt, r1 = nrows * (t + 1) / nt;
+                for (int64_t m = 0; m < nmat; ++m) {
+                    const int64_t *dr = dofnums_row + m * kr, *dc = dofnums_col + m * kc;
+                    for (int j = 0; j < kc; ++j) {
+                        const int64_t c = dc[j] - index_base;
+                        if (c >= ncols) continue;
+                        for (int i = 0; i < kr; ++i) {
+                            const int64_t r = dr[i] - index_base;
+                            if (r < r0 || r >= r1) continue;
+                            if (!fill) {
+                                cnt[r + 1]++;
+                            } else {
+                                const int64_t q = (*fill)[r]++;
+                                bcol[q] = c;
+                                bpos[q] = m * ke + (int64_t)j * kr + i;
+                            }
+                        }
+                    }
+                }
+            });
+        for (auto &x : th) x.join();
+    };
+    rowpass(nullptr, nullptr, nullptr);
+    for (int64_t r = 0; r < nrows; ++r) cnt[r + 1] += cnt[r];
+    const int64_t M = cnt[nrows];
+    std::vector<int64_t> bcol(M), bpos(M), fill(cnt.begin(), cnt.end());
+    rowpass(&fill, bcol.data(), bpos.data());
+    // per row: a stable sort by column keeps stream order among duplicates;
+    // each run of equal columns is one stored entry (zeros retained)
+    std::vector<int64_t> rlen(nrows, 0), perm(M);
+    parallel_chunks(nrows, [&](int64_t r0, int64_t r1) {
+        for (int64_t r = r0; r < r1; ++r) {
+            const int64_t b = cnt[r], e = cnt[r + 1];
+            for (int64_t q = b; q < e; ++q) perm[q] = q;
+            std::stable_sort(perm.begin() + b, perm.begin() + e, [&](int64_t x, int64_t y) { return bcol[x] < bcol[y]; });
+            int64_t u = 0;
+            for (int64_t q = b; q < e; ++q)
+                if (q == b || bcol[perm[q]] != bcol[perm[q - 1]]) ++u;
+            rlen[r] = u;
+        }
+    }, 4096);
+    h->rowptr.assign(nrows + 1, 0);
+    for (int64_t r = 0; r < nrows; ++r) h->rowptr[r + 1] = h->rowptr[r] + rlen[r];
+    const int64_t nnz = h->rowptr[nrows];
+    h->nnz = nnz;
+    h->colval.resize(nnz);
+    std::vector<int64_t> cptr(nnz + 1), clist(M);
+    parallel_chunks(nrows, [&](int64_t r0, int64_t r1) {
+        for (int64_t r = r0; r < r1; ++r) {
+            int64_t s = h->rowptr[r] - 1;
+            for (int64_t q = cnt[r]; q < cnt[r + 1]; ++q) {
+                if (q == cnt[r] || bcol[perm[q]] != bcol[perm[q - 1]]) {
+                    ++s;
+                    h->colval[s] = bcol[perm[q]];
+                    cptr[s] = q;
+                }
+                clist[q] = bpos[perm[q]];
+            }
+        }
+    }, 4096);
+    cptr[nnz] = M;
+    int st;
+    if ((st = upload(ctx, h->cptr, cptr.data(), cptr.size())) || (st = upload(ctx, h->clist, clist.data(), clist.size())) ||
+        (st = dalloc(ctx, h->vals, (size_t)nnz * 8))) {
+        delete h;
+        return st;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        delete h;
+        return fail(ctx, FS_ERR_DEVICE, "coo: upload of the reduction plan failed");
+    }
+    ++ctx->refs;
+    *out = h;
+    return FS_OK;
+}
+
+int fs_coo_pattern(const fs_coo *h, int64_t *nnz) {
+    if (!h || !nnz) return FS_ERR_ARG;
+    *nnz = h->nnz;
+    return FS_OK;
+}
+
+int fs_coo_pattern_get(const fs_coo *h, int64_t *rowptr, int64_t *colval, int index_base) {
+    if (!h || !rowptr || !colval) return FS_ERR_ARG;
+    for (int64_t r = 0; r <= h->nrows; ++r) rowptr[r] = h->rowptr[r] + index_base;
+    parallel_chunks(h->nnz, [&](int64_t a, int64_t b) {
+        for (int64_t q = a; q < b; ++q) colval[q] = h->colval[q] + index_base;
+    });
+    return FS_OK;
+}
+
+int fs_coo_values(fs_coo *h, const double *elmats, double *nzval) {
+    if (!h) return FS_ERR_ARG;
+    fs_ctx *c = h->ctx;
+    c->err.clear();
+    const size_t nvals = (size_t)h->nmat * h->kr * h->kc;
+    if (nvals > 0 && !elmats) return fail(c, FS_ERR_ARG, "coo: NULL element matrices");
+    FS_HIP(c, hipSetDevice(c->device));
+    int s;
+    if ((s = upload(c, h->mats, elmats, nvals))) return s;
+    FS_HIP(c, fs::launch_coo_reduce(h->nnz, h->cptr.as<int64_t>(), h->clist.as<int64_t>(), h->mats.as<double>(),
+                                    h->vals.as<double>(), c->stream));
+    if (nzval) return d2h(c, nzval, h->vals.p, (size_t)h->nnz * 8);
+    FS_HIP(c, hipStreamSynchronize(c->stream));
+    return FS_OK;
+}
+
+void fs_coo_destroy(fs_coo *h) {
+    if (!h) return;
+    fs_ctx *c = h->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete h;
+    ctx_release(c);
 }
 
 int fs_model_mixed_info(const fs_model *m, int64_t *info) {

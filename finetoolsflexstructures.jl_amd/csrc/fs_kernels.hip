@@ -396,7 +396,94 @@ __device__ __forceinline__ void q4_mitc_bs(const double ec[4][2], double r, doub
 }
 
 
-template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS>
+// ---------------------------------------------------------------------------
+// CSR writes of node-pair blocks (scatter, fused tiles, boundary pairs)
+// ---------------------------------------------------------------------------
+
+// Row r of the upper pair (I, J): writes sum[0..5] (row dof(I, r), J's
+// columns) and, for I != J, the same values as column dof(I, r) of the rows
+// dof(J, c) -- K is bitwise symmetric by construction.
+__device__ __forceinline__ void write_pair_row(const ScatterPlan &sp, int I, int J, int kI, int kJ, int r,
+                                               const double sum[6], double *__restrict__ vals) {
+    const int rsI = sp.node_rowstart[(int64_t)I * 6 + r];
+    if (rsI >= 0) {
+        const int8_t *codeJ = sp.node_code + (int64_t)J * 6;
+        const int of = sp.nbr_off[2 * kI], ox = sp.nbr_off[2 * kI + 1];
+        for (int c = 0; c < 6; ++c) {
+            const int cd = codeJ[c];
+            if (cd < 0) continue;
+            vals[rsI + (cd < 8 ? of + cd : ox + cd - 8)] = sum[c];
+        }
+    }
+    if (I != J) {
+        const int cdI = sp.node_code[(int64_t)I * 6 + r];
+        if (cdI >= 0) {
+            const int pos = cdI < 8 ? sp.nbr_off[2 * kJ] + cdI : sp.nbr_off[2 * kJ + 1] + cdI - 8;
+            for (int c = 0; c < 6; ++c) {
+                const int rsJ = sp.node_rowstart[(int64_t)J * 6 + c];
+                if (rsJ >= 0) vals[rsJ + pos] = sum[c];
+            }
+        }
+    }
+}
+
+// Row r of a pair from its placement record (TilePlan trec / brec): the
+// same writes as write_pair_row, with every position taken from the record.
+__device__ __forceinline__ void write_pair_row_rec(const int4 a, const int4 b, const int4 o, int r,
+                                                   const double sum[6], double *__restrict__ vals) {
+    const int crI = (a.w >> (4 * r)) & 15;
+    if (crI == 15) return;
+    const int rowI = crI < 8 ? a.x + crI * a.z : a.y + (crI - 8) * a.z;
+    for (int c = 0; c < 6; ++c) {
+        const int cc = (b.w >> (4 * c)) & 15;
+        if (cc == 15) continue;
+        vals[rowI + (cc < 8 ? o.x + cc : o.y + cc - 8)] = sum[c];
+    }
+    if ((a.w >> 30) & 1) return;  // I = J: the block is its own transpose
+    const int colI = crI < 8 ? o.z + crI : o.w + crI - 8;
+    for (int c = 0; c < 6; ++c) {
+        const int cr = (b.w >> (4 * c)) & 15;
+        if (cr == 15) continue;
+        vals[(cr < 8 ? b.x + cr * b.z : b.y + (cr - 8) * b.z) + colI] = sum[c];
+    }
+}
+
+// Output phase of the fused assembly (TilePlan): thread (pair entry, row r)
+// of tile t sums row r of the pair's blocks in the LDS stage (stage[(le NBLK
+// + blk) 36 + 6 r + c], a block stored with J's rows read transposed) in
+// element order, then writes the CSR values of a pair completed in the tile,
+// or the 6 values of its partial sum.
+template <int NBLK, int WG>
+__device__ __forceinline__ void tile_write(int t, const TilePlan &tp, const double *stage,
+                                           double *__restrict__ vals) {
+    const int qb = tp.tp_ptr[t], qe = tp.tp_ptr[t + 1];
+    const int n = (qe - qb) * 6;
+    for (int i = threadIdx.x; i < n; i += WG) {
+        const int q = qb + i / 6, r = i % 6;
+        const uint64_t cw = tp.tcontrib[q];
+        const int slot = tp.tslot[q];
+        const int4 a = tp.trec[3 * q], b = tp.trec[3 * q + 1], o = tp.trec[3 * q + 2];
+        double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const int cnt = (int)(cw & 7);
+        for (int k = 0; k < cnt; ++k) {
+            const int v = (int)((cw >> (3 + 10 * k)) & 1023);
+            const double *B = stage + ((v & 31) * NBLK + ((v >> 5) & 15)) * 36;
+            if ((v >> 9) == 0) {
+                for (int c = 0; c < 6; ++c) sum[c] += B[r * 6 + c];
+            } else {
+                for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
+            }
+        }
+        if (slot >= 0) {
+            double *dst = tp.partial + (int64_t)slot * 36 + r * 6;
+            for (int c = 0; c < 6; ++c) dst[c] = sum[c];
+        } else {
+            write_pair_row_rec(a, b, o, r, sum, vals);
+        }
+    }
+}
+
+template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS, bool FUSED>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_kernel(ElemParams p, const double *__restrict__ xs,
                                                   const double *__restrict__ ys,
                                                   const double *__restrict__ zs,
@@ -405,9 +492,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                                                   const double *__restrict__ nys,
                                                   const double *__restrict__ nzs,
                                                   const uint8_t *__restrict__ valid,
-                                                  double *__restrict__ Kb, int *__restrict__ status) {
+                                                  double *__restrict__ Kb, int *__restrict__ status,
+                                                  ScatterPlan sp, TilePlan tp, double *__restrict__ vals) {
     using T = ElemTraits<NEN, NQP, NR>;
     using S = ElemSmem<NEN, NQP, NR, NE>;
+    static_assert(!FUSED || SPLIT == 0, "the fused assembly writes whole blocks");
     extern __shared__ double smem[];
     double *sC = smem + S::OFF_C;
     double *sG = smem + S::OFF_G;
@@ -416,14 +505,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
 
     const int tid = threadIdx.x;
     const int E = p.nelem;
-    const int e0 = p.ebegin + blockIdx.x * NE;
-    const int ne = (p.eend - e0) < NE ? (p.eend - e0) : NE;
+    // fused: workgroups visit the tiles XCD by XCD, so that the neighbouring
+    // tiles writing adjacent stretches of the same CSR rows share an L2
+    const int tblk = FUSED ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int e0 = p.ebegin + tblk * NE;
+    const int ne = FUSED ? NE : ((p.eend - e0) < NE ? (p.eend - e0) : NE);
+    auto elem = [&](int le) -> int { return FUSED ? p.tile_elem[e0 + le] : e0 + le; };
 
     // ---------------- G1: geometry per (element, qp) ----------------
     if (tid < NE * NQP) {
         const int le = tid / NQP, q = tid % NQP;
         if (le < ne) {
-            const int e = e0 + le;
+            const int e = elem(le);
             double X[NEN][3];
             for (int a = 0; a < NEN; ++a) {
                 int n = conn[a * E + e];
@@ -653,7 +746,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     auto row_of = [&](int i) { return RS == 1 ? i : (i < 3 ? 3 * h2 + i : 6 + h2 * (T::NSH / 2) + (i - 3)); };
     double A[3][3];
     if (gact) {
-        const int e = e0 + le2;
+        const int e = elem(le2);
         const double *g = sG + (le2 * NQP + q2) * T::GEO;
         const int node = conn[j2 * E + e];
         double nk_e[3];
@@ -786,7 +879,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         }
         if (NEN == 3 && ba == bb) {
             // T3 drilling in the nodal basis (:714-727) as a global rank-one term
-            const int e = e0 + le3;
+            const int e = elem(le3);
             const int node = conn[ba * E + e];
             const double *ms = sM + le3 * T::MISC;
             const int grp = p.elem_group ? p.elem_group[e] : 0;
@@ -812,10 +905,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     if constexpr (NEN == 4) {
         // Q4 drilling (_drilling_penalty_kavg, _add_drilling_stiffness!, :807-859)
         __syncthreads();
-        const int grpd = cact ? (p.elem_group ? p.elem_group[e0 + le3] : 0) : 0;
+        const int grpd = cact ? (p.elem_group ? p.elem_group[elem(le3)] : 0) : 0;
         const double dscale = p.composite ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
         if (cact && ba == bb && dscale != 0.0) {
-            const int e = e0 + le3;
+            const int e = elem(le3);
             const double *kr = sM + le3 * T::MISC + T::M_KRR;
             double sum = 0.0;
             int cnt = 0;
@@ -869,7 +962,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             for (int c = 0; c < 6; ++c) ks[r * 6 + c] = acc[r][c];
     }
     __syncthreads();
-    {
+    if constexpr (FUSED) {
+        tile_write<T::NBLK, WG>(tblk, tp, sC, vals);
+    } else {
         const int total = ne * T::KSTAGE;  // contiguous in Kb
         double *dst = Kb + (size_t)e0 * T::KSTAGE;
         for (int i = tid; i < total; i += WG) dst[i] = sC[i];
@@ -888,7 +983,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
 // point) stores a compact geometry record: E_G, gradN_e, the material scale
 // factors (and the layup angle for layered shells) and the MITC shear row
 // entries; the material factors are rebuilt per node thread.
-template <int NE, int WG, int SPLIT, int RS, bool COMP>
+template <int NE, int WG, int SPLIT, int RS, bool COMP, bool FUSED>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_q4_seq_kernel(ElemParams p, const double *__restrict__ xs,
                                                          const double *__restrict__ ys,
                                                          const double *__restrict__ zs,
@@ -897,8 +992,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                                                          const double *__restrict__ nys,
                                                          const double *__restrict__ nzs,
                                                          const uint8_t *__restrict__ valid,
-                                                         double *__restrict__ Kb, int *__restrict__ status) {
+                                                         double *__restrict__ Kb, int *__restrict__ status,
+                                                         ScatterPlan sp, TilePlan tp, double *__restrict__ vals) {
     constexpr int NEN = 4, NQP = 4, NR = 8, K = 24, NBLK = 10;
+    static_assert(!FUSED || SPLIT == 0, "the fused assembly writes whole blocks");
     // geometry record per (element, Gauss point): EG 9 | gN 8 | s_m s_b s_s
     // (layered: s_mb 0 s_s m n) | bs 24 (layered: +1 pad)
     constexpr int NSC = COMP ? 5 : 3;
@@ -923,14 +1020,16 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
 
     const int tid = threadIdx.x;
     const int E = p.nelem;
-    const int e0 = p.ebegin + blockIdx.x * NE;
-    const int ne = (p.eend - e0) < NE ? (p.eend - e0) : NE;
+    const int tblk = FUSED ? (int)xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int e0 = p.ebegin + tblk * NE;
+    const int ne = FUSED ? NE : ((p.eend - e0) < NE ? (p.eend - e0) : NE);
+    auto elem = [&](int le) -> int { return FUSED ? p.tile_elem[e0 + le] : e0 + le; };
 
     // ---------------- G1: geometry per (element, Gauss point) ----------------
     if (tid < NE * NQP) {
         const int le = tid / NQP, q = tid % NQP;
         if (le < ne) {
-            const int e = e0 + le;
+            const int e = elem(le);
             double X[4][3];
             for (int a = 0; a < 4; ++a) {
                 const int n = conn[a * E + e];
@@ -1022,7 +1121,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     uint8_t valid2 = 0;
     double n20 = 0.0, n21 = 0.0, n22 = 0.0;
     if (gact) {
-        node2 = conn[j2 * E + e0 + le2];
+        node2 = conn[j2 * E + elem(le2)];
         valid2 = valid[node2];
         n20 = nxs[node2];
         n21 = nys[node2];
@@ -1070,7 +1169,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 nk_e[2] = 1.0;
             }
             nodal_triad(nk_e, A);
-            const MatRows mr = mat_rows(p, COMP, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
+            const MatRows mr = mat_rows(p, COMP, elem(le2), g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
                                         COMP ? g[G_SC + NSC - 2] : 0.0, COMP ? g[G_SC + NSC - 1] : 0.0);
             const double gNj0 = g[G_GN + 2 * j2], gNj1 = g[G_GN + 2 * j2 + 1];
             const double *bs = g + G_BS + j2 * 6;
@@ -1115,7 +1214,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 r2a[l] = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
                 r2b[l] = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
             }
-            const MatRows mr = mat_rows(p, COMP, e0 + le2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
+            const MatRows mr = mat_rows(p, COMP, elem(le2), g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
                                         COMP ? g[G_SC + NSC - 2] : 0.0, COMP ? g[G_SC + NSC - 1] : 0.0);
             const double *bs = g + G_BS + j2 * 6;
             double *cdst = sC + le2 * CSTR + 6 * j2;
@@ -1191,7 +1290,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     // the nodal normal; -1 marks a node the average skips
     if (tid < NE * NEN && tid / NEN < ne) {
         const int le = tid / NEN, k = tid % NEN;
-        const int node = conn[k * E + e0 + le];
+        const int node = conn[k * E + elem(le)];
         double t = -1.0;
         if (valid[node]) {
             const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
@@ -1214,14 +1313,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         sM[le * MISC + 36 + k] = t;
     }
     __syncthreads();
-    const int grpd = cact ? (p.elem_group ? p.elem_group[e0 + le3] : 0) : 0;
+    const int grpd = cact ? (p.elem_group ? p.elem_group[elem(le3)] : 0) : 0;
     const double dscale = COMP ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
     if (split_dr && cact && ba == bb) {
         double *dd = p.split_dr + ((size_t)(e0 + le3) * NEN + ba) * 9;
         for (int i = 0; i < 9; ++i) dd[i] = 0.0;
     }
     if (cact && ba == bb && dscale != 0.0) {
-        const int e = e0 + le3;
+        const int e = elem(le3);
         // kavg: mean over the counted nodes, summed in node order
         double sum = 0.0;
         int cnt = 0;
@@ -1255,7 +1354,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             for (int c = 0; c < 6; ++c) ks[r * 6 + c] = acc[r][c];
     }
     __syncthreads();
-    {
+    if constexpr (FUSED) {
+        tile_write<NBLK, WG>(tblk, tp, smem, vals);
+    } else {
         const int total = ne * NBLK * 36;
         double *dst = Kb + (size_t)e0 * NBLK * 36;
         for (int i = tid; i < total; i += WG) dst[i] = smem[i];
@@ -1299,26 +1400,27 @@ __global__ __launch_bounds__(SC_WG) void scatter_pair_kernel(ScatterPlan sp, con
             for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
         }
     }
-    const int rsI = sp.node_rowstart[(int64_t)I * 6 + r];
-    if (rsI >= 0) {
-        const int8_t *codeJ = sp.node_code + (int64_t)J * 6;
-        const int of = sp.nbr_off[2 * kI], ox = sp.nbr_off[2 * kI + 1];
-        for (int c = 0; c < 6; ++c) {
-            const int cd = codeJ[c];
-            if (cd < 0) continue;
-            vals[rsI + (cd < 8 ? of + cd : ox + cd - 8)] = sum[c];
-        }
+    write_pair_row(sp, I, J, kI, kJ, r, sum, vals);
+}
+
+// Boundary pairs of the fused assembly: thread (boundary pair, row r) adds
+// row r of the pair's partial sums in tile order and writes the CSR values
+// as the scatter does.
+constexpr int PF_WG = 256;
+__global__ __launch_bounds__(PF_WG) void pair_finalize_kernel(ScatterPlan sp, TilePlan tp,
+                                                              double *__restrict__ vals) {
+    const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * PF_WG + threadIdx.x;
+    const int64_t b = gid / 6;
+    if (b >= tp.nbnd) return;
+    const int r = (int)(gid % 6);
+    const int4 ra = tp.brec[3 * b], rb = tp.brec[3 * b + 1], ro = tp.brec[3 * b + 2];
+    double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int se = tp.bp_ptr[b + 1];
+    for (int k = tp.bp_ptr[b]; k < se; ++k) {
+        const double *P = tp.partial + (int64_t)k * 36 + r * 6;
+        for (int c = 0; c < 6; ++c) sum[c] += P[c];
     }
-    if (I != J) {
-        const int cdI = sp.node_code[(int64_t)I * 6 + r];
-        if (cdI >= 0) {
-            const int pos = cdI < 8 ? sp.nbr_off[2 * kJ] + cdI : sp.nbr_off[2 * kJ + 1] + cdI - 8;
-            for (int c = 0; c < 6; ++c) {
-                const int rsJ = sp.node_rowstart[(int64_t)J * 6 + c];
-                if (rsJ >= 0) vals[rsJ + pos] = sum[c];
-            }
-        }
-    }
+    write_pair_row_rec(ra, rb, ro, r, sum, vals);
 }
 
 // ---------------------------------------------------------------------------
@@ -2641,10 +2743,31 @@ __global__ __launch_bounds__(256) void scale_kernel(int64_t n, const double *__r
 #define FS_T3_RS 2
 #endif
 
-template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS = FS_T3_RS>
+// ---------------------------------------------------------------------------
+// Generic element-matrix reduction (assemble! / makematrix! of a FEMM with no
+// element kernel here)
+// ---------------------------------------------------------------------------
+//
+// Thread per stored entry of the CSR pattern: the entry's contributions are
+// positions in the buffered element matrices, listed in the order in which
+// assemble! appended them (element order, column-major within an element),
+// and are summed in that order, as sparse(I, J, V) combines duplicates.
+__global__ __launch_bounds__(256) void coo_reduce_kernel(int64_t nnz, const int64_t *__restrict__ cptr,
+                                                        const int64_t *__restrict__ clist,
+                                                        const double *__restrict__ mats, double *__restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nnz) return;
+    const int64_t q0 = cptr[s], q1 = cptr[s + 1];
+    double acc = mats[clist[q0]];
+    for (int64_t q = q0 + 1; q < q1; ++q) acc += mats[clist[q]];
+    out[s] = acc;
+}
+
+template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS = FS_T3_RS, bool FUSED = false>
 static hipError_t launch_elem(const ElemParams &p, const double *x, const double *y, const double *z,
                               const int32_t *conn, const double *nx, const double *ny, const double *nz,
-                              const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
+                              const uint8_t *valid, double *Kb, int *status, hipStream_t st,
+                              const ScatterPlan *sp = nullptr, const TilePlan *tp = nullptr, double *vals = nullptr) {
     using S = ElemSmem<NEN, NQP, NR, NE>;
     const size_t smem = S::TOTAL * sizeof(double);
     static_assert(S::TOTAL * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
@@ -2652,15 +2775,22 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
     // set on every launch: cheap, per device, and free of shared host state
     // when several contexts launch from different threads
     if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS>,
+        hipError_t e = hipFuncSetAttribute((const void *)elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS, FUSED>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
-    if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
-    const int nblocks = (p.eend - p.ebegin + NE - 1) / NE;
+    int nblocks;
+    if constexpr (FUSED) {
+        if (!sp || !tp || !vals || !p.tile_elem || p.ebegin != 0) return hipErrorInvalidValue;
+        nblocks = (int)tp->ntiles;
+    } else {
+        if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
+        nblocks = (p.eend - p.ebegin + NE - 1) / NE;
+    }
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z,
-                       conn, nx, ny, nz, valid, Kb, status);
+    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS, FUSED>), dim3(nblocks), dim3(WG), smem, st, p, x,
+                       y, z, conn, nx, ny, nz, valid, Kb, status, sp ? *sp : ScatterPlan{}, tp ? *tp : TilePlan{},
+                       vals);
     return hipGetLastError();
 }
 
@@ -2669,10 +2799,11 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
 #define FS_Q4_RS 2
 #endif
 
-template <int NE, int WG, int SPLIT, int RS, bool COMP>
+template <int NE, int WG, int SPLIT, int RS, bool COMP, bool FUSED = false>
 static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
-                                const uint8_t *valid, double *Kb, int *status, hipStream_t st) {
+                                const uint8_t *valid, double *Kb, int *status, hipStream_t st,
+                                const ScatterPlan *sp = nullptr, const TilePlan *tp = nullptr, double *vals = nullptr) {
     // geometry records (homogeneous 44, layered 47 doubles per Gauss point),
     // one Gauss point of C, node records (also the drilling scratch)
     constexpr int total = NE * 4 * (COMP ? 47 : 44) + NE * (8 * 24 + 2) + NE * 4 * 14;
@@ -2680,15 +2811,21 @@ static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const doub
     static_assert(NE * 10 <= WG && NE * 4 * RS <= WG, "workgroup too small");
     static_assert(total * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
     if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP>,
+        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP, FUSED>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
-    if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
-    const int nblocks = (p.eend - p.ebegin + NE - 1) / NE;
+    int nblocks;
+    if constexpr (FUSED) {
+        if (!sp || !tp || !vals || !p.tile_elem || p.ebegin != 0) return hipErrorInvalidValue;
+        nblocks = (int)tp->ntiles;
+    } else {
+        if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
+        nblocks = (p.eend - p.ebegin + NE - 1) / NE;
+    }
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP>), dim3(nblocks), dim3(WG), smem, st, p, x, y, z, conn,
-                       nx, ny, nz, valid, Kb, status);
+    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP, FUSED>), dim3(nblocks), dim3(WG), smem, st, p, x, y,
+                       z, conn, nx, ny, nz, valid, Kb, status, sp ? *sp : ScatterPlan{}, tp ? *tp : TilePlan{}, vals);
     return hipGetLastError();
 }
 
@@ -2749,6 +2886,33 @@ hipError_t launch_element_blocks(const ElemParams &p, const double *x, const dou
                                     : launch_q4_seq<16, 192, 0, FS_Q4_RS, false>(FS_ARGS);
     }
 #undef FS_ARGS
+}
+
+// Tile size of the fused assembly per family (elements per workgroup of the
+// fused element kernels).
+int fused_tile_elems(int nen) { return nen == 3 ? 32 : 12; }
+
+hipError_t launch_element_fused(const ElemParams &p, const double *x, const double *y, const double *z,
+                                const int32_t *conn, const double *nx, const double *ny, const double *nz,
+                                const uint8_t *valid, int *status, const ScatterPlan &sp, const TilePlan &tp,
+                                double *vals, hipStream_t st) {
+#define FS_FARGS p, x, y, z, conn, nx, ny, nz, valid, nullptr, status, st, &sp, &tp, vals
+    if (p.split_dr != nullptr || p.split_cs != nullptr) return hipErrorInvalidValue;
+    if (p.nen == 3) {
+        if (p.formulation == 1) return launch_elem<3, 1, 12, 32, 256, 0, FS_T3_RS, true>(FS_FARGS);
+        return launch_elem<3, 1, 8, 32, 256, 0, FS_T3_RS, true>(FS_FARGS);
+    }
+    return p.composite ? launch_q4_seq<12, 128, 0, FS_Q4_RS, true, true>(FS_FARGS)
+                       : launch_q4_seq<12, 128, 0, FS_Q4_RS, false, true>(FS_FARGS);
+#undef FS_FARGS
+}
+
+hipError_t launch_pair_finalize(const ScatterPlan &sp, const TilePlan &tp, double *vals, hipStream_t st) {
+    const int64_t threads = tp.nbnd * 6;
+    if (threads == 0) return hipSuccess;
+    hipLaunchKernelGGL(pair_finalize_kernel, dim3((unsigned)((threads + PF_WG - 1) / PF_WG)), dim3(PF_WG), 0, st, sp,
+                       tp, vals);
+    return hipGetLastError();
 }
 
 hipError_t launch_scatter(const ScatterPlan &sp, const double *Kb, double *vals, hipStream_t st) {
@@ -2987,6 +3151,14 @@ hipError_t launch_pack_xyz(int64_t N, const double *x, const double *y, const do
                            hipStream_t st) {
     if (N == 0) return hipSuccess;
     hipLaunchKernelGGL(pack_xyz_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, x, y, z, xyz);
+    return hipGetLastError();
+}
+
+hipError_t launch_coo_reduce(int64_t nnz, const int64_t *cptr, const int64_t *clist, const double *mats,
+                             double *out, hipStream_t st) {
+    if (nnz == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((nnz + 255) / 256);
+    hipLaunchKernelGGL(coo_reduce_kernel, dim3(blocks), dim3(256), 0, st, nnz, cptr, clist, mats, out);
     return hipGetLastError();
 }
 

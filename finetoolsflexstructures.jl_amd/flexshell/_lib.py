@@ -87,6 +87,7 @@ class CDParams(C.Structure):
         ("row_group", _i32p),
         ("group_dt", _dp),
         ("group_cdamp", _dp),
+        ("dt_mass", C.c_double),
     ]
 
 
@@ -121,7 +122,14 @@ SIGNATURES = {
     "fs_ctx_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i32p]),
     "fs_ctx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int32]),
     "fs_ctx_set_pipeline": (C.c_int, [C.c_void_p, C.c_int]),
+    "fs_ctx_set_assembly": (C.c_int, [C.c_void_p, C.c_int]),
     "fs_ctx_set_cd_lanes": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "fs_coo_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int64, _i64p, _i64p,
+                                C.c_int, C.POINTER(C.c_void_p)]),
+    "fs_coo_pattern": (C.c_int, [C.c_void_p, _i64p]),
+    "fs_coo_pattern_get": (C.c_int, [C.c_void_p, _i64p, _i64p, C.c_int]),
+    "fs_coo_values": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "fs_coo_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None
@@ -189,6 +197,11 @@ class Context:
         """Element ranges of the assembly pipeline (1: no overlap of the
         element kernel and the scatter)."""
         self.check(self.lib.fs_ctx_set_pipeline(self.h, int(ranges)), "fs_ctx_set_pipeline")
+
+    def set_assembly(self, mode):
+        """Stiffness assembly form: 1 fused element tiles (default), 0 element
+        blocks through HBM and the pair scatter."""
+        self.check(self.lib.fs_ctx_set_assembly(self.h, int(mode)), "fs_ctx_set_assembly")
 
     def set_cd_lanes(self, lanes_per_node, positions_per_lane=0):
         """Row products of the FP64 explicit step: 0 = four lanes per row,

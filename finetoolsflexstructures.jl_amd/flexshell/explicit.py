@@ -39,15 +39,17 @@ def cd_loop(femm, cdamp, U0, V0, tend, dt, Fmag, g=None, probe_dofs=(), probe_st
     model = femm._model
     if model is None:
         raise RuntimeError("cd_loop: the FEMM has no device model (call stiffness/mass first)")
+    dt_mass = dt  # invMC is formed before the adjustment (:70-71)
     nsteps, dt = adjust_steps(tend, dt)
     t = sample_times(nsteps, dt)
     gv = np.ones(nsteps + 1) if g is None else np.array([float(g(tt)) for tt in t])
-    return run(model, cdamp, U0, V0, dt, nsteps, Fmag, gv, probe_dofs, probe_stride, precision)
+    return run(model, cdamp, U0, V0, dt, nsteps, Fmag, gv, probe_dofs, probe_stride, precision, dt_mass=dt_mass)
 
 
 def run(model, cdamp, U0, V0, dt, nsteps, Fmag, gvals, probe_dofs=(), probe_stride=1,
-        precision=_lib.FS_PREC_FP64):
-    """fs_model_explicit with an explicit step count and sampled g values."""
+        precision=_lib.FS_PREC_FP64, dt_mass=None):
+    """fs_model_explicit with an explicit step count and sampled g values;
+    invMC is formed with dt_mass (None: dt)."""
     n = model.nfree
     U0 = np.ascontiguousarray(np.asarray(U0, dtype=np.float64))
     V0 = np.ascontiguousarray(np.asarray(V0, dtype=np.float64))
@@ -67,6 +69,7 @@ def run(model, cdamp, U0, V0, dt, nsteps, Fmag, gvals, probe_dofs=(), probe_stri
     p.nsteps = int(nsteps)
     p.precision = int(precision)
     p.probe_stride = stride
+    p.dt_mass = float(dt if dt_mass is None else dt_mass)
     ctx = model.ctx
     ctx.check(ctx.lib.fs_model_explicit(model.h, C.byref(p), _lib.ptr(U0), _lib.ptr(V0), _lib.ptr(Fmag),
                                         _lib.ptr(gvals), _lib.ptr(probes, C.c_int64) if len(probes) else None,

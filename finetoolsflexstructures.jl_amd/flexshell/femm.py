@@ -11,6 +11,7 @@ Julia's `!` suffix is dropped (associategeometry! -> associategeometry).  The
 stabilisation closure stab_fun(t, h) = t^2 / (t^2 + c h^2) is given by its
 coefficient c (`stab_c`), which covers every form the reference uses.
 """
+import ctypes as C
 import warnings
 
 import numpy as np
@@ -60,6 +61,80 @@ class SysmatAssemblerHIP:
     def __init__(self, block=_lib.FS_BLOCK_FF):
         self.block = block
         self.diag_only = False
+
+
+class SysmatAssemblerHIPGeneric:
+    """The assembler protocol of FinEtools' SysmatAssemblerSparse (extended to
+    CSR by SysmatAssemblerSparseCSRSymm, AssemblyModule.jl:28-52) for FEMMs
+    whose element matrices are computed on the host: startassembly sizes the
+    buffer, assemble appends one element matrix with its row and column dof
+    numbers (0-based), makematrix reduces the buffer on the device
+    (fs_coo_values) into a CSR matrix with the pattern of sparse(I, J, V)
+    (zeros retained) and duplicates summed in assemble order.  Dof numbers
+    at or beyond nrows / ncols are dropped (the free-free block when they are
+    the numbers of free dofs).  The reduction plan is kept while the dof
+    numbers of a new assembly equal the previous ones."""
+
+    def __init__(self, ctx=None):
+        self.ctx = ctx or _lib.default_context()
+        self._h = None
+        self._key = None
+
+    def startassembly(self, elem_mat_nrows, elem_mat_ncols, n_elem_mats, row_nalldofs, col_nalldofs):
+        self.kr, self.kc = int(elem_mat_nrows), int(elem_mat_ncols)
+        self.nrows, self.ncols = int(row_nalldofs), int(col_nalldofs)
+        self._mats = np.zeros((int(n_elem_mats), self.kc, self.kr))  # column-major element matrices
+        self._dr = np.zeros((int(n_elem_mats), self.kr), dtype=np.int64)
+        self._dc = np.zeros((int(n_elem_mats), self.kc), dtype=np.int64)
+        self._n = 0
+        return self
+
+    def assemble(self, mat, dofnums_row, dofnums_col):
+        mat = np.asarray(mat, dtype=np.float64)
+        if mat.shape != (self.kr, self.kc):
+            raise ValueError(f"element matrix {mat.shape}, expected {(self.kr, self.kc)}")
+        if self._n >= self._mats.shape[0]:
+            raise ValueError("more element matrices than startassembly announced")
+        self._mats[self._n] = mat.T
+        self._dr[self._n] = np.asarray(dofnums_row).ravel()
+        self._dc[self._n] = np.asarray(dofnums_col).ravel()
+        self._n += 1
+        return self
+
+    def makematrix(self):
+        n = self._n
+        dr = np.ascontiguousarray(self._dr[:n])
+        dc = np.ascontiguousarray(self._dc[:n])
+        key = (self.nrows, self.ncols, self.kr, self.kc, n, dr.tobytes(), dc.tobytes())
+        lib = self.ctx.lib
+        if self._h is None or key != self._key:
+            self.close()
+            h = C.c_void_p()
+            self.ctx.check(lib.fs_coo_create(self.ctx.h, self.nrows, self.ncols, self.kr, self.kc, n,
+                                             _lib.ptr(dr, C.c_int64), _lib.ptr(dc, C.c_int64), 0, C.byref(h)),
+                           "fs_coo_create")
+            self._h, self._key = h, key
+            nz = C.c_int64()
+            self.ctx.check(lib.fs_coo_pattern(h, C.byref(nz)), "fs_coo_pattern")
+            self._rp = np.zeros(self.nrows + 1, dtype=np.int64)
+            self._cv = np.zeros(nz.value, dtype=np.int64)
+            self.ctx.check(lib.fs_coo_pattern_get(h, _lib.ptr(self._rp, C.c_int64), _lib.ptr(self._cv, C.c_int64), 0),
+                           "fs_coo_pattern_get")
+        vals = np.zeros(len(self._cv))
+        mats = np.ascontiguousarray(self._mats[:n])
+        self.ctx.check(lib.fs_coo_values(self._h, _lib.ptr(mats), _lib.ptr(vals)), "fs_coo_values")
+        return sp.csr_matrix((vals, self._cv.copy(), self._rp.copy()), shape=(self.nrows, self.ncols))
+
+    def close(self):
+        if self._h is not None:
+            self.ctx.lib.fs_coo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceMatrix:

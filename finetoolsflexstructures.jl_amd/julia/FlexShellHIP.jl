@@ -88,6 +88,7 @@ struct CDParams                  # fs_cd_params
     row_group::Ptr{Int32}
     group_dt::Ptr{Float64}
     group_cdamp::Ptr{Float64}
+    dt_mass::Float64               # dt of invMC (before the tend/dt adjustment)
 end
 
 # ---------------------------------------------------------------- handles
@@ -128,8 +129,9 @@ mutable struct DeviceModel
     ctx::Context
     h::Ptr{Cvoid}
     nen::Int
-    geom_key::UInt
+    geom_key::Tuple{UInt,UInt,Int,Int}
     dofs_key::UInt
+    normals_key::UInt
 end
 
 function _destroy!(m::DeviceModel)
@@ -174,11 +176,17 @@ _block(a) = a.block === :ff ? FS_BLOCK_FF : FS_BLOCK_ALL
 _nen(::Union{FEMMShellT3FF,FEMMShellT3FFComp}) = 3
 _nen(::Union{FEMMShellQ4RS,FEMMShellQ4RSComp}) = 4
 
+# The device model of a FEMM is keyed by the identity and size of the
+# coordinate and connectivity arrays (no pass over their contents per call).
+# Coordinates changed in place need `invalidate!(assembler, femm)`; the
+# dofs and the normals are compared by content, since numberdofs! and
+# associategeometry! rewrite their arrays in place.
 function _device_model!(a, self, geom0::NodalField{Float64}, dchi::NodalField)
     ctx = a.ctx
     nen = _nen(self)
     X = geom0.values                                   # N x 3, column-major (SoA)
-    gkey = hash(X, hash(self.integdomain.fes.conn))
+    fconn = self.integdomain.fes.conn
+    gkey = (UInt(objectid(X)), UInt(objectid(fconn)), size(X, 1), length(fconn))
     m = get(a.models, self, nothing)
     if m === nothing || m.geom_key != gkey
         m === nothing || _destroy!(m)
@@ -190,7 +198,7 @@ function _device_model!(a, self, geom0::NodalField{Float64}, dchi::NodalField)
                              (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Cint, Ptr{Ptr{Cvoid}}),
                              ctx.h, nen, size(X, 1), X, size(conn, 1), connrm, 1, out), "fs_model_create")
         end
-        m = DeviceModel(ctx, out[], nen, gkey, UInt(0))
+        m = DeviceModel(ctx, out[], nen, gkey, UInt(0), UInt(0))
         finalizer(_destroy!, m)
         a.models[self] = m
     end
@@ -204,24 +212,62 @@ function _device_model!(a, self, geom0::NodalField{Float64}, dchi::NodalField)
         end
         m.dofs_key = dkey
     end
-    # normals from the reference associategeometry! (any mcsys closure)
-    nrm = self._normals
-    valid = UInt8.(self._normal_valid)
-    GC.@preserve nrm valid begin
-        check(ctx, ccall((:fs_model_normals_upload, libflexshell), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{UInt8}),
-                         m.h, nrm, valid), "fs_model_normals_upload")
+    # normals from the reference associategeometry! (any mcsys closure),
+    # uploaded when they differ from the resident ones
+    nkey = hash(self._normals, hash(self._normal_valid))
+    if m.normals_key != nkey
+        nrm = self._normals
+        valid = UInt8.(self._normal_valid)
+        GC.@preserve nrm valid begin
+            check(ctx, ccall((:fs_model_normals_upload, libflexshell), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{UInt8}),
+                             m.h, nrm, valid), "fs_model_normals_upload")
+        end
+        m.normals_key = nkey
     end
     return m
 end
 
-# stab_fun(t, h) = t^2 / (t^2 + c h^2) crosses the ABI as c
-_stab_coefficient(f) = 1 / f(1.0, 1.0) - 1.0
+"""
+    invalidate!(assembler, femm)
 
+Drop the device model of `femm` (after its coordinates or connectivity were
+changed in place); the next `stiffness`/`mass` call uploads the mesh again.
+"""
+function invalidate!(a, self)
+    m = pop!(a.models, self, nothing)
+    m === nothing || _destroy!(m)
+    return a
+end
+
+# stab_fun(t, h) = t^2 / (t^2 + c h^2) crosses the ABI as c.  c is read at
+# (t, h) = (1, 1) and the form is checked at three more points: a custom
+# stab_fun of another form is refused rather than approximated.
+function _stab_coefficient(f)
+    c = 1 / f(1.0, 1.0) - 1.0
+    for (t, h) in ((0.01, 0.1), (0.002, 0.5), (0.05, 0.013))
+        v = f(t, h)
+        ok = isapprox(v, t^2 / (t^2 + c * h^2); rtol = 1e-12)
+        ok || error("FlexShellHIP: stab_fun(t, h) is not of the form t^2/(t^2 + c h^2) " *
+                    "(c = $c from stab_fun(1, 1); stab_fun($t, $h) = $v)")
+    end
+    return c
+end
+
+# otherdimension evaluated at every element (FEMMShellT3FFModule.jl:677 and
+# FEMMShellQ4RSModule.jl:924 per Gauss point); the device path takes
+# one thickness per FEMM, so a thickness that varies is refused.
 function _thickness(self)
-    # constant thickness (every in-repo use): evaluated at the first element,
-    # as stiffness() does per element (FEMMShellT3FFModule.jl:677)
     fes = self.integdomain.fes
-    return self.integdomain.otherdimension([1/3 1/3 1/3], fes.conn[1], [1/3 1/3])
+    od = self.integdomain.otherdimension
+    loc = [1/3 1/3 1/3]
+    N = _nen(self) == 3 ? [1/3 1/3 1/3] : [1/4 1/4 1/4 1/4]
+    t1 = od(loc, fes.conn[1], N)
+    for e in 2:length(fes.conn)
+        te = od(loc, fes.conn[e], N)
+        te == t1 || error("FlexShellHIP: the thickness (otherdimension) differs between elements 1 and $e " *
+                          "($t1, $te); the device path takes a constant thickness")
+    end
+    return t1
 end
 
 function _homog_params(self)
@@ -239,6 +285,11 @@ function _layup_arrays(self)
         laminate_stiffnesses!(layup, A, B, D)
         laminate_transverse_stiffness!(layup, H)
         abdh[:, g] = vcat(vec(A), vec(B), vec(D), vec(H))
+        # the device path takes one layup frame per group: a position-dependent
+        # layup CSys (evaluated per quadrature point by the reference,
+        # FEMMShellT3FFCompModule.jl:612, FEMMShellQ4RSCompModule.jl:929) is refused
+        layup.csys.isconstant || error("FlexShellHIP: layup group $g has a position-dependent CSys; " *
+                                       "the device path takes a constant layup CSys per group")
         csys[:, g] = vec(csmat(layup.csys))
         tg[g] = thickness(layup)
         inert[:, g] .= laminate_inertia!(layup)
@@ -327,6 +378,7 @@ FS_PREC_MIXED_F32 (shear in FP32 too).  Returns (U, V, probe history).
 """
 function cd_loop!(femm, assembler, cdamp, U0, V0, tend, dt, Fmag, g; probes = Int[], stride = 1, mixed = false)
     m = assembler.models[femm]
+    dt_mass = dt                               # invMC uses the dt before the adjustment (:70-71)
     nsteps = Int(round(tend / dt))
     if nsteps * dt < tend                      # plate_expl_examples.jl:72-75
         dt = tend / (nsteps + 1)
@@ -343,7 +395,7 @@ function cd_loop!(femm, assembler, cdamp, U0, V0, tend, dt, Fmag, g; probes = In
     pr = Int64.(probes .- 1)
     pout = zeros(length(pr), nsteps ÷ stride + 1)
     prec = mixed === :f32 ? FS_PREC_MIXED_F32 : (mixed === true ? FS_PREC_MIXED : FS_PREC_FP64)
-    p = Ref(CDParams(dt, cdamp, nsteps, prec, stride, 0, 1, C_NULL, C_NULL, C_NULL))
+    p = Ref(CDParams(dt, cdamp, nsteps, prec, stride, 0, 1, C_NULL, C_NULL, C_NULL, dt_mass))
     check(m.ctx, ccall((:fs_model_explicit, libflexshell), Cint,
                        (Ptr{Cvoid}, Ptr{CDParams}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
                         Ptr{Int64}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),

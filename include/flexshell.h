@@ -136,6 +136,11 @@ typedef struct fs_cd_params {
     const int32_t *row_group;
     const double *group_dt;
     const double *group_cdamp;
+    /* dt of invMC = 1 / (M + dt_mass/2 C): _cd_loop! forms invMC (:70-71)
+     * from the dt it was called with, before the adjustment of :72-75, so a
+     * damped loop whose dt was adjusted uses both.  0 = dt.  Batches use
+     * group_dt for both. */
+    double dt_mass;
 } fs_cd_params;
 
 /* ---------------------------------------------------------------------- */
@@ -144,6 +149,11 @@ typedef struct fs_cd_params {
 
 /* Create a context on HIP device `device` (one stream per context). */
 int fs_ctx_create(int device, fs_ctx **out);
+/* Release the caller's handle.  Models of the context keep it alive: the
+ * context is torn down when the handle and every model are gone, so
+ * contexts and models may be destroyed in any order (after fs_ctx_destroy
+ * the only valid call on its models is fs_model_destroy).  Any call that
+ * fails returns with the context's queued work finished. */
 void fs_ctx_destroy(fs_ctx *ctx);
 /* Message of the last failing call on this context ("" if none). */
 const char *fs_last_error(const fs_ctx *ctx);
@@ -340,6 +350,38 @@ int fs_model_resultants(fs_model *m, const fs_shell_params *p, const double *u, 
                         const int32_t *felist, int32_t ocs_kind, const double *ocs, int32_t flags, double *out,
                         int32_t *mismatch);
 
+/* ---------------------------------------------------------------------- */
+/* Generic assembler: element matrices computed on the host               */
+/* ---------------------------------------------------------------------- */
+
+typedef struct fs_coo fs_coo;
+
+/*
+ * The assembler protocol for a FEMM with no element kernel in this library
+ * (FinEtools' SysmatAssemblerSparse, extended to CSR by
+ * SysmatAssemblerSparseCSRSymm, AssemblyModule.jl:28-52): the caller buffers
+ * what assemble!(a, mat, dofnums_row, dofnums_col) receives for each of the
+ * nmat element matrices and hands the whole buffer over in makematrix!.
+ * fs_coo_create takes the dof numbers (startassembly! sizes: nrows x ncols,
+ * kr x kc per element matrix; dofnums_row nmat x kr and dofnums_col nmat x kc,
+ * row-major, with index_base; numbers beyond nrows / ncols are dropped, as
+ * SysmatAssemblerFFBlock drops the fixed dofs) and builds the pattern: the
+ * structural union of the entries, zeros retained, columns sorted within
+ * each row.  fs_coo_values reduces the element matrices (nmat x kr x kc, each
+ * column-major as Julia stores it) on the device, summing the duplicates of
+ * each entry in the order assemble! appended them (element order, column by
+ * column), as sparse(I, J, V) does; nzval receives the nnz values (NULL:
+ * they stay on the device).  A handle may be reused for new values with the
+ * same dof numbers.
+ */
+int fs_coo_create(fs_ctx *ctx, int64_t nrows, int64_t ncols, int32_t kr, int32_t kc, int64_t nmat,
+                  const int64_t *dofnums_row, const int64_t *dofnums_col, int index_base, fs_coo **out);
+int fs_coo_pattern(const fs_coo *h, int64_t *nnz);
+/* rowptr (nrows + 1) and colval (nnz) with index_base. */
+int fs_coo_pattern_get(const fs_coo *h, int64_t *rowptr, int64_t *colval, int index_base);
+int fs_coo_values(fs_coo *h, const double *elmats, double *nzval);
+void fs_coo_destroy(fs_coo *h);
+
 /* Device time (ms) of the last fs_model_stiffness / fs_model_explicit call,
  * measured with HIP events on the context stream: which = 0 element kernels
  * (first launch to last element range done; with the assembly pipeline this
@@ -370,6 +412,23 @@ int fs_ctx_set_tuning(fs_ctx *ctx, int nen, int formulation, int32_t variant);
  * next range computes.  1 = element kernel, then scatter, on one stream.
  * Every setting gives bitwise the same matrix.  Default 1; 1 <= ranges <= 64. */
 int fs_ctx_set_pipeline(fs_ctx *ctx, int ranges);
+
+/* Form of the stiffness assembly of fs_model_stiffness (no counterpart in the
+ * reference, which assembles element by element on the CPU):
+ *   1 (default) fused element tiles: the elements are grouped into compact
+ *     tiles (per connected component, along a Morton curve of their
+ *     centroids); one kernel computes a tile's element matrices and writes
+ *     the CSR values of every node pair whose contributing elements all lie
+ *     in the tile, summed in element order; the other pairs leave partial
+ *     sums that a second kernel adds in tile order.  No element matrices go
+ *     through HBM.
+ *   0 element node-pair blocks to HBM, then the pair scatter (sums in
+ *     element order; the pipeline of fs_ctx_set_pipeline applies).
+ * Both give the same pattern; the values agree to rounding (the boundary
+ * pairs of form 1 add their tile partials).  Each form is bitwise
+ * reproducible and gives a model bitwise the same matrix alone and in a
+ * batch. */
+int fs_ctx_set_assembly(fs_ctx *ctx, int mode);
 
 /* Form of the row products of the FP64 explicit step and of the power
  * iteration of fs_model_omega_max (no counterpart in the reference, whose

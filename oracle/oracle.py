@@ -36,8 +36,12 @@ What pins each function (tests/test_oracle_pins.py unless noted):
   or_laminate / ply matrices  Barbero Ex 5.7 Dps and A, single isotropic ply A,
                               cross-ply B = 0 (test/test_composite_layup.jl:
                               321-328, 431-451, 594); H of one ply = 5/6 t G.
-  or_cd_loop                  undamped single-dof closed form (harmonic
-                              oscillator under the CD recurrence); energy.
+  or_cd_loop                  single-dof closed forms of the CD recurrence:
+                              undamped free vibration from the force-only
+                              A0; damped free decay (amplitude ratio and
+                              phase per step); a step load switched on at a
+                              given step (the g index); the invMC of a damped
+                              loop whose dt was adjusted (two step sizes).
   or_pwr                      largest eigenvalue by scipy on small models.
   surface_load, number_dofs   Scordelis-Lo deflections (the load of
                               test_shell_statics.jl:84-86); exactness of each
@@ -89,8 +93,10 @@ def lib():
         L.or_element_masses.restype = None
         L.or_assemble_csr.argtypes = [C.c_int, C.c_int64, _i64p, _i64p, C.c_int64, _dp, _i64p, _i64p, _dp]
         L.or_assemble_csr.restype = C.c_int64
-        L.or_cd_loop.argtypes = [C.c_int64, _i64p, _i64p, _dp, _dp, C.c_double, C.c_double, C.c_int64, _dp, _dp,
-                                 _dp, _dp, C.c_int, _i64p, _dp]
+        L.or_assemble_count.argtypes = [C.c_int, C.c_int64, _i64p, _i64p, C.c_int64]
+        L.or_assemble_count.restype = C.c_int64
+        L.or_cd_loop.argtypes = [C.c_int64, _i64p, _i64p, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                 _dp, _dp, _dp, _dp, C.c_int, _i64p, _dp]
         L.or_cd_loop.restype = None
         L.or_pwr.argtypes = [C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_int]
         L.or_pwr.restype = C.c_double
@@ -394,16 +400,17 @@ def assemble_csr(nen, conn, dofnums, nrows, Ke):
     dn = _c(dofnums, np.int64)
     Ke = _c(Ke, np.float64)
     E = conn.shape[0]
+    m = lib().or_assemble_count(nen, E, _p(conn, C.c_int64), _p(dn, C.c_int64), nrows)
+    if m < 0:
+        raise MemoryError("oracle assembly")
     rp = np.zeros(nrows + 1, dtype=np.int64)
-    nnz = lib().or_assemble_csr(nen, E, _p(conn, C.c_int64), _p(dn, C.c_int64), nrows, None, _p(rp, C.c_int64),
-                                None, None)
+    cv = np.zeros(max(m, 1), dtype=np.int64)
+    vals = np.zeros(max(m, 1))
+    nnz = lib().or_assemble_csr(nen, E, _p(conn, C.c_int64), _p(dn, C.c_int64), nrows, _p(Ke), _p(rp, C.c_int64),
+                                _p(cv, C.c_int64), _p(vals))
     if nnz < 0:
         raise MemoryError("oracle assembly")
-    cv = np.zeros(nnz, dtype=np.int64)
-    vals = np.zeros(nnz)
-    lib().or_assemble_csr(nen, E, _p(conn, C.c_int64), _p(dn, C.c_int64), nrows, _p(Ke), _p(rp, C.c_int64),
-                          _p(cv, C.c_int64), _p(vals))
-    return sp.csr_matrix((vals, cv, rp), shape=(nrows, nrows))
+    return sp.csr_matrix((vals[:nnz].copy(), cv[:nnz].copy(), rp), shape=(nrows, nrows))
 
 
 def lumped_diag(nen, conn, dofnums, nrows, tm, rm):
@@ -421,7 +428,9 @@ def lumped_diag(nen, conn, dofnums, nrows, tm, rm):
 
 
 # ---------------------------------------------------------------- dynamics
-def cd_loop(K, Mdiag, cdamp, dt, nsteps, Fmag, g, U0, V0, probes=()):
+def cd_loop(K, Mdiag, cdamp, dt, nsteps, Fmag, g, U0, V0, probes=(), dt_mass=None):
+    """nsteps of _cd_loop! at the (already adjusted) dt; invMC formed with
+    dt_mass (the dt before the adjustment; None = dt)."""
     K = sp.csr_matrix(K)
     n = K.shape[0]
     rp = _c(K.indptr, np.int64)
@@ -432,9 +441,29 @@ def cd_loop(K, Mdiag, cdamp, dt, nsteps, Fmag, g, U0, V0, probes=()):
     pr = _c(probes, np.int64)
     pout = np.zeros((nsteps + 1, len(pr)))
     lib().or_cd_loop(n, _p(rp, C.c_int64), _p(ci, C.c_int64), _p(kv), _p(_c(Mdiag, np.float64)), float(cdamp),
-                     float(dt), int(nsteps), _p(_c(Fmag, np.float64)), _p(_c(g, np.float64)), _p(U), _p(V), len(pr),
-                     _p(pr, C.c_int64), _p(pout))
+                     float(dt), float(dt if dt_mass is None else dt_mass), int(nsteps), _p(_c(Fmag, np.float64)),
+                     _p(_c(g, np.float64)), _p(U), _p(V), len(pr), _p(pr, C.c_int64), _p(pout))
     return U, V, pout
+
+
+def cd_loop_tend(K, Mdiag, ksi, omegad, U0, V0, tend, dt, Fmag, gfun, probes=()):
+    """_cd_loop!(M, K, ksi, U0, V0, tend, dt, force!, peek) as the reference
+    calls it (plate_expl_examples.jl:61-94): C = ksi 2 omegad diag(M) and
+    invMC from the given dt (:68-71), then nsteps = round(tend/dt) and dt
+    re-derived when nsteps dt < tend (:72-75), force!(F, t) = gfun(t) Fmag
+    at t accumulated as t = t + dt (:79-84).  Returns (U, V, probes, dt, nsteps)."""
+    nsteps = int(round(tend / dt))
+    dt_in = dt
+    if nsteps * dt < tend:
+        dt = tend / (nsteps + 1)
+    t = 0.0
+    g = np.zeros(nsteps + 1)
+    g[0] = gfun(t)
+    for n in range(1, nsteps + 1):
+        t = t + dt
+        g[n] = gfun(t)
+    U, V, pout = cd_loop(K, Mdiag, ksi * 2 * omegad, dt, nsteps, Fmag, g, U0, V0, probes, dt_mass=dt_in)
+    return U, V, pout, dt, nsteps
 
 
 def pwr(K, Mdiag, v0, maxit=30):

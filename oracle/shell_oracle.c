@@ -800,23 +800,60 @@ void or_element_masses(int nen, int64_t E, const double *xyz, const int64_t *con
  * sparse(I, J, V): a stable counting sort by row, then duplicates combined
  * in input order, then each row's columns sorted.
  * Call with nzval == NULL to get nnz (rowptr is filled).  rowptr has
- * nrows + 1 entries; colval/nzval nnz entries.  Returns nnz, or -1 on OOM.
+ * nrows + 1 entries; colval/nzval at least nnz entries (or_assemble_count
+ * gives an upper bound).  Returns nnz, or -1 on OOM.
+ * The counting and bucketing passes run on threads that each own a range of
+ * rows and scan the whole COO stream in order, so every row receives its
+ * entries in input order exactly as a serial pass would.
  */
+static void coo_rows_pass(int nen, int64_t E, const int64_t *conn, const int64_t *dofnums, int64_t nrows,
+                          const double *Ke, int64_t *cnt, int64_t *fill, int64_t *bc, double *bv) {
+    int k = NDOF * nen;
+#pragma omp parallel
+    {
+        int nth = 1, tid = 0;
+#ifdef _OPENMP
+        nth = omp_get_num_threads();
+        tid = omp_get_thread_num();
+#endif
+        const int64_t r0 = nrows * tid / nth, r1 = nrows * (tid + 1) / nth;
+        for (int64_t el = 0; el < E; ++el) {
+            int64_t dn[24];
+            for (int a = 0; a < nen; ++a)
+                for (int d = 0; d < NDOF; ++d) dn[a * NDOF + d] = dofnums[conn[el * nen + a] * NDOF + d];
+            for (int j = 0; j < k; ++j)
+                if (dn[j] < nrows)
+                    for (int i = 0; i < k; ++i)
+                        if (dn[i] >= r0 && dn[i] < r1) {
+                            if (!fill) {
+                                cnt[dn[i] + 1]++;
+                            } else {
+                                int64_t q = fill[dn[i]]++;
+                                bc[q] = dn[j];
+                                bv[q] = Ke ? Ke[el * k * k + i * k + j] : 0.0;
+                            }
+                        }
+        }
+    }
+}
+
+/* Number of COO entries of the block (an upper bound of its nnz). */
+int64_t or_assemble_count(int nen, int64_t E, const int64_t *conn, const int64_t *dofnums, int64_t nrows) {
+    int64_t *cnt = (int64_t *)calloc(nrows + 1, sizeof(int64_t));
+    if (!cnt) return -1;
+    coo_rows_pass(nen, E, conn, dofnums, nrows, NULL, cnt, NULL, NULL, NULL);
+    int64_t m = 0;
+    for (int64_t r = 0; r < nrows; ++r) m += cnt[r + 1];
+    free(cnt);
+    return m;
+}
+
 int64_t or_assemble_csr(int nen, int64_t E, const int64_t *conn, const int64_t *dofnums, int64_t nrows,
                         const double *Ke, int64_t *rowptr, int64_t *colval, double *nzval) {
-    int k = NDOF * nen;
     int64_t *cnt = (int64_t *)calloc(nrows + 1, sizeof(int64_t));
     if (!cnt) return -1;
     /* pass 1: entries per row */
-    for (int64_t el = 0; el < E; ++el) {
-        int64_t dn[24];
-        for (int a = 0; a < nen; ++a)
-            for (int d = 0; d < NDOF; ++d) dn[a * NDOF + d] = dofnums[conn[el * nen + a] * NDOF + d];
-        for (int j = 0; j < k; ++j)
-            if (dn[j] < nrows)
-                for (int i = 0; i < k; ++i)
-                    if (dn[i] < nrows) cnt[dn[i] + 1]++;
-    }
+    coo_rows_pass(nen, E, conn, dofnums, nrows, NULL, cnt, NULL, NULL, NULL);
     for (int64_t r = 0; r < nrows; ++r) cnt[r + 1] += cnt[r];
     int64_t m = cnt[nrows];
     int64_t *bc = (int64_t *)malloc(sizeof(int64_t) * (m > 0 ? m : 1));
@@ -828,19 +865,7 @@ int64_t or_assemble_csr(int nen, int64_t E, const int64_t *conn, const int64_t *
     }
     memcpy(fill, cnt, sizeof(int64_t) * (nrows + 1));
     /* pass 2: stable bucket by row (input order kept within a row) */
-    for (int64_t el = 0; el < E; ++el) {
-        int64_t dn[24];
-        for (int a = 0; a < nen; ++a)
-            for (int d = 0; d < NDOF; ++d) dn[a * NDOF + d] = dofnums[conn[el * nen + a] * NDOF + d];
-        for (int j = 0; j < k; ++j)
-            if (dn[j] < nrows)
-                for (int i = 0; i < k; ++i)
-                    if (dn[i] < nrows) {
-                        int64_t q = fill[dn[i]]++;
-                        bc[q] = dn[j];
-                        bv[q] = Ke ? Ke[el * k * k + i * k + j] : 0.0;
-                    }
-    }
+    coo_rows_pass(nen, E, conn, dofnums, nrows, Ke, cnt, fill, bc, bv);
     /* pass 3: per row, combine duplicates in input order, then sort columns */
     int64_t *rlen = (int64_t *)calloc(nrows + 1, sizeof(int64_t));
     int nth = 1;
@@ -852,6 +877,7 @@ int64_t or_assemble_csr(int nen, int64_t E, const int64_t *conn, const int64_t *
         free(cnt); free(bc); free(bv); free(fill); free(rlen); free(marks);
         return -1;
     }
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)nth * nrows; ++i) marks[i] = -1;
 #pragma omp parallel for schedule(dynamic, 1024)
     for (int64_t r = 0; r < nrows; ++r) {
@@ -921,13 +947,16 @@ static void csr_mv(int64_t n, const int64_t *rp, const int64_t *ci, const double
  * Runs nsteps of the central-difference loop with mass-proportional damping
  * C = cdamp * diag(M).  The external force is F(t_n) = g[n] * Fmag, with g
  * sampled by the caller at t_n = n dt (n = 0 .. nsteps).  dt is used as
- * given (the caller performs the tend/dt adjustment of :72-75).  U, V are
- * the initial conditions on entry and the final state on exit.  probe_out
+ * given (the caller performs the tend/dt adjustment of :72-75).  invMC is
+ * formed with dt_mass: the reference computes invMC (:70-71) from the dt it
+ * was called with, before the adjustment of :72-75 can change dt, so a
+ * damped loop whose dt was adjusted uses two step sizes.  U, V are the
+ * initial conditions on entry and the final state on exit.  probe_out
  * (nsteps+1) x nprobe receives U[probes] after each step (peek).
  */
 void or_cd_loop(int64_t n, const int64_t *rp, const int64_t *ci, const double *kv, const double *Mdiag,
-                double cdamp, double dt, int64_t nsteps, const double *Fmag, const double *g, double *U,
-                double *V, int nprobe, const int64_t *probes, double *probe_out) {
+                double cdamp, double dt, double dt_mass, int64_t nsteps, const double *Fmag, const double *g,
+                double *U, double *V, int nprobe, const int64_t *probes, double *probe_out) {
     double *A = (double *)malloc(sizeof(double) * n);
     double *F = (double *)malloc(sizeof(double) * n);
     double *Ev = (double *)malloc(sizeof(double) * n);
@@ -935,7 +964,7 @@ void or_cd_loop(int64_t n, const int64_t *rp, const int64_t *ci, const double *k
     double *invMC = (double *)malloc(sizeof(double) * n);
     for (int64_t i = 0; i < n; ++i) {
         C[i] = cdamp * Mdiag[i];
-        invMC[i] = 1.0 / (Mdiag[i] + (dt / 2) * C[i]);
+        invMC[i] = 1.0 / (Mdiag[i] + (dt_mass / 2) * C[i]);
     }
     double dt2_2 = ((dt * dt) / 2), dt_2 = (dt / 2);
     for (int64_t i = 0; i < n; ++i) {

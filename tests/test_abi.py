@@ -44,7 +44,7 @@ def test_struct_layouts_match_header():
     from flexshell import _lib
     assert ctypes.sizeof(_lib.ShellParams) == 4 + 4 + 3 * 8 + 9 * 8 + 4 * 8 + 4 + 4 + 5 * 8
     assert ctypes.sizeof(_lib.MassParams) == 4 + 4 + 2 * 8 + 3 * 8
-    assert ctypes.sizeof(_lib.CDParams) == 8 + 8 + 8 + 4 + 4 + 4 + 4 + 3 * 8
+    assert ctypes.sizeof(_lib.CDParams) == 8 + 8 + 8 + 4 + 4 + 4 + 4 + 3 * 8 + 8
 
 
 def test_dof_numbering_free_first_node_major():

@@ -56,3 +56,71 @@ def test_batch_equals_models_alone(ctx):
     Uo, Vo, _ = O.cd_loop(Ks[k], Ms[k], cd[k], dts[k], nsteps, Fm[k], gv[:, k], np.zeros_like(V0s[k]), V0s[k])
     assert np.linalg.norm(plain[k]["U"] - Uo) <= 1e-10 * np.linalg.norm(Uo)
     bt.close()
+
+
+@pytest.mark.timeout(900)
+def test_batch_c3_full_size(ctx):
+    """The 256-model T3 batch at its configured sizes (n per side 8..128,
+    2.99 M elements, the seed and initial velocities of the benchmark): a
+    seeded sample of models, including the largest and the smallest, gets
+    bitwise the same K, M and 500-step trajectory batched as alone; the
+    largest model's K is within 1e-12 of the oracle's assembly and its
+    trajectory within 1e-10 of the oracle's loop."""
+    from flexshell import _lib, femm as FM
+    print("[gpu] C3 full batch: 256 T3 models, n 8..128, 500 steps", flush=True)
+    models = W.batch_c3()
+    nm, nsteps = len(models), 500
+    E = np.array([m["conn"].shape[0] for m in models])
+    big = int(np.argmax(E))
+    sample = sorted({big, int(np.argmin(E))} | set(np.random.default_rng(11).choice(nm, 6, replace=False).tolist()))
+    V0s = [W.gaussian_velocity(m["dchi"], 1e-3, seed=100 + k) for k, m in enumerate(models)]
+    g = np.ones((nsteps + 1, nm))
+    dts, Ks, Ms, runs = [], {}, {}, {}
+    for k, m in enumerate(models):
+        f = FM.FEMMShellT3FF(m["conn"], m["material"], thickness=m["t"], ctx=ctx)
+        f.associategeometry(m["xyz"])
+        d = m["dchi"]
+        nf = d.nfreedofs
+        if k in sample:
+            Ks[k] = f.stiffness(FM.SysmatAssemblerFFBlock(nf), m["xyz"], None, None, d)
+            Ms[k] = f.mass(FM.SysmatAssemblerFFBlock(FM.SysmatAssemblerSparseDiag(), nf, nf), m["xyz"], d).diagonal()
+        else:
+            f.stiffness(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), m["xyz"], None, None, d)
+            f.mass(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), m["xyz"], d)
+        dts.append(explicit.stable_dt(explicit.omega_max(f, np.random.default_rng(k).random(nf), maxit=30)))
+        if k in sample:
+            z = np.zeros(nf)
+            runs[k] = explicit.run(f._model, 0.0, z, V0s[k], dts[k], nsteps, z, g[:, k])
+        f._model.close()
+    print(f"[gpu] C3 full batch: plain path done ({int(E.sum())} elements), sample {sample}", flush=True)
+    bt = B.ShellBatch(models, ctx=ctx)
+    Kb = bt.stiffness()
+    Mb = bt.mass()
+    for k in sample:
+        a, b = int(bt.free_off[k]), int(bt.free_off[k + 1])
+        lo, hi = int(Kb.indptr[a]), int(Kb.indptr[b])
+        assert np.array_equal(Kb.indptr[a:b + 1] - lo, Ks[k].indptr)
+        assert np.array_equal(Kb.indices[lo:hi] - a, Ks[k].indices)
+        assert np.array_equal(Kb.data[lo:hi], Ks[k].data), k
+        assert np.array_equal(Mb[a:b], Ms[k]), k
+    del Kb
+    nfree = bt.nfree
+    r = bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g, nsteps)
+    for k in sample:
+        sl = bt.model_slice(k)
+        assert np.array_equal(r["U"][sl], runs[k]["U"]), k
+        assert np.array_equal(r["V"][sl], runs[k]["V"]), k
+    bt.close()
+    m = models[big]
+    d = m["dchi"]
+    nf = d.nfreedofs
+    Dps, Dt = O.shell_D(O.iso_D6(m["material"].E, m["material"].nu))
+    nrm, val = O.normals(3, m["xyz"], m["conn"])
+    Ko = O.assemble_csr(3, m["conn"], d.dofnums, nf,
+                        O.element_matrices(3, m["xyz"], m["conn"], nrm, val, Dps=Dps, Dt=Dt, t=m["t"]))
+    assert np.array_equal(Ko.indices, Ks[big].indices)
+    assert np.linalg.norm(Ks[big].data - Ko.data) <= 1e-12 * np.linalg.norm(Ko.data)
+    z = np.zeros(nf)
+    Uo, Vo, _ = O.cd_loop(Ks[big], Ms[big], 0.0, dts[big], nsteps, z, g[:, big], z, V0s[big])
+    assert np.linalg.norm(runs[big]["U"] - Uo) <= 1e-10 * np.linalg.norm(Uo)
+    assert np.linalg.norm(runs[big]["V"] - Vo) <= 1e-10 * np.linalg.norm(Vo)

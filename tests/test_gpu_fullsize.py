@@ -4,8 +4,10 @@ properties that do not depend on size:
 - the 512 x 512 Q4RS cylinder panel (262,144 elements, 1,575,936 free dofs):
   the free-free pattern equals the structural union of the element dof
   pairs built independently here (scipy, from the connectivity and the dof
-  numbers), bit for bit; K is bitwise symmetric; a seeded sample of element
-  matrices matches the oracle's within 1e-12; 20 central-difference steps
+  numbers), bit for bit; K is bitwise symmetric; all 85 M values match the
+  oracle's COO assembly of every element matrix within 1e-12 (relative
+  Frobenius, and entrywise against the largest entry); a seeded sample of
+  element matrices matches the oracle's within 1e-12; 20 central-difference steps
   match the oracle's loop on the same K and M within 1e-10; the loop is
   linear, so doubling the initial velocity doubles the state exactly (a
   power-of-two scaling commutes with every rounding);
@@ -58,7 +60,8 @@ def _step_checks(f, xyz, d, seed, nsteps=20):
 
 
 def _matrix_checks(f, nen, xyz, conn, d, K, E, nu, t):
-    """Bit-exact pattern, bitwise symmetry, and a seeded element sample."""
+    """Bit-exact pattern, bitwise symmetry, every value against the oracle's
+    assembly of all element matrices, and a seeded element sample."""
     P = _pattern_reference(conn, d.dofnums, d.nfreedofs)
     assert np.array_equal(K.indptr, P.indptr) and np.array_equal(K.indices, P.indices)
     del P
@@ -67,15 +70,26 @@ def _matrix_checks(f, nen, xyz, conn, d, K, E, nu, t):
     assert np.array_equal(KT.indptr, K.indptr) and np.array_equal(KT.indices, K.indices)
     assert np.array_equal(KT.data, K.data)
     del KT
-    sample = np.sort(np.random.default_rng(7).choice(conn.shape[0], 256, replace=False))
     nrm, val = O.normals(nen, xyz, conn)
     Dps, Dt = O.shell_D(O.iso_D6(E, nu))
+    # all values: the oracle's COO -> CSR assembly of every element matrix
+    # (dense transforms, duplicates summed in element order)
+    Ko = O.assemble_csr(nen, conn, d.dofnums, d.nfreedofs,
+                        O.element_matrices(nen, xyz, conn, nrm, val, Dps=Dps, Dt=Dt, t=t))
+    assert np.array_equal(K.indptr, Ko.indptr) and np.array_equal(K.indices, Ko.indices)
+    rel = np.linalg.norm(K.data - Ko.data) / np.linalg.norm(Ko.data)
+    print(f"[gpu] full-size K vs oracle: {K.nnz} values, relative Frobenius error {rel:.2e}", flush=True)
+    assert rel <= 1e-12, rel
+    assert np.abs(K.data - Ko.data).max() <= 1e-12 * np.abs(Ko.data).max()
+    del Ko
+    sample = np.sort(np.random.default_rng(7).choice(conn.shape[0], 256, replace=False))
     Ko = O.element_matrices(nen, xyz, conn[sample], nrm, val, Dps=Dps, Dt=Dt, t=t)
     Kg = f.element_matrices(xyz)[sample]
     err = np.linalg.norm(Kg - Ko, axis=(1, 2)) / np.linalg.norm(Ko, axis=(1, 2))
     assert err.max() < 1e-12, err.max()
 
 
+@pytest.mark.timeout(600)
 def test_cylinder_q4_512_full_size(ctx):
     print("[gpu] full size: Q4RS cylinder 512x512", flush=True)
     c = W.cylinder_panel_c2(n=512)
@@ -86,6 +100,7 @@ def test_cylinder_q4_512_full_size(ctx):
     _matrix_checks(f, 4, xyz, conn, d, K, 70e9, 0.33, c["t"])
 
 
+@pytest.mark.timeout(600)
 def test_plate_t3_1024x512_full_size(ctx):
     print("[gpu] full size: T3 plate 1024x512", flush=True)
     c = W.plate_c4()

@@ -57,6 +57,7 @@ def test_assembly_pipeline_bitwise_equal(nen, shuffle):
     dch.numberdofs()
     f = FM.make(conn, MatDeforElastIso(70e9, 0.33, 2700.0), thickness=0.01, ctx=ctx)
     f.associategeometry(xyz)
+    ctx.set_assembly(0)  # element blocks through HBM + the scatter: the form the pipeline applies to
     ref = {}
     for v in range(3):
         ctx.set_tuning(nen, 0, v)

@@ -239,6 +239,95 @@ def test_cd_loop_single_dof_closed_form():
     np.testing.assert_allclose(pr[:, 0], u0 * np.cos(n * th - th / 2) / np.cos(th / 2), rtol=0, atol=1e-12)
 
 
+def _cd_closed_form(m, c, k, dt, f, u_a, u_b, n_a, n, dt_mass=None):
+    """Single-dof solution of the central-difference recurrence.  With the
+    velocity update V_{n+1} = V_n + dt/2 (A_n + A_{n+1}) the scheme is
+    Newmark (beta 0, gamma 1/2), so u_{n+1} - 2 u_n + u_{n-1} = dt^2 a_n and
+    v_n = (u_{n+1} - u_{n-1}) / (2 dt); the step's force balance
+    A_{n+1} = (f - k U_{n+1} - c (V_n + dt/2 A_n)) / (m + dt_m/2 c) then reads
+    m' a_n + c v_n + k u_n = f with m' = m + (dt_m - dt) c / 2.  That
+    three-term recurrence, fitted to u at steps n_a - 1 = u_a and n_a = u_b,
+    has the solution f/k + r^j (P cos j theta + Q sin j theta), j = n - n_a,
+    from the roots r e^(+-i theta) of alpha z^2 + beta z + gamma."""
+    dt_m = dt if dt_mass is None else dt_mass
+    mp = m + (dt_m - dt) * c / 2
+    al, be, ga = mp / dt ** 2 + c / (2 * dt), k - 2 * mp / dt ** 2, mp / dt ** 2 - c / (2 * dt)
+    r = np.sqrt(ga / al)
+    th = np.arccos(-be / (2 * np.sqrt(al * ga)))
+    up = f / k
+    P = u_b - up
+    # u_a = up + r^-1 (P cos th - Q sin th)
+    Q = (P * np.cos(th) - r * (u_a - up)) / np.sin(th)
+    j = np.asarray(n, dtype=float) - n_a
+    return up + r ** j * (P * np.cos(j * th) + Q * np.sin(j * th))
+
+
+def test_cd_loop_damped_free_decay():
+    """Damped free vibration of one dof (C = ksi 2 omega M, plate_expl_examples.jl:
+    69-71) from an initial displacement and velocity.  The damping force enters
+    as C (V + dt/2 A) and invMC = 1/(M + dt/2 C) (:86-89), which makes the
+    step an exact Newmark force balance; its closed form fixes the decay per
+    step r^2 = (m/dt^2 - c/(2 dt)) / (m/dt^2 + c/(2 dt)) and the phase, so
+    damping taken at V_n, or left out of invMC, fails."""
+    k, m, u0, v0 = 4.0e6, 2.0, 1e-3, 0.3
+    omega = np.sqrt(k / m)
+    ksi = 0.05
+    cdamp = ksi * 2 * omega
+    dt = 0.5 * 2 / omega * (np.sqrt(1 + ksi ** 2) - ksi)
+    nsteps = 1000
+    U, V, pr = O.cd_loop(sp.csr_matrix([[k]]), np.array([m]), cdamp, dt, nsteps, np.zeros(1), np.zeros(nsteps + 1),
+                         np.array([u0]), np.array([v0]), probes=[0])
+    u1 = u0 + dt * v0  # the reference's A0 = invMC F(0) = 0 (force only, :80-81)
+    n = np.arange(nsteps + 2)
+    u = _cd_closed_form(m, cdamp * m, k, dt, 0.0, u0, u1, 1, n)  # fitted at steps 0 and 1
+    scale = np.abs(pr[:, 0]).max()
+    np.testing.assert_allclose(pr[:, 0], u[:nsteps + 1], rtol=0, atol=1e-10 * scale)
+    np.testing.assert_allclose(V[0], (u[nsteps + 1] - u[nsteps - 1]) / (2 * dt), rtol=1e-8)
+    assert abs(pr[-1, 0]) < 0.05 * u0  # it decays
+
+
+@pytest.mark.parametrize("ksi", [0.0, 0.02])
+def test_cd_loop_step_load(ksi):
+    """A load switched on at step n0 (g[n] = 1 for n >= n0; F(t_n) = g[n] Fmag,
+    force! called after the displacement update, :84-85) on a dof at rest:
+    u stays 0 up to and including step n0 and then follows the closed form of
+    the forced recurrence about the static value F/k.  Reading the load of
+    the previous step (g[n-1]) delays the response by one step and fails."""
+    k, m, F0 = 9.0e6, 3.0, 250.0
+    omega = np.sqrt(k / m)
+    cdamp = ksi * 2 * omega
+    dt = 0.8 * 2 / omega * (np.sqrt(1 + ksi ** 2) - ksi)
+    nsteps, n0 = 600, 7
+    g = (np.arange(nsteps + 1) >= n0).astype(float)
+    U, V, pr = O.cd_loop(sp.csr_matrix([[k]]), np.array([m]), cdamp, dt, nsteps, np.array([F0]), g, np.zeros(1),
+                         np.zeros(1), probes=[0])
+    n = np.arange(nsteps + 1)
+    u = np.where(n <= n0, 0.0, _cd_closed_form(m, cdamp * m, k, dt, F0, 0.0, 0.0, n0, n))
+    np.testing.assert_allclose(pr[:, 0], u, rtol=0, atol=1e-10 * F0 / k)
+    assert pr[n0 + 1, 0] > 0.0
+
+
+def test_cd_loop_adjusted_dt_keeps_invmc_dt():
+    """_cd_loop! forms invMC from the dt it is called with (:70-71) and only
+    then re-derives dt = tend / (nsteps + 1) when round(tend/dt) dt < tend
+    (:72-75).  With damping the loop therefore runs at the new dt with the
+    old dt in invMC: the force balance carries m' = m + (dt_in - dt) c / 2,
+    which the closed form resolves from the single-dt one."""
+    k, m, u0 = 2.0e6, 1.0, 1e-3
+    omega = np.sqrt(k / m)
+    ksi = 0.05
+    dt_in, tend = 1e-4, 0.01234
+    U, V, pr, dt, nsteps = O.cd_loop_tend(sp.csr_matrix([[k]]), np.array([m]), ksi, omega, np.array([u0]),
+                                          np.zeros(1), tend, dt_in, np.zeros(1), lambda t: 0.0, probes=[0])
+    assert nsteps == 123 and dt == tend / 124
+    c = ksi * 2 * omega * m
+    n = np.arange(nsteps + 1)
+    u = _cd_closed_form(m, c, k, dt, 0.0, u0, u0, 1, n, dt_mass=dt_in)
+    np.testing.assert_allclose(pr[:, 0], u, rtol=0, atol=1e-11 * u0)
+    u_one_dt = _cd_closed_form(m, c, k, dt, 0.0, u0, u0, 1, n)
+    assert np.abs(u_one_dt - u).max() > 1e-8 * u0  # the two forms are told apart
+
+
 def test_pwr_largest_eigenvalue():
     n = 200
     kd = np.ones(n)
