@@ -140,15 +140,16 @@ def bench_batch(nmodels, nsteps):
 
 
 def mixed_step_bytes(N, E, nen, kmb_pairs, nsr, nnb, fmag_frac):
-    """Algorithmic bytes of one mixed-precision step: FP32 slices, the
+    """Algorithmic bytes of one mixed-precision step: the slices (FP32 value
+    and int16 residual, 6 bytes per stored value), the
     neighbour lists, FP64 shear factor rows with their incidence records and
-    connectivity (nsr > 0 only), drilling blocks, node coordinates, and the
-    FP64 state over 6 N positions (u gathered once; U V A invMC read, Fmag
+    connectivity (nsr > 0 only), drilling blocks, and the FP64 state over 6 N
+    positions (u gathered once; U V A invMC read, Fmag
     read on the steps with a nonzero load factor (fraction fmag_frac), U V A
     written)."""
     n6 = 6 * N
     shear = (E * nsr * 6 * nen * 8 + E * nen * 4 * 2 + (N + 1) * 4) if nsr > 0 else 0
-    return (kmb_pairs * 8 + nnb * 4 + (N + 1) * 4 + shear + N * 48 + N * 32
+    return (kmb_pairs * 12 + nnb * 4 + (N + 1) * 4 + shear + N * 48
             + n6 * (8 + 32 + 8 * fmag_frac + 24))
 
 

@@ -102,26 +102,28 @@ struct TilePlan {
 //          (float2 units).  Slot s of a row refers to the s-th neighbour of
 //          its node (nbr lists, sorted, including the node); slots past that
 //          node's count are zero.
+//   kml    residual of each kmb value, int16, at the same index (short2 per
+//          float2): value = hi + lo 2^(e - 165) with e the biased exponent of
+//          the FP32 value hi (zero for hi = 0), i.e. the FP64 value to a
+//          relative 2^-39 in 6 bytes
 //   cs     transverse-shear factor rows per element (FP64), K_s = sum C^T C
 //   dr     drilling 3 x 3 rotation block per node, FP64, symmetric storage
 //          (00, 01, 02, 11, 12, 22)
-// The membrane + bending product is formed on displacements relative to the
-// rigid motion of the row's node (K_mb annihilates rigid motions), so the
-// FP32 rounding of K_mb multiplies small differences instead of the full
-// displacement.
+// Each value of K_mb is stored as FP32 plus an int16 residual (kml), which
+// restores it to a relative 2^-39; the products run in FP64.
 struct MixedOp {
     int64_t nrows;            // 6 N
     int32_t nnodes;
     int32_t nelem;
     const int64_t *goff;      // nslices + 1
     const float2 *kmb;
+    const short2 *kml;        // residuals of kmb
     const int32_t *nbr_ptr;   // N + 1
     const int32_t *nbr_node;
     const double *dr;         // N x 6
     const int32_t *ne_ptr;    // node -> incident elements (ascending)
     const int32_t *ne_pack;   // per incidence: element, its nen nodes (stride 4 for T3, 8 for Q4)
     const double *cs;         // E x nsr x 6 nen
-    const double4 *xyz;       // node coordinates (x, y, z, 0)
 };
 
 // Scatter plan of one block (ALL or FF) -- see scatter_pair_kernel.

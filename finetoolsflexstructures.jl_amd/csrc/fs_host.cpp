@@ -79,12 +79,10 @@ hipError_t launch_cd_node_step(int nl, int pj, const NodeRows &nrw, const int2 *
 hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, const double *d, double *part,
                       double *out, hipStream_t st);
 hipError_t launch_scale(int64_t n, const double *ss, const double *d, const double *x, double *y, hipStream_t st);
-hipError_t launch_mixed_build(const ScatterPlan &sp, const double *Kb, const int64_t *goff, float2 *kmb,
+hipError_t launch_mixed_build(const ScatterPlan &sp, const double *Kb, const int64_t *goff, float2 *kmb, short2 *kml,
                               hipStream_t st);
 hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, const int32_t *ne_list,
                               const int32_t *conn, const double *dre, double *dr, hipStream_t st);
-hipError_t launch_pack_xyz(int64_t N, const double *x, const double *y, const double *z, double4 *xyz,
-                           hipStream_t st);
 hipError_t launch_coo_reduce(int64_t nnz, const int64_t *cptr, const int64_t *clist, const double *mats,
                              double *out, hipStream_t st);
 hipError_t launch_cd_mixed_step(int nen, int nsr, const MixedOp &op, const double *Ucur, double *Unext,
@@ -264,11 +262,11 @@ struct MixedStore {
     int nsr = 0;
     int64_t nslices = 0;
     int64_t kmb_pairs = 0;  // float2 values
-    DevBuf goff, kmb, cs, dr, ne_pack, xyz;
+    DevBuf goff, kmb, kml, cs, dr, ne_pack;
     float build_ms = 0.f;
     void reset() {
         built = false;
-        for (DevBuf *b : {&goff, &kmb, &cs, &dr, &ne_pack, &xyz}) b->release();
+        for (DevBuf *b : {&goff, &kmb, &kml, &cs, &dr, &ne_pack}) b->release();
     }
 };
 
@@ -1266,7 +1264,8 @@ int build_mixed(fs_model *m, bool shear_fp32) {
     if ((s = dalloc(c, kbm, (size_t)E * nblk * 36 * 8)) || (s = dalloc(c, dre, (size_t)E * nen * 9 * 8)) ||
         (nsr > 0 && (s = dalloc(c, X.cs, (size_t)E * nsr * K * 8))) || (s = dalloc(c, X.dr, (size_t)N * 6 * 8)) ||
         (s = upload(c, X.goff, goff.data(), goff.size())) ||
-        (s = dalloc(c, X.kmb, (size_t)goff[ns] * sizeof(float2))))
+        (s = dalloc(c, X.kmb, (size_t)goff[ns] * sizeof(float2))) ||
+        (s = dalloc(c, X.kml, (size_t)goff[ns] * sizeof(short2))))
         return s;
     if (nsr > 0) {
         // incidence records: element and its nodes, in each node's ascending element order
@@ -1279,9 +1278,6 @@ int build_mixed(fs_model *m, bool shear_fp32) {
         }
         if ((s = upload(c, X.ne_pack, pack.data(), pack.size()))) return s;
     }
-    if ((s = dalloc(c, X.xyz, (size_t)N * 32))) return s;
-    FS_HIP(c, fs::launch_pack_xyz(N, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(), X.xyz.as<double4>(),
-                                  c->stream));
     fs::ElemParams ep = m->last_ep;
     ep.split_cs = nsr > 0 ? X.cs.as<double>() : nullptr;  // nullptr: shear stays in the blocks
     ep.split_dr = dre.as<double>();
@@ -1290,11 +1286,12 @@ int build_mixed(fs_model *m, bool shear_fp32) {
     FS_HIP(c, hipEventCreate(&e1));
     FS_HIP(c, hipEventRecord(e0, c->stream));
     FS_HIP(c, hipMemsetAsync(X.kmb.p, 0, (size_t)goff[ns] * sizeof(float2), c->stream));
+    FS_HIP(c, hipMemsetAsync(X.kml.p, 0, (size_t)goff[ns] * sizeof(short2), c->stream));
     FS_HIP(c, fs::launch_element_blocks(ep, m->x.as<double>(), m->y.as<double>(), m->z.as<double>(),
                                         m->dconn.as<int32_t>(), m->nx.as<double>(), m->ny.as<double>(),
                                         m->nz.as<double>(), m->valid.as<uint8_t>(), kbm.as<double>(),
                                         m->status.as<int>(), c->stream, 0));
-    FS_HIP(c, fs::launch_mixed_build(plan_of(B), kbm.as<double>(), X.goff.as<int64_t>(), X.kmb.as<float2>(),
+    FS_HIP(c, fs::launch_mixed_build(plan_of(B), kbm.as<double>(), X.goff.as<int64_t>(), X.kmb.as<float2>(), X.kml.as<short2>(),
                                      c->stream));
     FS_HIP(c, fs::launch_mixed_drill(nen, N, (int)E, m->dne_ptr.as<int32_t>(), m->dne_list.as<int32_t>(),
                                      m->dconn.as<int32_t>(), dre.as<double>(), X.dr.as<double>(), c->stream));
@@ -1320,13 +1317,13 @@ fs::MixedOp mixed_op(const fs_model *m) {
     op.nelem = (int32_t)m->E;
     op.goff = m->mixed.goff.as<int64_t>();
     op.kmb = m->mixed.kmb.as<float2>();
+    op.kml = m->mixed.kml.as<short2>();
     op.nbr_ptr = B.nbr_ptr.as<int32_t>();
     op.nbr_node = B.nbr_node.as<int32_t>();
     op.dr = m->mixed.dr.as<double>();
     op.ne_ptr = m->dne_ptr.as<int32_t>();
     op.ne_pack = m->mixed.ne_pack.as<int32_t>();
     op.cs = m->mixed.cs.as<double>();
-    op.xyz = m->mixed.xyz.as<double4>();
     return op;
 }
 

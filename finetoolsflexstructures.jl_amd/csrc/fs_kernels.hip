@@ -2422,14 +2422,33 @@ __global__ __launch_bounds__(CDN_WG) void cd_node_step_kernel(NodeRows nrw, cons
 
 // K_mb slices from the split element blocks: thread per (node, row, neighbour
 // slot); the sum over the pair's contributions follows scatter_pair_kernel.
+// Split of an FP64 value into FP32 hi and an int16 residual lo in units of
+// 2^-15 of hi's ulp: v = hi + lo 2^(e - 165) (e = biased exponent of hi)
+// to within 2^(e - 166), a relative 2^-39.  |v - hi| <= ulp(hi) / 2 bounds
+// |lo| by 2^14.
+__device__ __forceinline__ double mx_scale(float hi) {
+    const unsigned e = (__float_as_uint(hi) >> 23) & 0xffu;
+    return e ? __longlong_as_double((long long)(e + 858u) << 52) : 0.0;
+}
+
+__device__ __forceinline__ void mx_split(double v, float &hi, short &lo) {
+    hi = (float)v;
+    const double s = mx_scale(hi);
+    lo = s != 0.0 ? (short)rint((v - (double)hi) / s) : (short)0;
+}
+
+__device__ __forceinline__ double mx_value(float hi, short lo) { return (double)hi + (double)lo * mx_scale(hi); }
+
 __global__ __launch_bounds__(256) void mixed_build_kernel(ScatterPlan sp, const double *__restrict__ Kb,
-                                                          const int64_t *__restrict__ goff, float2 *__restrict__ kmb) {
+                                                          const int64_t *__restrict__ goff, float2 *__restrict__ kmb,
+                                                          short2 *__restrict__ kml) {
     const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t I = gid / (6 * FS_SC_NB);
     if (I >= sp.nnodes) return;
     const int r = (int)((gid / FS_SC_NB) % 6);
     const int kk = (int)(gid % FS_SC_NB);
     float2 *dst = kmb + goff[I / 10] + 6 * (I % 10) + r;
+    short2 *dsl = kml + goff[I / 10] + 6 * (I % 10) + r;
     const int nb0 = sp.nbr_ptr[I], nb1 = sp.nbr_ptr[I + 1];
     for (int k = nb0 + kk; k < nb1; k += FS_SC_NB) {
         const int pr = sp.nbr_pair[k];
@@ -2447,17 +2466,15 @@ __global__ __launch_bounds__(256) void mixed_build_kernel(ScatterPlan sp, const 
             }
         }
         const int slot = k - nb0;
-        for (int c2 = 0; c2 < 3; ++c2)
-            dst[(3 * slot + c2) * 60] = make_float2((float)sum[2 * c2], (float)sum[2 * c2 + 1]);
+        for (int c2 = 0; c2 < 3; ++c2) {
+            float h0, h1;
+            short l0, l1;
+            mx_split(sum[2 * c2], h0, l0);
+            mx_split(sum[2 * c2 + 1], h1, l1);
+            dst[(3 * slot + c2) * 60] = make_float2(h0, h1);
+            dsl[(3 * slot + c2) * 60] = make_short2(l0, l1);
+        }
     }
-}
-
-// Coordinates of the mixed operator as one 32-byte record per node.
-__global__ __launch_bounds__(256) void pack_xyz_kernel(int64_t N, const double *__restrict__ x,
-                                                       const double *__restrict__ y, const double *__restrict__ z,
-                                                       double4 *__restrict__ xyz) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) xyz[i] = make_double4(x[i], y[i], z[i], 0.0);
 }
 
 // Drilling block per node: sum of the element-node blocks in element order.
@@ -2489,13 +2506,13 @@ __global__ __launch_bounds__(256) void mixed_drill_kernel(int64_t N, int E, cons
 // One fused central-difference step on the mixed operator: one wavefront per
 // slice of 10 nodes, lane 6 (i mod 10) + r for dof r of node i (lanes 60..63
 // idle); the update is the one of cd_step_kernel.
-//   K_mb: the row sum runs over the node's neighbours j on the displacement
-//     relative to the rigid motion of node i, du_j = u_j - t_i - th_i x
-//     (x_j - x_i), dth_j = th_j - th_i; K_mb annihilates rigid motions, so
-//     this equals (K_mb u)_R while the FP32 values multiply O(h) differences;
-//     products widened to FP64.  Transverse shear does no work on a rigid
-//     motion either, so in the FP32-shear form (NSR = 0) the slices carry it
-//     too.
+//   K_mb: the row sum runs over the node's neighbours j; each stored value
+//     is rebuilt from its FP32 part and int16 residual (a relative 2^-39)
+//     and multiplied in FP64.  In the form without factor rows (NSR = 0) the
+//     slices carry the transverse shear too.  (A product on displacements
+//     relative to node i's rigid motion is exact only where K_mb annihilates
+//     rigid rotations, which the nodal-triad transformation of a curved mesh
+//     does not; with 39-bit values it is not needed.)
 //   K_s (NSR > 0): for every incident element e (ascending), lane r forms the partial
 //     sums of s_e = C_e u_e over its own component r; the six lanes of the
 //     node add the partials in a fixed order, and each lane adds
@@ -2521,13 +2538,13 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
     const int r = l % 6;
     const int64_t R = 6 * i64 + r;
     const int base = (int)(threadIdx.x & 63) - r;  // lane of the node's component 0
-    const double4 xiv = op.xyz[i];
     const double2 *ui2 = reinterpret_cast<const double2 *>(Ucur + 6 * i64);  // 16-byte aligned
-    const double2 ut01 = ui2[0], ut2h0 = ui2[1], uh12 = ui2[2];
-    const double t0 = ut01.x, t1 = ut01.y, t2 = ut2h0.x, h0 = ut2h0.y, h1 = uh12.x, h2 = uh12.y;
+    const double2 ut2h0 = ui2[1], uh12 = ui2[2];
+    const double h0 = ut2h0.y, h1 = uh12.x, h2 = uh12.y;
     const int64_t g0 = op.goff[g];
     const int nsl = (int)((op.goff[g + 1] - g0) / 180);
     const float2 *kv = op.kmb + g0 + l;
+    const short2 *kl = op.kml + g0 + l;
     const int nb0 = op.nbr_ptr[i];
     const int cnt = min(op.nbr_ptr[i + 1] - nb0, nsl);
     double e = 0.0;
@@ -2538,33 +2555,35 @@ __global__ __launch_bounds__(256) void cd_mixed_step_kernel(
 #pragma unroll
         for (int t = 0; t < MX_CHUNK; ++t) jj[t] = (s0 + t < cnt) ? op.nbr_node[nb0 + s0 + t] : i;
         float2 kk[MX_CHUNK][3];
+        short2 ll[MX_CHUNK][3];
         double2 uu[MX_CHUNK][3];
-        double4 xx[MX_CHUNK];
 #pragma unroll
         for (int t = 0; t < MX_CHUNK; ++t) {
             const int s = s0 + t;
             if (s < cnt) {
-                for (int c2 = 0; c2 < 3; ++c2) kk[t][c2] = kv[(3 * s + c2) * 60];
+                for (int c2 = 0; c2 < 3; ++c2) {
+                    kk[t][c2] = kv[(3 * s + c2) * 60];
+                    ll[t][c2] = kl[(3 * s + c2) * 60];
+                }
             } else {
-                for (int c2 = 0; c2 < 3; ++c2) kk[t][c2] = make_float2(0.f, 0.f);
+                for (int c2 = 0; c2 < 3; ++c2) {
+                    kk[t][c2] = make_float2(0.f, 0.f);
+                    ll[t][c2] = make_short2(0, 0);
+                }
             }
             const double2 *uj = reinterpret_cast<const double2 *>(Ucur + 6 * (int64_t)jj[t]);
             uu[t][0] = uj[0];
             uu[t][1] = uj[1];
             uu[t][2] = uj[2];
-            xx[t] = op.xyz[jj[t]];
         }
 #pragma unroll
         for (int t = 0; t < MX_CHUNK; ++t) {
-            const double dx = xx[t].x - xiv.x, dy = xx[t].y - xiv.y, dz = xx[t].z - xiv.z;
-            // u_j minus the rigid motion of node i evaluated at node j (zero for padding: j = i)
-            const double d0 = uu[t][0].x - (t0 + (h1 * dz - h2 * dy));
-            const double d1 = uu[t][0].y - (t1 + (h2 * dx - h0 * dz));
-            const double d2 = uu[t][1].x - (t2 + (h0 * dy - h1 * dx));
-            const double d3 = uu[t][1].y - h0, d4 = uu[t][2].x - h1, d5 = uu[t][2].y - h2;
+            const double d0 = uu[t][0].x, d1 = uu[t][0].y, d2 = uu[t][1].x;
+            const double d3 = uu[t][1].y, d4 = uu[t][2].x, d5 = uu[t][2].y;
             const float2 k0 = kk[t][0], k1 = kk[t][1], k2 = kk[t][2];
-            e += (double)k0.x * d0 + (double)k0.y * d1 + (double)k1.x * d2 + (double)k1.y * d3 +
-                 (double)k2.x * d4 + (double)k2.y * d5;
+            const short2 q0 = ll[t][0], q1 = ll[t][1], q2 = ll[t][2];
+            e += mx_value(k0.x, q0.x) * d0 + mx_value(k0.y, q0.y) * d1 + mx_value(k1.x, q1.x) * d2 +
+                 mx_value(k1.y, q1.y) * d3 + mx_value(k2.x, q2.x) * d4 + mx_value(k2.y, q2.y) * d5;
         }
     }
     // drilling (FP64)
@@ -3128,11 +3147,11 @@ hipError_t launch_scale(int64_t n, const double *ss, const double *d, const doub
 }
 
 hipError_t launch_mixed_build(const ScatterPlan &sp, const double *Kb, const int64_t *goff, float2 *kmb,
-                              hipStream_t st) {
+                              short2 *kml, hipStream_t st) {
     const int64_t threads = sp.nnodes * 6 * FS_SC_NB;
     if (threads == 0) return hipSuccess;
     hipLaunchKernelGGL(mixed_build_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, sp, Kb, goff,
-                       kmb);
+                       kmb, kml);
     return hipGetLastError();
 }
 
@@ -3144,13 +3163,6 @@ hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, 
         hipLaunchKernelGGL(mixed_drill_kernel<3>, dim3(blocks), dim3(256), 0, st, N, E, ne_ptr, ne_list, conn, dre, dr);
     else
         hipLaunchKernelGGL(mixed_drill_kernel<4>, dim3(blocks), dim3(256), 0, st, N, E, ne_ptr, ne_list, conn, dre, dr);
-    return hipGetLastError();
-}
-
-hipError_t launch_pack_xyz(int64_t N, const double *x, const double *y, const double *z, double4 *xyz,
-                           hipStream_t st) {
-    if (N == 0) return hipSuccess;
-    hipLaunchKernelGGL(pack_xyz_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, x, y, z, xyz);
     return hipGetLastError();
 }
 

@@ -65,7 +65,9 @@ def test_batch_c3_full_size(ctx):
     seeded sample of models, including the largest and the smallest, gets
     bitwise the same K, M and 500-step trajectory batched as alone; the
     largest model's K is within 1e-12 of the oracle's assembly and its
-    trajectory within 1e-10 of the oracle's loop."""
+    trajectory within 1e-10 of the oracle's loop.  Both mixed precisions on
+    the same inputs stay within 1e-6 of the FP64 final state of every model
+    (and of the oracle's for the largest)."""
     from flexshell import _lib, femm as FM
     print("[gpu] C3 full batch: 256 T3 models, n 8..128, 500 steps", flush=True)
     models = W.batch_c3()
@@ -110,6 +112,18 @@ def test_batch_c3_full_size(ctx):
         sl = bt.model_slice(k)
         assert np.array_equal(r["U"][sl], runs[k]["U"]), k
         assert np.array_equal(r["V"][sl], runs[k]["V"]), k
+    # both mixed precisions on the same inputs (the figures the batch
+    # throughput is quoted with): final-state error of every model against
+    # its FP64 trajectory, max |dU| / max |U|, within 1e-6
+    mixed = {}
+    for prec in (_lib.FS_PREC_MIXED, _lib.FS_PREC_MIXED_F32):
+        rm = bt.explicit(dts, np.zeros(nm), np.zeros(nfree), np.concatenate(V0s), np.zeros(nfree), g, nsteps,
+                         precision=prec)
+        err = max(float(np.abs(rm["U"][bt.model_slice(k)] - r["U"][bt.model_slice(k)]).max()
+                        / np.abs(r["U"][bt.model_slice(k)]).max()) for k in range(nm))
+        print(f"[gpu] C3 full batch: precision {prec} max final-state error {err:.2e}", flush=True)
+        assert err <= 1e-6, (prec, err)
+        mixed[prec] = rm["U"][bt.model_slice(big)].copy()
     bt.close()
     m = models[big]
     d = m["dchi"]
@@ -124,3 +138,5 @@ def test_batch_c3_full_size(ctx):
     Uo, Vo, _ = O.cd_loop(Ks[big], Ms[big], 0.0, dts[big], nsteps, z, g[:, big], z, V0s[big])
     assert np.linalg.norm(runs[big]["U"] - Uo) <= 1e-10 * np.linalg.norm(Uo)
     assert np.linalg.norm(runs[big]["V"] - Vo) <= 1e-10 * np.linalg.norm(Vo)
+    for prec, Um in mixed.items():
+        assert np.abs(Um - Uo).max() <= 1e-6 * np.abs(Uo).max(), prec

@@ -10,7 +10,9 @@ properties that do not depend on size:
   element matrices matches the oracle's within 1e-12; 20 central-difference steps
   match the oracle's loop on the same K and M within 1e-10; the loop is
   linear, so doubling the initial velocity doubles the state exactly (a
-  power-of-two scaling commutes with every rounding);
+  power-of-two scaling commutes with every rounding); both mixed precisions
+  stay within 1e-6 of the oracle over those steps and of the FP64 run over
+  the benchmark's 1000 Gaussian-velocity steps;
 - the 1024 x 512 T3 plate (1,048,576 elements, 3,136,518 free dofs): the
   same checks."""
 import numpy as np
@@ -56,6 +58,17 @@ def _step_checks(f, xyz, d, seed, nsteps=20):
     assert np.linalg.norm(r["V"] - Vo) <= 1e-10 * np.linalg.norm(Vo)
     r2 = explicit.run(f._model, 0.0, z, 2.0 * V0, dt, nsteps, z, g)
     assert np.array_equal(r2["U"], 2.0 * r["U"]) and np.array_equal(r2["V"], 2.0 * r["V"])
+    # mixed precisions: the same steps against the oracle, and the benchmark's
+    # 1000 Gaussian-velocity steps against the FP64 run (max over the state)
+    gl = np.zeros(1001)
+    rl = explicit.run(f._model, 0.0, z, V0, dt, 1000, z, gl)
+    for prec in (_lib.FS_PREC_MIXED, _lib.FS_PREC_MIXED_F32):
+        rm = explicit.run(f._model, 0.0, z, V0, dt, nsteps, z, g, precision=prec)
+        assert np.abs(rm["U"] - Uo).max() <= 1e-6 * np.abs(Uo).max(), prec
+        rm = explicit.run(f._model, 0.0, z, V0, dt, 1000, z, gl, precision=prec)
+        err = np.abs(rm["U"] - rl["U"]).max() / np.abs(rl["U"]).max()
+        print(f"[gpu] full size: precision {prec}, 1000 steps, max error {err:.2e}", flush=True)
+        assert err <= 1e-6, (prec, err)
     return K
 
 
@@ -109,3 +122,21 @@ def test_plate_t3_1024x512_full_size(ctx):
     f.associategeometry(xyz)
     K = _step_checks(f, xyz, d, seed=32)
     _matrix_checks(f, 3, xyz, conn, d, K, W.STEEL["E"], W.STEEL["nu"], c["t"])
+    # the mixed-precision benchmark inputs: Gaussian pressure pulse (sigma
+    # 0.1 m, 1e4 Pa, 1e-4 s), 5000 steps at 0.9 of the stable dt, 64 probe
+    # dofs every 10 steps, both mixed precisions against the FP64 run
+    nf = d.nfreedofs
+    x0 = c["x0"]
+    F = W.pressure_load_t3(xyz, conn, d, lambda X: 1e4 * np.exp(-np.sum((X - x0) ** 2, axis=1) / (2 * 0.1 ** 2)))
+    dt = explicit.stable_dt(explicit.omega_max(f, np.random.default_rng(7).random(nf), maxit=30))
+    nsteps = 5000
+    g = (explicit.sample_times(nsteps, dt) <= 1e-4).astype(float)
+    wd = d.dofnums[:, 2][d.dofnums[:, 2] < nf]
+    probes = wd[np.linspace(0, len(wd) - 1, 64).astype(int)]
+    z = np.zeros(nf)
+    ref = explicit.run(f._model, 0.0, z, z, dt, nsteps, F, g, probes, 10)["probes"]
+    for prec in (_lib.FS_PREC_MIXED, _lib.FS_PREC_MIXED_F32):
+        pm = explicit.run(f._model, 0.0, z, z, dt, nsteps, F, g, probes, 10, precision=prec)["probes"]
+        err = np.abs(pm - ref).max() / np.abs(ref).max()
+        print(f"[gpu] full size plate pulse: precision {prec}, {nsteps} steps, max error {err:.2e}", flush=True)
+        assert err <= 1e-6, (prec, err)
