@@ -16,6 +16,12 @@
 
 namespace fs {
 
+// Rows of the largest free-free block the allocation-free entry points take:
+// their per-device workspace holds three vectors of this length, 512 dot
+// partials and 8 scalars (24 MiB + 4 KiB of FP64).
+constexpr int64_t NOALLOC_MAX_ROWS = int64_t(1) << 20;
+constexpr int64_t NOALLOC_DOUBLES = 3 * NOALLOC_MAX_ROWS + 512 + 8;
+
 // Parameters of one element-stiffness launch.  Material factors are Cholesky
 // factors (upper triangular, row-major) so that K_e = sum C^T C with
 // C = L * B * T; see the element kernel for the derivation.

@@ -79,6 +79,8 @@ hipError_t launch_cd_node_step(int nl, int pj, const NodeRows &nrw, const int2 *
 hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, const double *d, double *part,
                       double *out, hipStream_t st);
 hipError_t launch_scale(int64_t n, const double *ss, const double *d, const double *x, double *y, hipStream_t st);
+hipError_t launch_pwr_init(int64_t n, const double *mdiag, double *inv, double *ss, hipStream_t st);
+hipError_t noalloc_workspace(int device, double **out);
 hipError_t launch_mixed_build(const ScatterPlan &sp, const double *Kb, const int64_t *goff, float2 *kmb, short2 *kml,
                               hipStream_t st);
 hipError_t launch_mixed_drill(int nen, int64_t N, int E, const int32_t *ne_ptr, const int32_t *ne_list,
@@ -138,6 +140,11 @@ struct fs_ctx {
     // cd_pj positions per lane (0: from the block's longest row)
     int cd_lanes = 8;
     int cd_pj = 0;
+    // the device's workspace of the allocation-free entry points (address of
+    // its copy of the file-scope array, taken at context creation), and the
+    // status of that lookup
+    double *scratch = nullptr;
+    hipError_t scratch_status = hipSuccess;
     // stiffness assembly: 1 = fused element tiles (element kernel writes the
     // CSR values of the pairs it completes, boundary pairs finalised after);
     // 0 = element blocks to HBM, then the pair scatter
@@ -1352,6 +1359,9 @@ int fs_ctx_create(int device, fs_ctx **out) {
         delete c;
         return FS_ERR_DEVICE;
     }
+    // with `device` current: the device's copy of the allocation-free workspace
+    c->scratch_status = fs::noalloc_workspace(device, &c->scratch);
+    if (c->scratch_status != hipSuccess) (void)hipGetLastError();
     *out = c;
     return FS_OK;
 }
@@ -2248,24 +2258,17 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     return FS_OK;
 }
 
-int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *omega_max) {
-    if (!m || !v0 || !omega_max || maxit < 0) return FS_ERR_ARG;
+}  // extern "C"
+
+namespace {
+// Power iteration of pwr (clamp_cyl_transient_examples.jl:45-60) on the
+// resident FF-block K and M, from v (device, overwritten): w, inv (n each),
+// part (512) and ss (4) are device work arrays.
+int power_iteration(fs_model *m, double *v, double *w, double *inv, double *part, double *ss, int32_t maxit,
+                    double *omega_max) {
     fs_ctx *c = m->ctx;
-    c->err.clear();
     Block &B = m->blk[FS_BLOCK_FF];
-    if (!B.k_ready || !B.m_ready) return fail(c, FS_ERR_STATE, "omega_max: FF-block K and M must be computed first");
-    FS_HIP(c, hipSetDevice(c->device));
     const int64_t n = B.nrows;
-    std::vector<double> M(n), invM(n);
-    int s;
-    if ((s = d2h(c, M.data(), B.mdiag.p, n * 8))) return s;
-    for (int64_t i = 0; i < n; ++i) invM[i] = 1.0 / M[i];  // clamp_cyl_transient_examples.jl:47
-    DevBuf dv, dw, dinv, dpart, dss;
-    const double one[4] = {1.0, 1.0, 1.0, 1.0};
-    if ((s = upload(c, dv, v0, n)) || (s = dalloc(c, dw, n * 8)) || (s = upload(c, dinv, invM.data(), n)) ||
-        (s = dalloc(c, dpart, 512 * 8)) || (s = upload(c, dss, one, 4)))
-        return s;
-    double *v = dv.as<double>(), *w = dw.as<double>(), *ss = dss.as<double>();
     const fs::RunRows rows = runrows_of(B);
     const fs::NodeRows nrw = noderows_of(B);
     const double *kv = B.kval.as<double>();
@@ -2275,22 +2278,64 @@ int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *ome
         return c->cd_lanes > 0 ? fs::launch_spmv_node(c->cd_lanes, pj, nrw, B.slotrec.as<int2>(), kv, v, w, c->stream)
                                : fs::launch_spmv(rows, kv, v, w, c->stream);
     };
+    FS_HIP(c, fs::launch_pwr_init(n, B.mdiag.as<double>(), inv, ss, c->stream));  // invM, scale factors 1
     for (int it = 0; it < maxit; ++it) {
-        FS_HIP(c, spmv());                                                                  // w = K v
-        FS_HIP(c, fs::launch_dot(n, 0, w, nullptr, nullptr, dpart.as<double>(), ss, c->stream));
-        FS_HIP(c, fs::launch_scale(n, ss, nullptr, w, w, c->stream));                       // w /= |w|
-        FS_HIP(c, fs::launch_scale(n, ss + 3, dinv.as<double>(), w, v, c->stream));          // v = invM .* w
-        FS_HIP(c, fs::launch_dot(n, 0, v, nullptr, nullptr, dpart.as<double>(), ss + 1, c->stream));
-        FS_HIP(c, fs::launch_scale(n, ss + 1, nullptr, v, v, c->stream));                    // v /= |v|
+        FS_HIP(c, spmv());                                                            // w = K v
+        FS_HIP(c, fs::launch_dot(n, 0, w, nullptr, nullptr, part, ss, c->stream));
+        FS_HIP(c, fs::launch_scale(n, ss, nullptr, w, w, c->stream));                 // w /= |w|
+        FS_HIP(c, fs::launch_scale(n, ss + 3, inv, w, v, c->stream));                 // v = invM .* w
+        FS_HIP(c, fs::launch_dot(n, 0, v, nullptr, nullptr, part, ss + 1, c->stream));
+        FS_HIP(c, fs::launch_scale(n, ss + 1, nullptr, v, v, c->stream));             // v /= |v|
     }
     FS_HIP(c, spmv());
-    FS_HIP(c, fs::launch_dot(n, 1, w, v, nullptr, dpart.as<double>(), ss, c->stream));
-    FS_HIP(c, fs::launch_dot(n, 2, v, nullptr, B.mdiag.as<double>(), dpart.as<double>(), ss + 1, c->stream));
+    FS_HIP(c, fs::launch_dot(n, 1, w, v, nullptr, part, ss, c->stream));
+    FS_HIP(c, fs::launch_dot(n, 2, v, nullptr, B.mdiag.as<double>(), part, ss + 1, c->stream));
     double h[2];
     FS_HIP(c, hipMemcpyAsync(h, ss, 16, hipMemcpyDeviceToHost, c->stream));
     FS_HIP(c, hipStreamSynchronize(c->stream));
     *omega_max = std::sqrt(h[0] / h[1]);
     return FS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *omega_max) {
+    if (!m || !v0 || !omega_max || maxit < 0) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    Block &B = m->blk[FS_BLOCK_FF];
+    if (!B.k_ready || !B.m_ready) return fail(c, FS_ERR_STATE, "omega_max: FF-block K and M must be computed first");
+    FS_HIP(c, hipSetDevice(c->device));
+    const int64_t n = B.nrows;
+    DevBuf dv, dw, dinv, dpart, dss;
+    int s;
+    if ((s = upload(c, dv, v0, n)) || (s = dalloc(c, dw, n * 8)) || (s = dalloc(c, dinv, n * 8)) ||
+        (s = dalloc(c, dpart, 512 * 8)) || (s = dalloc(c, dss, 4 * 8)))
+        return s;
+    return power_iteration(m, dv.as<double>(), dw.as<double>(), dinv.as<double>(), dpart.as<double>(),
+                           dss.as<double>(), maxit, omega_max);
+}
+
+int fs_model_omega_max_noalloc(fs_model *m, const double *v0, int32_t maxit, double *omega_max) {
+    if (!m || !v0 || !omega_max || maxit < 0) return FS_ERR_ARG;
+    fs_ctx *c = m->ctx;
+    c->err.clear();
+    Block &B = m->blk[FS_BLOCK_FF];
+    if (!B.k_ready || !B.m_ready) return fail(c, FS_ERR_STATE, "omega_max: FF-block K and M must be computed first");
+    const int64_t n = B.nrows;
+    if (n > fs::NOALLOC_MAX_ROWS)
+        return fail(c, FS_ERR_UNSUPPORTED,
+                    "omega_max without allocation: %lld free dofs exceed the workspace's %lld (use fs_model_omega_max)",
+                    (long long)n, (long long)fs::NOALLOC_MAX_ROWS);
+    if (!c->scratch)
+        return fail(c, FS_ERR_DEVICE, "omega_max without allocation: no workspace on device %d (%s)", c->device,
+                    hipGetErrorString(c->scratch_status));
+    FS_HIP(c, hipSetDevice(c->device));
+    double *v = c->scratch, *w = v + fs::NOALLOC_MAX_ROWS, *inv = w + fs::NOALLOC_MAX_ROWS;
+    double *part = inv + fs::NOALLOC_MAX_ROWS, *ss = part + 512;
+    if (n) FS_HIP(c, hipMemcpyAsync(v, v0, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    return power_iteration(m, v, w, inv, part, ss, maxit, omega_max);
 }
 
 int fs_ctx_tuning(const fs_ctx *ctx, int nen, int formulation, int32_t *shape) {

@@ -41,6 +41,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <mutex>
 
 #include "fs_device.h"
 
@@ -2743,6 +2744,16 @@ __global__ __launch_bounds__(256) void dot_final_kernel(int nb, const double *__
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// Start of the power iteration: inv[i] = 1 / mdiag[i] (the invM of pwr,
+// clamp_cyl_transient_examples.jl:47) and the four scale factors set to one
+// (a fill kernel rather than a memset, so that the sequence can be captured).
+__global__ __launch_bounds__(256) void pwr_init_kernel(int64_t n, const double *__restrict__ mdiag,
+                                                       double *__restrict__ inv, double *__restrict__ ss) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4) ss[i] = 1.0;
+    if (i < n) inv[i] = 1.0 / mdiag[i];
+}
+
 // y = (d ? d : 1) * x / sqrt(ss[0])  (normalisation steps of pwr)
 __global__ __launch_bounds__(256) void scale_kernel(int64_t n, const double *__restrict__ ss,
                                                     const double *__restrict__ d, const double *__restrict__ x,
@@ -3138,6 +3149,45 @@ hipError_t launch_dot(int64_t n, int mode, const double *x, const double *y, con
     hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(256), 0, st, n, mode, x, y, d, part);
     hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, st, kDotBlocks, part, out);
     return hipGetLastError();
+}
+
+hipError_t launch_pwr_init(int64_t n, const double *mdiag, double *inv, double *ss, hipStream_t st) {
+    const int64_t m = n > 4 ? n : 4;
+    hipLaunchKernelGGL(pwr_init_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, n, mdiag, inv, ss);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Workspace of the allocation-free entry points
+// ---------------------------------------------------------------------------
+// One copy per device, held until the process ends (NOALLOC_DOUBLES FP64).
+// Every call on a device shares it: calls that overlap on one device race.
+// Host code must not use the array's name as a device address: it reaches
+// the device copy through the address hipGetSymbolAddress returns with that
+// device current, taken once per device by noalloc_workspace.
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[256] = {};
+}  // namespace
+
+hipError_t noalloc_workspace(int device, double **out) {
+    *out = nullptr;
+    if (device < 0 || device >= 256) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_scale(int64_t n, const double *ss, const double *d, const double *x, double *y, hipStream_t st) {

@@ -112,6 +112,7 @@ SIGNATURES = {
     "fs_model_explicit": (C.c_int, [C.c_void_p, C.POINTER(CDParams), _dp, _dp, _dp, _dp, _i64p, C.c_int32,
                                     _dp, _dp, _dp]),
     "fs_model_omega_max": (C.c_int, [C.c_void_p, _dp, C.c_int32, _dp]),
+    "fs_model_omega_max_noalloc": (C.c_int, [C.c_void_p, _dp, C.c_int32, _dp]),
     "fs_model_number_dofs": (C.c_int, [C.c_void_p, _u8p, _i64p, _i64p, _i64p]),
     "fs_model_surface_load": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int, _dp]),
     "fs_model_load_points": (C.c_int, [C.c_void_p, C.c_int32, _dp]),

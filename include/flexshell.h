@@ -262,6 +262,18 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
 int fs_model_omega_max(fs_model *m, const double *v0, int32_t maxit, double *omega_max);
 
 /*
+ * fs_model_omega_max without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments and result (bitwise); its work vectors live in a per-device
+ * workspace that the library holds until the process ends (24 MiB + 4 KiB),
+ * so the free-free block may have at most 1,048,576 rows (FS_ERR_UNSUPPORTED
+ * beyond).  Every call on a device shares that workspace: calls that
+ * overlap on one device, from any context, race.
+ */
+int fs_model_omega_max_noalloc(fs_model *m, const double *v0, int32_t maxit, double *omega_max);
+
+/*
  * Element matrices before assembly: the elmat that stiffness() hands to
  * assemble!(assembler, elmat, dofnums, dofnums) (FEMMShellT3FFModule.jl:733,
  * FEMMShellQ4RSModule.jl:944), for the generic-assembler path and for tests.
