@@ -47,33 +47,165 @@ def resultant_bytes(E, N, nen, npts):
     return 4 * nen * E + (24 + 25 + 48) * N + 64 * npts * E
 
 
-def cpu_baseline(case, nsample, K_host, Mdiag, dt, cd_steps_cpu):
-    """Time the oracle (C restatement, OpenMP) on a bounded sample of the same
-    workload: element matrices + COO assembly for nsample elements, and
-    cd_steps_cpu central-difference steps on the full K."""
-    import numpy as np
+def _omp_threads(n):
+    """Set the OpenMP thread count of the oracle library (libgomp, shared with
+    the oracle's C code); returns the count in effect."""
+    import ctypes as C
+    gomp = C.CDLL("libgomp.so.1")
+    if n:
+        gomp.omp_set_num_threads(int(n))
+    return int(gomp.omp_get_max_threads())
 
+
+def _cpu_assembly_rate(nen, xyz, conn, dofnums, nrows, material, t, nsample):
+    """Elements/s of the oracle's element matrices + COO -> CSR assembly over
+    the first nsample elements."""
     from oracle import oracle as O
-    xyz, conn, d = case["xyz"], case["conn"], case["dchi"]
-    mat = case["material"]
-    Dps, Dt = O.shell_D(O.iso_D6(mat.E, mat.nu))
-    nrm, val = O.normals(4, xyz, conn)
+    Dps, Dt = O.shell_D(O.iso_D6(material.E, material.nu))
+    nrm, val = O.normals(nen, xyz, conn)
     sub = conn[:nsample]
     t0 = time.perf_counter()
-    Ke = O.element_matrices(4, xyz, sub, nrm, val, Dps=Dps, Dt=Dt, t=case["t"])
-    O.assemble_csr(4, sub, d.dofnums, d.nfreedofs, Ke)
-    t_asm = time.perf_counter() - t0
-    n = d.nfreedofs
-    V0 = np.zeros(n)
-    t0 = time.perf_counter()
-    O.cd_loop(K_host, Mdiag, 0.0, dt, cd_steps_cpu, np.zeros(n), np.zeros(cd_steps_cpu + 1), np.zeros(n), V0)
-    t_cd = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nsample / t_asm, "unit": "elements/s", "cores": cores, "kind": "port",
-            "sample": f"{nsample} elements of the workload (element matrices + COO->CSR assembly), "
-                      f"oracle C restatement with OpenMP",
-            "dof_steps_per_s": n * cd_steps_cpu / t_cd,
-            "cd_sample": f"{cd_steps_cpu} CD steps on the full free-free K ({n} dofs), OpenMP CSR SpMV"}
+    Ke = O.element_matrices(nen, xyz, sub, nrm, val, Dps=Dps, Dt=Dt, t=t)
+    O.assemble_csr(nen, sub, dofnums, nrows, Ke)
+    return nsample / (time.perf_counter() - t0)
+
+
+def cpu_baseline(case, nsample, K_host, Mdiag, dt, cd_steps_cpu):
+    """The oracle (C restatement of the reference's algorithm: dense
+    transforms, COO -> CSR with duplicates summed in input order, row CSR
+    products; OpenMP) timed on this host, on all cores and on one core:
+    - the 32 x 32 plate in full: K and M assembled, static solve under 1e3 Pa
+      (scipy spsolve) and the 10 lowest frequencies (scipy eigsh,
+      shift-invert), as the reference's statics/dynamics tests drive them;
+    - the 512 x 512 cylinder: element matrices + assembly of the first
+      nsample elements (rate extrapolated to the whole model) and
+      cd_steps_cpu central-difference steps on the full K;
+    - the 1024 x 512 plate: element matrices + assembly of a 2 x nsample
+      element sample."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    from flexshell import workloads as W
+    from oracle import oracle as O
+    all_cores = _omp_threads(0)
+    out = {"unit": "elements/s", "kind": "port", "cores": all_cores}
+    n = case["dchi"].nfreedofs
+    c1 = W.plate_c1(n=32)
+    c4 = W.plate_c4()
+    for label, nth in (("all_cores", all_cores), ("one_core", 1)):
+        _omp_threads(nth)
+        r = {"threads": nth}
+        # C1 in full
+        d1 = c1["dchi"]
+        mat1 = c1["material"]
+        t0 = time.perf_counter()
+        Dps, Dt = O.shell_D(O.iso_D6(mat1.E, mat1.nu))
+        nrm, val = O.normals(3, c1["xyz"], c1["conn"])
+        Ke = O.element_matrices(3, c1["xyz"], c1["conn"], nrm, val, Dps=Dps, Dt=Dt, t=c1["t"])
+        K1 = O.assemble_csr(3, c1["conn"], d1.dofnums, d1.nfreedofs, Ke)
+        tm, rm = O.element_masses(3, c1["xyz"], c1["conn"], rho=mat1.rho, t=c1["t"])
+        M1 = O.lumped_diag(3, c1["conn"], d1.dofnums, d1.nfreedofs, tm, rm)
+        t_asm = time.perf_counter() - t0
+        F1 = O.surface_load(3, c1["xyz"], c1["conn"], [0, 0, -1e3, 0, 0, 0], d1.nalldofs, d1.dofnums, 3)[
+            :d1.nfreedofs]
+        t0 = time.perf_counter()
+        sla.spsolve(K1.tocsc(), F1)
+        w = sla.eigsh(K1.tocsc(), k=10, M=sp.diags(M1).tocsc(), sigma=0.0, which="LM", return_eigenvectors=False)
+        t_solve = time.perf_counter() - t0
+        r["plate_32x32"] = {"elements": int(c1["conn"].shape[0]), "assembly_s": t_asm,
+                            "elements_per_s": c1["conn"].shape[0] / t_asm,
+                            "solve_and_10_frequencies_s": t_solve,
+                            "f1_hz": float(np.sqrt(np.sort(w)[0]) / (2 * np.pi))}
+        # C2 sample and steps
+        ns = nsample if nth > 1 else max(2000, nsample // 8)
+        r["cylinder_512_elements_per_s"] = _cpu_assembly_rate(4, case["xyz"], case["conn"], case["dchi"].dofnums,
+                                                              n, case["material"], case["t"], ns)
+        r["cylinder_512_sample_elements"] = ns
+        steps = cd_steps_cpu if nth > 1 else max(2, cd_steps_cpu // 5)
+        t0 = time.perf_counter()
+        O.cd_loop(K_host, Mdiag, 0.0, dt, steps, np.zeros(n), np.zeros(steps + 1), np.zeros(n), np.zeros(n))
+        r["cylinder_512_dof_steps_per_s"] = n * steps / (time.perf_counter() - t0)
+        r["cylinder_512_cd_steps"] = steps
+        # C4 sample
+        ns4 = 2 * ns
+        d4 = c4["dchi"]
+        r["plate_1024x512_elements_per_s"] = _cpu_assembly_rate(3, c4["xyz"], c4["conn"], d4.dofnums, d4.nfreedofs,
+                                                                c4["material"], c4["t"], ns4)
+        r["plate_1024x512_sample_elements"] = ns4
+        out[label] = r
+    _omp_threads(all_cores)
+    a = out["all_cores"]
+    out["value"] = a["cylinder_512_elements_per_s"]
+    out["dof_steps_per_s"] = a["cylinder_512_dof_steps_per_s"]
+    out["sample"] = (f"oracle C restatement (OpenMP) on {all_cores} threads and on 1: the 32x32 plate in full "
+                     f"(assembly, static solve, 10 frequencies); 512x512 cylinder element matrices + COO->CSR "
+                     f"assembly of the first {nsample} elements ({out['one_core']['cylinder_512_sample_elements']} "
+                     f"on 1 thread), rate extrapolated, and {cd_steps_cpu} CD steps on the full K "
+                     f"({out['one_core']['cylinder_512_cd_steps']} on 1 thread); 1024x512 plate assembly of "
+                     f"{2 * nsample} elements")
+    return out
+
+
+def time_assembly(ctx, model, params, steps, warmup):
+    """FP64 assembly of the free-free K into CSR values (pattern and plans
+    prebuilt, inputs resident in HBM), timed with HIP events on the library's
+    stream: the fused tile form (the default) and, for reference, the
+    two-kernel form (element blocks through HBM + pair scatter).  Returns
+    per-form device times (ms) of the whole assembly and of its two kernels,
+    and the wall time per fused assembly."""
+    import numpy as np
+
+    from flexshell import _lib
+
+    def assemble():
+        ctx.check(ctx.lib.fs_model_stiffness(model.h, _lib.C.byref(params), _lib.FS_BLOCK_FF, None),
+                  "fs_model_stiffness")
+    out = {}
+    for mode, name in ((0, "two_kernel"), (1, "fused")):
+        ctx.set_assembly(mode)
+        for _ in range(max(1, warmup)):
+            assemble()
+        t = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            assemble()
+            t.append((model.timing(0), model.timing(1), model.timing(2)))
+        wall = (time.perf_counter() - t0) / steps * 1e3
+        t = np.array(t)
+        out[name] = {"device_ms": float(t[:, 2].mean()), "kernel1_ms": float(t[:, 0].mean()),
+                     "kernel2_ms": float(t[:, 1].mean()), "wall_ms": wall}
+    ctx.set_assembly(1)
+    return out
+
+
+def bench_assembly_plate(steps, warmup):
+    """FP64 assembly of the 1024 x 512 T3 plate (1,048,576 elements): elements
+    per second and the HBM roofline of the fused form."""
+    from flexshell import _lib, femm as FM, workloads as W
+    ctx = _lib.default_context()
+    c = W.plate_c4()
+    d = c["dchi"]
+    f = FM.FEMMShellT3FF(c["conn"], c["material"], thickness=c["t"], ctx=ctx)
+    f.associategeometry(c["xyz"])
+    f._ensure_dofs(d)
+    nnz = len(f._pattern(_lib.FS_BLOCK_FF)[2])
+    E, N = c["conn"].shape[0], c["xyz"].shape[0]
+    t = time_assembly(ctx, f._model, f._shell_params(), steps, warmup)
+    b = assembly_bytes(E, N, 3, nnz)
+    dev = t["fused"]["device_ms"]
+    ach = b / (dev / 1e3) / 1e9
+    f._model.close()
+    return {"workload": "plate_t3_1024x512 steel t0.004 clamped, FF-block K into CSR", "elements": E, "nodes": N,
+            "nnz": nnz, "elements_per_s": E / (dev / 1e3), "device_ms": dev,
+            "kernels_ms": {"fused_element_kernel": t["fused"]["kernel1_ms"],
+                           "pair_finalize_kernel": t["fused"]["kernel2_ms"]},
+            "two_kernel_ms": {"elem_kernel": t["two_kernel"]["kernel1_ms"],
+                              "scatter_kernel": t["two_kernel"]["kernel2_ms"],
+                              "total": t["two_kernel"]["device_ms"]},
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": b,
+                         "kernel": "assembly: fused element kernel (T3, 32 elements per tile) + pair_finalize_kernel"}}
 
 
 def bench_batch(nmodels, nsteps):
@@ -134,9 +266,10 @@ def bench_batch(nmodels, nsteps):
             "plain_elements_per_s": E / (plain_asm_ms / 1e3), "batched_elements_per_s": E / (batch_asm_ms / 1e3),
             "bitwise_equal_to_plain": bool(same),
             "note": "plain = one model per launch sequence (FP64, no graph); batched = all models per launch, "
-                    "hipGraph step loop, FP64; batched_mixed = the same with FP32 membrane+bending stiffness "
-                    "(FP64 shear, drilling, mass, state); batched_mixed_f32 = shear in FP32 too; "
-                    "error = max over models of max|dU| / max|U_fp64| (white-noise initial velocities)"}
+                    "hipGraph step loop, FP64; batched_mixed = membrane+bending stiffness stored as FP32 + int16 "
+                    "residual (48-bit values, FP64 products), FP64 shear factor rows, drilling, mass, state; "
+                    "batched_mixed_f32 = the shear in the 48-bit slices too; error = max over models of "
+                    "max|dU| / max|U_fp64| after the 500 steps (Gaussian initial velocities)"}
 
 
 def mixed_step_bytes(N, E, nen, kmb_pairs, nsr, nnb, fmag_frac):
@@ -212,36 +345,45 @@ def bench_mixed(nsteps, nx=1024, ny=512):
     return out
 
 
-def bench_serve(nreq, rate, slots, plain=True, steps=500, plain_requests=300):
+def bench_serve(nreq, rate, slots, plain=True, steps=500, plain_requests=300, saturation_requests=300):
     """Served layered-shell workload (seed 6): Poisson arrivals at `rate`
     requests/s of assembly / assembly + modal matrices / explicit requests on
     T3FFComp or Q4RSComp plates of 1e3..5e5 elements (log-uniform), 3..8
-    orthotropic plies.  The request stream is replayed through the server
-    with `slots` concurrent slots (continuous batching, admission by HBM
-    footprint) and, for comparison, its first `plain_requests` requests
-    through one slot (the plain GPU path: one request at a time in arrival
-    order; when the offered rate exceeds what one slot sustains its queue
-    grows, so its latencies depend on the prefix length).  Meshes are
-    generated before the clock starts; latency is arrival -> result on the
-    host."""
+    orthotropic plies; explicit requests step in FS_PREC_MIXED_F32 (48-bit
+    stiffness values, FP64 products and state).  The request stream is
+    replayed through the server with `slots` concurrent slots (continuous
+    batching, admission by HBM footprint) and, for comparison, its first
+    `plain_requests` requests through one slot (the plain GPU path: one
+    request at a time in arrival order; when the offered rate exceeds what
+    one slot sustains its queue grows, so its latencies depend on the prefix
+    length).  Saturation: the first `saturation_requests` requests all
+    offered at once, so the server runs at capacity; its throughput is the
+    sustainable rate.  Meshes are generated before the clock starts; latency
+    is arrival -> result on the host."""
     import numpy as np
 
-    from flexshell import serve as S, workloads as W
+    from flexshell import _lib, serve as S, workloads as W
     t0 = time.perf_counter()
     specs = W.served_c5(nreq=nreq, seed=6, rate=rate, explicit_steps=steps)
     models = [W.c5_model(s) for s in specs]
     print(f"[bench] serve: {nreq} requests generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    prec = _lib.FS_PREC_MIXED_F32
 
-    def stream():
-        return [S.Request(s["rid"], s["kind"], m, steps=s["steps"], arrival=s["arrival"], seed=s["rid"])
+    def stream(at_once=False):
+        return [S.Request(s["rid"], s["kind"], m, steps=s["steps"], arrival=0.0 if at_once else s["arrival"],
+                          seed=s["rid"], precision=prec)
                 for s, m in zip(specs, models)]
 
-    out = {"requests": nreq, "rate_req_s": rate, "explicit_steps": steps, "explicit_precision": "fp64",
+    out = {"requests": nreq, "rate_req_s": rate, "explicit_steps": steps, "explicit_precision": "mixed_f32",
            "elements_total": int(sum(m["conn"].shape[0] for m in models)),
            "kinds": {k: sum(s["kind"] == k for s in specs) for k in ("assembly", "modal", "explicit")}}
-    runs = [("served", slots, nreq)] + ([("plain", 1, min(plain_requests, nreq))] if plain else [])
-    for name, ns, nr in runs:
-        reqs = stream()[:nr]
+    runs = [("served", slots, nreq, False)]
+    if saturation_requests:
+        runs.append(("saturation", slots, min(saturation_requests, nreq), True))
+    if plain:
+        runs.append(("plain", 1, min(plain_requests, nreq), False))
+    for name, ns, nr, at_once in runs:
+        reqs = stream(at_once)[:nr]
         with S.ShellServer(nslots=ns) as srv:
             # warm the kernels of both element families on every slot before the clock starts
             warm = [S.Request(-1, "modal", W.c5_model(dict(specs[0], nel=1000.0, nen=nen, kind="modal")))
@@ -254,6 +396,9 @@ def bench_serve(nreq, rate, slots, plain=True, steps=500, plain_requests=300):
         print(f"[bench] serve {name}: {summ}", file=sys.stderr, flush=True)
         if summ["failed"]:
             print("[bench] serve failures: " + "; ".join(r.error for r in reqs if r.error)[:2000], file=sys.stderr)
+    if saturation_requests:
+        out["saturation_req_s"] = out["saturation"]["throughput_req_s"]
+        out["saturation_elements_per_s"] = out["saturation"]["elements_per_s"]
     if plain:
         # served latencies of the same prefix, for a like-for-like ratio
         pre = [r for r in served_reqs if r.rid < out["plain"]["requests"]]
@@ -290,6 +435,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--cd-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plate-assembly", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-models", type=int, default=256)
@@ -336,38 +482,14 @@ def main():
     print(f"[bench] {case['name']}: E={E} N={N} nfree={nf} nnz={nnz} setup {time.perf_counter() - t_setup:.1f}s",
           file=sys.stderr, flush=True)
 
-    def assemble():
-        ctx.check(ctx.lib.fs_model_stiffness(model.h, _lib.C.byref(params), _lib.FS_BLOCK_FF, None),
-                  "fs_model_stiffness")
-
-    # element kernel and scatter one after the other on one stream: the
-    # duration of each kernel alone
-    ctx.set_pipeline(1)
-    for _ in range(max(1, args.warmup)):
-        assemble()
-    elem_ms, scat_ms = [], []
-    for _ in range(max(3, min(args.steps, 10))):
-        assemble()
-        elem_ms.append(model.timing(0))
-        scat_ms.append(model.timing(1))
-    print(f"[bench] assembly kernels alone: elem {np.mean(elem_ms):.3f} ms + scatter {np.mean(scat_ms):.3f} ms",
-          file=sys.stderr, flush=True)
-    # timed: the assembly pipeline (scatter of completed node pairs overlapping
-    # the element kernels of the later element ranges)
     ctx.set_pipeline(args.pipeline)
-    for _ in range(args.warmup):
-        assemble()
-    asm_ms, span_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        assemble()
-        asm_ms.append(model.timing(2))
-        span_ms.append(model.timing(0))
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
-    dev_ms = float(np.mean(asm_ms))
-    print(f"[bench] assembly: wall {ms_per_step:.3f} ms/assembly, device {dev_ms:.3f} ms with {args.pipeline} "
-          f"element ranges (element span {np.mean(span_ms):.3f} ms)", file=sys.stderr, flush=True)
+    ta = time_assembly(ctx, model, params, args.steps, args.warmup)
+    dev_ms = ta["fused"]["device_ms"]
+    ms_per_step = ta["fused"]["wall_ms"]
+    elem_ms = [ta["fused"]["kernel1_ms"]]
+    print(f"[bench] assembly (fused tiles): device {dev_ms:.3f} ms = element kernel {ta['fused']['kernel1_ms']:.3f} + "
+          f"finalize {ta['fused']['kernel2_ms']:.3f}; wall {ms_per_step:.3f} ms; two-kernel form "
+          f"{ta['two_kernel']['device_ms']:.3f} ms", file=sys.stderr, flush=True)
 
     # explicit loop on the resident K and M
     Kd = f.stiffness(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), case["xyz"], None, None, d)
@@ -409,17 +531,13 @@ def main():
     b_cd = cd_step_bytes(nf, nnz, model.mixed_info()[2], N)
     ach = b_asm / (dev_ms / 1e3) / 1e9
     ach_cd = b_cd / (cd_ms / ncd / 1e3) / 1e9
-    # HBM bytes per assembly from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # summary of this workload (512 x 512 only)
+    # HBM bytes per assembly (fused element kernel + finalize) from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    # (512 x 512 only), scaled by the copy-kernel calibration
     traffic = None
-    elem_flops = None
     tf = os.path.join(ROOT, "profiles", "traffic_q4_cylinder_512x512.json")
     if args.n == 512 and os.path.exists(tf):
-        tj = json.load(open(tf))
-        traffic = tj.get("assembly_bytes_per_launch")
-        # FP64 operations the element kernel issues per launch (rocprofv3
-        # SQ_INSTS_VALU_*_F64 of this workload)
-        elem_flops = tj.get("elem_kernel", {}).get("fp64_ops_issued")
+        traffic = json.load(open(tf)).get("fused_assembly_bytes_per_launch")
 
     out = {
         "metric": "FP64 shell elements assembled/s + DOF-steps/s; p50/p99 latency vs plain GPU",
@@ -439,27 +557,19 @@ def main():
                    "elements": E, "nodes": N, "free_dofs": nf, "nnz": nnz, "cd_steps": ncd},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("assembly: elem_q4_seq_kernel then scatter_pair_kernel on one stream, HIP events "
-                                "around both" if args.pipeline == 1 else
-                                f"assembly: elem_q4_seq_kernel over {args.pipeline} element ranges + "
-                                "scatter_pair_kernel on a second stream, HIP events around the whole"),
+                     "kernel": "assembly: fused element kernel (Q4, 12 elements per tile) + pair_finalize_kernel, "
+                               "HIP events on the library stream around both",
                      "algorithmic_bytes": b_asm, "device_ms": dev_ms},
         "dof_steps_per_s": dof_steps,
         "cd_us_per_step": cd_ms / ncd * 1e3,
         "roofline_cd": {"bound": "hbm", "achieved": ach_cd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach_cd / HBM_PEAK_GBS, "kernel": "cd_node_step_kernel (8 lanes per node)",
                         "algorithmic_bytes": b_cd},
-        "roofline_elem": (None if elem_flops is None else
-                          {"bound": "fp64-valu", "achieved": elem_flops / (np.mean(elem_ms) / 1e3) / 1e12,
-                           "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-                           "frac": elem_flops / (np.mean(elem_ms) / 1e3) / 1e12 / FP64_PEAK_TFS,
-                           "kernel": "elem_q4_seq_kernel (FP64 operations issued, from the committed counters)",
-                           "fp64_ops_per_launch": elem_flops}),
-        "kernels_ms": {"elem_kernel": float(np.mean(elem_ms)), "scatter_kernel": float(np.mean(scat_ms)),
-                       "note": "each kernel alone (pipeline of 1 range)"},
-        "assembly_pipeline": {"ranges": args.pipeline, "device_ms": dev_ms,
-                              "element_span_ms": float(np.mean(span_ms)),
-                              "unpipelined_ms": float(np.mean(elem_ms) + np.mean(scat_ms))},
+        "kernels_ms": {"fused_element_kernel": ta["fused"]["kernel1_ms"],
+                       "pair_finalize_kernel": ta["fused"]["kernel2_ms"],
+                       "two_kernel_form": {"elem_kernel": ta["two_kernel"]["kernel1_ms"],
+                                           "scatter_kernel": ta["two_kernel"]["kernel2_ms"],
+                                           "total": ta["two_kernel"]["device_ms"]}},
         "elem_launch_shape": dict(zip(("variant", "elements_per_workgroup", "workgroup"), ctx.tuning(4, 0))),
         "resultants": {"kernel_us": res_dev * 1e3, "points_per_s": 4 * E / (res_dev / 1e3),
                        "call_ms": res_wall,
@@ -467,6 +577,11 @@ def main():
                                     "frac": ach_res / HBM_PEAK_GBS, "kernel": "resultants_kernel<4, false, false>",
                                     "algorithmic_bytes": b_res}},
     }
+    if not args.no_plate_assembly:
+        t0 = time.perf_counter()
+        out["assembly_plate_t3_1024x512"] = bench_assembly_plate(args.steps, args.warmup)
+        print(f"[bench] plate assembly section {time.perf_counter() - t0:.1f}s: {out['assembly_plate_t3_1024x512']}",
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         import scipy.sparse as sp
         Kh = sp.csr_matrix((np.zeros(nnz), cv, rp), shape=(nf, nf))

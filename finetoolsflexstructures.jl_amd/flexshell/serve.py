@@ -152,10 +152,17 @@ def device_memory(device=0):
     return free.value, total.value
 
 
-def default_budget(device=0, fraction=0.9):
-    """HBM budget for admission: a fraction of the device memory free now."""
+# device bytes a context may keep as spare buffers after its requests end
+# (the buffer cache of libflexshell, 4 GiB per context)
+CONTEXT_SPARE_BYTES = 4 << 30
+
+
+def default_budget(device=0, fraction=0.9, nslots=0):
+    """HBM budget for admission: a fraction of the device memory free now,
+    less the spare buffers each of the nslots contexts may keep cached between
+    its requests (memory that in-flight footprints do not count)."""
     free, _total = device_memory(device)
-    return int(fraction * free)
+    return max(int(fraction * free) - nslots * CONTEXT_SPARE_BYTES, 1 << 30)
 
 
 class ShellServer:
@@ -164,8 +171,13 @@ class ShellServer:
     budget is admitted only when nothing else runs)."""
 
     def __init__(self, nslots=4, hbm_budget=None, device=0):
-        self.budget = int(default_budget(device) if hbm_budget is None else hbm_budget)
+        self.budget = int(default_budget(device, nslots=nslots) if hbm_budget is None else hbm_budget)
         self.contexts = [_lib.Context(device) for _ in range(nslots)]
+        # a served request assembles its model once: the two-kernel form needs
+        # only the pair plan, while the fused tiles' host plan (tens to
+        # hundreds of ms on large meshes) pays off over repeated assemblies
+        for c in self.contexts:
+            c.set_assembly(0)
         self.in_use = 0
         self.queue = collections.deque()
         self.cv = threading.Condition()

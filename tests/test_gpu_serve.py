@@ -87,3 +87,35 @@ def test_served_layered_request_matches_oracle(ctx):
     tm, rm = O.element_masses(4, m["xyz"], m["conn"], m0=md, m1=mi)
     Mo = O.lumped_diag(4, m["conn"], d.dofnums, nf, tm, rm)
     assert np.allclose(res["M"], Mo, rtol=1e-13, atol=0)
+
+
+def test_served_large_requests_alone_and_in_mix(ctx):
+    """Requests of 1e5 to 2.5e5 elements (layered T3 and Q4, assembly and
+    explicit dynamics in FS_PREC_MIXED_F32) among small ones on three slots:
+    each result bitwise the same as alone."""
+    from flexshell import _lib
+    print("[gpu] serve: large requests alone vs mix", flush=True)
+    specs = W.served_c5(nreq=6, seed=12, rate=1000.0, emin=1e3, emax=3e3, explicit_steps=60)
+    big = [dict(specs[0], rid=100, kind="assembly", nen=4, nel=2.5e5),
+           dict(specs[1], rid=101, kind="explicit", nen=3, nel=1.2e5, steps=60),
+           dict(specs[2], rid=102, kind="explicit", nen=4, nel=1.0e5, steps=60)]
+    specs = [big[0]] + specs[:2] + [big[1]] + specs[2:] + [big[2]]
+    for k, sp in enumerate(specs):
+        sp["arrival"] = 0.002 * k
+
+    def reqs():
+        out = []
+        for sp in specs:
+            prec = _lib.FS_PREC_MIXED_F32 if sp["kind"] == "explicit" else _lib.FS_PREC_FP64
+            out.append(S.Request(sp["rid"], sp["kind"], W.c5_model(sp), steps=sp["steps"], arrival=sp["arrival"],
+                                 seed=sp["rid"], precision=prec))
+        return out
+    alone = [S.execute(r, ctx) for r in reqs()]
+    assert max(a["elements"] for a in alone) >= 2.4e5
+    mixed = reqs()
+    with S.ShellServer(nslots=3, hbm_budget=64 << 30) as srv:
+        summary = S.replay(srv, mixed)
+        assert summary["failed"] == 0, [r.error for r in mixed if r.error]
+        assert srv.max_concurrent >= 2
+    for r, a in zip(mixed, alone):
+        assert _same(r.result, a), (r.rid, r.kind)
