@@ -28,6 +28,7 @@ using SparseArrays
 using SparseMatricesCSR
 using FinEtools
 using FinEtools.AssemblyModule: AbstractSysmatAssembler
+import FinEtools.AssemblyModule: startassembly!, assemble!, makematrix!
 using FinEtoolsFlexStructures.FEMMShellT3FFModule: FEMMShellT3FF
 using FinEtoolsFlexStructures.FEMMShellQ4RSModule: FEMMShellQ4RS
 using FinEtoolsFlexStructures.FEMMShellT3FFCompModule: FEMMShellT3FFComp
@@ -122,6 +123,14 @@ set_cd_lanes!(ctx::Context, lanes_per_node::Integer, positions_per_lane::Integer
     check(ctx, ccall((:fs_ctx_set_cd_lanes, libflexshell), Cint, (Ptr{Cvoid}, Cint, Cint), ctx.h,
                      lanes_per_node, positions_per_lane), "fs_ctx_set_cd_lanes")
 
+# Assembly form of fs_model_stiffness: 1 = fused element tiles writing CSR
+# values (default), 0 = element blocks + pair scatter (no tile plan to build:
+# suits models assembled once).  Both give the same pattern; the values
+# differ by rounding only (within 1e-13).
+set_assembly!(ctx::Context, mode::Integer) =
+    check(ctx, ccall((:fs_ctx_set_assembly, libflexshell), Cint, (Ptr{Cvoid}, Cint), ctx.h, mode),
+          "fs_ctx_set_assembly")
+
 const _default_ctx = Ref{Union{Nothing,Context}}(nothing)
 default_context() = something(_default_ctx[], (_default_ctx[] = Context(parse(Int, get(ENV, "FLEXSHELL_DEVICE", "0")))))
 
@@ -171,6 +180,98 @@ SysmatAssemblerHIPDiag(ctx::Context = default_context(); block::Symbol = :all) =
     SysmatAssemblerHIPDiag(ctx, block, IdDict{Any,DeviceModel}())
 
 _block(a) = a.block === :ff ? FS_BLOCK_FF : FS_BLOCK_ALL
+
+"""
+    SysmatAssemblerHIPGeneric(ctx = default_context())
+
+The assembler protocol of `SysmatAssemblerSparseCSRSymm`
+(`AssemblyModule.jl:28-52`) for any FEMM, whose element matrices stay on the
+host: `assemble!` buffers each element matrix with its dof numbers,
+`makematrix!` reduces the buffer on the device (`fs_coo_values`) into the
+`SparseMatrixCSR{1}` that `SysmatAssemblerSparseCSRSymm` returns: the
+pattern of `sparse(I, J, V)` with zeros retained, duplicates summed in
+`assemble!` order.  Dof numbers beyond the `startassembly!` sizes are
+dropped, so the sizes of the free-free block give the free-free matrix.
+The reduction plan is kept while the dof numbers of a new assembly repeat
+the previous ones.
+"""
+mutable struct SysmatAssemblerHIPGeneric <: AbstractSysmatAssembler
+    ctx::Context
+    sizes::NTuple{4,Int}              # kr, kc, nrows, ncols
+    n::Int                            # matrices buffered
+    mats::Vector{Float64}             # kr kc per matrix, column-major
+    dr::Vector{Int64}                 # kr per matrix
+    dc::Vector{Int64}                 # kc per matrix
+    h::Ptr{Cvoid}                     # fs_coo handle of the current plan
+    plan_key::Tuple{NTuple{4,Int},Int,Vector{Int64},Vector{Int64}}
+    rowptr::Vector{Int64}
+    colval::Vector{Int64}
+    function SysmatAssemblerHIPGeneric(ctx::Context = default_context())
+        a = new(ctx, (0, 0, 0, 0), 0, Float64[], Int64[], Int64[], C_NULL, ((0, 0, 0, 0), 0, Int64[], Int64[]),
+                Int64[], Int64[])
+        finalizer(_coo_destroy!, a)      # the handle holds its own reference to the context
+        return a
+    end
+end
+
+function _coo_destroy!(a::SysmatAssemblerHIPGeneric)
+    a.h == C_NULL || ccall((:fs_coo_destroy, libflexshell), Cvoid, (Ptr{Cvoid},), a.h)
+    a.h = C_NULL
+end
+
+function startassembly!(self::SysmatAssemblerHIPGeneric, elem_mat_nrows::Integer, elem_mat_ncols::Integer,
+                        n_elem_mats::Integer, row_nalldofs::Integer, col_nalldofs::Integer; force_init = false)
+    kr, kc = Int(elem_mat_nrows), Int(elem_mat_ncols)
+    self.sizes = (kr, kc, Int(row_nalldofs), Int(col_nalldofs))
+    self.n = 0
+    self.mats = Vector{Float64}(undef, kr * kc * n_elem_mats)
+    self.dr = Vector{Int64}(undef, kr * n_elem_mats)
+    self.dc = Vector{Int64}(undef, kc * n_elem_mats)
+    return self
+end
+
+function assemble!(self::SysmatAssemblerHIPGeneric, mat::AbstractMatrix, dofnums_row::AbstractVector,
+                   dofnums_col::AbstractVector)
+    kr, kc = self.sizes[1], self.sizes[2]
+    size(mat) == (kr, kc) || error("assemble!: element matrix $(size(mat)), expected ($kr, $kc)")
+    length(dofnums_row) == kr && length(dofnums_col) == kc || error("assemble!: dof numbers do not match the matrix")
+    k = self.n
+    (k + 1) * kr <= length(self.dr) || error("assemble!: more element matrices than startassembly! announced")
+    copyto!(self.mats, k * kr * kc + 1, vec(mat), 1, kr * kc)
+    copyto!(self.dr, k * kr + 1, dofnums_row, 1, kr)
+    copyto!(self.dc, k * kc + 1, dofnums_col, 1, kc)
+    self.n = k + 1
+    return self
+end
+
+function makematrix!(self::SysmatAssemblerHIPGeneric)
+    kr, kc, nrows, ncols = self.sizes
+    n = self.n
+    dr, dc = self.dr[1:(kr * n)], self.dc[1:(kc * n)]
+    key = (self.sizes, n, dr, dc)
+    ctx = self.ctx
+    if self.h == C_NULL || key != self.plan_key
+        _coo_destroy!(self)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ctx, ccall((:fs_coo_create, libflexshell), Cint,
+                         (Ptr{Cvoid}, Int64, Int64, Int32, Int32, Int64, Ptr{Int64}, Ptr{Int64}, Cint, Ptr{Ptr{Cvoid}}),
+                         ctx.h, nrows, ncols, kr, kc, n, dr, dc, 1, out), "fs_coo_create")
+        self.h, self.plan_key = out[], key
+        nz = Ref{Int64}(0)
+        check(ctx, ccall((:fs_coo_pattern, libflexshell), Cint, (Ptr{Cvoid}, Ptr{Int64}), self.h, nz),
+              "fs_coo_pattern")
+        self.rowptr = Vector{Int64}(undef, nrows + 1)
+        self.colval = Vector{Int64}(undef, nz[])
+        check(ctx, ccall((:fs_coo_pattern_get, libflexshell), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Cint),
+                         self.h, self.rowptr, self.colval, 1), "fs_coo_pattern_get")
+    end
+    nzval = Vector{Float64}(undef, length(self.colval))
+    GC.@preserve self begin
+        check(ctx, ccall((:fs_coo_values, libflexshell), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                         self.h, pointer(self.mats), nzval), "fs_coo_values")
+    end
+    return SparseMatrixCSR{1}(nrows, ncols, copy(self.rowptr), copy(self.colval), nzval)
+end
 
 # ---------------------------------------------------------------- model upload
 _nen(::Union{FEMMShellT3FF,FEMMShellT3FFComp}) = 3
@@ -576,7 +677,8 @@ function inspectintegpoints_hip(self, geom0::NodalField{Float64}, u::NodalField{
     return idat
 end
 
-export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, cd_loop!, Context, default_context,
-    inspectintegpoints_hip, numberdofs_hip!, distribloads_hip, set_pipeline!, set_cd_lanes!
+export SysmatAssemblerHIPCSR, SysmatAssemblerHIPDiag, SysmatAssemblerHIPGeneric, cd_loop!, Context,
+    default_context, inspectintegpoints_hip, numberdofs_hip!, distribloads_hip, set_pipeline!, set_cd_lanes!,
+    set_assembly!
 
 end # module
