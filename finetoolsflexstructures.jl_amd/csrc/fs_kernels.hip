@@ -868,7 +868,25 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         for (int c = 0; c < 6; ++c) acc[r][c] = 0.0;
     if (cact) {
         const double *cb = sC + le3 * T::CSTR;
-        for (int s = 0; s < T::NS; ++s) {
+        // Homogeneous shells with one quadrature point: rows 0..2 of C are
+        // the membrane rows, whose factor rows have exact zeros in the
+        // bending columns (Lmb is block diagonal), so cx = cy = 0 and their
+        // rotation columns are signed zeros.  Their products with those
+        // columns add signed zeros to +0 or to nonzero sums: leaving them out
+        // keeps every block bitwise the same.
+        const bool memb3 = NQP == 1 && !p.composite;
+        if (memb3) {
+            for (int s = 0; s < 3; ++s) {
+                double va[3], vb[3];
+                for (int r = 0; r < 3; ++r) {
+                    va[r] = cb[s * T::K + 6 * ba + r];
+                    vb[r] = cb[s * T::K + 6 * bb + r];
+                }
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) acc[r][c] += va[r] * vb[c];
+            }
+        }
+        for (int s = memb3 ? 3 : 0; s < T::NS; ++s) {
             if (split_sh && (s % NR) >= 6) continue;
             double va[6], vb[6];
             for (int r = 0; r < 6; ++r) {
@@ -1254,7 +1272,24 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         // ---------------- C: accumulate node-pair blocks ----------------
         if (cact) {
             const double *cbp = sC + le3 * CSTR;
-            for (int s = 0; s < NR; ++s) {
+            // Homogeneous shells: rows 0..2 of C are the membrane rows (with
+            // either row split), which have zero rotation columns (c_row_mb
+            // with lrow[3..5] = 0 gives cx = cy = 0, so no coupling term and
+            // cgr = 0).  Their products with the rotation columns are signed
+            // zeros added to +0 or to nonzero sums, so leaving them out keeps
+            // every block bitwise the same.
+            if (!COMP) {
+                for (int s = 0; s < 3; ++s) {
+                    double va[3], vb[3];
+                    for (int r = 0; r < 3; ++r) {
+                        va[r] = cbp[s * K + 6 * ba + r];
+                        vb[r] = cbp[s * K + 6 * bb + r];
+                    }
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) acc[r][c] += va[r] * vb[c];
+                }
+            }
+            for (int s = COMP ? 0 : 3; s < NR; ++s) {
                 double va[6], vb[6];
                 for (int r = 0; r < 6; ++r) {
                     va[r] = cbp[s * K + 6 * ba + r];
