@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <map>
@@ -22,6 +23,7 @@
 #include <new>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/flexshell.h"
@@ -277,6 +279,27 @@ struct MixedStore {
     }
 };
 
+// Device state of the explicit loop, kept with its step graph between
+// calls: a call whose launch parameters (sizes, scalars, load factors,
+// buffer addresses and the operator's records) equal those of the graph
+// uploads its initial state into the same buffers and replays the graph,
+// instead of capturing and instantiating its nsteps launches again.
+struct StepCache {
+    DevBuf dU[2], dV, dA, dF, dInv, dC, dSlot, dProbe, dRG, dGdt, dG;
+    std::vector<uint64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t replays = 0;  // launches of a graph taken from the cache
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key.clear();
+    }
+    ~StepCache() { drop_graph(); }
+};
+
 struct fs_model {
     fs_ctx *ctx = nullptr;
     int nen = 0;
@@ -292,12 +315,13 @@ struct fs_model {
     Block blk[2];
     // group data for the last stiffness/mass call
     DevBuf g_Labd, g_Lh, g_csys, g_t, g_group, g_inertia, m_group, g_homog, g_rho_t;
-    double timing[6] = {0, 0, 0, 0, 0, 0};
+    double timing[7] = {0, 0, 0, 0, 0, 0, 0};
     // parameters of the last element-stiffness launch (device pointers into
     // the group buffers above) and their generation
     fs::ElemParams last_ep{};
     uint64_t kgen = 0;
     MixedStore mixed;
+    StepCache step;
 };
 
 // Buffered element matrices of a FEMM with no element kernel here, reduced
@@ -420,6 +444,30 @@ int upload(fs_ctx *c, DevBuf &b, const T *src, size_t n) {
     if (s) return s;
     if (n) FS_HIP(c, hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
     return FS_OK;
+}
+
+// dalloc that keeps the buffer when it is already large enough (its
+// address then stays the same from call to call).
+int ensure(fs_ctx *c, DevBuf &b, size_t bytes) {
+    if (b.p && b.bytes >= std::max<size_t>(bytes, 8)) return FS_OK;
+    return dalloc(c, b, bytes);
+}
+
+template <class T>
+int put(fs_ctx *c, DevBuf &b, const T *src, size_t n) {
+    int s = ensure(c, b, n * sizeof(T));
+    if (s) return s;
+    if (n) FS_HIP(c, hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return FS_OK;
+}
+
+// Appends the bytes of a trivially copyable value to a cache key.
+template <class T>
+void key_add(std::vector<uint64_t> &k, const T &v) {
+    static_assert(std::is_trivially_copyable<T>::value, "key values are plain data");
+    uint64_t w[(sizeof(T) + 7) / 8] = {};
+    std::memcpy(w, &v, sizeof(T));
+    k.insert(k.end(), w, w + (sizeof(T) + 7) / 8);
 }
 
 constexpr size_t FS_STAGE_BYTES = size_t(32) << 20;
@@ -2147,21 +2195,31 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     }
     bool damped = false;
     for (int gi = 0; gi < ng; ++gi) damped |= (batch ? p->group_cdamp[gi] : p->cdamp) != 0.0;
-    DevBuf dU[2], dV, dA, dF, dInv, dC, dSlot, dProbe, dRG, dGdt, dG;
-    if ((s = upload(c, dU[0], U0v.data(), nv)) || (s = dalloc(c, dU[1], nv * 8)) || (s = upload(c, dV, V0v.data(), nv)) ||
-        (s = dalloc(c, dA, nv * 8)) || (s = upload(c, dF, Fv.data(), nv)) || (s = upload(c, dInv, invMC.data(), nv)) ||
-        (s = upload(c, dC, Cd.data(), nv)))
+    StepCache &sc = m->step;
+    DevBuf(&dU)[2] = sc.dU;
+    DevBuf &dV = sc.dV, &dA = sc.dA, &dF = sc.dF, &dInv = sc.dInv, &dC = sc.dC, &dSlot = sc.dSlot,
+           &dProbe = sc.dProbe, &dRG = sc.dRG, &dGdt = sc.dGdt, &dG = sc.dG;
+    if ((s = put(c, dU[0], U0v.data(), nv)) || (s = ensure(c, dU[1], nv * 8)) || (s = put(c, dV, V0v.data(), nv)) ||
+        (s = ensure(c, dA, nv * 8)) || (s = put(c, dF, Fv.data(), nv)) || (s = put(c, dInv, invMC.data(), nv)) ||
+        (s = put(c, dC, Cd.data(), nv))) {
+        sc.drop_graph();
         return s;
+    }
     if (batch) {
-        if ((s = upload(c, dRG, rgv.data(), nv)) || (s = upload(c, dGdt, p->group_dt, ng)) ||
-            (s = upload(c, dG, g, (size_t)(p->nsteps + 1) * ng)))
+        if ((s = put(c, dRG, rgv.data(), nv)) || (s = put(c, dGdt, p->group_dt, ng)) ||
+            (s = put(c, dG, g, (size_t)(p->nsteps + 1) * ng))) {
+            sc.drop_graph();
             return s;
+        }
     }
     const int64_t nrec = p->nsteps / stride + 1;
     if (nprobes > 0) {
         std::vector<int32_t> slot(nv, -1);
         for (int k = 0; k < nprobes; ++k) slot[pos[probe_dofs[k]]] = k;
-        if ((s = upload(c, dSlot, slot.data(), nv)) || (s = dalloc(c, dProbe, (size_t)nrec * nprobes * 8))) return s;
+        if ((s = put(c, dSlot, slot.data(), nv)) || (s = ensure(c, dProbe, (size_t)nrec * nprobes * 8))) {
+            sc.drop_graph();
+            return s;
+        }
         for (int k = 0; k < nprobes; ++k) probe_out[k] = U0[probe_dofs[k]];  // peek(0, U0)
     }
     // steps whose load factor is zero for every model skip the Fmag read (mixed path)
@@ -2211,24 +2269,53 @@ int fs_model_explicit(fs_model *m, const fs_cd_params *p, const double *U0, cons
     hipEvent_t e0, e1;
     FS_HIP(c, hipEventCreate(&e0));
     FS_HIP(c, hipEventCreate(&e1));
+    m->timing[6] = 0.0;
     if (p->use_graph) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        FS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t ce = enqueue(c->stream);
-        hipError_t ee = hipStreamEndCapture(c->stream, &graph);
-        if (ce != hipSuccess || ee != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return fail(c, FS_ERR_DEVICE, "explicit: graph capture failed: %s",
-                        hipGetErrorString(ce != hipSuccess ? ce : ee));
+        // everything the captured launches take as arguments
+        std::vector<uint64_t> key;
+        key.reserve(64 + (size_t)p->nsteps + 1);
+        for (const int64_t v : {nv, n, (int64_t)p->nsteps, (int64_t)p->precision, (int64_t)stride, (int64_t)nprobes,
+                                (int64_t)ng, (int64_t)batch, (int64_t)damped, (int64_t)c->cd_lanes, (int64_t)cd_pj,
+                                (int64_t)tpr, (int64_t)m->nen, (int64_t)m->mixed.nsr})
+            key_add(key, v);
+        key_add(key, p->dt);
+        for (const DevBuf *b : {&dU[0], &dU[1], &dV, &dA, &dF, &dInv, &dC, &dSlot, &dProbe, &dRG, &dGdt, &dG})
+            key_add(key, b->p);
+        key_add(key, B.kval.p);
+        key_add(key, B.slotrec.p);
+        if (mixed) key_add(key, op);
+        else if (c->cd_lanes > 0) key_add(key, noderows_of(B));
+        else key_add(key, runrows_of(B));
+        for (int64_t step = 0; step <= p->nsteps; ++step) {
+            key.push_back(forced[step]);
+            if (!batch) key_add(key, g[step]);
         }
-        FS_HIP(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        if (!sc.exec || key != sc.key) {
+            const auto tc0 = std::chrono::steady_clock::now();
+            sc.drop_graph();
+            FS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = enqueue(c->stream);
+            hipError_t ee = hipStreamEndCapture(c->stream, &sc.graph);
+            if (ce != hipSuccess || ee != hipSuccess) {
+                sc.drop_graph();
+                return fail(c, FS_ERR_DEVICE, "explicit: graph capture failed: %s",
+                            hipGetErrorString(ce != hipSuccess ? ce : ee));
+            }
+            hipError_t ie = hipGraphInstantiate(&sc.exec, sc.graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                sc.exec = nullptr;
+                sc.drop_graph();
+                return fail(c, FS_ERR_DEVICE, "explicit: graph instantiation failed: %s", hipGetErrorString(ie));
+            }
+            sc.key.swap(key);
+            m->timing[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0).count();
+        } else {
+            ++sc.replays;
+        }
         FS_HIP(c, hipEventRecord(e0, c->stream));
-        FS_HIP(c, hipGraphLaunch(exec, c->stream));
+        FS_HIP(c, hipGraphLaunch(sc.exec, c->stream));
         FS_HIP(c, hipEventRecord(e1, c->stream));
         FS_HIP(c, hipStreamSynchronize(c->stream));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
     } else {
         FS_HIP(c, hipEventRecord(e0, c->stream));
         FS_HIP(c, enqueue(c->stream));
@@ -2550,7 +2637,7 @@ int fs_model_mixed_info(const fs_model *m, int64_t *info) {
 }
 
 int fs_model_last_timing(const fs_model *m, int which, double *ms) {
-    if (!m || !ms || which < 0 || which > 5) return FS_ERR_ARG;
+    if (!m || !ms || which < 0 || which > 6) return FS_ERR_ARG;
     *ms = m->timing[which];
     return FS_OK;
 }

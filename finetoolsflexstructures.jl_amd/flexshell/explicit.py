@@ -47,9 +47,12 @@ def cd_loop(femm, cdamp, U0, V0, tend, dt, Fmag, g=None, probe_dofs=(), probe_st
 
 
 def run(model, cdamp, U0, V0, dt, nsteps, Fmag, gvals, probe_dofs=(), probe_stride=1,
-        precision=_lib.FS_PREC_FP64, dt_mass=None):
+        precision=_lib.FS_PREC_FP64, dt_mass=None, use_graph=False):
     """fs_model_explicit with an explicit step count and sampled g values;
-    invMC is formed with dt_mass (None: dt)."""
+    invMC is formed with dt_mass (None: dt).  use_graph: the steps run as one
+    captured graph, which the model keeps and replays on later calls with the
+    same sizes, dt and load factors (result key "graph_build_ms": host time
+    spent capturing, 0 on a replay)."""
     n = model.nfree
     U0 = np.ascontiguousarray(np.asarray(U0, dtype=np.float64))
     V0 = np.ascontiguousarray(np.asarray(V0, dtype=np.float64))
@@ -70,13 +73,14 @@ def run(model, cdamp, U0, V0, dt, nsteps, Fmag, gvals, probe_dofs=(), probe_stri
     p.precision = int(precision)
     p.probe_stride = stride
     p.dt_mass = float(dt if dt_mass is None else dt_mass)
+    p.use_graph = 1 if use_graph else 0
     ctx = model.ctx
     ctx.check(ctx.lib.fs_model_explicit(model.h, C.byref(p), _lib.ptr(U0), _lib.ptr(V0), _lib.ptr(Fmag),
                                         _lib.ptr(gvals), _lib.ptr(probes, C.c_int64) if len(probes) else None,
                                         int(len(probes)), _lib.ptr(pout) if len(probes) else None,
                                         _lib.ptr(U), _lib.ptr(V)), "fs_model_explicit")
     return {"U": U, "V": V, "probes": pout, "dt": dt, "nsteps": nsteps, "ms": model.timing(3),
-            "build_ms": model.timing(4)}
+            "build_ms": model.timing(4), "graph_build_ms": model.timing(6)}
 
 
 def omega_max(femm, v0, maxit=30):

@@ -401,7 +401,10 @@ void fs_coo_destroy(fs_coo *h);
  * (the whole scatter kernel when the pipeline has one range), 2 whole
  * stiffness (0 + 1), 3 explicit loop, 4 construction of the mixed-precision
  * operator (0 when the last loop ran in FP64 or reused an operator built
- * earlier), 5 the stress-resultant kernel of the last fs_model_resultants. */
+ * earlier), 5 the stress-resultant kernel of the last fs_model_resultants,
+ * 6 host time of the last fs_model_explicit with use_graph spent capturing
+ * and instantiating its step graph (0 when it replayed the model's cached
+ * graph: same sizes, scalars, load factors and buffers as the graph's). */
 int fs_model_last_timing(const fs_model *m, int which, double *ms);
 
 /* Element-kernel launch shape chosen by the autotuner for a family (FS_T3 /
