@@ -161,6 +161,115 @@ class ShellBatch:
         self.model.close()
 
 
+# Working set of one group's explicit loop (operator values and state) that
+# stays resident in the 256 MB Infinity Cache of an MI355X while the group's
+# steps run.  Measured on the 256-model T3 batch (500 steps) against one
+# group over all models, which streams the whole operator from HBM every
+# step: groups of about 225 MB gave 1.34x (FP64) and 1.35x
+# (FS_PREC_MIXED_F32) the DOF-steps/s, 180 MB 1.31x and 1.34x, 135 MB 1.23x
+# and 1.24x (more, smaller launches per step).
+CACHE_GROUP_BYTES = 224 << 20
+
+
+def step_bytes(model, value_bytes=8):
+    """Bytes one central-difference step moves for a model: its free-free
+    values at value_bytes each (f_I f_J per node pair and f_I^2 per node,
+    f_I the free dofs of node I: the structural union of the element dof
+    pairs) plus 72 B of state per free dof (U, V, A, invMC read; U, V, A
+    written)."""
+    conn = np.asarray(model["conn"])
+    nen = conn.shape[1]
+    a, b = np.triu_indices(nen, 1)
+    pairs = np.sort(np.stack([conn[:, a].ravel(), conn[:, b].ravel()], axis=1), axis=1).astype(np.int64)
+    key = np.unique(pairs[:, 0] * (1 << 32) + pairs[:, 1])
+    d = model["dchi"]
+    f = (np.asarray(d.dofnums) < d.nfreedofs).sum(axis=1).astype(np.int64)
+    nnz = int((f * f).sum() + 2 * (f[key >> 32] * f[key & 0xffffffff]).sum())
+    return int(nnz * value_bytes + 72 * d.nfreedofs)
+
+
+def cache_groups(models, group_bytes=CACHE_GROUP_BYTES, value_bytes=8):
+    """Consecutive groups of model indices whose step_bytes sum to at most
+    group_bytes (a larger model forms a group of its own)."""
+    out, cur, cb = [], [], 0
+    for k, m in enumerate(models):
+        sb = step_bytes(m, value_bytes)
+        if cur and cb + sb > group_bytes:
+            out.append(cur)
+            cur, cb = [], 0
+        cur.append(k)
+        cb += sb
+    if cur:
+        out.append(cur)
+    return out
+
+
+class ShellBatchGroups:
+    """A batch run as consecutive groups (cache_groups), each a ShellBatch:
+    the explicit loop runs every step of one group before the next group,
+    so that a group's operator is read from the Infinity Cache rather than
+    from HBM on all but its first step.  Vectors are the concatenation over
+    all models in model order, as for ShellBatch, and every model's results
+    are bitwise those of running it alone."""
+
+    def __init__(self, models, ctx=None, group_bytes=CACHE_GROUP_BYTES, value_bytes=8, threshold_angle=30.0):
+        if not models:
+            raise ValueError("empty batch")
+        self.ctx = ctx or _lib.default_context()
+        self.models = models
+        self.groups = cache_groups(models, group_bytes, value_bytes)
+        self.batches = [ShellBatch([models[k] for k in g], ctx=self.ctx, threshold_angle=threshold_angle)
+                        for g in self.groups]
+        nfree = np.array([m["dchi"].nfreedofs for m in models])
+        self.free_off = np.concatenate([[0], np.cumsum(nfree)])
+        self.nfree = int(nfree.sum())
+        self.group_off = np.concatenate([[0], np.cumsum([b.nfree for b in self.batches])])
+
+    def assemble_device(self):
+        for b in self.batches:
+            b.assemble_device()
+
+    def mass(self):
+        return np.concatenate([b.mass() for b in self.batches])
+
+    def model_slice(self, k):
+        return slice(int(self.free_off[k]), int(self.free_off[k + 1]))
+
+    def timing(self, which):
+        """Sum over the groups of fs_model_last_timing(which)."""
+        return float(sum(b.model.timing(which) for b in self.batches))
+
+    def explicit(self, dts, cdamps, U0, V0, Fmag, gvals, nsteps, use_graph=True, probe_dofs=(), probe_stride=1,
+                 precision=_lib.FS_PREC_FP64):
+        """ShellBatch.explicit over the groups in turn; "ms" is the summed
+        device time of the groups' step loops."""
+        dts = np.asarray(dts, dtype=np.float64)
+        cd = np.asarray(cdamps, dtype=np.float64)
+        gv = np.asarray(gvals, dtype=np.float64)
+        probes = np.asarray(probe_dofs, dtype=np.int64)
+        stride = max(1, int(probe_stride))
+        pout = np.zeros((nsteps // stride + 1, len(probes)))
+        U = np.zeros(self.nfree)
+        V = np.zeros(self.nfree)
+        ms = build = 0.0
+        for gi, (g, b) in enumerate(zip(self.groups, self.batches)):
+            lo, hi = int(self.group_off[gi]), int(self.group_off[gi + 1])
+            mine = np.nonzero((probes >= lo) & (probes < hi))[0]
+            r = b.explicit(dts[g], cd[g], U0[lo:hi], V0[lo:hi], Fmag[lo:hi], gv[:, g], nsteps, use_graph=use_graph,
+                           probe_dofs=probes[mine] - lo, probe_stride=stride, precision=precision)
+            U[lo:hi] = r["U"]
+            V[lo:hi] = r["V"]
+            if len(mine):
+                pout[:, mine] = r["probes"]
+            ms += r["ms"]
+            build += r["build_ms"]
+        return {"U": U, "V": V, "probes": pout, "ms": ms, "build_ms": build}
+
+    def close(self):
+        for b in self.batches:
+            b.close()
+
+
 def run_plain(models, dts, cdamps, U0s, V0s, Fmags, gvals, nsteps, ctx=None):
     """The plain path: one model per launch sequence (own device model,
     assembly, mass and step loop per model, no graph).  Returns the per-model

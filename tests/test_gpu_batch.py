@@ -51,6 +51,21 @@ def test_batch_equals_models_alone(ctx):
             sl = bt.model_slice(k)
             assert np.array_equal(r["U"][sl], plain[k]["U"]), (use_graph, k)
             assert np.array_equal(r["V"][sl], plain[k]["V"]), (use_graph, k)
+    # the same batch in Infinity-Cache groups (small groups force several),
+    # probes across group boundaries: bitwise the single batch
+    bg = B.ShellBatchGroups(models, ctx=ctx, group_bytes=int(1.2 * max(B.step_bytes(m) for m in models)))
+    assert len(bg.groups) >= 3
+    bg.assemble_device()
+    assert np.array_equal(bg.mass(), Mb)
+    probes = np.array([0, bt.free_off[4] - 1, bt.free_off[4], bt.nfree - 1])
+    for prec in (0, 2):
+        ra = bt.explicit(dts, cd, np.zeros(bt.nfree), np.concatenate(V0s), np.concatenate(Fm), gv, nsteps,
+                         probe_dofs=probes, probe_stride=3, precision=prec)
+        rb = bg.explicit(dts, cd, np.zeros(bt.nfree), np.concatenate(V0s), np.concatenate(Fm), gv, nsteps,
+                         probe_dofs=probes, probe_stride=3, precision=prec)
+        assert np.array_equal(ra["U"], rb["U"]) and np.array_equal(ra["V"], rb["V"]), prec
+        assert np.array_equal(ra["probes"], rb["probes"]), prec
+    bg.close()
     # one model against the oracle
     k = 3
     Uo, Vo, _ = O.cd_loop(Ks[k], Ms[k], cd[k], dts[k], nsteps, Fm[k], gv[:, k], np.zeros_like(V0s[k]), V0s[k])
