@@ -453,33 +453,61 @@ __device__ __forceinline__ void write_pair_row_rec(const int4 a, const int4 b, c
 // of tile t sums row r of the pair's blocks in the LDS stage (stage[(le NBLK
 // + blk) 36 + 6 r + c], a block stored with J's rows read transposed) in
 // element order, then writes the CSR values of a pair completed in the tile,
-// or the 6 values of its partial sum.
-template <int NBLK, int WG>
+// or the 6 values of its partial sum.  A thread takes TW items per pass and
+// loads all their records before the first sum, so that the record loads of
+// the items overlap (C2 cylinder, fused Q4 kernel: 0.756 ms with one item,
+// 0.727 ms with three; C4 plate, T3: 1.032 ms and 0.990 ms with four; the
+// values are bitwise the same).
+struct TileItem {
+    int4 a, b, o;
+    uint64_t cw;
+    int slot;
+};
+
+__device__ __forceinline__ void tile_item_load(const TilePlan &tp, int q, TileItem &it) {
+    it.cw = tp.tcontrib[q];
+    it.slot = tp.tslot[q];
+    it.a = tp.trec[3 * q];
+    it.b = tp.trec[3 * q + 1];
+    it.o = tp.trec[3 * q + 2];
+}
+
+template <int NBLK>
+__device__ __forceinline__ void tile_item_write(const TileItem &it, int r, const TilePlan &tp, const double *stage,
+                                                double *__restrict__ vals) {
+    double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int cnt = (int)(it.cw & 7);
+    for (int k = 0; k < cnt; ++k) {
+        const int v = (int)((it.cw >> (3 + 10 * k)) & 1023);
+        const double *B = stage + ((v & 31) * NBLK + ((v >> 5) & 15)) * 36;
+        if ((v >> 9) == 0) {
+            for (int c = 0; c < 6; ++c) sum[c] += B[r * 6 + c];
+        } else {
+            for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
+        }
+    }
+    if (it.slot >= 0) {
+        double *dst = tp.partial + (int64_t)it.slot * 36 + r * 6;
+        for (int c = 0; c < 6; ++c) dst[c] = sum[c];
+    } else {
+        write_pair_row_rec(it.a, it.b, it.o, r, sum, vals);
+    }
+}
+
+template <int NBLK, int WG, int TW>
 __device__ __forceinline__ void tile_write(int t, const TilePlan &tp, const double *stage,
                                            double *__restrict__ vals) {
     const int qb = tp.tp_ptr[t], qe = tp.tp_ptr[t + 1];
     const int n = (qe - qb) * 6;
-    for (int i = threadIdx.x; i < n; i += WG) {
-        const int q = qb + i / 6, r = i % 6;
-        const uint64_t cw = tp.tcontrib[q];
-        const int slot = tp.tslot[q];
-        const int4 a = tp.trec[3 * q], b = tp.trec[3 * q + 1], o = tp.trec[3 * q + 2];
-        double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        const int cnt = (int)(cw & 7);
-        for (int k = 0; k < cnt; ++k) {
-            const int v = (int)((cw >> (3 + 10 * k)) & 1023);
-            const double *B = stage + ((v & 31) * NBLK + ((v >> 5) & 15)) * 36;
-            if ((v >> 9) == 0) {
-                for (int c = 0; c < 6; ++c) sum[c] += B[r * 6 + c];
-            } else {
-                for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
-            }
+    for (int i0 = threadIdx.x; i0 < n; i0 += TW * WG) {
+        TileItem it[TW];
+        for (int w = 0; w < TW; ++w) {
+            const int i = i0 + w * WG;
+            if (i < n) tile_item_load(tp, qb + i / 6, it[w]);
         }
-        if (slot >= 0) {
-            double *dst = tp.partial + (int64_t)slot * 36 + r * 6;
-            for (int c = 0; c < 6; ++c) dst[c] = sum[c];
-        } else {
-            write_pair_row_rec(a, b, o, r, sum, vals);
+        for (int w = 0; w < TW; ++w) {
+            const int i = i0 + w * WG;
+            if (i < n) tile_item_write<NBLK>(it[w], i % 6, tp, stage, vals);
         }
     }
 }
@@ -982,7 +1010,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     }
     __syncthreads();
     if constexpr (FUSED) {
-        tile_write<T::NBLK, WG>(tblk, tp, sC, vals);
+        tile_write<T::NBLK, WG, 4>(tblk, tp, sC, vals);  // 4 items per pass: measured best of 1-4 on T3
     } else {
         const int total = ne * T::KSTAGE;  // contiguous in Kb
         double *dst = Kb + (size_t)e0 * T::KSTAGE;
@@ -1391,7 +1419,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     }
     __syncthreads();
     if constexpr (FUSED) {
-        tile_write<NBLK, WG>(tblk, tp, smem, vals);
+        tile_write<NBLK, WG, 3>(tblk, tp, smem, vals);  // 3 items per pass: measured best of 1-4 on Q4
     } else {
         const int total = ne * NBLK * 36;
         double *dst = Kb + (size_t)e0 * NBLK * 36;
