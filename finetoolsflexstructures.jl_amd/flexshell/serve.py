@@ -165,6 +165,18 @@ def default_budget(device=0, fraction=0.9, nslots=0):
     return max(int(fraction * free) - nslots * CONTEXT_SPARE_BYTES, 1 << 30)
 
 
+def request_context(device=0):
+    """A context set up as the server's slots are: a served request
+    assembles its model once, so it takes the two-kernel assembly form, which
+    needs only the pair plan, while the fused tiles' host plan (tens to
+    hundreds of ms on large meshes) pays off over repeated assemblies.  A
+    request run alone on such a context gives bitwise what it gives in a
+    server."""
+    c = _lib.Context(device)
+    c.set_assembly(0)
+    return c
+
+
 class ShellServer:
     """nslots worker threads, one HIP context each; FIFO admission by
     footprint against `hbm_budget` bytes (a request larger than the whole
@@ -172,12 +184,7 @@ class ShellServer:
 
     def __init__(self, nslots=4, hbm_budget=None, device=0):
         self.budget = int(default_budget(device, nslots=nslots) if hbm_budget is None else hbm_budget)
-        self.contexts = [_lib.Context(device) for _ in range(nslots)]
-        # a served request assembles its model once: the two-kernel form needs
-        # only the pair plan, while the fused tiles' host plan (tens to
-        # hundreds of ms on large meshes) pays off over repeated assemblies
-        for c in self.contexts:
-            c.set_assembly(0)
+        self.contexts = [request_context(device) for _ in range(nslots)]
         self.in_use = 0
         self.queue = collections.deque()
         self.cv = threading.Condition()

@@ -35,9 +35,11 @@ def _same(a, b):
 def test_served_results_identical_alone_and_in_mix(ctx):
     print("[gpu] serve: alone vs mix", flush=True)
     specs = W.served_c5(nreq=12, seed=6, rate=400.0, emin=1e3, emax=3e4, explicit_steps=40)
+    c1 = S.request_context()
     alone = []
     for r in _requests(specs):
-        alone.append(S.execute(r, ctx))
+        alone.append(S.execute(r, c1))
+    c1.close()
     mixed = _requests(specs)
     with S.ShellServer(nslots=4, hbm_budget=8 << 30) as srv:
         summary = S.replay(srv, mixed)
@@ -110,7 +112,9 @@ def test_served_large_requests_alone_and_in_mix(ctx):
             out.append(S.Request(sp["rid"], sp["kind"], W.c5_model(sp), steps=sp["steps"], arrival=sp["arrival"],
                                  seed=sp["rid"], precision=prec))
         return out
-    alone = [S.execute(r, ctx) for r in reqs()]
+    c1 = S.request_context()
+    alone = [S.execute(r, c1) for r in reqs()]
+    c1.close()
     assert max(a["elements"] for a in alone) >= 2.4e5
     mixed = reqs()
     with S.ShellServer(nslots=3, hbm_budget=64 << 30) as srv:
