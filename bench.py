@@ -280,6 +280,19 @@ def bench_batch(nmodels, nsteps):
             out[f"{name}_over_batched"] = rate / out["batched_dof_steps_per_s"]
             out[f"{name}_max_rel_err_final_U"] = max(errs)
         bg.close()
+    # wall clock, two contexts: groups of half the size dealt to two streams,
+    # so that one group's host work and launch gaps overlap another's steps
+    for name, prec, vb in (("batched", _lib.FS_PREC_FP64, 8), ("batched_mixed_f32", _lib.FS_PREC_MIXED_F32, 6)):
+        bg = B.ShellBatchGroups(models, ctx=ctx, group_bytes=B.CACHE_GROUP_BYTES // 2, value_bytes=vb, streams=2)
+        bg.assemble_device()
+        bg.mass()
+        bg.explicit(dts, cd0, np.zeros(nfree), V0, np.zeros(nfree), g, nsteps, precision=prec)  # operators, graphs
+        rm = bg.explicit(dts, cd0, np.zeros(nfree), V0, np.zeros(nfree), g, nsteps, precision=prec)
+        rate = nfree * nsteps / (rm["wall_ms"] / 1e3)
+        out.update({f"{name}_2streams_wall_dof_steps_per_s": rate, f"{name}_2streams_wall_over_plain": rate / plain})
+        if prec == _lib.FS_PREC_FP64:
+            out["bitwise_equal_2streams"] = bool(np.array_equal(rm["U"], r1["U"]))
+        bg.close()
     return {**out,
             "plain_elements_per_s": E / (plain_asm_ms / 1e3), "batched_elements_per_s": E / (batch_asm_ms / 1e3),
             "note": "plain = one model per launch sequence (FP64, no graph); one_group = all models per launch, "
@@ -289,7 +302,9 @@ def bench_batch(nmodels, nsteps):
                     "256 MB Infinity Cache; batched_mixed = membrane+bending stiffness stored as FP32 + int16 "
                     "residual (FP64 products), FP64 shear factor rows, drilling, mass, state; batched_mixed_f32 = "
                     "the shear in the slices too; error = max over models of max|dU| / max|U_fp64| after the "
-                    "steps (Gaussian initial velocities)"}
+                    "steps (Gaussian initial velocities); *_2streams_wall = groups of half the size on two contexts "
+                    "run concurrently, rate by host wall clock of the whole step loop (replay of the captured "
+                    "graphs)"}
 
 
 def mixed_step_bytes(N, E, nen, kmb_pairs, nsr, nnb, fmag_frac):

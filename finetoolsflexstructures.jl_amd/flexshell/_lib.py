@@ -177,6 +177,7 @@ class Context:
                                  "(a HIP device is required)")
         self.lib = lib
         self.h = h
+        self.device = int(device)
 
     def check(self, st, what):
         if st != FS_OK:

@@ -212,14 +212,22 @@ class ShellBatchGroups:
     all models in model order, as for ShellBatch, and every model's results
     are bitwise those of running it alone."""
 
-    def __init__(self, models, ctx=None, group_bytes=CACHE_GROUP_BYTES, value_bytes=8, threshold_angle=30.0):
+    def __init__(self, models, ctx=None, group_bytes=CACHE_GROUP_BYTES, value_bytes=8, threshold_angle=30.0,
+                 streams=1):
+        """streams > 1: the groups are dealt round-robin to that many
+        contexts (streams) of ctx's device, and explicit() runs the contexts
+        concurrently, one host thread each, so that one group's launches
+        fill the gaps of another's."""
         if not models:
             raise ValueError("empty batch")
+        if streams < 1:
+            raise ValueError("streams must be at least 1")
         self.ctx = ctx or _lib.default_context()
+        self.ctxs = [self.ctx] + [_lib.Context(self.ctx.device) for _ in range(streams - 1)]
         self.models = models
         self.groups = cache_groups(models, group_bytes, value_bytes)
-        self.batches = [ShellBatch([models[k] for k in g], ctx=self.ctx, threshold_angle=threshold_angle)
-                        for g in self.groups]
+        self.batches = [ShellBatch([models[k] for k in g], ctx=self.ctxs[i % streams], threshold_angle=threshold_angle)
+                        for i, g in enumerate(self.groups)]
         nfree = np.array([m["dchi"].nfreedofs for m in models])
         self.free_off = np.concatenate([[0], np.cumsum(nfree)])
         self.nfree = int(nfree.sum())
@@ -241,8 +249,11 @@ class ShellBatchGroups:
 
     def explicit(self, dts, cdamps, U0, V0, Fmag, gvals, nsteps, use_graph=True, probe_dofs=(), probe_stride=1,
                  precision=_lib.FS_PREC_FP64):
-        """ShellBatch.explicit over the groups in turn; "ms" is the summed
-        device time of the groups' step loops."""
+        """ShellBatch.explicit over the groups (in turn on each context);
+        "ms" is the summed device time of the groups' step loops, "wall_ms"
+        the host time of the whole call."""
+        import threading
+        import time
         dts = np.asarray(dts, dtype=np.float64)
         cd = np.asarray(cdamps, dtype=np.float64)
         gv = np.asarray(gvals, dtype=np.float64)
@@ -251,23 +262,48 @@ class ShellBatchGroups:
         pout = np.zeros((nsteps // stride + 1, len(probes)))
         U = np.zeros(self.nfree)
         V = np.zeros(self.nfree)
+        res = [None] * len(self.groups)
+        errs = []
+
+        def run(s):
+            try:
+                for gi in range(s, len(self.groups), len(self.ctxs)):
+                    g, b = self.groups[gi], self.batches[gi]
+                    lo, hi = int(self.group_off[gi]), int(self.group_off[gi + 1])
+                    mine = np.nonzero((probes >= lo) & (probes < hi))[0]
+                    res[gi] = (mine, b.explicit(dts[g], cd[g], U0[lo:hi], V0[lo:hi], Fmag[lo:hi], gv[:, g], nsteps,
+                                                use_graph=use_graph, probe_dofs=probes[mine] - lo,
+                                                probe_stride=stride, precision=precision))
+            except Exception as e:  # re-raised in the caller's thread
+                errs.append(e)
+        t0 = time.perf_counter()
+        if len(self.ctxs) == 1:
+            run(0)
+        else:
+            th = [threading.Thread(target=run, args=(s,)) for s in range(len(self.ctxs))]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+        wall = (time.perf_counter() - t0) * 1e3
+        if errs:
+            raise errs[0]
         ms = build = 0.0
-        for gi, (g, b) in enumerate(zip(self.groups, self.batches)):
+        for gi, (mine, r) in enumerate(res):
             lo, hi = int(self.group_off[gi]), int(self.group_off[gi + 1])
-            mine = np.nonzero((probes >= lo) & (probes < hi))[0]
-            r = b.explicit(dts[g], cd[g], U0[lo:hi], V0[lo:hi], Fmag[lo:hi], gv[:, g], nsteps, use_graph=use_graph,
-                           probe_dofs=probes[mine] - lo, probe_stride=stride, precision=precision)
             U[lo:hi] = r["U"]
             V[lo:hi] = r["V"]
             if len(mine):
                 pout[:, mine] = r["probes"]
             ms += r["ms"]
             build += r["build_ms"]
-        return {"U": U, "V": V, "probes": pout, "ms": ms, "build_ms": build}
+        return {"U": U, "V": V, "probes": pout, "ms": ms, "build_ms": build, "wall_ms": wall}
 
     def close(self):
         for b in self.batches:
             b.close()
+        for c in self.ctxs[1:]:
+            c.close()
 
 
 def run_plain(models, dts, cdamps, U0s, V0s, Fmags, gvals, nsteps, ctx=None):

@@ -58,14 +58,24 @@ def test_batch_equals_models_alone(ctx):
     bg.assemble_device()
     assert np.array_equal(bg.mass(), Mb)
     probes = np.array([0, bt.free_off[4] - 1, bt.free_off[4], bt.nfree - 1])
+    ra0 = None
     for prec in (0, 2):
         ra = bt.explicit(dts, cd, np.zeros(bt.nfree), np.concatenate(V0s), np.concatenate(Fm), gv, nsteps,
                          probe_dofs=probes, probe_stride=3, precision=prec)
         rb = bg.explicit(dts, cd, np.zeros(bt.nfree), np.concatenate(V0s), np.concatenate(Fm), gv, nsteps,
                          probe_dofs=probes, probe_stride=3, precision=prec)
         assert np.array_equal(ra["U"], rb["U"]) and np.array_equal(ra["V"], rb["V"]), prec
+        ra0 = ra if ra0 is None else ra0
         assert np.array_equal(ra["probes"], rb["probes"]), prec
     bg.close()
+    # the groups dealt to two contexts run concurrently: the same results
+    bg2 = B.ShellBatchGroups(models, ctx=ctx, group_bytes=int(1.2 * max(B.step_bytes(m) for m in models)), streams=2)
+    bg2.assemble_device()
+    bg2.mass()
+    rc = bg2.explicit(dts, cd, np.zeros(bt.nfree), np.concatenate(V0s), np.concatenate(Fm), gv, nsteps,
+                      probe_dofs=probes, probe_stride=3)
+    assert np.array_equal(rc["U"], ra0["U"]) and np.array_equal(rc["probes"], ra0["probes"])
+    bg2.close()
     # one model against the oracle
     k = 3
     Uo, Vo, _ = O.cd_loop(Ks[k], Ms[k], cd[k], dts[k], nsteps, Fm[k], gv[:, k], np.zeros_like(V0s[k]), V0s[k])
