@@ -96,6 +96,22 @@ def gaussian_velocity(dchi, sigma, seed):
     return V
 
 
+def converged_dt(femm, nfree, seed=0, maxit=300, factor=0.9):
+    """Stable time step of a workload: factor * 2 / omega_max with omega_max
+    from the device power iteration (fs_model_omega_max) started from a
+    seeded Gaussian vector and run for maxit iterations.  The reference's
+    start (rand(n), all positive, 30 iterations,
+    clamp_cyl_transient_examples.jl:45-60) has little weight in the
+    oscillatory top modes: on one model of the 256-model batch it
+    underestimated omega_max by 11 %, so that 0.9 of its dt was unstable
+    and the trajectory grew to 1e62 in 500 steps.  A Gaussian start reached
+    omega_max within 1e-4 in 300 iterations on every batch model, and within
+    0.4 % (0.06 %) on the 512 x 512 cylinder (1024 x 512 plate)."""
+    from . import explicit
+    v0 = np.random.default_rng(seed).standard_normal(nfree)
+    return explicit.stable_dt(explicit.omega_max(femm, v0, maxit=maxit), factor=factor)
+
+
 def batch_c3(nmodels=256, seed=4, nmin=8, nmax=128):
     """256 independent T3 plate models: n per side U{nmin..nmax} (2 n^2
     triangles), aspect ratio U(1, 4), thickness log-U(0.002, 0.05), E

@@ -114,7 +114,7 @@ def test_batch_c3_full_size(ctx):
         else:
             f.stiffness(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), m["xyz"], None, None, d)
             f.mass(FM.SysmatAssemblerHIP(_lib.FS_BLOCK_FF), m["xyz"], d)
-        dts.append(explicit.stable_dt(explicit.omega_max(f, np.random.default_rng(k).random(nf), maxit=30)))
+        dts.append(W.converged_dt(f, nf, seed=k))
         if k in sample:
             z = np.zeros(nf)
             runs[k] = explicit.run(f._model, 0.0, z, V0s[k], dts[k], nsteps, z, g[:, k])
@@ -137,6 +137,14 @@ def test_batch_c3_full_size(ctx):
         sl = bt.model_slice(k)
         assert np.array_equal(r["U"][sl], runs[k]["U"]), k
         assert np.array_equal(r["V"][sl], runs[k]["V"]), k
+    # the dt of every model is stable: no trajectory grows.  Free vibration
+    # from velocities of sigma 1e-3 m/s moves a model by about sigma times
+    # the elapsed time at most (100 x that is allowed); an unstable model
+    # grows without bound (1e62 in 500 steps for the one whose dt came from
+    # an unconverged omega_max)
+    for k in range(nm):
+        uk = float(np.abs(r["U"][bt.model_slice(k)]).max())
+        assert np.isfinite(uk) and uk <= 100 * 1e-3 * nsteps * dts[k], (k, uk, dts[k])
     # both mixed precisions on the same inputs (the figures the batch
     # throughput is quoted with): final-state error of every model against
     # its FP64 trajectory, max |dU| / max |U|, within 1e-6
