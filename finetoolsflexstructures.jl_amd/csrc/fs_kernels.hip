@@ -289,7 +289,7 @@ __device__ __forceinline__ void c_row_q4_split(const MatRows &mr, int i, int h, 
 }
 
 // ---------------------------------------------------------------------------
-// Element kernel
+// T3 element kernel
 // ---------------------------------------------------------------------------
 
 template <int NEN, int NQP, int NR>
@@ -308,12 +308,10 @@ struct ElemTraits {
     static constexpr int GEO = G_BS + NEN * 3 * NSH;
     // node record per (element, qp, node): A rows 0..1 and coupling vector d
     static constexpr int NODE = 6 + NR;
-    // per element: drilling vectors, kavg parts, Q4 rotation blocks, kavg
+    // per element: drilling vectors and the parts of the drilling average
     static constexpr int M_G = 0;                      // NEN*3
     static constexpr int M_KP = NEN * 3;               // NEN*2
-    static constexpr int M_KRR = M_KP + NEN * 2;       // NEN*9
-    static constexpr int M_KAVG = M_KRR + NEN * 9;     // 1
-    static constexpr int MISC = M_KAVG + 2;
+    static constexpr int MISC = M_KP + NEN * 2;
     static constexpr int KSTAGE = NBLK * 36;           // staged output per element
 };
 
@@ -541,7 +539,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     const int ne = FUSED ? NE : ((p.eend - e0) < NE ? (p.eend - e0) : NE);
     auto elem = [&](int le) -> int { return FUSED ? p.tile_elem[e0 + le] : e0 + le; };
 
-    // ---------------- G1: geometry per (element, qp) ----------------
+    // ---------------- G1: geometry per element ----------------
+    static_assert(NEN == 3 && NQP == 1, "elem_kernel is the T3 kernel");
     if (tid < NE * NQP) {
         const int le = tid / NQP, q = tid % NQP;
         if (le < ne) {
@@ -560,7 +559,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             const double *hrec = p.composite ? nullptr : p.g_homog + 16 * grp;
             const double tt = p.composite ? p.g_t[grp] : hrec[13];
             const double stab_c = p.composite ? p.stab_c : hrec[14];
-            if constexpr (NEN == 3) {
+            {
                 // FEMMShellT3FFModule.jl:269-321,465-484
                 double J0[3][2];
                 for (int j = 0; j < 3; ++j) {
@@ -636,114 +635,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 for (int j = 0; j < 3; ++j)
                     for (int cc = 0; cc < 3; ++cc)
                         for (int r = 0; r < T::NSH; ++r) g[T::G_BS + (j * 3 + cc) * T::NSH + r] = bs[j][cc][r];
-            } else {
-                // Q4RS at Gauss point q (FEMMShellQ4RSModule.jl:919-936)
-                const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
-                const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
-                double gNp[4][2];
-                q4_gradn(xi, eta, gNp);
-                double J[3][2];
-                for (int i = 0; i < 3; ++i)
-                    for (int k = 0; k < 2; ++k) {
-                        double acc = 0.0;
-                        for (int a = 0; a < 4; ++a) acc += X[a][i] * gNp[a][k];
-                        J[i][k] = acc;
-                    }
-                double cr0 = -J[2][0] * J[1][1] + J[1][0] * J[2][1];
-                double cr1 = J[2][0] * J[0][1] - J[0][0] * J[2][1];
-                double cr2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
-                const double Jac = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
-                e_g(EG, J);
-                double cen[3] = {0.0, 0.0, 0.0};
-                for (int a = 0; a < 4; ++a)
-                    for (int k = 0; k < 3; ++k) cen[k] += X[a][k];
-                for (int k = 0; k < 3; ++k) cen[k] /= 4;
-                double ec[4][2];
-                for (int a = 0; a < 4; ++a)
-                    for (int k = 0; k < 2; ++k) {
-                        double dd = 0.0;
-                        for (int m = 0; m < 3; ++m) dd += (X[a][m] - cen[m]) * EG[m][k];
-                        ec[a][k] = dd;
-                    }
-                double G00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
-                double G01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
-                double G11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
-                double detG = G00 * G11 - G01 * G01;
-                if (detG == 0.0) status[0] = 1;
-                double i00 = G11 / detG, i01 = -G01 / detG, i11 = G00 / detG;
-                for (int a = 0; a < 4; ++a) {
-                    double t0 = i00 * gNp[a][0] + i01 * gNp[a][1];
-                    double t1 = i01 * gNp[a][0] + i11 * gNp[a][1];
-                    double gg[3];
-                    for (int i = 0; i < 3; ++i) gg[i] = J[i][0] * t0 + J[i][1] * t1;
-                    gN[a][0] = EG[0][0] * gg[0] + EG[1][0] * gg[1] + EG[2][0] * gg[2];
-                    gN[a][1] = EG[0][1] * gg[0] + EG[1][1] * gg[1] + EG[2][1] * gg[2];
-                }
-                // _quad_diameter (:861-870)
-                double maxd = 0.0;
-                for (int i = 1; i < 4; ++i) {
-                    double dx = X[i][0] - X[0][0], dy = X[i][1] - X[0][1], dz = X[i][2] - X[0][2];
-                    double dist = (dx * dx + dy * dy + dz * dz);
-                    maxd = dist > maxd ? dist : maxd;
-                }
-                const double h = sqrt(maxd);
-                const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
-                if (!p.composite) {
-                    homog_factors(hrec, hrec + 9, sqrt(tt * Jac), sqrt((tt * tt * tt / 12.0) * Jac),
-                                  sqrt(tt * stab * Jac), Lmb, Ls);
-                } else {
-                    double m, n;
-                    layup_angle(EG, p.g_csys + 9 * grp, m, n);
-                    composite_factors(p.g_Labd + 36 * grp, p.g_Lh + 4 * grp, m, n, sqrt(Jac), sqrt(stab * Jac),
-                                      Lmb, Ls);
-                }
-                // MITC transverse shear (_Bsmat, :753-804)
-                const double X1 = ec[0][0], Y1 = ec[0][1], X2 = ec[1][0], Y2 = ec[1][1];
-                const double X3 = ec[2][0], Y3 = ec[2][1], X4 = ec[3][0], Y4 = ec[3][1];
-                const double r = xi, s = eta;
-                const double J11 = (X1 * (s - 1) / 4 - X2 * (s - 1) / 4 + X3 * (s + 1) / 4 - X4 * (s + 1) / 4);
-                const double J21 = (Y1 * (s - 1) / 4 - Y2 * (s - 1) / 4 + Y3 * (s + 1) / 4 - Y4 * (s + 1) / 4);
-                const double J12 = (X1 * (r - 1) / 4 - X2 * (r + 1) / 4 + X3 * (r + 1) / 4 - X4 * (r - 1) / 4);
-                const double J22 = (Y1 * (r - 1) / 4 - Y2 * (r + 1) / 4 + Y3 * (r + 1) / 4 - Y4 * (r - 1) / 4);
-                const double A = sqrt(J11 * J11 + J21 * J21);
-                const double B = sqrt(J12 * J12 + J22 * J22);
-                const double ca = J11 / A, sa = J21 / A, cb = J12 / B, sb = J22 / B;
-                const double detJ = J11 * J22 - J12 * J21;
-                const double Ax = X1 - X2 - X3 + X4, Ay = Y1 - Y2 - Y3 + Y4;
-                const double Bx = X1 - X2 + X3 - X4, By = Y1 - Y2 + Y3 - Y4;
-                const double Cx = X1 + X2 - X3 - X4, Cy = Y1 + Y2 - Y3 - Y4;
-                const double P = sqrt((Ax + Bx * s) * (Ax + Bx * s) + (Ay + By * s) * (Ay + By * s));
-                const double Q = sqrt((Bx * r + Cx) * (Bx * r + Cx) + (By * r + Cy) * (By * r + Cy));
-                const double i16 = 1.0 / (16 * detJ), i32 = 1.0 / (32 * detJ);
-                double *bs = g + T::G_BS;  // [node][comp][row]
-                // node 1
-                bs[0 * 6 + 0 * 2 + 0] = (sa * (r + 1) * P - sb * (s + 1) * Q) * i16;
-                bs[0 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
-                bs[0 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
-                bs[0 * 6 + 0 * 2 + 1] = (-ca * (r + 1) * P + cb * (s + 1) * Q) * i16;
-                bs[0 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
-                bs[0 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
-                // node 2
-                bs[1 * 6 + 0 * 2 + 0] = (-sa * (r - 1) * P + sb * (s + 1) * Q) * i16;
-                bs[1 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y1 - Y2) * (s + 1) * Q) * i32;
-                bs[1 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X1 - X2) * (s + 1) * Q) * i32;
-                bs[1 * 6 + 0 * 2 + 1] = (ca * (r - 1) * P - cb * (s + 1) * Q) * i16;
-                bs[1 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y1 - Y2) * (s + 1) * Q) * i32;
-                bs[1 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X1 - X2) * (s + 1) * Q) * i32;
-                // node 3
-                bs[2 * 6 + 0 * 2 + 0] = (sa * (r - 1) * P - sb * (s - 1) * Q) * i16;
-                bs[2 * 6 + 1 * 2 + 0] = (sa * (Y2 - Y3) * (r - 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
-                bs[2 * 6 + 2 * 2 + 0] = (-sa * (X2 - X3) * (r - 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
-                bs[2 * 6 + 0 * 2 + 1] = (-ca * (r - 1) * P + cb * (s - 1) * Q) * i16;
-                bs[2 * 6 + 1 * 2 + 1] = (-ca * (Y2 - Y3) * (r - 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
-                bs[2 * 6 + 2 * 2 + 1] = (ca * (X2 - X3) * (r - 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
-                // node 4
-                bs[3 * 6 + 0 * 2 + 0] = (-sa * (r + 1) * P + sb * (s - 1) * Q) * i16;
-                bs[3 * 6 + 1 * 2 + 0] = (-sa * (Y1 - Y4) * (r + 1) * P + sb * (Y3 - Y4) * (s - 1) * Q) * i32;
-                bs[3 * 6 + 2 * 2 + 0] = (sa * (X1 - X4) * (r + 1) * P - sb * (X3 - X4) * (s - 1) * Q) * i32;
-                bs[3 * 6 + 0 * 2 + 1] = (ca * (r + 1) * P - cb * (s - 1) * Q) * i16;
-                bs[3 * 6 + 1 * 2 + 1] = (ca * (Y1 - Y4) * (r + 1) * P - cb * (Y3 - Y4) * (s - 1) * Q) * i32;
-                bs[3 * 6 + 2 * 2 + 1] = (-ca * (X1 - X4) * (r + 1) * P + cb * (X3 - X4) * (s - 1) * Q) * i32;
             }
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j) g[T::G_EG + i * 3 + j] = EG[i][j];
@@ -943,54 +834,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                     for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (gv[r] * gv[c]);
             }
         }
-        if (NEN == 4 && ba == bb) {
-            double *ms = sM + le3 * T::MISC + T::M_KRR + 9 * ba;
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) ms[r * 3 + c] = acc[3 + r][3 + c];
-        }
-    }
-    if constexpr (NEN == 4) {
-        // Q4 drilling (_drilling_penalty_kavg, _add_drilling_stiffness!, :807-859)
-        __syncthreads();
-        const int grpd = cact ? (p.elem_group ? p.elem_group[elem(le3)] : 0) : 0;
-        const double dscale = p.composite ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
-        if (cact && ba == bb && dscale != 0.0) {
-            const int e = elem(le3);
-            const double *kr = sM + le3 * T::MISC + T::M_KRR;
-            double sum = 0.0;
-            int cnt = 0;
-            for (int k = 0; k < 4; ++k) {
-                const int node = conn[k * E + e];
-                if (!valid[node]) continue;
-                const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
-                const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-                if (nn == 0.0) continue;
-                const double u[3] = {n0 / nn, n1 / nn, n2 / nn};
-                // tr(P K P) with P = I - u u'
-                double Pm[3][3], tmp[3][3];
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) Pm[a][b] = ((a == b) ? 1.0 : 0.0) - u[a] * u[b];
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b)
-                        tmp[a][b] = Pm[a][0] * kr[9 * k + 0 * 3 + b] + Pm[a][1] * kr[9 * k + 1 * 3 + b] +
-                                    Pm[a][2] * kr[9 * k + 2 * 3 + b];
-                double tr = 0.0;
-                for (int a = 0; a < 3; ++a) tr += tmp[a][0] * Pm[0][a] + tmp[a][1] * Pm[1][a] + tmp[a][2] * Pm[2][a];
-                tr = tr / 2.0;
-                sum += tr > 0.0 ? tr : 0.0;
-                ++cnt;
-            }
-            const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * dscale;
-            const int node = conn[ba * E + e];
-            if (kavg != 0.0 && valid[node]) {
-                const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
-                if (sqrt(n0 * n0 + n1 * n1 + n2 * n2) != 0.0) {
-                    const double nv[3] = {n0, n1, n2};
-                    for (int r = 0; r < 3; ++r)
-                        for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (nv[r] * nv[c]);
-                }
-            }
-        }
     }
     if (split_sh) {
         // shear rows of C (global basis) of every element of the block
@@ -1022,11 +865,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
 // Q4 element kernel, Gauss points processed in sequence
 // ---------------------------------------------------------------------------
 //
-// Same mathematics as elem_kernel<4, 4, 8>; the LDS holds the C matrix of one
-// Gauss point at a time (8 x 24) and the K blocks accumulate in registers of
-// the (element, node pair) threads across the four Gauss points, so an
-// element needs ~3.7 KB of LDS instead of ~11 KB and three times as many
-// elements are resident per CU.  Phase G1 (thread per element and Gauss
+// The phases of the T3 kernel (G1 geometry, G2/G3 rows of C, C blocks),
+// with the LDS holding the C matrix of one Gauss point at a time (8 x 24):
+// the K blocks accumulate in registers of the (element, node pair) threads
+// across the four Gauss points, so an element needs ~3.7 KB of LDS where all
+// four points of C would take ~11 KB.  Phase G1 (per element and Gauss
 // point) stores a compact geometry record: E_G, gradN_e, the material scale
 // factors (and the layup angle for layered shells) and the MITC shear row
 // entries; the material factors are rebuilt per node thread.
@@ -1073,8 +916,16 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     auto elem = [&](int le) -> int { return FUSED ? p.tile_elem[e0 + le] : e0 + le; };
 
     // ---------------- G1: geometry per (element, Gauss point) ----------------
-    if (tid < NE * NQP) {
-        const int le = tid / NQP, q = tid % NQP;
+    // With two or more waves, wave 0 computes the frame, gradients and
+    // material scales of every (element, Gauss point) of the tile and wave 1
+    // its MITC shear entries (both from the same tangents and basis); the
+    // branches are uniform per wave.  A single wave does both.
+    constexpr int NW = WG / 64;
+    constexpr bool G1_TWO = NW >= 2 && NE * NQP <= 64;
+    const int g1_roles = G1_TWO ? ((tid >> 6) == 0 ? 1 : (tid >> 6) == 1 ? 2 : 0) : 3;
+    const int g1_rec = G1_TWO ? (tid & 63) : tid;
+    if (g1_roles != 0 && g1_rec < NE * NQP) {
+        const int le = g1_rec / NQP, q = g1_rec % NQP;
         if (le < ne) {
             const int e = elem(le);
             double X[4][3];
@@ -1084,10 +935,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 X[a][1] = ys[n];
                 X[a][2] = zs[n];
             }
-            const int grp = p.elem_group ? p.elem_group[e] : 0;
-            const double *hrec = COMP ? nullptr : p.g_homog + 16 * grp;
-            const double tt = COMP ? p.g_t[grp] : hrec[13];
-            const double stab_c = COMP ? p.stab_c : hrec[14];
             double *g = sG + (le * NQP + q) * GEO;
             const double xi = (q / 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
             const double eta = (q % 2 == 0) ? -FS_GAUSS2 : FS_GAUSS2;
@@ -1100,62 +947,70 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                     for (int a = 0; a < 4; ++a) acc += X[a][i] * gNp[a][k];
                     J[i][k] = acc;
                 }
-            const double cr0 = -J[2][0] * J[1][1] + J[1][0] * J[2][1];
-            const double cr1 = J[2][0] * J[0][1] - J[0][0] * J[2][1];
-            const double cr2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
-            const double Jac = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
             double EG[3][3];
             e_g(EG, J);
-            double cen[3] = {0.0, 0.0, 0.0};
-            for (int a = 0; a < 4; ++a)
-                for (int k = 0; k < 3; ++k) cen[k] += X[a][k];
-            for (int k = 0; k < 3; ++k) cen[k] /= 4;
-            double ec[4][2];
-            for (int a = 0; a < 4; ++a)
-                for (int k = 0; k < 2; ++k) {
-                    double dd = 0.0;
-                    for (int m = 0; m < 3; ++m) dd += (X[a][m] - cen[m]) * EG[m][k];
-                    ec[a][k] = dd;
+            if (g1_roles & 1) {
+                const int grp = p.elem_group ? p.elem_group[e] : 0;
+                const double *hrec = COMP ? nullptr : p.g_homog + 16 * grp;
+                const double tt = COMP ? p.g_t[grp] : hrec[13];
+                const double stab_c = COMP ? p.stab_c : hrec[14];
+                const double cr0 = -J[2][0] * J[1][1] + J[1][0] * J[2][1];
+                const double cr1 = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+                const double cr2 = -J[1][0] * J[0][1] + J[0][0] * J[1][1];
+                const double Jac = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+                const double G00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
+                const double G01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
+                const double G11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
+                const double detG = G00 * G11 - G01 * G01;
+                if (detG == 0.0) status[0] = 1;
+                const double idetG = 1.0 / detG;
+                const double i00 = G11 * idetG, i01 = -G01 * idetG, i11 = G00 * idetG;
+                for (int a = 0; a < 4; ++a) {
+                    const double t0 = i00 * gNp[a][0] + i01 * gNp[a][1];
+                    const double t1 = i01 * gNp[a][0] + i11 * gNp[a][1];
+                    double gg[3];
+                    for (int i = 0; i < 3; ++i) gg[i] = J[i][0] * t0 + J[i][1] * t1;
+                    g[G_GN + 2 * a] = EG[0][0] * gg[0] + EG[1][0] * gg[1] + EG[2][0] * gg[2];
+                    g[G_GN + 2 * a + 1] = EG[0][1] * gg[0] + EG[1][1] * gg[1] + EG[2][1] * gg[2];
                 }
-            const double G00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
-            const double G01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
-            const double G11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
-            const double detG = G00 * G11 - G01 * G01;
-            if (detG == 0.0) status[0] = 1;
-            const double idetG = 1.0 / detG;
-            const double i00 = G11 * idetG, i01 = -G01 * idetG, i11 = G00 * idetG;
-            for (int a = 0; a < 4; ++a) {
-                const double t0 = i00 * gNp[a][0] + i01 * gNp[a][1];
-                const double t1 = i01 * gNp[a][0] + i11 * gNp[a][1];
-                double gg[3];
-                for (int i = 0; i < 3; ++i) gg[i] = J[i][0] * t0 + J[i][1] * t1;
-                g[G_GN + 2 * a] = EG[0][0] * gg[0] + EG[1][0] * gg[1] + EG[2][0] * gg[2];
-                g[G_GN + 2 * a + 1] = EG[0][1] * gg[0] + EG[1][1] * gg[1] + EG[2][1] * gg[2];
+                double maxd = 0.0;
+                for (int i = 1; i < 4; ++i) {
+                    const double dx = X[i][0] - X[0][0], dy = X[i][1] - X[0][1], dz = X[i][2] - X[0][2];
+                    const double dist = (dx * dx + dy * dy + dz * dz);
+                    maxd = dist > maxd ? dist : maxd;
+                }
+                const double h = sqrt(maxd);
+                const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
+                if (!COMP) {
+                    g[G_SC + 0] = sqrt(tt * Jac);
+                    g[G_SC + 1] = sqrt((tt * tt * tt / 12.0) * Jac);
+                    g[G_SC + 2] = sqrt(tt * stab * Jac);
+                } else {
+                    double m, n;
+                    layup_angle(EG, p.g_csys + 9 * grp, m, n);
+                    g[G_SC + 0] = sqrt(Jac);
+                    g[G_SC + 1] = 0.0;
+                    g[G_SC + 2] = sqrt(stab * Jac);
+                    g[G_SC + 3] = m;
+                    g[G_SC + 4] = n;
+                }
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) g[G_EG + i * 3 + j] = EG[i][j];
             }
-            double maxd = 0.0;
-            for (int i = 1; i < 4; ++i) {
-                const double dx = X[i][0] - X[0][0], dy = X[i][1] - X[0][1], dz = X[i][2] - X[0][2];
-                const double dist = (dx * dx + dy * dy + dz * dz);
-                maxd = dist > maxd ? dist : maxd;
+            if (g1_roles & 2) {
+                double cen[3] = {0.0, 0.0, 0.0};
+                for (int a = 0; a < 4; ++a)
+                    for (int k = 0; k < 3; ++k) cen[k] += X[a][k];
+                for (int k = 0; k < 3; ++k) cen[k] /= 4;
+                double ec[4][2];
+                for (int a = 0; a < 4; ++a)
+                    for (int k = 0; k < 2; ++k) {
+                        double dd = 0.0;
+                        for (int m = 0; m < 3; ++m) dd += (X[a][m] - cen[m]) * EG[m][k];
+                        ec[a][k] = dd;
+                    }
+                q4_mitc_bs(ec, xi, eta, g + G_BS);
             }
-            const double h = sqrt(maxd);
-            const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
-            if (!COMP) {
-                g[G_SC + 0] = sqrt(tt * Jac);
-                g[G_SC + 1] = sqrt((tt * tt * tt / 12.0) * Jac);
-                g[G_SC + 2] = sqrt(tt * stab * Jac);
-            } else {
-                double m, n;
-                layup_angle(EG, p.g_csys + 9 * grp, m, n);
-                g[G_SC + 0] = sqrt(Jac);
-                g[G_SC + 1] = 0.0;
-                g[G_SC + 2] = sqrt(stab * Jac);
-                g[G_SC + 3] = m;
-                g[G_SC + 4] = n;
-            }
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) g[G_EG + i * 3 + j] = EG[i][j];
-            q4_mitc_bs(ec, xi, eta, g + G_BS);
         }
     }
 
