@@ -447,6 +447,34 @@ __device__ __forceinline__ void write_pair_row_rec(const int4 a, const int4 b, c
     }
 }
 
+// Lane c of a pair from its placement record: col = column c of the pair's
+// block K_IJ (col[r] = K_IJ[r][c]), row = row c (row[j] = K_IJ[c][j]).  The
+// same values and positions as write_pair_row_rec over r = 0..5, grouped by
+// column: value K_IJ[r][c] goes to row dof(I, r) at J's dof c, and for
+// I != J, K_IJ[c][j] to row dof(J, j) at I's dof c.
+__device__ __forceinline__ void write_pair_lane_rec(const int4 a, const int4 b, const int4 o, int c,
+                                                    const double col[6], const double row[6],
+                                                    double *__restrict__ vals) {
+    const int ccJ = (b.w >> (4 * c)) & 15;
+    if (ccJ != 15) {
+        const int colJ = ccJ < 8 ? o.x + ccJ : o.y + ccJ - 8;
+        for (int r = 0; r < 6; ++r) {
+            const int crI = (a.w >> (4 * r)) & 15;
+            if (crI == 15) continue;
+            vals[(crI < 8 ? a.x + crI * a.z : a.y + (crI - 8) * a.z) + colJ] = col[r];
+        }
+    }
+    if ((a.w >> 30) & 1) return;  // I = J: the block is its own transpose
+    const int ccI = (a.w >> (4 * c)) & 15;
+    if (ccI == 15) return;
+    const int colI = ccI < 8 ? o.z + ccI : o.w + ccI - 8;
+    for (int j = 0; j < 6; ++j) {
+        const int cr = (b.w >> (4 * j)) & 15;
+        if (cr == 15) continue;
+        vals[(cr < 8 ? b.x + cr * b.z : b.y + (cr - 8) * b.z) + colI] = row[j];
+    }
+}
+
 // Output phase of the fused assembly (TilePlan): thread (pair entry, row r)
 // of tile t sums row r of the pair's blocks in the LDS stage (stage[(le NBLK
 // + blk) 36 + 6 r + c], a block stored with J's rows read transposed) in
@@ -1322,24 +1350,28 @@ __global__ __launch_bounds__(SC_WG) void scatter_pair_kernel(ScatterPlan sp, con
     write_pair_row(sp, I, J, kI, kJ, r, sum, vals);
 }
 
-// Boundary pairs of the fused assembly: thread (boundary pair, row r) adds
-// row r of the pair's partial sums in tile order and writes the CSR values
-// as the scatter does.
+// Boundary pairs of the fused assembly: the six threads of a boundary pair
+// add the pair's partial sums in tile order and write its CSR values with
+// lane c taking column c of the pair's block for the rows of I and row c for
+// the transposed rows of J, so that each store instruction writes six
+// consecutive values per pair (a run of J's columns in a row of I, or of I's
+// columns in a row of J) instead of six values in six rows.
 constexpr int PF_WG = 256;
 __global__ __launch_bounds__(PF_WG) void pair_finalize_kernel(ScatterPlan sp, TilePlan tp,
                                                               double *__restrict__ vals) {
     const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * PF_WG + threadIdx.x;
     const int64_t b = gid / 6;
     if (b >= tp.nbnd) return;
-    const int r = (int)(gid % 6);
+    const int c = (int)(gid % 6);
     const int4 ra = tp.brec[3 * b], rb = tp.brec[3 * b + 1], ro = tp.brec[3 * b + 2];
-    double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double col[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, row[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int se = tp.bp_ptr[b + 1];
     for (int k = tp.bp_ptr[b]; k < se; ++k) {
-        const double *P = tp.partial + (int64_t)k * 36 + r * 6;
-        for (int c = 0; c < 6; ++c) sum[c] += P[c];
+        const double *P = tp.partial + (int64_t)k * 36;
+        for (int r = 0; r < 6; ++r) col[r] += P[r * 6 + c];
+        for (int j = 0; j < 6; ++j) row[j] += P[c * 6 + j];
     }
-    write_pair_row_rec(ra, rb, ro, r, sum, vals);
+    write_pair_lane_rec(ra, rb, ro, c, col, row, vals);
 }
 
 // ---------------------------------------------------------------------------
