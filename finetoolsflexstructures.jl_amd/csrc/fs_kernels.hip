@@ -921,9 +921,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     constexpr int G_EG = 0, G_GN = 9, G_SC = 17, G_BS = 17 + NSC;
     constexpr int CSTR = NR * K + 2;    // one Gauss point of C, padded
     constexpr int NODE = 6 + NR;
-    constexpr int OFF_G = 0;
-    constexpr int OFF_C = OFF_G + NE * NQP * GEO;
-    constexpr int OFF_N = OFF_C + NE * CSTR;
+    // C first: the accumulation's LDS reads then address row s of C at
+    // immediate offsets (< 2 KB) from one base per node
+    constexpr int OFF_C = 0;
+    constexpr int OFF_G = OFF_C + NE * CSTR;
+    constexpr int OFF_N = OFF_G + NE * NQP * GEO;
     // after the Gauss-point loop the node records are dead: the drilling
     // scratch (rotation blocks K_rr of the 4 nodes | trace terms) takes their place
     constexpr int OFF_M = OFF_N;
