@@ -1223,8 +1223,11 @@ int prepare_elements(fs_model *m, const fs_shell_params *p, fs::ElemParams &ep, 
             for (int64_t e = 0; e < m->E; ++e)
                 if (p->elem_group[e] < 0 || p->elem_group[e] >= ng)
                     return fail(c, FS_ERR_ARG, "stiffness: element %lld has group %d", (long long)e, p->elem_group[e]);
-            if ((s = upload(c, m->g_group, p->elem_group, (size_t)m->E))) return s;
-            ep.elem_group = m->g_group.as<int32_t>();
+            // one group: no group array, so the kernels read one material record
+            if (ng > 1) {
+                if ((s = upload(c, m->g_group, p->elem_group, (size_t)m->E))) return s;
+                ep.elem_group = m->g_group.as<int32_t>();
+            }
         }
     } else {
         const int ng = p->ngroups;
@@ -1264,8 +1267,10 @@ int prepare_elements(fs_model *m, const fs_shell_params *p, fs::ElemParams &ep, 
             for (int64_t e = 0; e < m->E; ++e)
                 if (p->elem_group[e] < 0 || p->elem_group[e] >= ng)
                     return fail(c, FS_ERR_ARG, "stiffness: element %lld has group %d", (long long)e, p->elem_group[e]);
-            if ((s = upload(c, m->g_group, p->elem_group, (size_t)m->E))) return s;
-            ep.elem_group = m->g_group.as<int32_t>();
+            if (ng > 1) {
+                if ((s = upload(c, m->g_group, p->elem_group, (size_t)m->E))) return s;
+                ep.elem_group = m->g_group.as<int32_t>();
+            }
         }
     }
     FS_HIP(c, hipMemsetAsync(m->status.p, 0, sizeof(int), c->stream));

@@ -176,10 +176,9 @@ struct MatRows {
     double Ls[4];
 };
 
-__device__ __forceinline__ MatRows mat_rows(const ElemParams &p, bool composite, int e, double sc0, double sc1,
+__device__ __forceinline__ MatRows mat_rows(const ElemParams &p, bool composite, int grp, double sc0, double sc1,
                                             double sc2, double sc3, double sc4) {
     MatRows r;
-    const int grp = p.elem_group ? p.elem_group[e] : 0;
     r.composite = composite;
     if (!r.composite) {
         const double *hrec = p.g_homog + 16 * grp;
@@ -901,7 +900,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
 // point) stores a compact geometry record: E_G, gradN_e, the material scale
 // factors (and the layup angle for layered shells) and the MITC shear row
 // entries; the material factors are rebuilt per node thread.
-template <int NE, int WG, int SPLIT, int RS, bool COMP, bool FUSED>
+template <int NE, int WG, int SPLIT, int RS, bool COMP, bool FUSED, bool UG>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_q4_seq_kernel(ElemParams p, const double *__restrict__ xs,
                                                          const double *__restrict__ ys,
                                                          const double *__restrict__ zs,
@@ -980,7 +979,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             double EG[3][3];
             e_g(EG, J);
             if (g1_roles & 1) {
-                const int grp = p.elem_group ? p.elem_group[e] : 0;
+                const int grp = UG ? 0 : (p.elem_group ? p.elem_group[e] : 0);
                 const double *hrec = COMP ? nullptr : p.g_homog + 16 * grp;
                 const double tt = COMP ? p.g_t[grp] : hrec[13];
                 const double stab_c = COMP ? p.stab_c : hrec[14];
@@ -1052,6 +1051,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     int node2 = 0;
     uint8_t valid2 = 0;
     double n20 = 0.0, n21 = 0.0, n22 = 0.0;
+    // material group of the lane's element: with one group (UG) a constant,
+    // so the material factor rows come through scalar loads of one record
+    int grp2 = 0;
+    if (!UG && gact && p.elem_group) grp2 = p.elem_group[elem(le2)];
     if (gact) {
         node2 = conn[j2 * E + elem(le2)];
         valid2 = valid[node2];
@@ -1101,7 +1104,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 nk_e[2] = 1.0;
             }
             nodal_triad(nk_e, A);
-            const MatRows mr = mat_rows(p, COMP, elem(le2), g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
+            const MatRows mr = mat_rows(p, COMP, grp2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
                                         COMP ? g[G_SC + NSC - 2] : 0.0, COMP ? g[G_SC + NSC - 1] : 0.0);
             const double gNj0 = g[G_GN + 2 * j2], gNj1 = g[G_GN + 2 * j2 + 1];
             const double *bs = g + G_BS + j2 * 6;
@@ -1146,7 +1149,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
                 r2a[l] = A[0][l] - (1 / A[2][2]) * A[0][2] * A[l][2];
                 r2b[l] = A[1][l] - (1 / A[2][2]) * A[1][2] * A[l][2];
             }
-            const MatRows mr = mat_rows(p, COMP, elem(le2), g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
+            const MatRows mr = mat_rows(p, COMP, grp2, g[G_SC + 0], g[G_SC + 1], g[G_SC + 2],
                                         COMP ? g[G_SC + NSC - 2] : 0.0, COMP ? g[G_SC + NSC - 1] : 0.0);
             const double *bs = g + G_BS + j2 * 6;
             double *cdst = sC + le2 * CSTR + 6 * j2;
@@ -1262,7 +1265,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         sM[le * MISC + 36 + k] = t;
     }
     __syncthreads();
-    const int grpd = cact ? (p.elem_group ? p.elem_group[elem(le3)] : 0) : 0;
+    const int grpd = UG ? 0 : (cact ? (p.elem_group ? p.elem_group[elem(le3)] : 0) : 0);
     const double dscale = COMP ? p.drill_scale : (cact ? p.g_homog[16 * grpd + 15] : 0.0);
     if (split_dr && cact && ba == bb) {
         double *dd = p.split_dr + ((size_t)(e0 + le3) * NEN + ba) * 9;
@@ -2781,8 +2784,8 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
 #define FS_Q4_RS 2
 #endif
 
-template <int NE, int WG, int SPLIT, int RS, bool COMP, bool FUSED = false>
-static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
+template <int NE, int WG, int SPLIT, int RS, bool COMP, bool FUSED, bool UG>
+static hipError_t launch_q4_seq_ug(const ElemParams &p, const double *x, const double *y, const double *z,
                                 const int32_t *conn, const double *nx, const double *ny, const double *nz,
                                 const uint8_t *valid, double *Kb, int *status, hipStream_t st,
                                 const ScatterPlan *sp = nullptr, const TilePlan *tp = nullptr, double *vals = nullptr) {
@@ -2793,7 +2796,7 @@ static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const doub
     static_assert(NE * 10 <= WG && NE * 4 * RS <= WG, "workgroup too small");
     static_assert(total * sizeof(double) <= 160 * 1024, "element kernel LDS exceeds 160 KiB");
     if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP, FUSED>,
+        hipError_t e = hipFuncSetAttribute((const void *)elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP, FUSED, UG>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
@@ -2806,9 +2809,24 @@ static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const doub
         nblocks = (p.eend - p.ebegin + NE - 1) / NE;
     }
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP, FUSED>), dim3(nblocks), dim3(WG), smem, st, p, x, y,
+    hipLaunchKernelGGL((elem_q4_seq_kernel<NE, WG, SPLIT, RS, COMP, FUSED, UG>), dim3(nblocks), dim3(WG), smem, st, p, x, y,
                        z, conn, nx, ny, nz, valid, Kb, status, sp ? *sp : ScatterPlan{}, tp ? *tp : TilePlan{}, vals);
     return hipGetLastError();
+}
+
+// Models with one material group (no per-element group array) run the
+// uniform-group instantiation; the split forms (built once per stiffness
+// for the mixed step) keep the general one.
+template <int NE, int WG, int SPLIT, int RS, bool COMP, bool FUSED = false>
+static hipError_t launch_q4_seq(const ElemParams &p, const double *x, const double *y, const double *z,
+                                const int32_t *conn, const double *nx, const double *ny, const double *nz,
+                                const uint8_t *valid, double *Kb, int *status, hipStream_t st,
+                                const ScatterPlan *sp = nullptr, const TilePlan *tp = nullptr, double *vals = nullptr) {
+    if (SPLIT == 0 && p.elem_group == nullptr)
+        return launch_q4_seq_ug<NE, WG, SPLIT, RS, COMP, FUSED, SPLIT == 0>(p, x, y, z, conn, nx, ny, nz, valid, Kb,
+                                                                             status, st, sp, tp, vals);
+    return launch_q4_seq_ug<NE, WG, SPLIT, RS, COMP, FUSED, false>(p, x, y, z, conn, nx, ny, nz, valid, Kb, status,
+                                                                   st, sp, tp, vals);
 }
 
 // Launch shapes (elements per workgroup, workgroup size) the autotuner

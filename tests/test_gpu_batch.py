@@ -173,3 +173,35 @@ def test_batch_c3_full_size(ctx):
     assert np.linalg.norm(runs[big]["V"] - Vo) <= 1e-10 * np.linalg.norm(Vo)
     for prec, Um in mixed.items():
         assert np.abs(Um - Uo).max() <= 1e-6 * np.abs(Uo).max(), prec
+
+
+def test_q4_batch_equals_models_alone(ctx):
+    """Q4RS models of different sizes, thicknesses and materials in one
+    batch (one material group per model, so the element kernel reads each
+    element's group) against each model alone (one group): bitwise the same
+    stiffness, and the oracle's element matrices within 1e-12."""
+    print("[gpu] Q4 batch vs alone", flush=True)
+    from flexshell.femm import FEMMShellQ4RS, SysmatAssemblerFFBlock
+    from flexshell.material import MatDeforElastIso
+    models = []
+    for k, (n, t, E) in enumerate([(6, 0.005, 70e9), (9, 0.02, 200e9), (7, 0.001, 110e9)]):
+        m = W.cylinder_panel_c2(n=n, seed=40 + k, t=t, E=E)
+        m["material"] = MatDeforElastIso(E, 0.3 + 0.01 * k, 2700.0)
+        models.append(m)
+    bt = B.ShellBatch(models, ctx=ctx)
+    Kb = bt.stiffness()
+    for k, m in enumerate(models):
+        f = FEMMShellQ4RS(m["conn"], m["material"], thickness=m["t"], ctx=ctx)
+        f.associategeometry(m["xyz"])
+        nf = m["dchi"].nfreedofs
+        Kk = f.stiffness(SysmatAssemblerFFBlock(nf), m["xyz"], None, None, m["dchi"])
+        sl = bt.model_slice(k)
+        Kbk = Kb[sl, sl]
+        assert np.array_equal(Kbk.indices, Kk.indices) and np.array_equal(Kbk.data, Kk.data), k
+        Emod, nu = m["material"].E, m["material"].nu
+        Dps, Dt = O.shell_D(O.iso_D6(Emod, nu))
+        Ko = O.element_matrices(4, m["xyz"], m["conn"], f._normals, f._normal_valid, Dps=Dps, Dt=Dt, t=m["t"])
+        Kg = f.element_matrices(m["xyz"])
+        err = np.linalg.norm(Kg - Ko, axis=(1, 2)) / np.linalg.norm(Ko, axis=(1, 2))
+        assert err.max() < 1e-12, (k, err.max())
+        f._model.close()

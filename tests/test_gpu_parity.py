@@ -133,6 +133,50 @@ def test_layered_barbero(ctx, nen):
 
 
 @pytest.mark.parametrize("nen", [3, 4])
+def test_layered_two_layup_groups(ctx, nen):
+    """Two layup groups on alternating elements of the Barbero plate (the
+    reference's layup_groups): a [0/90] and a [30/-60/30] laminate of other
+    thicknesses whose frame is turned about x. Every element matrix must use
+    its own group's laminate and frame: K against the oracle within 1e-12."""
+    print(f"[gpu] layered two groups nen={nen}", flush=True)
+    c = SC.barbero(nen)
+    d = c["dchi"]
+    D6 = c["plies"][0][0]
+    plies_b = [(D6, 3 * SC.mm, 30.0, 0.0), (D6, 4 * SC.mm, -60.0, 0.0), (D6, 3 * SC.mm, 30.0, 0.0)]
+    ca, sa = np.cos(0.3), np.sin(0.3)
+    csys_b = np.array([[1.0, 0.0, 0.0], [0.0, ca, -sa], [0.0, sa, ca]])
+    lay_a = _flexshell_layup(c)
+    lay_b = _flexshell_layup(dict(c, plies=plies_b, csys=csys_b))
+    E = c["conn"].shape[0]
+    eg = (np.arange(E) % 2).astype(np.int32)
+    cls = FM.FEMMShellT3FFComp if nen == 3 else FM.FEMMShellQ4RSComp
+    f = cls(c["conn"], None, layup_groups=[(lay_a, np.nonzero(eg == 0)[0]), (lay_b, np.nonzero(eg == 1)[0])],
+            ctx=ctx)
+    f.associategeometry(c["xyz"])
+    Kg = f.stiffness(FM.SysmatAssemblerSparseCSRSymm(), c["xyz"], None, None, d)
+    A, B, D, H, _ = O.laminate(c["plies"])
+    Ab, Bb, Db, Hb, _ = O.laminate(plies_b)
+    groups = [(A, B, D, H, c["csys"], c["th"]), (Ab, Bb, Db, Hb, csys_b, 10 * SC.mm)]
+    Ko = O.assemble_csr(nen, c["conn"], d.dofnums, d.nalldofs,
+                        O.element_matrices(nen, c["xyz"], c["conn"], f._normals, f._normal_valid, groups=groups,
+                                           elem_group=eg))
+    assert np.array_equal(Kg.indices, Ko.indices)
+    assert _rel(Kg, Ko) < KTOL
+    # and each group alone: a model of only its elements gives the same element matrices
+    Ke = f.element_matrices(c["xyz"])
+    for gi, lay in enumerate((lay_a, lay_b)):
+        sel = np.nonzero(eg == gi)[0]
+        fg = cls(c["conn"][sel], lay, ctx=ctx)
+        fg.associategeometry(c["xyz"])
+        fg._normals, fg._normal_valid = f._normals.copy(), f._normal_valid.copy()
+        fg._model.normals_upload(fg._normals, fg._normal_valid)
+        Kge = fg.element_matrices(c["xyz"])
+        assert np.abs(Kge - Ke[sel]).max() <= 1e-12 * np.abs(Ke[sel]).max(), gi
+        fg._model.close()
+    f._model.close()
+
+
+@pytest.mark.parametrize("nen", [3, 4])
 @pytest.mark.parametrize("case", ["c1", "c2", "c6", "c7", "c8+", "c9-", "c10w", "c10u"])
 def test_composite_reference_values_gpu(ctx, case, nen):
     """The T3FFComp / Q4RSComp plates of test_composite_shell_statics.jl and
