@@ -476,13 +476,21 @@ __device__ __forceinline__ void write_pair_lane_rec(const int4 a, const int4 b, 
 
 // Output phase of the fused assembly (TilePlan): thread (pair entry, row r)
 // of tile t sums row r of the pair's blocks in the LDS stage (stage[(le NBLK
-// + blk) 36 + 6 r + c], a block stored with J's rows read transposed) in
-// element order, then writes the CSR values of a pair completed in the tile,
-// or the 6 values of its partial sum.  A thread takes TW items per pass and
+// + blk) 36 + 6 r + c], every block stored in the pair's orientation, rows
+// of the lower-numbered node first) in element order, then writes the CSR
+// values of a pair completed in the tile, or the 6 values of its partial
+// sum.  A thread takes TW items per pass and
 // loads all their records before the first sum, so that the record loads of
 // the items overlap (C2 cylinder, fused Q4 kernel: 0.756 ms with one item,
 // 0.727 ms with three; C4 plate, T3: 1.032 ms and 0.990 ms with four; the
 // values are bitwise the same).
+// Whether element e's block (a, b) is staged transposed: the fused output
+// phase reads every block in the orientation of its node pair (I < J), the
+// transposed bit of the host's contribution code.
+__device__ __forceinline__ bool stage_transposed(const int32_t *__restrict__ conn, int E, int e, int a, int b) {
+    return a != b && conn[a * E + e] > conn[b * E + e];
+}
+
 struct TileItem {
     int4 a, b, o;
     uint64_t cw;
@@ -504,12 +512,8 @@ __device__ __forceinline__ void tile_item_write(const TileItem &it, int r, const
     const int cnt = (int)(it.cw & 7);
     for (int k = 0; k < cnt; ++k) {
         const int v = (int)((it.cw >> (3 + 10 * k)) & 1023);
-        const double *B = stage + ((v & 31) * NBLK + ((v >> 5) & 15)) * 36;
-        if ((v >> 9) == 0) {
-            for (int c = 0; c < 6; ++c) sum[c] += B[r * 6 + c];
-        } else {
-            for (int c = 0; c < 6; ++c) sum[c] += B[c * 6 + r];
-        }
+        const double *B = stage + ((v & 31) * NBLK + ((v >> 5) & 15)) * 36 + r * 6;
+        for (int c = 0; c < 6; ++c) sum[c] += B[c];
     }
     if (it.slot >= 0) {
         double *dst = tp.partial + (int64_t)it.slot * 36 + r * 6;
@@ -875,8 +879,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     __syncthreads();  // every read of sC and sG is done: reuse them as the output stage
     if (cact) {
         double *ks = sC + (le3 * T::NBLK + blk) * 36;
-        for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c) ks[r * 6 + c] = acc[r][c];
+        if (FUSED && stage_transposed(conn, E, elem(le3), ba, bb)) {
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) ks[c * 6 + r] = acc[r][c];
+        } else {
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) ks[r * 6 + c] = acc[r][c];
+        }
     }
     __syncthreads();
     if constexpr (FUSED) {
@@ -1302,8 +1311,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     // ---------------- stage and store ----------------
     if (cact) {
         double *ks = smem + (le3 * NBLK + blk) * 36;  // G and C regions are dead here
-        for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c) ks[r * 6 + c] = acc[r][c];
+        if (FUSED && stage_transposed(conn, E, elem(le3), ba, bb)) {
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) ks[c * 6 + r] = acc[r][c];
+        } else {
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) ks[r * 6 + c] = acc[r][c];
+        }
     }
     __syncthreads();
     if constexpr (FUSED) {
