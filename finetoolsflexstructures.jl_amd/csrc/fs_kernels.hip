@@ -427,22 +427,37 @@ __device__ __forceinline__ void write_pair_row(const ScatterPlan &sp, int I, int
 
 // Row r of a pair from its placement record (TilePlan trec / brec): the
 // same writes as write_pair_row, with every position taken from the record.
+// Codes of a node whose six dofs are all in the block, in order: the
+// common case, written through one base address per run.
+constexpr int FS_ALL_DOFS = 0x543210;
+
 __device__ __forceinline__ void write_pair_row_rec(const int4 a, const int4 b, const int4 o, int r,
                                                    const double sum[6], double *__restrict__ vals) {
     const int crI = (a.w >> (4 * r)) & 15;
     if (crI == 15) return;
     const int rowI = crI < 8 ? a.x + crI * a.z : a.y + (crI - 8) * a.z;
-    for (int c = 0; c < 6; ++c) {
-        const int cc = (b.w >> (4 * c)) & 15;
-        if (cc == 15) continue;
-        vals[rowI + (cc < 8 ? o.x + cc : o.y + cc - 8)] = sum[c];
+    const bool allJ = (b.w & 0xffffff) == FS_ALL_DOFS;
+    if (allJ) {
+        double *d = vals + rowI + o.x;
+        for (int c = 0; c < 6; ++c) d[c] = sum[c];
+    } else {
+        for (int c = 0; c < 6; ++c) {
+            const int cc = (b.w >> (4 * c)) & 15;
+            if (cc == 15) continue;
+            vals[rowI + (cc < 8 ? o.x + cc : o.y + cc - 8)] = sum[c];
+        }
     }
     if ((a.w >> 30) & 1) return;  // I = J: the block is its own transpose
     const int colI = crI < 8 ? o.z + crI : o.w + crI - 8;
-    for (int c = 0; c < 6; ++c) {
-        const int cr = (b.w >> (4 * c)) & 15;
-        if (cr == 15) continue;
-        vals[(cr < 8 ? b.x + cr * b.z : b.y + (cr - 8) * b.z) + colI] = sum[c];
+    if (allJ) {
+        double *d = vals + b.x + colI;
+        for (int c = 0; c < 6; ++c) d[(int64_t)c * b.z] = sum[c];
+    } else {
+        for (int c = 0; c < 6; ++c) {
+            const int cr = (b.w >> (4 * c)) & 15;
+            if (cr == 15) continue;
+            vals[(cr < 8 ? b.x + cr * b.z : b.y + (cr - 8) * b.z) + colI] = sum[c];
+        }
     }
 }
 
