@@ -1100,6 +1100,21 @@ int build_tiles(fs_model *m, int which, int NE) {
                 }
                 i = j;
             }
+            // entries of one kind next to each other, so that the waves of the
+            // output phase take one path: completed pairs of two nodes whose
+            // dofs are all in the block, completed diagonal pairs, other
+            // completed pairs, boundary pairs.  The order within a kind (and
+            // of the entries of one pair) stays, so the sums do not change.
+            auto kind = [&](const Ent &en) {
+                if (en.boundary) return 3;
+                const int64_t I = B.h_pnode[2 * (int64_t)en.p], J = B.h_pnode[2 * (int64_t)en.p + 1];
+                if (I == J) return 1;
+                const bool all = (B.h_node_rec[4 * I + 3] & 0xffffff) == 0x543210 &&
+                                 (B.h_node_rec[4 * J + 3] & 0xffffff) == 0x543210;
+                return all ? 0 : 2;
+            };
+            std::stable_sort(out.begin(), out.end(),
+                             [&](const Ent &x, const Ent &y) { return kind(x) < kind(y); });
         }
     }, 256);
     if (bad) return fail(c, FS_ERR_STATE, "fused assembly: tile contributions exceed the pair's");
