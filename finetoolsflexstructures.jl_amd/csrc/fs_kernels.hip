@@ -585,111 +585,116 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     const int ne = FUSED ? NE : ((p.eend - e0) < NE ? (p.eend - e0) : NE);
     auto elem = [&](int le) -> int { return FUSED ? p.tile_elem[e0 + le] : e0 + le; };
 
-    // ---------------- G1: geometry per element ----------------
+    // ---------------- G1: geometry per element (FEMMShellT3FFModule.jl:269-321,465-484) ----------------
+    // One thread per element; each part of the record (frame and gradients,
+    // material factors, DSG shear entries) is stored as soon as it is
+    // computed, which keeps the thread's live registers (and the kernel's) low.
     static_assert(NEN == 3 && NQP == 1, "elem_kernel is the T3 kernel");
-    if (tid < NE * NQP) {
-        const int le = tid / NQP, q = tid % NQP;
-        if (le < ne) {
-            const int e = elem(le);
-            double X[NEN][3];
-            for (int a = 0; a < NEN; ++a) {
-                int n = conn[a * E + e];
-                X[a][0] = xs[n];
-                X[a][1] = ys[n];
-                X[a][2] = zs[n];
+    if (tid < NE && tid < ne) {
+        const int le = tid;
+        const int e = elem(le);
+        double X[NEN][3];
+        for (int a = 0; a < NEN; ++a) {
+            const int n = conn[a * E + e];
+            X[a][0] = xs[n];
+            X[a][1] = ys[n];
+            X[a][2] = zs[n];
+        }
+        double *g = sG + le * T::GEO;
+        double EG[3][3];
+        double J0[3][2];
+        for (int j = 0; j < 3; ++j) {
+            J0[j][0] = X[1][j] - X[0][j];
+            J0[j][1] = X[2][j] - X[0][j];
+        }
+        e_g(EG, J0);
+        double ec[3][2];
+        ec[0][0] = 0.0;
+        ec[0][1] = 0.0;
+        ec[1][0] = J0[0][0] * EG[0][0] + J0[1][0] * EG[1][0] + J0[2][0] * EG[2][0];
+        ec[1][1] = J0[0][0] * EG[0][1] + J0[1][0] * EG[1][1] + J0[2][0] * EG[2][1];
+        ec[2][0] = J0[0][1] * EG[0][0] + J0[1][1] * EG[1][0] + J0[2][1] * EG[2][0];
+        ec[2][1] = J0[0][1] * EG[0][1] + J0[1][1] * EG[1][1] + J0[2][1] * EG[2][1];
+        const double a = ec[1][0], b = ec[1][1], c = ec[2][0], d = ec[2][1];
+        const double Jd = (a * d - b * c);
+        const double Ae = Jd / 2;
+        {
+            double gN[NEN][2];
+            gN[0][0] = (b - d) / Jd;
+            gN[1][0] = d / Jd;
+            gN[2][0] = -b / Jd;
+            gN[0][1] = (c - a) / Jd;
+            gN[1][1] = -c / Jd;
+            gN[2][1] = a / Jd;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) g[T::G_EG + i * 3 + j] = EG[i][j];
+            for (int k = 0; k < NEN; ++k) {
+                g[T::G_GN + 2 * k] = gN[k][0];
+                g[T::G_GN + 2 * k + 1] = gN[k][1];
             }
-            double *g = sG + (le * NQP + q) * T::GEO;
-            double EG[3][3], gN[NEN][2];
+        }
+        {
             double Lmb[36], Ls[4];
             const int grp = p.elem_group ? p.elem_group[e] : 0;
             const double *hrec = p.composite ? nullptr : p.g_homog + 16 * grp;
             const double tt = p.composite ? p.g_t[grp] : hrec[13];
             const double stab_c = p.composite ? p.stab_c : hrec[14];
-            {
-                // FEMMShellT3FFModule.jl:269-321,465-484
-                double J0[3][2];
-                for (int j = 0; j < 3; ++j) {
-                    J0[j][0] = X[1][j] - X[0][j];
-                    J0[j][1] = X[2][j] - X[0][j];
-                }
-                e_g(EG, J0);
-                double ec[3][2];
-                ec[0][0] = 0.0;
-                ec[0][1] = 0.0;
-                ec[1][0] = J0[0][0] * EG[0][0] + J0[1][0] * EG[1][0] + J0[2][0] * EG[2][0];
-                ec[1][1] = J0[0][0] * EG[0][1] + J0[1][0] * EG[1][1] + J0[2][0] * EG[2][1];
-                ec[2][0] = J0[0][1] * EG[0][0] + J0[1][1] * EG[1][0] + J0[2][1] * EG[2][0];
-                ec[2][1] = J0[0][1] * EG[0][1] + J0[1][1] * EG[1][1] + J0[2][1] * EG[2][1];
-                const double a = ec[1][0], b = ec[1][1], c = ec[2][0], d = ec[2][1];
-                const double Jd = (a * d - b * c);
-                gN[0][0] = (b - d) / Jd;
-                gN[1][0] = d / Jd;
-                gN[2][0] = -b / Jd;
-                gN[0][1] = (c - a) / Jd;
-                gN[1][1] = -c / Jd;
-                gN[2][1] = a / Jd;
-                const double Ae = Jd / 2;
-                const double h = sqrt(2 * Ae);
-                const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
-                const double wsh = (NR == 12) ? Ae / 3 : Ae;  // AVERAGE_K splits the weight
-                if (!p.composite) {
-                    homog_factors(hrec, hrec + 9, sqrt(tt * Ae), sqrt((tt * tt * tt) / 12 * Ae),
-                                  sqrt(tt * stab * wsh), Lmb, Ls);
-                } else {
-                    double m, n;
-                    layup_angle(EG, p.g_csys + 9 * grp, m, n);
-                    composite_factors(p.g_Labd + 36 * grp, p.g_Lh + 4 * grp, m, n, sqrt(Ae), sqrt(stab * wsh),
-                                      Lmb, Ls);
-                }
-                // DSG transverse shear entries (_add_Bsmat_o!, :486-537)
-                double bs[3][3][T::NSH];
-                for (int j = 0; j < 3; ++j)
-                    for (int cc = 0; cc < 3; ++cc)
-                        for (int r = 0; r < T::NSH; ++r) bs[j][cc][r] = 0.0;
-                const int ord[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
-                for (int o = 0; o < 3; ++o) {
-                    const int s = ord[o][0], pp = ord[o][1], qq = ord[o][2];
-                    const double aa = ec[pp][0] - ec[s][0];
-                    const double bb = ec[pp][1] - ec[s][1];
-                    const double cc = ec[qq][0] - ec[s][0];
-                    const double dd = ec[qq][1] - ec[s][1];
-                    const double mm = (1.0 / 2 / Ae);
-                    const int r0 = (NR == 12) ? 2 * o : 0;
-                    // comps: 0 = w, 1 = theta_x, 2 = theta_y
-                    bs[s][0][r0] += mm * (bb - dd);
-                    bs[s][2][r0] += mm * (Ae);
-                    bs[s][0][r0 + 1] += mm * (cc - aa);
-                    bs[s][1][r0 + 1] += mm * (-Ae);
-                    bs[pp][0][r0] += mm * (dd);
-                    bs[pp][1][r0] += mm * (-bb * dd / 2);
-                    bs[pp][2][r0] += mm * (aa * dd / 2);
-                    bs[pp][0][r0 + 1] += mm * (-cc);
-                    bs[pp][1][r0 + 1] += mm * (bb * cc / 2);
-                    bs[pp][2][r0 + 1] += mm * (-aa * cc / 2);
-                    bs[qq][0][r0] += mm * (-bb);
-                    bs[qq][1][r0] += mm * (bb * dd / 2);
-                    bs[qq][2][r0] += mm * (-bb * cc / 2);
-                    bs[qq][0][r0 + 1] += mm * (aa);
-                    bs[qq][1][r0 + 1] += mm * (-aa * dd / 2);
-                    bs[qq][2][r0 + 1] += mm * (aa * cc / 2);
-                }
-                if (NR == 8) {
-                    for (int j = 0; j < 3; ++j)
-                        for (int cc = 0; cc < 3; ++cc)
-                            for (int r = 0; r < 2; ++r) bs[j][cc][r] *= (1.0 / 3);
-                }
-                for (int j = 0; j < 3; ++j)
-                    for (int cc = 0; cc < 3; ++cc)
-                        for (int r = 0; r < T::NSH; ++r) g[T::G_BS + (j * 3 + cc) * T::NSH + r] = bs[j][cc][r];
-            }
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) g[T::G_EG + i * 3 + j] = EG[i][j];
-            for (int a = 0; a < NEN; ++a) {
-                g[T::G_GN + 2 * a] = gN[a][0];
-                g[T::G_GN + 2 * a + 1] = gN[a][1];
+            const double h = sqrt(2 * Ae);
+            const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
+            const double wsh = (NR == 12) ? Ae / 3 : Ae;  // AVERAGE_K splits the weight
+            if (!p.composite) {
+                homog_factors(hrec, hrec + 9, sqrt(tt * Ae), sqrt((tt * tt * tt) / 12 * Ae),
+                              sqrt(tt * stab * wsh), Lmb, Ls);
+            } else {
+                double m, n;
+                layup_angle(EG, p.g_csys + 9 * grp, m, n);
+                composite_factors(p.g_Labd + 36 * grp, p.g_Lh + 4 * grp, m, n, sqrt(Ae), sqrt(stab * wsh),
+                                  Lmb, Ls);
             }
             for (int i = 0; i < 36; ++i) g[T::G_LMB + i] = Lmb[i];
             for (int i = 0; i < 4; ++i) g[T::G_LS + i] = Ls[i];
+        }
+        {
+            // DSG transverse shear entries (_add_Bsmat_o!, :486-537)
+            double bs[3][3][T::NSH];
+            for (int j = 0; j < 3; ++j)
+                for (int cc = 0; cc < 3; ++cc)
+                    for (int r = 0; r < T::NSH; ++r) bs[j][cc][r] = 0.0;
+            const int ord[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+            for (int o = 0; o < 3; ++o) {
+                const int s = ord[o][0], pp = ord[o][1], qq = ord[o][2];
+                const double aa = ec[pp][0] - ec[s][0];
+                const double bb = ec[pp][1] - ec[s][1];
+                const double cc = ec[qq][0] - ec[s][0];
+                const double dd = ec[qq][1] - ec[s][1];
+                const double mm = (1.0 / 2 / Ae);
+                const int r0 = (NR == 12) ? 2 * o : 0;
+                // comps: 0 = w, 1 = theta_x, 2 = theta_y
+                bs[s][0][r0] += mm * (bb - dd);
+                bs[s][2][r0] += mm * (Ae);
+                bs[s][0][r0 + 1] += mm * (cc - aa);
+                bs[s][1][r0 + 1] += mm * (-Ae);
+                bs[pp][0][r0] += mm * (dd);
+                bs[pp][1][r0] += mm * (-bb * dd / 2);
+                bs[pp][2][r0] += mm * (aa * dd / 2);
+                bs[pp][0][r0 + 1] += mm * (-cc);
+                bs[pp][1][r0 + 1] += mm * (bb * cc / 2);
+                bs[pp][2][r0 + 1] += mm * (-aa * cc / 2);
+                bs[qq][0][r0] += mm * (-bb);
+                bs[qq][1][r0] += mm * (bb * dd / 2);
+                bs[qq][2][r0] += mm * (-bb * cc / 2);
+                bs[qq][0][r0 + 1] += mm * (aa);
+                bs[qq][1][r0 + 1] += mm * (-aa * dd / 2);
+                bs[qq][2][r0 + 1] += mm * (aa * cc / 2);
+            }
+            if (NR == 8) {
+                for (int j = 0; j < 3; ++j)
+                    for (int cc = 0; cc < 3; ++cc)
+                        for (int r = 0; r < 2; ++r) bs[j][cc][r] *= (1.0 / 3);
+            }
+            for (int j = 0; j < 3; ++j)
+                for (int cc = 0; cc < 3; ++cc)
+                    for (int r = 0; r < T::NSH; ++r) g[T::G_BS + (j * 3 + cc) * T::NSH + r] = bs[j][cc][r];
         }
     }
     __syncthreads();
