@@ -556,7 +556,7 @@ __device__ __forceinline__ void tile_write(int t, const TilePlan &tp, const doub
     }
 }
 
-template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS, bool FUSED>
+template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS, bool FUSED, bool COMP>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void elem_kernel(ElemParams p, const double *__restrict__ xs,
                                                   const double *__restrict__ ys,
                                                   const double *__restrict__ zs,
@@ -636,13 +636,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         {
             double Lmb[36], Ls[4];
             const int grp = p.elem_group ? p.elem_group[e] : 0;
-            const double *hrec = p.composite ? nullptr : p.g_homog + 16 * grp;
-            const double tt = p.composite ? p.g_t[grp] : hrec[13];
-            const double stab_c = p.composite ? p.stab_c : hrec[14];
+            const double *hrec = COMP ? nullptr : p.g_homog + 16 * grp;
+            const double tt = COMP ? p.g_t[grp] : hrec[13];
+            const double stab_c = COMP ? p.stab_c : hrec[14];
             const double h = sqrt(2 * Ae);
             const double stab = (tt * tt) / (tt * tt + stab_c * (h * h));
             const double wsh = (NR == 12) ? Ae / 3 : Ae;  // AVERAGE_K splits the weight
-            if (!p.composite) {
+            if constexpr (!COMP) {
                 homog_factors(hrec, hrec + 9, sqrt(tt * Ae), sqrt((tt * tt * tt) / 12 * Ae),
                               sqrt(tt * stab * wsh), Lmb, Ls);
             } else {
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         // rotation columns are signed zeros.  Their products with those
         // columns add signed zeros to +0 or to nonzero sums: leaving them out
         // keeps every block bitwise the same.
-        const bool memb3 = NQP == 1 && !p.composite;
+        constexpr bool memb3 = NQP == 1 && !COMP;
         if (memb3) {
             for (int s = 0; s < 3; ++s) {
                 double va[3], vb[3];
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             const int node = conn[ba * E + e];
             const double *ms = sM + le3 * T::MISC;
             const int grp = p.elem_group ? p.elem_group[e] : 0;
-            const double dscale = p.composite ? p.drill_scale : p.g_homog[16 * grp + 15];
+            const double dscale = COMP ? p.drill_scale : p.g_homog[16 * grp + 15];
             const double kavg = (((((ms[T::M_KP + 0] + ms[T::M_KP + 2]) + ms[T::M_KP + 4]) + ms[T::M_KP + 1]) +
                                    ms[T::M_KP + 3]) + ms[T::M_KP + 5]) / 6 * dscale;
             const double *gv = ms + T::M_G + 3 * ba;
@@ -2782,8 +2782,8 @@ __global__ __launch_bounds__(256) void coo_reduce_kernel(int64_t nnz, const int6
     out[s] = acc;
 }
 
-template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS = FS_T3_RS, bool FUSED = false>
-static hipError_t launch_elem(const ElemParams &p, const double *x, const double *y, const double *z,
+template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS, bool FUSED, bool COMP>
+static hipError_t launch_elem_kind(const ElemParams &p, const double *x, const double *y, const double *z,
                               const int32_t *conn, const double *nx, const double *ny, const double *nz,
                               const uint8_t *valid, double *Kb, int *status, hipStream_t st,
                               const ScatterPlan *sp = nullptr, const TilePlan *tp = nullptr, double *vals = nullptr) {
@@ -2794,7 +2794,7 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
     // set on every launch: cheap, per device, and free of shared host state
     // when several contexts launch from different threads
     if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS, FUSED>,
+        hipError_t e = hipFuncSetAttribute((const void *)elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS, FUSED, COMP>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return e;
     }
@@ -2807,10 +2807,25 @@ static hipError_t launch_elem(const ElemParams &p, const double *x, const double
         nblocks = (p.eend - p.ebegin + NE - 1) / NE;
     }
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS, FUSED>), dim3(nblocks), dim3(WG), smem, st, p, x,
+    hipLaunchKernelGGL((elem_kernel<NEN, NQP, NR, NE, WG, SPLIT, RS, FUSED, COMP>), dim3(nblocks), dim3(WG), smem, st, p, x,
                        y, z, conn, nx, ny, nz, valid, Kb, status, sp ? *sp : ScatterPlan{}, tp ? *tp : TilePlan{},
                        vals);
     return hipGetLastError();
+}
+
+// The T3 kernel is compiled once per material kind (homogeneous / layered),
+// so that the membrane-row skip and the kind's branches resolve at compile
+// time and the accumulation over the rows of C unrolls.
+template <int NEN, int NQP, int NR, int NE, int WG, int SPLIT, int RS = FS_T3_RS, bool FUSED = false>
+static hipError_t launch_elem(const ElemParams &p, const double *x, const double *y, const double *z,
+                              const int32_t *conn, const double *nx, const double *ny, const double *nz,
+                              const uint8_t *valid, double *Kb, int *status, hipStream_t st,
+                              const ScatterPlan *sp = nullptr, const TilePlan *tp = nullptr, double *vals = nullptr) {
+    if (p.composite)
+        return launch_elem_kind<NEN, NQP, NR, NE, WG, SPLIT, RS, FUSED, true>(p, x, y, z, conn, nx, ny, nz, valid, Kb,
+                                                                             status, st, sp, tp, vals);
+    return launch_elem_kind<NEN, NQP, NR, NE, WG, SPLIT, RS, FUSED, false>(p, x, y, z, conn, nx, ny, nz, valid, Kb,
+                                                                          status, st, sp, tp, vals);
 }
 
 // lanes per node in the G2/G3 phases of the Q4 kernel (1 or 2)
