@@ -57,8 +57,10 @@ struct ElemParams {
     double *split_dr;
     // Fused assembly (TilePlan): workgroup b computes the elements
     // tile_elem[b NE .. b NE + NE - 1] (element numbers; a tile's unused
-    // slots repeat one of its elements) instead of ebegin + b NE + le.
+    // slots repeat one of its elements) instead of ebegin + b NE + le;
+    // tile_conn[(b NE + le) nen + a] is node a of that element.
     const int32_t *tile_elem;
+    const int32_t *tile_conn;
 };
 
 // Fused assembly: the element kernel of each tile (a compact patch of NE

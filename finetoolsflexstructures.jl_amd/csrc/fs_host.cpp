@@ -196,11 +196,11 @@ struct DevBuf {
 struct TileStore {
     int ne = 0;  // elements per tile it was built for (0: not built)
     int64_t ntiles = 0, nent = 0, nbnd = 0, nslots = 0;
-    DevBuf tile_elem, tp_ptr, trec, tcontrib, tslot, brec, bp_ptr, partial;
+    DevBuf tile_elem, tile_conn, tp_ptr, trec, tcontrib, tslot, brec, bp_ptr, partial;
     void reset() {
         ne = 0;
         ntiles = nent = nbnd = nslots = 0;
-        for (DevBuf *b : {&tile_elem, &tp_ptr, &trec, &tcontrib, &tslot, &brec, &bp_ptr, &partial})
+        for (DevBuf *b : {&tile_elem, &tile_conn, &tp_ptr, &trec, &tcontrib, &tslot, &brec, &bp_ptr, &partial})
             b->release();
     }
 };
@@ -1165,7 +1165,12 @@ int build_tiles(fs_model *m, int which, int NE) {
             tslot[q] = en.boundary ? bfill[bidx[en.p]]++ : -1;
         }
     int st;
-    if ((st = upload(c, T.tile_elem, telem.data(), telem.size())) || (st = upload(c, T.tp_ptr, tp_ptr.data(), tp_ptr.size())) ||
+    // the tiles' own connectivity (node a of tile element le at (t NE + le) nen + a)
+    std::vector<int32_t> tconn(telem.size() * nen);
+    for (size_t i = 0; i < telem.size(); ++i)
+        for (int a = 0; a < nen; ++a) tconn[i * nen + a] = m->conn[(size_t)telem[i] * nen + a];
+    if ((st = upload(c, T.tile_elem, telem.data(), telem.size())) || (st = upload(c, T.tile_conn, tconn.data(), tconn.size())) ||
+        (st = upload(c, T.tp_ptr, tp_ptr.data(), tp_ptr.size())) ||
         (st = upload(c, T.trec, trec.data(), trec.size())) || (st = upload(c, T.tcontrib, tcw.data(), tcw.size())) ||
         (st = upload(c, T.tslot, tslot.data(), tslot.size())) ||
         (st = upload(c, T.brec, brec.data(), brec.size())) ||
@@ -1923,6 +1928,7 @@ int fs_model_stiffness(fs_model *m, const fs_shell_params *p, int block, double 
         ep.ebegin = 0;
         ep.eend = (int32_t)(B.tiles.ntiles * NE);
         ep.tile_elem = B.tiles.tile_elem.as<int32_t>();
+        ep.tile_conn = B.tiles.tile_conn.as<int32_t>();
         const fs::ScatterPlan sp = plan_of(B);
         const fs::TilePlan tp = tileplan_of(B);
         hipEvent_t t0, t1, t2;
@@ -1947,6 +1953,7 @@ int fs_model_stiffness(fs_model *m, const fs_shell_params *p, int block, double 
         (void)hipEventDestroy(t1);
         (void)hipEventDestroy(t2);
         ep.tile_elem = nullptr;
+        ep.tile_conn = nullptr;
         ep.eend = (int32_t)m->E;
         if ((s = finish_elements(m, ep, st))) return s;
         m->timing[0] = ms_elem;

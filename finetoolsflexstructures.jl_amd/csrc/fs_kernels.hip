@@ -502,8 +502,8 @@ __device__ __forceinline__ void write_pair_lane_rec(const int4 a, const int4 b, 
 // Whether element e's block (a, b) is staged transposed: the fused output
 // phase reads every block in the orientation of its node pair (I < J), the
 // transposed bit of the host's contribution code.
-__device__ __forceinline__ bool stage_transposed(const int32_t *__restrict__ conn, int E, int e, int a, int b) {
-    return a != b && conn[a * E + e] > conn[b * E + e];
+__device__ __forceinline__ bool stage_transposed(int a, int b, int node_a, int node_b) {
+    return a != b && node_a > node_b;
 }
 
 struct TileItem {
@@ -584,6 +584,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     const int e0 = p.ebegin + tblk * NE;
     const int ne = FUSED ? NE : ((p.eend - e0) < NE ? (p.eend - e0) : NE);
     auto elem = [&](int le) -> int { return FUSED ? p.tile_elem[e0 + le] : e0 + le; };
+    // node a of tile element le: the fused tiles carry their own copy of the
+    // connectivity, so coordinate and normal loads wait on one index load
+    auto enode = [&](int le, int a) -> int { return FUSED ? p.tile_conn[(e0 + le) * NEN + a] : conn[a * E + e0 + le]; };
 
     // ---------------- G1: geometry per element (FEMMShellT3FFModule.jl:269-321,465-484) ----------------
     // One thread per element; each part of the record (frame and gradients,
@@ -595,7 +598,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         const int e = elem(le);
         double X[NEN][3];
         for (int a = 0; a < NEN; ++a) {
-            const int n = conn[a * E + e];
+            const int n = enode(le, a);
             X[a][0] = xs[n];
             X[a][1] = ys[n];
             X[a][2] = zs[n];
@@ -717,9 +720,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     auto row_of = [&](int i) { return RS == 1 ? i : (i < 3 ? 3 * h2 + i : 6 + h2 * (T::NSH / 2) + (i - 3)); };
     double A[3][3];
     if (gact) {
-        const int e = elem(le2);
         const double *g = sG + (le2 * NQP + q2) * T::GEO;
-        const int node = conn[j2 * E + e];
+        const int node = enode(le2, j2);
         double nk_e[3];
         if (valid[node]) {
             const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
@@ -869,7 +871,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         if (NEN == 3 && ba == bb) {
             // T3 drilling in the nodal basis (:714-727) as a global rank-one term
             const int e = elem(le3);
-            const int node = conn[ba * E + e];
+            const int node = enode(le3, ba);
             const double *ms = sM + le3 * T::MISC;
             const int grp = p.elem_group ? p.elem_group[e] : 0;
             const double dscale = COMP ? p.drill_scale : p.g_homog[16 * grp + 15];
@@ -899,7 +901,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     __syncthreads();  // every read of sC and sG is done: reuse them as the output stage
     if (cact) {
         double *ks = sC + (le3 * T::NBLK + blk) * 36;
-        if (FUSED && stage_transposed(conn, E, elem(le3), ba, bb)) {
+        if (FUSED && stage_transposed(ba, bb, enode(le3, ba), enode(le3, bb))) {
             for (int r = 0; r < 6; ++r)
                 for (int c = 0; c < 6; ++c) ks[c * 6 + r] = acc[r][c];
         } else {
@@ -972,6 +974,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     const int e0 = p.ebegin + tblk * NE;
     const int ne = FUSED ? NE : ((p.eend - e0) < NE ? (p.eend - e0) : NE);
     auto elem = [&](int le) -> int { return FUSED ? p.tile_elem[e0 + le] : e0 + le; };
+    // node a of tile element le: the fused tiles carry their own copy of the
+    // connectivity, so coordinate and normal loads wait on one index load
+    auto enode = [&](int le, int a) -> int { return FUSED ? p.tile_conn[(e0 + le) * NEN + a] : conn[a * E + e0 + le]; };
 
     // ---------------- G1: geometry per (element, Gauss point) ----------------
     // With two or more waves, wave 0 computes the frame, gradients and
@@ -988,7 +993,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             const int e = elem(le);
             double X[4][3];
             for (int a = 0; a < 4; ++a) {
-                const int n = conn[a * E + e];
+                const int n = enode(le, a);
                 X[a][0] = xs[n];
                 X[a][1] = ys[n];
                 X[a][2] = zs[n];
@@ -1085,7 +1090,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     int grp2 = 0;
     if (!UG && gact && p.elem_group) grp2 = p.elem_group[elem(le2)];
     if (gact) {
-        node2 = conn[j2 * E + elem(le2)];
+        node2 = enode(le2, j2);
         valid2 = valid[node2];
         n20 = nxs[node2];
         n21 = nys[node2];
@@ -1271,7 +1276,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     // the nodal normal; -1 marks a node the average skips
     if (tid < NE * NEN && tid / NEN < ne) {
         const int le = tid / NEN, k = tid % NEN;
-        const int node = conn[k * E + elem(le)];
+        const int node = enode(le, k);
         double t = -1.0;
         if (valid[node]) {
             const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
@@ -1312,7 +1317,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             ++cnt;
         }
         const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * dscale;
-        const int node = conn[ba * E + e];
+        const int node = enode(le3, ba);
         if (kavg != 0.0 && valid[node]) {
             const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
             if (sqrt(n0 * n0 + n1 * n1 + n2 * n2) != 0.0) {
@@ -1331,7 +1336,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     // ---------------- stage and store ----------------
     if (cact) {
         double *ks = smem + (le3 * NBLK + blk) * 36;  // G and C regions are dead here
-        if (FUSED && stage_transposed(conn, E, elem(le3), ba, bb)) {
+        if (FUSED && stage_transposed(ba, bb, enode(le3, ba), enode(le3, bb))) {
             for (int r = 0; r < 6; ++r)
                 for (int c = 0; c < 6; ++c) ks[c * 6 + r] = acc[r][c];
         } else {
@@ -2800,7 +2805,7 @@ static hipError_t launch_elem_kind(const ElemParams &p, const double *x, const d
     }
     int nblocks;
     if constexpr (FUSED) {
-        if (!sp || !tp || !vals || !p.tile_elem || p.ebegin != 0) return hipErrorInvalidValue;
+        if (!sp || !tp || !vals || !p.tile_elem || !p.tile_conn || p.ebegin != 0) return hipErrorInvalidValue;
         nblocks = (int)tp->ntiles;
     } else {
         if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
@@ -2851,7 +2856,7 @@ static hipError_t launch_q4_seq_ug(const ElemParams &p, const double *x, const d
     }
     int nblocks;
     if constexpr (FUSED) {
-        if (!sp || !tp || !vals || !p.tile_elem || p.ebegin != 0) return hipErrorInvalidValue;
+        if (!sp || !tp || !vals || !p.tile_elem || !p.tile_conn || p.ebegin != 0) return hipErrorInvalidValue;
         nblocks = (int)tp->ntiles;
     } else {
         if (p.ebegin < 0 || p.eend > p.nelem || p.ebegin > p.eend) return hipErrorInvalidValue;
