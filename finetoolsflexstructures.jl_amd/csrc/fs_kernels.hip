@@ -587,6 +587,20 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     // node a of tile element le: the fused tiles carry their own copy of the
     // connectivity, so coordinate and normal loads wait on one index load
     auto enode = [&](int le, int a) -> int { return FUSED ? p.tile_conn[(e0 + le) * NEN + a] : conn[a * E + e0 + le]; };
+    // node data of the thread's G2 lane (element, node), loaded before G1 so
+    // that the loads overlap it
+    uint8_t valid2 = 0;
+    double n20 = 0.0, n21 = 0.0, n22 = 0.0;
+    {
+        const int le = tid / (NQP * NEN * RS), j = (tid / RS) % NEN;
+        if (tid < NE * NQP * NEN * RS && le < ne) {
+            const int node = enode(le, j);
+            valid2 = valid[node];
+            n20 = nxs[node];
+            n21 = nys[node];
+            n22 = nzs[node];
+        }
+    }
 
     // ---------------- G1: geometry per element (FEMMShellT3FFModule.jl:269-321,465-484) ----------------
     // One thread per element; each part of the record (frame and gradients,
@@ -721,12 +735,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     double A[3][3];
     if (gact) {
         const double *g = sG + (le2 * NQP + q2) * T::GEO;
-        const int node = enode(le2, j2);
         double nk_e[3];
-        if (valid[node]) {
-            const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
+        if (valid2) {
             for (int i = 0; i < 3; ++i)
-                nk_e[i] = g[T::G_EG + 0 * 3 + i] * n0 + g[T::G_EG + 1 * 3 + i] * n1 + g[T::G_EG + 2 * 3 + i] * n2;
+                nk_e[i] = g[T::G_EG + 0 * 3 + i] * n20 + g[T::G_EG + 1 * 3 + i] * n21 + g[T::G_EG + 2 * 3 + i] * n22;
         } else {
             nk_e[0] = 0.0;
             nk_e[1] = 0.0;
@@ -978,6 +990,25 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     // connectivity, so coordinate and normal loads wait on one index load
     auto enode = [&](int le, int a) -> int { return FUSED ? p.tile_conn[(e0 + le) * NEN + a] : conn[a * E + e0 + le]; };
 
+    // thread roles: G2/G3 lane (element, node, row half) -- RS lanes share
+    // the eight rows of C of a node
+    static_assert(RS == 1 || RS == 2, "row split");
+    const bool gact = tid < NE * NEN * RS && (tid / (NEN * RS)) < ne;
+    const int le2 = tid / (NEN * RS), j2 = (tid / RS) % NEN, h2 = tid % RS;
+    int node2 = 0;
+    uint8_t valid2 = 0;
+    double n20 = 0.0, n21 = 0.0, n22 = 0.0;
+    // material group of the lane's element: with one group (UG) a constant,
+    // so the material factor rows come through scalar loads of one record
+    int grp2 = 0;
+    if (!UG && gact && p.elem_group) grp2 = p.elem_group[elem(le2)];
+    if (gact) {  // loaded before G1 so that the loads overlap it
+        node2 = enode(le2, j2);
+        valid2 = valid[node2];
+        n20 = nxs[node2];
+        n21 = nys[node2];
+        n22 = nzs[node2];
+    }
     // ---------------- G1: geometry per (element, Gauss point) ----------------
     // With two or more waves, wave 0 computes the frame, gradients and
     // material scales of every (element, Gauss point) of the tile and wave 1
@@ -1077,25 +1108,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         }
     }
 
-    // thread roles: G2/G3 lane (element, node, row half) -- RS lanes share
-    // the eight rows of C of a node
-    static_assert(RS == 1 || RS == 2, "row split");
-    const bool gact = tid < NE * NEN * RS && (tid / (NEN * RS)) < ne;
-    const int le2 = tid / (NEN * RS), j2 = (tid / RS) % NEN, h2 = tid % RS;
-    int node2 = 0;
-    uint8_t valid2 = 0;
-    double n20 = 0.0, n21 = 0.0, n22 = 0.0;
-    // material group of the lane's element: with one group (UG) a constant,
-    // so the material factor rows come through scalar loads of one record
-    int grp2 = 0;
-    if (!UG && gact && p.elem_group) grp2 = p.elem_group[elem(le2)];
-    if (gact) {
-        node2 = enode(le2, j2);
-        valid2 = valid[node2];
-        n20 = nxs[node2];
-        n21 = nys[node2];
-        n22 = nzs[node2];
-    }
     const bool cact = tid < NE * NBLK && (tid / NBLK) < ne;
     const int le3 = tid / NBLK, blk = tid % NBLK;
     int ba = 0, bb = 0;
