@@ -601,6 +601,32 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             n22 = nzs[node];
         }
     }
+    // block lane (element, node pair) of the C phase
+    const bool cthread = tid < NE * T::NBLK;
+    const int le3 = tid / T::NBLK;
+    const int blk = tid % T::NBLK;
+    int ba = 0, bb = 0;
+    {
+        int cnt = 0;
+        for (int a = 0; a < NEN; ++a)
+            for (int b = a; b < NEN; ++b) {
+                if (cnt == blk) {
+                    ba = a;
+                    bb = b;
+                }
+                ++cnt;
+            }
+    }
+    const bool cact = cthread && (le3 < ne);
+    // fused: the block lane's node data, loaded now as well: the flag of
+    // node a (the drilling term of a diagonal block) and the staged
+    // orientation (the two-kernel form measured faster loading the flag late)
+    bool cvalid = false, ctrans = false;
+    if (FUSED && cact) {
+        const int na = enode(le3, ba), nb = enode(le3, bb);
+        cvalid = valid[na];
+        ctrans = stage_transposed(ba, bb, na, nb);
+    }
 
     // ---------------- G1: geometry per element (FEMMShellT3FFModule.jl:269-321,465-484) ----------------
     // One thread per element; each part of the record (frame and gradients,
@@ -829,22 +855,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     __syncthreads();
 
     // ---------------- C: node-pair blocks ----------------
-    const bool cthread = tid < NE * T::NBLK;
-    const int le3 = tid / T::NBLK;
-    const int blk = tid % T::NBLK;
-    int ba = 0, bb = 0;
-    {
-        int cnt = 0;
-        for (int a = 0; a < NEN; ++a)
-            for (int b = a; b < NEN; ++b) {
-                if (cnt == blk) {
-                    ba = a;
-                    bb = b;
-                }
-                ++cnt;
-            }
-    }
-    const bool cact = cthread && (le3 < ne);
     constexpr bool split_sh = SPLIT == 1;  // shear rows leave as factors
     constexpr bool split_dr = SPLIT != 0;  // drilling blocks leave separately
     double acc[6][6];
@@ -883,7 +893,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         if (NEN == 3 && ba == bb) {
             // T3 drilling in the nodal basis (:714-727) as a global rank-one term
             const int e = elem(le3);
-            const int node = enode(le3, ba);
+            if (!FUSED) cvalid = valid[enode(le3, ba)];
             const double *ms = sM + le3 * T::MISC;
             const int grp = p.elem_group ? p.elem_group[e] : 0;
             const double dscale = COMP ? p.drill_scale : p.g_homog[16 * grp + 15];
@@ -893,8 +903,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             if (split_dr) {
                 double *dd = p.split_dr + ((size_t)e * NEN + ba) * 9;
                 for (int r = 0; r < 3; ++r)
-                    for (int c = 0; c < 3; ++c) dd[r * 3 + c] = valid[node] ? kavg * (gv[r] * gv[c]) : 0.0;
-            } else if (valid[node]) {
+                    for (int c = 0; c < 3; ++c) dd[r * 3 + c] = cvalid ? kavg * (gv[r] * gv[c]) : 0.0;
+            } else if (cvalid) {
                 for (int r = 0; r < 3; ++r)
                     for (int c = 0; c < 3; ++c) acc[3 + r][3 + c] += kavg * (gv[r] * gv[c]);
             }
@@ -913,7 +923,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     __syncthreads();  // every read of sC and sG is done: reuse them as the output stage
     if (cact) {
         double *ks = sC + (le3 * T::NBLK + blk) * 36;
-        if (FUSED && stage_transposed(ba, bb, enode(le3, ba), enode(le3, bb))) {
+        if (FUSED && ctrans) {
             for (int r = 0; r < 6; ++r)
                 for (int c = 0; c < 6; ++c) ks[c * 6 + r] = acc[r][c];
         } else {
@@ -1009,6 +1019,23 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         n21 = nys[node2];
         n22 = nzs[node2];
     }
+    // block lane (element, node pair) of the accumulation, and its staged
+    // orientation, loaded now too
+    const bool cact = tid < NE * NBLK && (tid / NBLK) < ne;
+    const int le3 = tid / NBLK, blk = tid % NBLK;
+    int ba = 0, bb = 0;
+    {
+        int cnt = 0;
+        for (int a = 0; a < NEN; ++a)
+            for (int b = a; b < NEN; ++b) {
+                if (cnt == blk) {
+                    ba = a;
+                    bb = b;
+                }
+                ++cnt;
+            }
+    }
+    const bool ctrans = FUSED && cact && stage_transposed(ba, bb, enode(le3, ba), enode(le3, bb));
     // ---------------- G1: geometry per (element, Gauss point) ----------------
     // With two or more waves, wave 0 computes the frame, gradients and
     // material scales of every (element, Gauss point) of the tile and wave 1
@@ -1108,20 +1135,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
         }
     }
 
-    const bool cact = tid < NE * NBLK && (tid / NBLK) < ne;
-    const int le3 = tid / NBLK, blk = tid % NBLK;
-    int ba = 0, bb = 0;
-    {
-        int cnt = 0;
-        for (int a = 0; a < NEN; ++a)
-            for (int b = a; b < NEN; ++b) {
-                if (cnt == blk) {
-                    ba = a;
-                    bb = b;
-                }
-                ++cnt;
-            }
-    }
     constexpr bool split_sh = SPLIT == 1;  // shear rows leave as factors
     constexpr bool split_dr = SPLIT != 0;  // drilling blocks leave separately
     double acc[6][6];
@@ -1348,7 +1361,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
     // ---------------- stage and store ----------------
     if (cact) {
         double *ks = smem + (le3 * NBLK + blk) * 36;  // G and C regions are dead here
-        if (FUSED && stage_transposed(ba, bb, enode(le3, ba), enode(le3, bb))) {
+        if (ctrans) {
             for (int r = 0; r < 6; ++r)
                 for (int c = 0; c < 6; ++c) ks[c * 6 + r] = acc[r][c];
         } else {
