@@ -1296,19 +1296,22 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             for (int c = 0; c < 3; ++c) ms[r * 3 + c] = acc[3 + r][3 + c] + accs[r][c];
     }
     __syncthreads();
-    // trace term of each element node, once per (element, node): thread
-    // (element, k) projects node k's rotation block onto the plane normal to
-    // the nodal normal; -1 marks a node the average skips
-    if (tid < NE * NEN && tid / NEN < ne) {
-        const int le = tid / NEN, k = tid % NEN;
-        const int node = enode(le, k);
+    // trace term of each element node, once per (element, node), by the
+    // node's first G3 lane (whose node flag and normal are in registers):
+    // projects node k's rotation block onto the plane normal to the nodal
+    // normal; -1 marks a node the average skips.  The lane then leaves the
+    // node's flag (valid, nonzero normal) and normal in place of the block,
+    // for the diagonal-block lanes below.
+    if (gact && h2 == 0) {
+        const int le = le2, k = j2;
+        double *kr = sM + le * MISC + 9 * k;
         double t = -1.0;
-        if (valid[node]) {
-            const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
-            const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        bool counted = false;
+        if (valid2) {
+            const double nn = sqrt(n20 * n20 + n21 * n21 + n22 * n22);
             if (nn != 0.0) {
-                const double *kr = sM + le * MISC + 9 * k;
-                const double u[3] = {n0 / nn, n1 / nn, n2 / nn};
+                counted = true;
+                const double u[3] = {n20 / nn, n21 / nn, n22 / nn};
                 double Pm[3][3], tmp[3][3];
                 for (int a = 0; a < 3; ++a)
                     for (int b = 0; b < 3; ++b) Pm[a][b] = ((a == b) ? 1.0 : 0.0) - u[a] * u[b];
@@ -1322,6 +1325,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             }
         }
         sM[le * MISC + 36 + k] = t;
+        kr[0] = counted ? 1.0 : 0.0;
+        kr[1] = n20;
+        kr[2] = n21;
+        kr[3] = n22;
     }
     __syncthreads();
     const int grpd = UG ? 0 : (cact ? (p.elem_group ? p.elem_group[elem(le3)] : 0) : 0);
@@ -1342,11 +1349,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(2))) void el
             ++cnt;
         }
         const double kavg = (cnt == 0) ? 0.0 : (sum / cnt) * dscale;
-        const int node = enode(le3, ba);
-        if (kavg != 0.0 && valid[node]) {
-            const double n0 = nxs[node], n1 = nys[node], n2 = nzs[node];
-            if (sqrt(n0 * n0 + n1 * n1 + n2 * n2) != 0.0) {
-                const double nv[3] = {n0, n1, n2};
+        const double *nrec = sM + le3 * MISC + 9 * ba;  // node ba: flag, normal
+        if (kavg != 0.0) {
+            if (nrec[0] != 0.0) {
+                const double nv[3] = {nrec[1], nrec[2], nrec[3]};
                 if (split_dr) {
                     double *dd = p.split_dr + ((size_t)e * NEN + ba) * 9;
                     for (int r = 0; r < 3; ++r)
