@@ -4,9 +4,9 @@ two-kernel form (element blocks through HBM + pair scatter, sums in element
 order) and against the oracle: identical pattern, values within 1e-13 of
 the scatter form and 1e-12 of the oracle, bitwise reproducible, for every
 element family and shear formulation, both blocks, element numberings
-shuffled so that tiles and element order disagree, and a fan of 48
+shuffled so that tiles and element order disagree, a fan of 48
 triangles around one node (a pair with more contributions in one tile than
-one tile entry holds)."""
+one tile entry holds), a one-element model and two disconnected patches."""
 import numpy as np
 import pytest
 
@@ -101,4 +101,46 @@ def test_fused_fan_hub():
         Ko = O.assemble_csr(3, conn, d.dofnums, nrows, Ke)
         assert np.linalg.norm(Kt.data - Ko.data) <= 1e-12 * np.linalg.norm(Ko.data)
     f._model.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("nen", [3, 4])
+def test_fused_single_element_and_split_patches(nen):
+    """Tiles that are mostly empty and tiles cut by components: a model of
+    one element (one tile whose unused slots repeat that element), and a
+    model of two disconnected patches of 3 and 14 (T3: 6 and 28) elements,
+    the second curved and rotated, each patch filling a partial tile of its
+    own.  Fused form against the two-kernel form and the oracle."""
+    print(f"[gpu] fused assembly: one element and two patches, nen={nen}", flush=True)
+    ctx = _lib.Context(0)
+    xa, ca = SC.distorted_mesh(nen, 1, 1, seed=3, warp=0.3)
+    one = (xa, ca[:1])
+    xb, cb = SC.distorted_mesh(nen, 3, 1, seed=4, warp=0.2)
+    xc, cc = SC.distorted_mesh(nen, 7, 2, seed=6, warp=0.6)
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    xc = xc @ R.T + np.array([3.0, -1.0, 0.5])
+    two = (np.vstack([xb, xc]), np.vstack([cb, cc + xb.shape[0]]))
+    for name, (xyz, conn) in (("one element", one), ("two patches", two)):
+        d = fe.DofField(xyz.shape[0])
+        d.setebc([0], [1, 2, 3, 4, 5, 6])
+        d.setebc([xyz.shape[0] - 1], [3])
+        d.numberdofs()
+        f = FM.make(conn, MatDeforElastIso(70e9, 0.33, 2700.0), thickness=0.01, ctx=ctx)
+        f.associategeometry(xyz)
+        forms = _both_forms(ctx, f, xyz, d)
+        Dps, Dt = O.shell_D(O.iso_D6(70e9, 0.33))
+        nrm, val = O.normals(nen, xyz, conn)
+        Ke = O.element_matrices(nen, xyz, conn, nrm, val, Dps=Dps, Dt=Dt, t=0.01)
+        for i, nrows in enumerate((d.nalldofs, d.nfreedofs)):
+            Ks, Kt = forms[0][i], forms[1][i]
+            assert np.array_equal(Ks.indptr, Kt.indptr) and np.array_equal(Ks.indices, Kt.indices), name
+            assert np.linalg.norm(Kt.data - Ks.data) <= 1e-13 * np.linalg.norm(Ks.data), name
+            KT = Kt.T.tocsr()
+            KT.sort_indices()
+            assert np.array_equal(KT.data, Kt.data), name
+            Ko = O.assemble_csr(nen, conn, d.dofnums, nrows, Ke)
+            assert np.array_equal(Ko.indices, Kt.indices), name
+            assert np.linalg.norm(Kt.data - Ko.data) <= 1e-12 * np.linalg.norm(Ko.data), name
+        f._model.close()
     ctx.close()
