@@ -327,7 +327,7 @@ class Model:
         return cm.reshape(6, self.N).T.copy(), int(nf.value)
 
     def load_points(self, npts):
-        out = np.zeros((self.E, npts, 3))
+        out = np.empty((self.E, npts, 3))  # every entry is written
         self.ctx.check(self.ctx.lib.fs_model_load_points(self.h, int(npts), ptr(out)), "fs_model_load_points")
         return out
 
@@ -363,7 +363,7 @@ class Model:
             else:
                 raise ValueError("ocs must be one 3 x 3 matrix or nlist x npts x 3 x 3")
             oc = np.ascontiguousarray(np.swapaxes(m.reshape(-1, 3, 3), 1, 2))  # column-major matrices
-        out = np.zeros((nlist, npts, 8))
+        out = np.empty((nlist, npts, 8))  # every entry is written
         mis = C.c_int32(0)
         self.ctx.check(self.ctx.lib.fs_model_resultants(self.h, C.byref(params), ptr(u), int(nlist),
                                                         ptr(fl, C.c_int32), int(kind), ptr(oc), int(flags), ptr(out),

@@ -376,8 +376,8 @@ class _ShellFEMM:
         """Location (nlist x npts x 3) and, if asked, tangent matrix (nlist x
         npts x 3 x 2) of every integration point of the listed elements: the
         centroid and J0 for T3 (FEMMShellT3FFModule.jl:908-911), loc and J of
-        locjac! at the Gauss points for Q4."""
-        X = xyz[self.conn[fl]]
+        locjac! at the Gauss points for Q4.  fl None: all elements."""
+        X = xyz[self.conn] if fl is None else xyz[self.conn[fl]]
         if self.nen == 3:
             loc = X.mean(axis=1)[:, None, :]
             J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]], axis=2)[:, None] if tangents else None
@@ -404,17 +404,25 @@ class _ShellFEMM:
         xyz = np.asarray(geom0, dtype=float)
         m = self._ensure_model(xyz)
         vals = np.asarray(getattr(u, "values", u), dtype=float).reshape(xyz.shape[0], 6)
-        fl = np.arange(self.conn.shape[0]) if felist is None else np.asarray(felist, dtype=np.int64).ravel()
-        if fl.size and (fl.min() < 0 or fl.max() >= self.conn.shape[0]):
+        fl = None if felist is None else np.asarray(felist, dtype=np.int64).ravel()
+        if fl is not None and fl.size and (fl.min() < 0 or fl.max() >= self.conn.shape[0]):
             raise IndexError("inspectintegpoints: element number out of range")
         ocs = outputcsys
         if ocs is None:
             ocs = self.layup_groups[0][0].csys if self.composite else self.mcsys
-        loc, J = self._points(xyz, fl, tangents=callable(ocs))
+        if fl is None and not callable(ocs):
+            # every element, no tangents needed: the point locations come from
+            # the device (the load-point kernel's TriRule(1) / GaussRule(2, 2)
+            # points are these integration points, in this order)
+            npts = 1 if self.nen == 3 else 4
+            loc, J = m.load_points(npts).reshape(self.conn.shape[0], npts, 3), None
+        else:
+            loc, J = self._points(xyz, fl, tangents=callable(ocs))
         if callable(ocs):
             npts = loc.shape[1]
-            ocs = np.array([[np.asarray(ocs(loc[l, j], J[l, j], int(fl[l]), j), dtype=float)
-                             for j in range(npts)] for l in range(fl.shape[0])]).reshape(-1, 3, 3)
+            enum = np.arange(self.conn.shape[0]) if fl is None else fl
+            ocs = np.array([[np.asarray(ocs(loc[l, j], J[l, j], int(enum[l]), j), dtype=float)
+                             for j in range(npts)] for l in range(enum.shape[0])]).reshape(-1, 3, 3)
         flags = _lib.FS_RES_LAYERED_Q4_SINGLE_T if transform_once else 0
         out, mismatch = m.resultants(self._shell_params(), vals, fl, ocs, flags)
         if mismatch:
