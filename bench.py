@@ -586,7 +586,7 @@ def main():
     res_dev = float(np.mean(res_ms))
     b_res = resultant_bytes(E, N, 4, 4)
     ach_res = b_res / (res_dev / 1e3) / 1e9
-    print(f"[bench] resultants: kernel {res_dev * 1e3:.1f} us, call {res_wall:.1f} ms (host copies, point "
+    print(f"[bench] resultants: kernel {res_dev * 1e3:.1f} us, call {res_wall:.1f} ms (copies of u, the results and the point "
           f"locations)", file=sys.stderr, flush=True)
 
     b_asm = assembly_bytes(E, N, 4, nnz)
