@@ -1,6 +1,6 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle.
 
-Tolerances: stiffness within 1e-12 relative Frobenius
+Tolerances (the accuracy contract of the drop-in): stiffness within 1e-12 relative Frobenius
 error with a bit-exact CSR pattern; lowest frequencies from the GPU-assembled
 K and M within 1e-9 of the oracle's; explicit trajectories within 1e-10
 relative to the FP64 oracle trajectory; integer structure bit-exact."""
