@@ -63,6 +63,10 @@ def test_normals_match_oracle(ctx, nen):
     n_o, v_o = O.normals(nen, xyz, conn)
     np.testing.assert_allclose(f._normals, n_o, rtol=0, atol=1e-14)
     assert (f._normal_valid == v_o).all()
+    # a second call recomputes from the coordinates: bitwise the same
+    n1, v1 = f._normals.copy(), f._normal_valid.copy()
+    f.associategeometry(xyz)
+    assert np.array_equal(f._normals, n1) and np.array_equal(f._normal_valid, v1)
     # folded plate: the crease invalidates the fold nodes
     xy, c3 = fe.T3block(1.0, 1.0, 4, 4) if nen == 3 else fe.Q4block(1.0, 1.0, 4, 4)
     fold = fe.xyz3(xy)
