@@ -596,10 +596,15 @@ def main():
     # HBM bytes per assembly (fused element kernel + finalize) from the
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
     # (512 x 512 only), scaled by the copy-kernel calibration
-    traffic = None
+    # and the FP64 operations the fused element kernel issues per launch
+    # (SQ_INSTS_VALU_*_F64 of the same summary), against its live duration
+    traffic, elem_flops = None, None
     tf = os.path.join(ROOT, "profiles", "traffic_q4_cylinder_512x512.json")
     if args.n == 512 and os.path.exists(tf):
-        traffic = json.load(open(tf)).get("fused_assembly_bytes_per_launch")
+        tj = json.load(open(tf))
+        traffic = tj.get("fused_assembly_bytes_per_launch")
+        elem_flops = tj.get("fused_elem_kernel", {}).get("fp64_ops_issued")
+    elem_s = ta["fused"]["kernel1_ms"] / 1e3
 
     out = {
         "metric": "FP64 shell elements assembled/s + DOF-steps/s; p50/p99 latency vs plain GPU",
@@ -627,6 +632,12 @@ def main():
         "roofline_cd": {"bound": "hbm", "achieved": ach_cd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach_cd / HBM_PEAK_GBS, "kernel": "cd_node_step_kernel (8 lanes per node)",
                         "algorithmic_bytes": b_cd},
+        "roofline_elem": (None if elem_flops is None else
+                          {"bound": "fp64-valu", "achieved": elem_flops / elem_s / 1e12, "peak": FP64_PEAK_TFS,
+                           "unit": "TFLOP/s", "frac": elem_flops / elem_s / 1e12 / FP64_PEAK_TFS,
+                           "kernel": "fused Q4 element kernel (FP64 operations issued, from the committed "
+                                     "counters, over its live duration)",
+                           "fp64_ops_per_launch": elem_flops}),
         "kernels_ms": {"fused_element_kernel": ta["fused"]["kernel1_ms"],
                        "pair_finalize_kernel": ta["fused"]["kernel2_ms"],
                        "two_kernel_form": {"elem_kernel": ta["two_kernel"]["kernel1_ms"],
