@@ -463,10 +463,45 @@ def copy_probe(nbytes):
     return 2 * nbytes
 
 
-def hbm_measured(nbytes=2 << 30, reps=5):
-    """What plain streaming kernels reach on this device: a device-to-device
-    copy (read + write) and a sum (read) of nbytes of FP64, timed with events
-    on torch's stream; a practical ceiling beside the published 8 TB/s."""
+def hip_copy_GBs(nbytes=2 << 30, reps=5):
+    """Device-to-device hipMemcpyAsync of nbytes (read + write), timed with HIP
+    events on the null stream through the HIP runtime itself."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+
+    def ok(status, what):
+        if status != 0:
+            raise RuntimeError(f"{what} returned {status}")
+
+    try:
+        ok(hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)), "hipMalloc")
+        ok(hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)), "hipMalloc")
+        ok(hip.hipMemset(src, 1, C.c_size_t(nbytes)), "hipMemset")
+        ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        best = None
+        for _ in range(reps + 1):
+            ok(hip.hipEventRecord(e0, None), "hipEventRecord")
+            ok(hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None), "hipMemcpyAsync")  # device to device
+            ok(hip.hipEventRecord(e1, None), "hipEventRecord")
+            ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+            ms = C.c_float()
+            ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+            best = ms.value if best is None else min(best, ms.value)
+        return 2 * nbytes / (best / 1e3) / 1e9
+    finally:
+        for ev in (e0, e1):
+            if ev.value:
+                hip.hipEventDestroy(ev)
+        for ptr in (src, dst):
+            if ptr.value:
+                hip.hipFree(ptr)
+
+
+def torch_stream_GBs(nbytes, reps):
+    """Copy (read + write) and sum (read) of nbytes of FP64 on torch's stream,
+    timed with torch's events."""
     import torch
     a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
@@ -487,6 +522,25 @@ def hbm_measured(nbytes=2 << 30, reps=5):
     del a, b
     torch.cuda.empty_cache()
     return out
+
+
+def hbm_measured(nbytes=2 << 30, reps=5):
+    """What plain streaming kernels reach on this device: a device-to-device
+    copy (read + write) and a sum (read) of nbytes of FP64 (torch_stream_GBs);
+    a practical ceiling beside the published 8 TB/s.  Where torch cannot open
+    the device, the copy is timed through the HIP runtime (hip_copy_GBs,
+    smaller if the allocation fails) and the read rate is not measured; where
+    that fails too, both are null."""
+    try:
+        return torch_stream_GBs(nbytes, reps)
+    except Exception as exc:  # torch's own device discovery or allocation failed
+        print(f"[bench] streaming kernels: torch unavailable ({exc}); HIP copy only", file=sys.stderr, flush=True)
+    for nb in (nbytes, nbytes // 8):
+        try:
+            return {"copy_GBs": hip_copy_GBs(nb, reps), "read_GBs": None, "source": f"hipMemcpyAsync of {nb} bytes"}
+        except Exception as exc:
+            print(f"[bench] streaming kernels: HIP copy of {nb} bytes failed ({exc})", file=sys.stderr, flush=True)
+    return {"copy_GBs": None, "read_GBs": None, "source": "not measured"}
 
 
 def main():
@@ -676,9 +730,11 @@ def main():
     else:
         hm = hbm_measured()
         out["hbm_measured"] = hm
-        out["roofline"]["frac_of_copy"] = ach / hm["copy_GBs"]
-        out["roofline_cd"]["frac_of_read"] = ach_cd / hm["read_GBs"]
-        print(f"[bench] streaming kernels: copy {hm['copy_GBs']:.0f} GB/s, read {hm['read_GBs']:.0f} GB/s",
+        if hm["copy_GBs"]:
+            out["roofline"]["frac_of_copy"] = ach / hm["copy_GBs"]
+        if hm["read_GBs"]:
+            out["roofline_cd"]["frac_of_read"] = ach_cd / hm["read_GBs"]
+        print(f"[bench] streaming kernels: copy {hm['copy_GBs']} GB/s, read {hm['read_GBs']} GB/s",
               file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
