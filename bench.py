@@ -526,15 +526,26 @@ def torch_stream_GBs(nbytes, reps):
 
 def hbm_measured(nbytes=2 << 30, reps=5):
     """What plain streaming kernels reach on this device: a device-to-device
-    copy (read + write) and a sum (read) of nbytes of FP64 (torch_stream_GBs);
-    a practical ceiling beside the published 8 TB/s.  Where torch cannot open
-    the device, the copy is timed through the HIP runtime (hip_copy_GBs,
-    smaller if the allocation fails) and the read rate is not measured; where
-    that fails too, both are null."""
+    copy (read + write) and a sum (read) of nbytes of FP64 (torch_stream_GBs),
+    a practical ceiling beside the published 8 TB/s.  torch runs in a child
+    process: it ships its own HIP runtime, and starting it in this process,
+    after libflexshell has initialised the system one, has failed on some
+    machines and left the process without a device.  If the child fails, the
+    copy is timed here through the HIP runtime (hip_copy_GBs, smaller if the
+    allocation fails) and the read rate is not measured; if that fails too,
+    both are null."""
+    import subprocess
+    code = f"import json, bench; print(json.dumps(bench.torch_stream_GBs({int(nbytes)}, {int(reps)})))"
     try:
-        return torch_stream_GBs(nbytes, reps)
-    except Exception as exc:  # torch's own device discovery or allocation failed
-        print(f"[bench] streaming kernels: torch unavailable ({exc}); HIP copy only", file=sys.stderr, flush=True)
+        r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=600)
+        if r.returncode == 0:
+            out = json.loads(r.stdout.strip().splitlines()[-1])
+            out["source"] = "torch copy_ and sum in a child process"
+            return out
+        print(f"[bench] streaming kernels: torch child failed ({r.stderr.strip().splitlines()[-1:]}); HIP copy only",
+              file=sys.stderr, flush=True)
+    except Exception as exc:
+        print(f"[bench] streaming kernels: torch child failed ({exc}); HIP copy only", file=sys.stderr, flush=True)
     for nb in (nbytes, nbytes // 8):
         try:
             return {"copy_GBs": hip_copy_GBs(nb, reps), "read_GBs": None, "source": f"hipMemcpyAsync of {nb} bytes"}
