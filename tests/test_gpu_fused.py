@@ -38,8 +38,8 @@ def _both_forms(ctx, f, xyz, d):
 
 
 @pytest.mark.parametrize("nen,layered,form,shuffle", [(3, False, 0, False), (3, False, 1, False), (3, True, 0, True),
-                                                      (4, False, 0, False), (4, True, 0, False), (4, False, 0, True),
-                                                      (3, False, 0, True)])
+                                                      (3, True, 1, False), (4, False, 0, False), (4, True, 0, False),
+                                                      (4, False, 0, True), (4, True, 0, True), (3, False, 0, True)])
 def test_fused_matches_scatter_and_oracle(nen, layered, form, shuffle):
     print(f"[gpu] fused assembly nen={nen} layered={layered} form={form} shuffle={shuffle}", flush=True)
     ctx = _lib.Context(0)
